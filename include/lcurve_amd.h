@@ -1,0 +1,242 @@
+/*
+ * lcurve_amd.h -- C ABI of the MI355X light-curve engine.
+ *
+ * Plain C types only (no C++ or HIP types in any signature).  Every entry
+ * point returns an lc_status; on failure lc_last_error() holds the message.
+ *
+ * The reference (Fabmat1/lcurve_re) has no FFI: its light-curve path is
+ * called from C++ through Lcurve::light_curve_comp / light_curve_comp_fast
+ * and the Lcurve::Model JSON/parameter API.  Each entry point below names
+ * the reference interface it replaces; include/lcurve_re/lcurve.h is the
+ * C++ drop-in layer built on top of this ABI.
+ */
+#ifndef LCURVE_AMD_H
+#define LCURVE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef enum lc_status {
+    LC_OK = 0,
+    LC_E_ROCHE_OVERFLOW = 1, /* star larger than its Roche lobe
+                                (light_curve_comp.cpp:135-138, set_star_grid.cpp:64-79) */
+    LC_E_FACE = 2,           /* surface solve failed (face.cpp:229-265)              */
+    LC_E_ECLIPSE_SEARCH = 3, /* pot_min/linmin/dbrent/fblink failed
+                                (pot_min.cpp:98,271,291,401; dbrent.cpp:649)         */
+    LC_E_PARAM = 4,          /* malformed or illegal parameter (model.cpp:10-360)     */
+    LC_E_DEVICE = 5,         /* HIP runtime error                                     */
+    LC_E_UNSUPPORTED = 6,    /* disc / bright spot / iscale: outside this engine      */
+    LC_E_ARG = 7,            /* invalid argument (null pointer, bad size)             */
+    LC_E_LENSING = 8         /* lensing configuration rejected (set_star_grid.cpp:80-100) */
+} lc_status;
+
+/* Precision of the per-face flux arithmetic. */
+enum {
+    LC_PREC_MIXED = 0, /* FP32 per face, FP64 per-phase accumulation (default) */
+    LC_PREC_FP64 = 1   /* FP64 per face in the reference's block order         */
+};
+
+enum { LC_STAR1 = 0, LC_STAR2 = 1 };      /* Roche::PRIMARY / SECONDARY (roche.h:13) */
+enum { LC_LDC_POLY = 0, LC_LDC_CLARET = 1 }; /* LDC::LDCtype (lcurve.h:204)        */
+
+/* ------------------------------------------------------------ parameters */
+/* One physical parameter: Lcurve::Pparam (lcurve.h:159-195). */
+typedef struct lc_pparam {
+    double value, range, dstep;
+    int32_t vary, defined;
+} lc_pparam;
+
+/* Lcurve::Model (lcurve.h:317-421), field for field. */
+typedef struct lc_model {
+    lc_pparam q, iangle, r1, r2, cphi3, cphi4, spin1, spin2;
+    lc_pparam t1, t2;
+    lc_pparam ldc1_1, ldc1_2, ldc1_3, ldc1_4;
+    lc_pparam ldc2_1, ldc2_2, ldc2_3, ldc2_4;
+    lc_pparam velocity_scale, beam_factor1, beam_factor2;
+    lc_pparam t0, period, pdot, deltat;
+    lc_pparam gravity_dark1, gravity_dark2;
+    lc_pparam absorb, slope, quad, cube, third;
+    lc_pparam rdisc1, rdisc2, height_disc, beta_disc;
+    lc_pparam temp_disc, texp_disc, lin_limb_disc, quad_limb_disc;
+    lc_pparam temp_edge, absorb_edge;
+    lc_pparam radius_spot, length_spot, height_spot;
+    lc_pparam expon_spot, epow_spot, angle_spot, yaw_spot;
+    lc_pparam temp_spot, tilt_spot, cfrac_spot;
+    lc_pparam stsp11_long, stsp11_lat, stsp11_fwhm, stsp11_tcen;
+    lc_pparam stsp12_long, stsp12_lat, stsp12_fwhm, stsp12_tcen;
+    lc_pparam stsp13_long, stsp13_lat, stsp13_fwhm, stsp13_tcen;
+    lc_pparam stsp21_long, stsp21_lat, stsp21_fwhm, stsp21_tcen;
+    lc_pparam stsp22_long, stsp22_lat, stsp22_fwhm, stsp22_tcen;
+    lc_pparam uesp_long1, uesp_long2, uesp_lathw, uesp_taper, uesp_temp;
+    /* computational parameters */
+    double delta_phase, lfudge, llo, lhi, phase1, phase2, wavelength;
+    double tperiod, mucrit1, mucrit2;
+    int32_t nlat1f, nlat2f, nlat1c, nlat2c, npole, nlatfill, nlngfill;
+    int32_t roche1, roche2, eclipse1, eclipse2, glens1, use_radii;
+    int32_t gdark_bolom1, gdark_bolom2, limb1, limb2, mirror;
+    int32_t add_disc, nrad, opaque, add_spot, nspot, iscale;
+    int32_t reserved_;
+} lc_model;
+
+/* Lcurve::Datum (lcurve.h:427-445); same field order and padding. */
+typedef struct lc_datum {
+    double time, expose, flux, ferr, weight;
+    int32_t ndiv;
+    int32_t reserved_;
+} lc_datum;
+
+/* Scalar outputs of light_curve_comp (lcurve.h:669-674).  sfac is in/out:
+ * with scale=0, sfac[0] is applied to the curve (light_curve_comp.cpp:562-573). */
+typedef struct lc_scalars {
+    double sfac[5];
+    double wdwarf, chisq, wnok, logg1, logg2, rv1, rv2;
+} lc_scalars;
+
+typedef struct lc_eval_opts {
+    int32_t scale;            /* fit the scale factor (re_scale)                  */
+    int32_t rdata;            /* real data: compute chi**2                         */
+    int32_t info;             /* print grid sizes to stderr                        */
+    int32_t diagnostics;      /* wdwarf/logg/rv (light_curve_comp.cpp:504-596)     */
+    int32_t max_model_points; /* 0: light_curve_comp; >0: light_curve_comp_fast   */
+    int32_t precision;        /* LC_PREC_MIXED or LC_PREC_FP64                     */
+} lc_eval_opts;
+
+/* Limb darkening: Lcurve::LDC (lcurve.h:202-251). */
+typedef struct lc_ldc {
+    double c1, c2, c3, c4, mucrit;
+    int32_t type; /* LC_LDC_POLY / LC_LDC_CLARET */
+    int32_t reserved_;
+} lc_ldc;
+
+/* Structure-of-arrays star grid: Lcurve::FlatGrid (lcurve.h:260-272).
+ * Faces [0,n0) are never eclipsed, [n0,n1) have one range in1/out1
+ * (indexed i-n0), [n1,n) have ragged ranges moff/min_/mout_. */
+typedef struct lc_flat_grid {
+    size_t n, n0, n1;
+    const double *dx, *dy, *dz, *px, *py, *pz, *flux;
+    const double *in1, *out1;
+    const int32_t *moff;
+    const double *min_, *mout_;
+} lc_flat_grid;
+
+/* Sub-phases of one sum: Lcurve::PhaseBatch (lcurve.h:279-295), minus the
+ * scatter fields (w, idx) that the sum does not read. */
+typedef struct lc_phase_batch {
+    size_t n;
+    const double *ex, *ey, *ez, *phin;
+} lc_phase_batch;
+
+/* One star grid as built by set_star_grid (+ set_star_continuum):
+ * a vector<Lcurve::Point> (lcurve.h:49-90) in SoA form.  necl[i] is 0 or 1
+ * (a star can only be eclipsed by its companion). */
+typedef struct lc_star_faces {
+    size_t n;
+    double *px, *py, *pz, *dx, *dy, *dz;
+    float *area, *gravity, *flux;
+    int32_t *necl;
+    double *ingress, *egress;
+} lc_star_faces;
+
+/* Engine timing counters (hipEvent based, on the engine's stream). */
+typedef struct lc_profile {
+    double flux_ms_total;   /* summed duration of flux-sum kernel launches      */
+    double grid_ms_total;   /* face geometry kernels                            */
+    double eclipse_ms_total;/* eclipse search kernels                           */
+    double cont_ms_total;   /* continuum kernels                                */
+    double other_ms_total;  /* phases, reductions, scatter, chi**2              */
+    int64_t flux_launches;
+    int64_t evaluations;    /* light curves evaluated                            */
+    double flux_elements;   /* algorithmic face x sub-phase elements summed      */
+    double flux_flops;      /* algorithmic FLOP of those elements                */
+    double flux_bytes;      /* algorithmic HBM bytes of the flux kernels         */
+} lc_profile;
+
+typedef struct lc_ctx lc_ctx;
+
+/* ------------------------------------------------------------ context */
+/* No reference counterpart: the reference keeps thread_local state
+ * (light_curve_comp.cpp:161, comp_light_cuda.cu:462-466).  One ctx per host
+ * thread; a ctx is not thread-safe. */
+lc_status lc_ctx_create(int device, lc_ctx **out);
+void lc_ctx_destroy(lc_ctx *ctx);
+const char *lc_last_error(const lc_ctx *ctx);
+lc_status lc_ctx_get_profile(const lc_ctx *ctx, lc_profile *out);
+lc_status lc_ctx_reset_profile(lc_ctx *ctx);
+/* Library version string; usable without a GPU. */
+const char *lc_version(void);
+/* Message for the last failure of a ctx-free call on this thread. */
+const char *lc_thread_error(void);
+/* sizeof of a struct of this header by name ("lc_model", ...), 0 if
+ * unknown: lets a foreign-language binding verify its layouts. */
+size_t lc_sizeof(const char *struct_name);
+
+/* ------------------------------------------------------ model API (host) */
+/* Lcurve::Model::Model(const json&) (model.cpp:10-360): parses a JSON
+ * document with a "model_parameters" object of string values. */
+lc_status lc_model_from_json(const char *json_text, size_t len, lc_model *out);
+/* Model::nvary (model.cpp:363) */
+int lc_model_nvary(const lc_model *m);
+/* Model::set_param (model.cpp:463) */
+lc_status lc_model_set_param(lc_model *m, const double *vpar, size_t n);
+/* Model::get_param (model.cpp:1160) */
+lc_status lc_model_get_param(const lc_model *m, double *vpar, size_t n);
+/* Model::get_range (model.cpp:1412) / get_dstep (model.cpp:1539) */
+lc_status lc_model_get_range(const lc_model *m, double *v, size_t n);
+lc_status lc_model_get_dstep(const lc_model *m, double *v, size_t n);
+/* Model::get_limit (model.cpp:1286): lo[i], hi[i] */
+lc_status lc_model_get_limit(const lc_model *m, double *lo, double *hi, size_t n);
+/* Model::is_not_legal (model.cpp:788): returns 1 if illegal, 0 if legal */
+int lc_model_is_not_legal(const lc_model *m, const double *vpar, size_t n);
+/* Model::get_name (model.cpp:562): writes NUL-terminated name */
+lc_status lc_model_get_name(const lc_model *m, int i, char *buf, size_t buflen);
+/* Lcurve::numface via set_star_grid's face count (set_star_grid.cpp:110-162) */
+lc_status lc_star_grid_size(const lc_model *m, int star, int fine, size_t *nface);
+
+/* ------------------------------------------------------- light curves */
+/* Lcurve::light_curve_comp (lcurve.h:669-674; light_curve_comp.cpp:119-599),
+ * or light_curve_comp_fast (lcurve.h:474-483) when opts->max_model_points > 0.
+ * calc has n entries; io->sfac is read when opts->scale == 0. */
+lc_status lc_light_curve_comp(lc_ctx *ctx, const lc_model *model,
+                              const lc_datum *data, size_t n,
+                              const lc_eval_opts *opts, double *calc,
+                              lc_scalars *io);
+
+/* Batched form: m parameter vectors against one data set, one launch
+ * sequence.  calc is m x n (row per model), io has m entries, status[i] is
+ * the lc_status of instance i.  Returns LC_OK if the batch ran (instances
+ * may still fail individually), or the error that stopped it. */
+lc_status lc_light_curve_comp_batch(lc_ctx *ctx, const lc_model *models,
+                                    size_t m, const lc_datum *data, size_t n,
+                                    const lc_eval_opts *opts, double *calc,
+                                    lc_scalars *io, int32_t *status);
+
+/* ------------------------------------------------ stage-level entry points */
+/* Lcurve::flat_sum_star1_multi (lcurve.h:299-301; comp_light_flat.cpp:413) */
+lc_status lc_flat_sum_star1_multi(lc_ctx *ctx, const lc_flat_grid *g,
+                                  const lc_phase_batch *pb, const lc_ldc *ldc,
+                                  double beam, double spin, double VFAC,
+                                  double XCOFM, int precision, double *out);
+/* Lcurve::flat_sum_star2_multi (lcurve.h:304-307; comp_light_flat.cpp:434) */
+lc_status lc_flat_sum_star2_multi(lc_ctx *ctx, const lc_flat_grid *g,
+                                  const lc_phase_batch *pb, const lc_ldc *ldc,
+                                  double beam, double spin, double VFAC,
+                                  double XCOFM, int glens1, double rlens1,
+                                  int precision, double *out);
+/* Lcurve::set_star_grid (lcurve.h:528; set_star_grid.cpp:49-523): fills out
+ * (arrays of lc_star_grid_size() entries, caller-owned host memory). */
+lc_status lc_set_star_grid(lc_ctx *ctx, const lc_model *m, int star, int fine,
+                           lc_star_faces *out);
+/* Lcurve::set_star_continuum (lcurve.h:550-552; set_star_continuum.cpp:26-297):
+ * sets star1->flux and star2->flux from the other fields. */
+lc_status lc_set_star_continuum(lc_ctx *ctx, const lc_model *m,
+                                lc_star_faces *star1, lc_star_faces *star2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LCURVE_AMD_H */

@@ -1,0 +1,1434 @@
+// engine.hip -- host side of the MI355X light-curve engine and its C ABI.
+//
+// One evaluation (or a batch of model instances sharing one data set) runs
+// as one launch sequence on the context's stream:
+//
+//   faces -> eclipse search -> continuum -> [Ginterp sums] -> FlatGrid build
+//   -> phases -> [fine/coarse routing] -> flux sums -> chunk merge
+//   -> scatter + scale + chi**2 -> [diagnostics]
+//
+// The host only plans: per grid it derives the scalars of set_star_grid
+// (set_star_grid.cpp:52-207) and the latitude-ring table of add_faces
+// (set_star_grid.cpp:256-320), which fixes every face's slot in HBM; all
+// per-face and per-phase work runs on the GPU.  Tables are uploaded once per
+// call; results come back in one copy at the end.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lcurve_amd.h"
+#include "lc_host.h"
+#include "lc_kernels.h"
+#include "lc_math.h"
+
+using namespace lck;
+using lcx::V3;
+
+double lcx_host_l1(int kind, double q, double spin, int *st) { return lcx::lagrange_l1(kind, q, spin, st); }
+
+namespace {
+
+struct EngineError : std::runtime_error {
+    int code;
+    EngineError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            throw EngineError(LC_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+const double PI = lcx::PI;
+const double TWOPI = lcx::TWOPI;
+
+// ------------------------------------------------------------- arenas
+// Device memory grows geometrically and is reused across calls, so the
+// steady state of repeated evaluations allocates nothing.
+struct DevArena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void reset() { used = 0; }
+    void reserve(size_t bytes) {
+        if (bytes <= cap) return;
+        if (base) HIPCHK(hipFree(base));
+        base = nullptr;
+        size_t nc = std::max(bytes, cap + cap / 2);
+        HIPCHK(hipMalloc(&base, nc));
+        cap = nc;
+    }
+    template <class T> T *take(size_t n) {
+        size_t a = (used + 255) & ~(size_t)255;
+        T *p = reinterpret_cast<T *>(base + a);
+        used = a + n * sizeof(T);
+        if (used > cap) throw EngineError(LC_E_DEVICE, "device arena overflow");
+        return p;
+    }
+    ~DevArena() {
+        if (base) (void)hipFree(base);
+    }
+};
+
+struct HostPinned {
+    char *base = nullptr;
+    size_t cap = 0;
+    void reserve(size_t bytes) {
+        if (bytes <= cap) return;
+        if (base) HIPCHK(hipHostFree(base));
+        base = nullptr;
+        size_t nc = std::max(bytes, cap + cap / 2);
+        HIPCHK(hipHostMalloc((void **)&base, nc, hipHostMallocDefault));
+        cap = nc;
+    }
+    ~HostPinned() {
+        if (base) (void)hipHostFree(base);
+    }
+};
+
+// Byte-size accounting helper: sizes computed in one pass, then taken.
+struct Sizer {
+    size_t total = 0;
+    template <class T> void add(size_t n) { total = ((total + 255) & ~(size_t)255) + n * sizeof(T); }
+};
+
+// ---------------------------------------------------------- planning
+struct ModelScalars {
+    double r1, r2;          // get_r1r2 (lcurve.h:399-410)
+    double cosi, sini;      // of deg2rad(iangle)
+};
+
+void get_r1r2(const lc_model &m, double &rr1, double &rr2) {
+    if (m.use_radii) {
+        rr1 = m.r1.value;
+        rr2 = m.r2.value;
+    } else {
+        double sini = std::sin(lcx::deg2rad(m.iangle.value));
+        double r2pr1 = std::sqrt(1.0 - std::pow(sini * std::cos(2.0 * M_PI * m.cphi4.value), 2.0));
+        double r2mr1 = std::sqrt(1.0 - std::pow(sini * std::cos(2.0 * M_PI * m.cphi3.value), 2.0));
+        rr1 = (r2pr1 - r2mr1) / 2.0;
+        rr2 = (r2pr1 + r2mr1) / 2.0;
+    }
+}
+
+double l1(int kind, double q, double spin) {
+    int st = 0;
+    double x = lcx::lagrange_l1(kind, q, spin, &st);
+    if (st) throw EngineError(LC_E_PARAM, kind == 0 ? "Roche::xl1: q <= 0 or no convergence"
+                                                    : (kind == 1 ? "Roche::xl11(double, double): q <= 0 or no convergence"
+                                                                 : "Roche::xl12(double,double): q <= 0 or no convergence"));
+    return x;
+}
+
+// numface (numface.cpp:14-53)
+int numface(int nlat, bool infill, double thelo, double thehi, int nlatfill, int nlngfill) {
+    int nface = 0;
+    double theta, dphi, dtheta = PI / nlat;
+    if (infill) {
+        int nl1 = int(std::ceil(thelo / dtheta));
+        for (int i = 0; i < nl1; i++) {
+            theta = thelo * (i + 0.5) / nl1;
+            dphi = dtheta / std::sin(theta);
+            nface += std::max(16, int(2 * PI / dphi));
+        }
+        int nl2 = int(std::ceil((1 + nlatfill) * (thehi - thelo) / dtheta));
+        for (int i = 0; i < nl2; i++) {
+            theta = thelo + (thehi - thelo) * (i + 0.5) / nl2;
+            dphi = dtheta / std::sin(theta) / (1 + nlngfill);
+            nface += std::max(8, int(PI / dphi));
+        }
+        int nl3 = int(std::ceil((thehi - thelo) / dtheta));
+        for (int i = 0; i < nl3; i++) {
+            theta = thelo + (thehi - thelo) * (i + 0.5) / nl3;
+            dphi = dtheta / std::sin(theta);
+            nface += std::max(8, int(PI / dphi));
+        }
+        int nl4 = int(std::ceil((PI - thehi) / dtheta));
+        for (int i = 0; i < nl4; i++) {
+            theta = thehi + (PI - thehi) * (i + 0.5) / nl4;
+            dphi = dtheta / std::sin(theta);
+            nface += std::max(16, int(2 * PI / dphi));
+        }
+    } else {
+        nlat = int(std::ceil(PI / dtheta));
+        for (int i = 0; i < nlat; i++) {
+            theta = PI * (i + 0.5) / nlat;
+            dphi = dtheta / std::sin(theta);
+            nface += std::max(16, int(2 * PI / dphi));
+        }
+    }
+    return nface;
+}
+
+struct RingHost {
+    double sint, cost, phi1, dphi_span, tspan_nl;
+    int nphi;
+};
+
+struct GridPlan {
+    GridDev dev;
+    std::vector<RingHost> rings;
+    int ring_faces;  // faces laid out by the rings (<= nface)
+};
+
+// Everything set_star_grid derives before add_faces, plus the ring table
+// of every add_faces segment (set_star_grid.cpp:49-320).
+GridPlan plan_grid(const lc_model &m, int star, bool fine) {
+    GridPlan gp;
+    GridDev &G = gp.dev;
+    std::memset(&G, 0, sizeof(G));
+    const double q = m.q.value;
+    double r1, r2;
+    get_r1r2(m, r1, r2);
+    G.star = star;
+    G.fine = fine ? 1 : 0;
+    G.eclipse = star == 0 ? m.eclipse1 : m.eclipse2;
+    const int nlat = star == 0 ? (fine ? m.nlat1f : m.nlat1c) : (fine ? m.nlat2f : m.nlat2c);
+    const int nlatfill = star == 0 ? 0 : (fine ? m.nlatfill : 0);
+    const int nlngfill = star == 0 ? 0 : (fine ? m.nlngfill : 0);
+    const double latlo = star == 0 ? 0 : m.llo;
+    const double lathi = star == 0 ? 0 : m.lhi;
+    if (nlat < 0) throw EngineError(LC_E_PARAM, "set_star_grid: number of latitude strips must not be negative");
+
+    const double rl1 = l1(1, q, m.spin1.value);
+    if (r1 < 0)
+        r1 = rl1;
+    else if (r1 > rl1)
+        throw EngineError(LC_E_ROCHE_OVERFLOW, "set_star_grid: the primary star is larger than its Roche lobe!");
+    const double rl2 = 1. - l1(2, q, m.spin2.value);
+    if (r2 < 0)
+        r2 = rl2;
+    else if (r2 > rl2)
+        throw EngineError(LC_E_ROCHE_OVERFLOW, "set_star_grid: the secondary star is larger than its Roche lobe!");
+    if (m.glens1 && star == 1 && m.roche1 && m.eclipse2)
+        throw EngineError(LC_E_LENSING, "set_star_grid: cannot have gravitational lensing, eclipse and Roche lobe "
+                                        "geometry at the same time");
+    if (m.glens1 && star == 1) {
+        double gm = std::pow(1000. * m.velocity_scale.value, 3.0) * m.tperiod * lcx::DAY / TWOPI;
+        double a = std::pow(gm / lcx::sqr(TWOPI / lcx::DAY / m.tperiod), 1. / 3.);
+        double rlens1 = 4. * gm / (1. + q) / a / lcx::sqr(lcx::C_SI);
+        r1 -= rlens1 / (r1 / (1. - r2 / 2.));
+        if (r1 < 0)
+            throw EngineError(LC_E_LENSING, "set_star_grid: gravitational lensing correction more than the current "
+                                            "program can cope with.");
+    }
+    const double ffac1 = r1 / rl1, ffac2 = r2 / rl2;
+    double rref1, pref1, rref2, pref2;
+    lcx::ref_sphere(q, 0, m.spin1.value, ffac1, rref1, pref1);
+    lcx::ref_sphere(q, 1, m.spin2.value, ffac2, rref2, pref2);
+
+    bool infill = m.npole && star == 1 && (nlatfill > 0 || nlngfill > 0) && r2 > r1;
+    double thelo = 0, thehi = 0;
+    if (infill) {
+        double rangle = lcx::deg2rad(m.iangle.value);
+        double cosi = std::cos(rangle);
+        double ratio = (cosi + r1) / r2;
+        if (ratio >= 1.) {
+            thehi = PI / 2. + rangle;
+        } else {
+            // bisect where the lower envelope swept by star 1 crosses the
+            // limb of star 2 (set_star_grid.cpp:125-144, envelope 528-538)
+            double llo = 0., lhi = PI / 2., ex = 0., ey = 0.;
+            while (lhi > llo + 1.e-7) {
+                double lmid = (llo + lhi) / 2.;
+                double si = std::sin(rangle), co = std::cos(rangle);
+                double sl = std::sin(lmid), cl = std::cos(lmid);
+                double norm = std::sqrt(co * co + si * si * cl * cl);
+                ex = sl + r1 * co * sl / norm;
+                ey = -co * cl - r1 * cl / norm;
+                if (ex * ex + ey * ey < r2 * r2)
+                    llo = lmid;
+                else
+                    lhi = lmid;
+            }
+            double sini = std::sin(rangle);
+            thehi = std::max(PI / 2. - lcx::deg2rad(latlo),
+                             std::min(PI / 2. + rangle, std::acos(ey * sini / r2) + lcx::deg2rad(m.lfudge)));
+        }
+        ratio = (cosi - r1) / r2;
+        if (ratio >= 1.)
+            infill = false;
+        else if (ratio <= -1.)
+            thelo = 0.;
+        else
+            thelo = std::min(PI / 2. - lcx::deg2rad(lathi),
+                             std::max(0., PI / 2. - std::acos(ratio) + rangle - lcx::deg2rad(m.lfudge)));
+    }
+    G.nface = numface(nlat, infill, thelo, thehi, nlatfill, nlngfill);
+
+    const double delta = m.delta_phase;
+    const double ACC = delta / 10.;
+    G.gref = 1.;
+    if ((star == 0 && m.roche1) || (star == 1 && m.roche2)) {
+        V3 pv, dv;
+        double rad, g;
+        int st = star == 0 ? lcx::face_scalar(q, 0, m.spin1.value, lcx::v3(-1., 0., 0.), rref1, pref1, ACC, pv, dv,
+                                              rad, g)
+                           : lcx::face_scalar(q, 1, m.spin2.value, lcx::v3(1., 0., 0.), rref2, pref2, ACC, pv, dv,
+                                              rad, g);
+        if (st) throw EngineError(LC_E_FACE, "Roche::face error: could not solve for the reference gravity");
+        G.gref = g;
+    }
+
+    // ring tables of the add_faces segments (set_star_grid.cpp:256-320)
+    auto segment = [&](double tlo, double thi, int nlf, int nlg) {
+        const double dtheta = PI / nlat;
+        const bool inf = (nlf > 0) || (nlg > 0);
+        int nl, nl1 = 0, nl2 = 0;
+        if (inf) {
+            nl1 = int(std::ceil((1 + nlf) * (thi - tlo) / dtheta));
+            nl2 = int(std::ceil((thi - tlo) / dtheta));
+            nl = nl1 + nl2;
+        } else {
+            nl = int(std::ceil((thi - tlo) / dtheta));
+        }
+        for (int nt = 0; nt < nl; nt++) {
+            RingHost R;
+            double theta, phi1, phi2;
+            int nls;
+            if (inf) {
+                if (nt < nl1) {
+                    theta = tlo + (thi - tlo) * (nt + 0.5) / nl1;
+                    R.sint = std::sin(theta);
+                    R.cost = std::cos(theta);
+                    R.nphi = std::max(8, int(PI * R.sint * (1 + nlg) / dtheta));
+                    nls = nl1;
+                    phi1 = star == 0 ? -PI / 2. : +PI / 2.;
+                    phi2 = star == 0 ? +PI / 2. : +3. * PI / 2.;
+                } else {
+                    theta = tlo + (thi - tlo) * (nt - nl1 + 0.5) / nl2;
+                    R.sint = std::sin(theta);
+                    R.cost = std::cos(theta);
+                    R.nphi = std::max(8, int(PI * R.sint / dtheta));
+                    nls = nl2;
+                    phi1 = star == 0 ? +PI / 2. : -PI / 2.;
+                    phi2 = star == 0 ? +3. * PI / 2. : +PI / 2.;
+                }
+            } else {
+                theta = tlo + (thi - tlo) * (nt + 0.5) / nl;
+                R.sint = std::sin(theta);
+                R.cost = std::cos(theta);
+                R.nphi = std::max(16, int(2 * PI * R.sint / dtheta));
+                nls = nl;
+                phi1 = 0.;
+                phi2 = TWOPI;
+            }
+            R.phi1 = phi1;
+            R.dphi_span = phi2 - phi1;
+            R.tspan_nl = (thi - tlo) / nls;
+            gp.rings.push_back(R);
+        }
+    };
+    if (nlat > 0) {
+        if (infill) {
+            segment(0., thelo, 0, 0);
+            segment(thelo, thehi, nlatfill, nlngfill);
+            segment(thehi, PI, 0, 0);
+        } else {
+            segment(0., PI, 0, 0);
+        }
+    }
+    gp.ring_faces = 0;
+    for (auto &R : gp.rings) gp.ring_faces += R.nphi;
+    if (gp.ring_faces > G.nface)
+        throw EngineError(LC_E_PARAM, "set_star_grid: latitude rings hold more faces than numface() allows");
+
+    // geometry scalars of the face and eclipse kernels
+    const double ri = lcx::deg2rad(m.iangle.value);
+    G.cosi = std::cos(ri);
+    G.sini = std::sin(ri);
+    G.q = q;
+    G.npole = m.npole ? 1 : 0;
+    G.acc = ACC;
+    G.delta = delta;
+    G.roche_self = star == 0 ? m.roche1 : m.roche2;
+    G.roche_other = star == 0 ? m.roche2 : m.roche1;
+    G.spin_self = star == 0 ? m.spin1.value : m.spin2.value;
+    G.spin_other = star == 0 ? m.spin2.value : m.spin1.value;
+    G.rref_self = star == 0 ? rref1 : rref2;
+    G.pref_self = star == 0 ? pref1 : pref2;
+    G.rref_other = star == 0 ? rref2 : rref1;
+    G.pref_other = star == 0 ? pref2 : pref1;
+    G.r_self = star == 0 ? r1 : r2;
+    G.r_other = star == 0 ? r2 : r1;
+    return gp;
+}
+
+// Continuum and flux-sum constants of a grid (set_star_continuum.cpp:30-101,
+// 205-237; light_curve_comp.cpp:141-154, 419-423).
+void fill_physics(const lc_model &m, GridDev &G) {
+    const int star = G.star;
+    const double q = m.q.value;
+    double r1, r2;
+    get_r1r2(m, r1, r2);
+    double rl1 = l1(0, q, 0.);
+    if (r1 < 0) r1 = rl1;
+    double rl2 = 1. - rl1;
+    if (r2 < 0) r2 = rl2;
+    const double T1 = m.t1.value, T2 = m.t2.value;
+    G.wave = m.wavelength;
+    G.absorb = m.absorb.value;
+    G.mirror = m.mirror ? 1 : 0;
+    if (star == 0) {
+        G.r_cont_other = r2;
+        G.t_base = T1;
+        G.t_irr = T2;
+        G.t_mirror = std::fabs(T2);
+        G.gdcbol = m.gdark_bolom1 ? m.gravity_dark1.value : m.gravity_dark1.value / lcx::dlpdlt(m.wavelength, T1);
+        G.spot_running = 0;
+    } else {
+        G.r_cont_other = r1;
+        G.t_base = std::fabs(T2);
+        G.t_irr = T1;
+        G.t_mirror = T1;
+        G.gdcbol = m.gdark_bolom2 ? m.gravity_dark2.value
+                                  : m.gravity_dark2.value / lcx::dlpdlt(m.wavelength, std::fabs(T2));
+        G.spot_running = 1;
+    }
+    const lc_pparam *spots[5][4] = {{&m.stsp11_long, &m.stsp11_lat, &m.stsp11_fwhm, &m.stsp11_tcen},
+                                    {&m.stsp12_long, &m.stsp12_lat, &m.stsp12_fwhm, &m.stsp12_tcen},
+                                    {&m.stsp13_long, &m.stsp13_lat, &m.stsp13_fwhm, &m.stsp13_tcen},
+                                    {&m.stsp21_long, &m.stsp21_lat, &m.stsp21_fwhm, &m.stsp21_tcen},
+                                    {&m.stsp22_long, &m.stsp22_lat, &m.stsp22_fwhm, &m.stsp22_tcen}};
+    G.nspot = 0;
+    for (int g = star == 0 ? 0 : 3; g < (star == 0 ? 3 : 5); g++) {
+        bool on = spots[g][0]->defined && spots[g][1]->defined && spots[g][2]->defined && spots[g][3]->defined;
+        if (!on) continue;
+        double clong = std::cos(lcx::deg2rad(spots[g][0]->value));
+        double slong = std::sin(lcx::deg2rad(spots[g][0]->value));
+        double clat = std::cos(lcx::deg2rad(spots[g][1]->value));
+        double slat = std::sin(lcx::deg2rad(spots[g][1]->value));
+        int k = G.nspot++;
+        G.spot_dir[k][0] = clat * clong;
+        G.spot_dir[k][1] = clat * slong;
+        G.spot_dir[k][2] = slat;
+        G.spot_fwhm[k] = spots[g][2]->value;
+        G.spot_tcen[k] = spots[g][3]->value;
+    }
+    G.uespot = 0;
+    if (star == 0 && m.uesp_long1.defined && m.uesp_long2.defined && m.uesp_lathw.defined && m.uesp_taper.defined &&
+        m.uesp_temp.defined) {
+        G.uespot = 1;
+        G.ue_longhw = (m.uesp_long2.value - m.uesp_long1.value) / 2.;
+        double lcen = lcx::deg2rad((m.uesp_long1.value + m.uesp_long2.value) / 2.);
+        G.ue_dir[0] = std::cos(lcen);
+        G.ue_dir[1] = std::sin(lcen);
+        G.ue_dir[2] = 0.;
+        G.ue_lathw = m.uesp_lathw.value;
+        G.ue_taper = m.uesp_taper.value;
+        G.ue_temp = m.uesp_temp.value;
+    }
+    // flux sums
+    const float vs = (float)m.velocity_scale.value;
+    G.VFAC = vs / (lcx::C_SI / 1.e3);
+    G.XCOFM = q / (1.0 + q);
+    G.beam = star == 0 ? m.beam_factor1.value : m.beam_factor2.value;
+    G.spin_beam = star == 0 ? m.spin1.value : m.spin2.value;
+    const lc_pparam *l = star == 0 ? &m.ldc1_1 : &m.ldc2_1;
+    const lc_pparam *ls[4] = {star == 0 ? &m.ldc1_1 : &m.ldc2_1, star == 0 ? &m.ldc1_2 : &m.ldc2_2,
+                              star == 0 ? &m.ldc1_3 : &m.ldc2_3, star == 0 ? &m.ldc1_4 : &m.ldc2_4};
+    (void)l;
+    for (int k = 0; k < 4; k++) G.ldc[k] = ls[k]->value;
+    G.mucrit = star == 0 ? m.mucrit1 : m.mucrit2;
+    G.ltype = star == 0 ? m.limb1 : m.limb2;
+    G.glens = (star == 1 && m.glens1) ? 1 : 0;
+    G.rlens1 = 0.;
+    if (m.glens1) {
+        double gm = std::pow(1000. * m.velocity_scale.value, 3.0) * m.tperiod * lcx::DAY / TWOPI;
+        double a = std::pow(gm / (TWOPI / lcx::DAY / m.tperiod * TWOPI / lcx::DAY / m.tperiod), 1. / 3.);
+        G.rlens1 = 4. * gm / (1. + q) / a / (lcx::C_SI * lcx::C_SI);
+    }
+}
+
+int flux_variant(const GridDev &G) { return (G.ltype ? 1 : 0) | (G.beam != 0.0 ? 2 : 0) | (G.glens ? 4 : 0); }
+
+struct InstPlan {
+    int status = LC_OK;
+    std::string msg;
+    int g[4] = {-1, -1, -1, -1};  // 1f, 2f, 1c, 2c
+    bool copy1 = true, copy2 = true;
+    double gref_diag[2] = {0., 0.};
+    double gscale = 0.;
+};
+
+}  // namespace
+
+// ================================================================== ctx
+struct lc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevArena dev;
+    HostPinned pin;
+    lc_profile prof;
+    hipEvent_t ev[16];
+    int flux_rphase = 2;
+    int flux_chunk = 2048;
+};
+
+namespace {
+
+const char *status_msg(int st) {
+    switch (st) {
+    case LC_E_FACE: return "Roche::face error: could not solve for a face radius";
+    case LC_E_ECLIPSE_SEARCH: return "Roche::ingress_egress: eclipse search failed (pot_min/linmin/dbrent)";
+    default: return "device evaluation failed";
+    }
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.;
+    return ms;
+}
+
+// Algorithmic FLOP per face x sub-phase element (POLY/CLARET x beaming):
+// the counts the roofline uses.
+double elem_flops(const GridDev &G) {
+    static const double c[4] = {24., 26., 39., 41.};
+    return c[(G.ltype ? 1 : 0) + (G.beam != 0.0 ? 2 : 0)];
+}
+
+struct EvalRequest {
+    const lc_model *models;
+    size_t m;
+    const lc_datum *data;
+    size_t n;
+    lc_eval_opts opts;
+    double *calc;       // m x n
+    lc_scalars *io;     // m
+    int32_t *status;    // m
+};
+
+// Runs the whole pipeline for a batch of instances sharing the data set.
+void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
+    const size_t M = rq.m, n = rq.n;
+    const lc_eval_opts &o = rq.opts;
+    const bool fp64 = o.precision == LC_PREC_FP64;
+
+    // ---------------- data layout (shared by all instances)
+    std::vector<int> sub_off(n + 1);
+    sub_off[0] = 0;
+    for (size_t i = 0; i < n; i++) sub_off[i + 1] = sub_off[i] + std::max(0, rq.data[i].ndiv);
+    const int S = sub_off[n];
+    double xmin = rq.data[0].time, xmax = rq.data[0].time;
+    for (size_t i = 1; i < n; i++) {
+        xmin = rq.data[i].time > xmin ? xmin : rq.data[i].time;
+        xmax = rq.data[i].time < xmax ? xmax : rq.data[i].time;
+    }
+    const double middle = (xmin + xmax) / 2., range = (xmax - xmin) / 2.;
+
+    // ---------------- per instance planning
+    std::vector<InstPlan> ip(M);
+    std::vector<GridPlan> grids;
+    std::vector<InstDev> insts(M);
+    for (size_t i = 0; i < M; i++) {
+        const lc_model &m = rq.models[i];
+        InstPlan &P = ip[i];
+        InstDev &I = insts[i];
+        std::memset(&I, 0, sizeof(I));
+        try {
+            if (m.add_disc || m.add_spot || m.iscale)
+                throw EngineError(LC_E_UNSUPPORTED, "disc, bright spot and iscale components are not supported");
+            double r1, r2;
+            get_r1r2(m, r1, r2);
+            double rl2 = 1. - l1(2, m.q.value, m.spin2.value);
+            if (r2 < 0)
+                r2 = rl2;
+            else if (r2 > rl2)
+                throw EngineError(LC_E_ROCHE_OVERFLOW,
+                                  "light_curve_comp: the secondary star is larger than its Roche lobe!");
+            std::vector<GridPlan> local;
+            local.push_back(plan_grid(m, 0, true));
+            local.push_back(plan_grid(m, 1, true));
+            P.copy1 = (m.nlat1f == m.nlat1c);
+            P.copy2 = (m.nlat2f == m.nlat2c) && (!m.npole || r1 >= r2 || (m.nlatfill == 0 && m.nlngfill == 0));
+            if (!P.copy1) local.push_back(plan_grid(m, 0, false));
+            if (!P.copy2) local.push_back(plan_grid(m, 1, false));
+            int k = 0;
+            for (auto &gp : local) {
+                gp.dev.inst = (int)i;
+                fill_physics(m, gp.dev);
+                int slot = gp.dev.star + (gp.dev.fine ? 0 : 2);
+                P.g[slot] = (int)grids.size() + k;
+                k++;
+            }
+            if (P.copy1) P.g[2] = P.g[0];
+            if (P.copy2) P.g[3] = P.g[1];
+            for (auto &gp : local) grids.push_back(std::move(gp));
+            if (o.info) {
+                std::fprintf(stderr, "Number of points for star 1 (fine) = %d\n", grids[P.g[0]].dev.nface);
+                std::fprintf(stderr, "Number of points for star 2 (fine) = %d\n", grids[P.g[1]].dev.nface);
+                std::fprintf(stderr, "Number of points for star 1 (coarse) = %d\n", grids[P.g[2]].dev.nface);
+                std::fprintf(stderr, "Number of points for star 2 (coarse) = %d\n", grids[P.g[3]].dev.nface);
+            }
+            // instance parameters
+            const double ri = lcx::deg2rad(m.iangle.value);
+            I.t0 = m.t0.value;
+            I.period = m.period.value;
+            I.pdot = m.pdot.value;
+            I.deltat = m.deltat.value;
+            I.slope = m.slope.value;
+            I.quad = m.quad.value;
+            I.cube = m.cube.value;
+            I.third = m.third.value;
+            I.cosi = std::cos(ri);
+            I.sini = std::sin(ri);
+            I.phase1 = m.phase1;
+            I.phase2 = m.phase2;
+            I.scale11 = I.scale12 = I.scale21 = I.scale22 = 1.;
+            I.sfac0 = rq.io[i].sfac[0];
+            I.copy1 = P.copy1;
+            I.copy2 = P.copy2;
+            I.grid1f = P.g[0];
+            I.grid2f = P.g[1];
+            I.grid1c = P.g[2];
+            I.grid2c = P.g[3];
+            I.scale = o.scale;
+            I.rdata = o.rdata;
+            I.active = 1;
+        } catch (const EngineError &e) {
+            P.status = e.code;
+            P.msg = e.what();
+            I.active = 0;
+        }
+    }
+
+    // ---------------- global tables
+    const int ngrid = (int)grids.size();
+    std::vector<GridDev> gdev(ngrid);
+    std::vector<RingDev> rdev;
+    int face_cur = 0, half_cur = 0;
+    for (int g = 0; g < ngrid; g++) {
+        GridPlan &gp = grids[g];
+        GridDev &G = gp.dev;
+        G.face_off = face_cur;
+        G.ring_off = (int)rdev.size();
+        G.nring = (int)gp.rings.size();
+        G.half_off = half_cur;
+        int fo = face_cur;
+        for (auto &R : gp.rings) {
+            RingDev d;
+            d.sint = R.sint;
+            d.cost = R.cost;
+            d.phi1 = R.phi1;
+            d.dphi_span = R.dphi_span;
+            d.tspan_nl = R.tspan_nl;
+            d.nphi = R.nphi;
+            d.face_off = fo;
+            d.half_off = half_cur;
+            d.grid = g;
+            rdev.push_back(d);
+            fo += R.nphi;
+            half_cur += (R.nphi + 1) / 2;
+        }
+        G.nhalf = half_cur - G.half_off;
+        face_cur += G.nface;
+        gdev[g] = G;
+    }
+    const int nface_total = face_cur, nhalf_total = half_cur, nring = (int)rdev.size();
+    if ((long long)face_cur > (1LL << 30)) throw EngineError(LC_E_ARG, "batch too large: more than 2^30 faces");
+
+    // gaps between the ring layout and numface() hold default faces
+    std::vector<std::pair<int, int>> gaps;
+    for (int g = 0; g < ngrid; g++)
+        if (grids[g].ring_faces < gdev[g].nface)
+            gaps.push_back({gdev[g].face_off + grids[g].ring_faces, gdev[g].nface - grids[g].ring_faces});
+
+    // ---------------- flux batches and tasks
+    const int R = ctx->flux_rphase;
+    const int TILE = R * 256;
+    const int chunk = fp64 ? 1024 : ctx->flux_chunk;
+    const bool any_route = std::any_of(ip.begin(), ip.end(), [](const InstPlan &p) {
+        return p.status == LC_OK && (!p.copy1 || !p.copy2);
+    });
+    std::vector<BatchDev> batches;
+    std::vector<std::vector<TaskDev>> tasks_by_var(8);
+    long long part_total = 0;
+    double alg_flops = 0., alg_elems = 0., alg_bytes = 0.;
+    const int ntile = S > 0 ? (S + TILE - 1) / TILE : 0;
+    for (size_t i = 0; i < M; i++) {
+        if (ip[i].status != LC_OK || S == 0) continue;
+        for (int star = 0; star < 2; star++) {
+            bool copy = star == 0 ? ip[i].copy1 : ip[i].copy2;
+            for (int part = 0; part < (copy ? 1 : 2); part++) {
+                int gi = ip[i].g[star + (part ? 2 : 0)];
+                const GridDev &G = gdev[gi];
+                if (G.nface == 0) continue;
+                BatchDev B;
+                B.inst = (int)i;
+                B.star = star;
+                B.grid = gi;
+                B.list_off = copy ? -1 : (int)(((size_t)i * 2 + star) * 2 * S + (part ? S : 0));
+                B.count_idx = copy ? -1 : (int)((i * 2 + star) * 2 + part);
+                B.chunk = chunk;
+                B.nchunk = (G.nface + chunk - 1) / chunk;
+                B.ntile = ntile;
+                B.part_off = part_total;
+                part_total += (long long)B.nchunk * ntile * TILE;
+                int bi = (int)batches.size();
+                batches.push_back(B);
+                int var = flux_variant(G);
+                for (int t = 0; t < ntile; t++)
+                    for (int c = 0; c < B.nchunk; c++) tasks_by_var[var].push_back(TaskDev{bi, t, c, 0});
+                if (copy) {  // algorithmic work: every face x every sub-phase of the grid
+                    alg_elems += (double)G.nface * S;
+                    alg_flops += (double)G.nface * S * elem_flops(G);
+                    alg_bytes += (double)G.nface * 40. + (double)S * 32.;
+                }
+            }
+        }
+    }
+    std::vector<TaskDev> tasks;
+    std::vector<std::pair<int, int>> var_range(8, {0, 0});
+    for (int v = 0; v < 8; v++) {
+        var_range[v] = {(int)tasks.size(), (int)tasks_by_var[v].size()};
+        tasks.insert(tasks.end(), tasks_by_var[v].begin(), tasks_by_var[v].end());
+    }
+
+    // diagnostics grids: star 1 fine, star 2 fine, and the wdwarf grid
+    std::vector<int> diag_idx;
+    std::vector<std::array<int, 3>> diag_of(M, std::array<int, 3>{-1, -1, -1});
+    if (o.diagnostics)
+        for (size_t i = 0; i < M; i++) {
+            if (ip[i].status != LC_OK) continue;
+            const lc_model &m = rq.models[i];
+            double pn = 0.5;
+            bool t1 = (pn <= m.phase1 || pn >= 1. - m.phase1);
+            int wgrid = t1 ? ip[i].g[0] : ip[i].g[2];
+            diag_of[i][0] = (int)diag_idx.size();
+            diag_idx.push_back(ip[i].g[0]);
+            diag_of[i][1] = (int)diag_idx.size();
+            diag_idx.push_back(ip[i].g[1]);
+            if (wgrid == ip[i].g[0]) {
+                diag_of[i][2] = diag_of[i][0];
+            } else {
+                diag_of[i][2] = (int)diag_idx.size();
+                diag_idx.push_back(wgrid);
+            }
+        }
+
+    // ---------------- device memory
+    const size_t FN = (size_t)std::max(nface_total, 1);
+    const size_t SM = (size_t)std::max(S, 1) * std::max<size_t>(M, 1);
+    Sizer sz;
+    sz.add<GridDev>(std::max(ngrid, 1));
+    sz.add<RingDev>(std::max(nring, 1));
+    sz.add<InstDev>(std::max<size_t>(M, 1));
+    sz.add<int>(M);  // status
+    for (int k = 0; k < 6; k++) sz.add<double>(FN);  // px..dz
+    for (int k = 0; k < 3; k++) sz.add<float>(FN);   // area grav flux
+    sz.add<int>(FN);
+    sz.add<double>(FN);
+    sz.add<double>(FN);
+    sz.add<int>(FN);  // cand
+    sz.add<int>(1);
+    sz.add<float4>(FN);
+    sz.add<float4>(FN);
+    sz.add<float4>(FN);
+    sz.add<double2>(FN);
+    sz.add<int>(FN);
+    if (fp64)
+        for (int k = 0; k < 7; k++) sz.add<double>(FN);
+    sz.add<int>(std::max(ngrid, 1));
+    for (int k = 0; k < 5; k++) sz.add<double>(std::max<size_t>(n, 1));
+    sz.add<int>(std::max<size_t>(n, 1));
+    sz.add<int>(n + 1);
+    for (int k = 0; k < 6; k++) sz.add<double>(SM);
+    sz.add<signed char>(SM);
+    if (any_route) {
+        sz.add<int>(SM * 4);
+        sz.add<int>(std::max<size_t>(M, 1) * 4);
+    }
+    sz.add<double>(SM * 2);  // out
+    sz.add<double>((size_t)std::max<long long>(part_total, 1));
+    sz.add<BatchDev>(std::max<size_t>(batches.size(), 1));
+    sz.add<TaskDev>(std::max<size_t>(tasks.size(), 1));
+    sz.add<double>(std::max<size_t>(M, 1) * n);  // calc
+    sz.add<double>(std::max<size_t>(M, 1) * 8);  // scalars
+    sz.add<double>(std::max<size_t>(M, 1) * 8);  // gint sums
+    sz.add<int>(std::max<size_t>(diag_idx.size(), 1));
+    sz.add<double>(std::max<size_t>(diag_idx.size(), 1) * 8);
+    ctx->dev.reserve(sz.total + 4096);
+    ctx->dev.reset();
+    DevArena &A = ctx->dev;
+    GridDev *d_grids = A.take<GridDev>(std::max(ngrid, 1));
+    RingDev *d_rings = A.take<RingDev>(std::max(nring, 1));
+    InstDev *d_insts = A.take<InstDev>(std::max<size_t>(M, 1));
+    int *d_status = A.take<int>(M);
+    FaceArrays fa;
+    fa.px = A.take<double>(FN); fa.py = A.take<double>(FN); fa.pz = A.take<double>(FN);
+    fa.dx = A.take<double>(FN); fa.dy = A.take<double>(FN); fa.dz = A.take<double>(FN);
+    fa.area = A.take<float>(FN); fa.grav = A.take<float>(FN); fa.flux = A.take<float>(FN);
+    fa.ecl = A.take<int>(FN);
+    fa.ein = A.take<double>(FN);
+    fa.eout = A.take<double>(FN);
+    int *d_cand = A.take<int>(FN);
+    int *d_cand_count = A.take<int>(1);
+    FlatArrays fl;
+    std::memset(&fl, 0, sizeof(fl));
+    fl.rec0 = A.take<float4>(FN);
+    fl.rec1 = A.take<float4>(FN);
+    fl.rec2 = A.take<float4>(FN);
+    fl.ecl = A.take<double2>(FN);
+    fl.perm = A.take<int>(FN);
+    if (fp64) {
+        fl.fdx = A.take<double>(FN); fl.fdy = A.take<double>(FN); fl.fdz = A.take<double>(FN);
+        fl.fpx = A.take<double>(FN); fl.fpy = A.take<double>(FN); fl.fpz = A.take<double>(FN);
+        fl.fflux = A.take<double>(FN);
+    }
+    fl.n0 = A.take<int>(std::max(ngrid, 1));
+    double *d_time = A.take<double>(std::max<size_t>(n, 1));
+    double *d_expose = A.take<double>(std::max<size_t>(n, 1));
+    double *d_flux = A.take<double>(std::max<size_t>(n, 1));
+    double *d_ferr = A.take<double>(std::max<size_t>(n, 1));
+    double *d_weight = A.take<double>(std::max<size_t>(n, 1));
+    int *d_ndiv = A.take<int>(std::max<size_t>(n, 1));
+    int *d_sub_off = A.take<int>(n + 1);
+    PhaseArrays pa;
+    pa.ex = A.take<double>(SM); pa.ey = A.take<double>(SM); pa.ez = A.take<double>(SM);
+    pa.phin = A.take<double>(SM); pa.w1 = A.take<double>(SM); pa.w2 = A.take<double>(SM);
+    pa.type = A.take<signed char>(SM);
+    int *d_lists = nullptr, *d_counts = nullptr;
+    if (any_route) {
+        d_lists = A.take<int>(SM * 4);
+        d_counts = A.take<int>(std::max<size_t>(M, 1) * 4);
+    }
+    double *d_out = A.take<double>(SM * 2);
+    double *d_partial = A.take<double>((size_t)std::max<long long>(part_total, 1));
+    BatchDev *d_batches = A.take<BatchDev>(std::max<size_t>(batches.size(), 1));
+    TaskDev *d_tasks = A.take<TaskDev>(std::max<size_t>(tasks.size(), 1));
+    double *d_calc = A.take<double>(std::max<size_t>(M, 1) * n);
+    double *d_scal = A.take<double>(std::max<size_t>(M, 1) * 8);
+    double *d_gsums = A.take<double>(std::max<size_t>(M, 1) * 8);
+    int *d_diag_idx = A.take<int>(std::max<size_t>(diag_idx.size(), 1));
+    double *d_diag = A.take<double>(std::max<size_t>(diag_idx.size(), 1) * 8);
+
+    // ---------------- uploads through one pinned staging block
+    std::vector<int> ndiv(n);
+    std::vector<double> col[5];
+    for (int k = 0; k < 5; k++) col[k].resize(n);
+    for (size_t i = 0; i < n; i++) {
+        col[0][i] = rq.data[i].time;
+        col[1][i] = rq.data[i].expose;
+        col[2][i] = rq.data[i].flux;
+        col[3][i] = rq.data[i].ferr;
+        col[4][i] = rq.data[i].weight;
+        ndiv[i] = std::max(0, rq.data[i].ndiv);
+    }
+    std::vector<int> st0(M);
+    for (size_t i = 0; i < M; i++) st0[i] = ip[i].status;
+    struct Up {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    };
+    std::vector<Up> ups = {
+        {d_grids, gdev.data(), gdev.size() * sizeof(GridDev)},
+        {d_rings, rdev.data(), rdev.size() * sizeof(RingDev)},
+        {d_insts, insts.data(), insts.size() * sizeof(InstDev)},
+        {d_status, st0.data(), st0.size() * sizeof(int)},
+        {d_time, col[0].data(), n * 8}, {d_expose, col[1].data(), n * 8}, {d_flux, col[2].data(), n * 8},
+        {d_ferr, col[3].data(), n * 8}, {d_weight, col[4].data(), n * 8},
+        {d_ndiv, ndiv.data(), n * sizeof(int)},
+        {d_sub_off, sub_off.data(), (n + 1) * sizeof(int)},
+        {d_batches, batches.data(), batches.size() * sizeof(BatchDev)},
+        {d_tasks, tasks.data(), tasks.size() * sizeof(TaskDev)},
+        {d_diag_idx, diag_idx.data(), diag_idx.size() * sizeof(int)},
+    };
+    size_t up_total = 0;
+    for (auto &u : ups) up_total += (u.bytes + 255) & ~(size_t)255;
+    const size_t down_bytes = M * n * 8 + M * 8 * 8 + M * sizeof(int) + diag_idx.size() * 8 * 8 + M * sizeof(InstDev);
+    ctx->pin.reserve(std::max(up_total, down_bytes) + 4096);
+    hipStream_t s = ctx->stream;
+    {
+        size_t off = 0;
+        for (auto &u : ups) {
+            if (!u.bytes) continue;
+            std::memcpy(ctx->pin.base + off, u.src, u.bytes);
+            HIPCHK(hipMemcpyAsync(u.dst, ctx->pin.base + off, u.bytes, hipMemcpyHostToDevice, s));
+            off += (u.bytes + 255) & ~(size_t)255;
+        }
+    }
+    HIPCHK(hipMemsetAsync(d_cand_count, 0, sizeof(int), s));
+    for (auto &g : gaps) {
+        HIPCHK(hipMemsetAsync(fa.px + g.first, 0, g.second * 8, s));
+        HIPCHK(hipMemsetAsync(fa.py + g.first, 0, g.second * 8, s));
+        HIPCHK(hipMemsetAsync(fa.pz + g.first, 0, g.second * 8, s));
+        HIPCHK(hipMemsetAsync(fa.dx + g.first, 0, g.second * 8, s));
+        HIPCHK(hipMemsetAsync(fa.dy + g.first, 0, g.second * 8, s));
+        HIPCHK(hipMemsetAsync(fa.dz + g.first, 0, g.second * 8, s));
+        HIPCHK(hipMemsetAsync(fa.area + g.first, 0, g.second * 4, s));
+        std::vector<float> ones(g.second, 1.f);
+        HIPCHK(hipMemcpy(fa.grav + g.first, ones.data(), g.second * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemsetAsync(fa.ecl + g.first, 0, g.second * 4, s));
+        HIPCHK(hipMemsetAsync(fa.ein + g.first, 0, g.second * 8, s));
+        HIPCHK(hipMemsetAsync(fa.eout + g.first, 0, g.second * 8, s));
+    }
+
+    DataDev dd;
+    dd.time = d_time; dd.expose = d_expose; dd.flux = d_flux; dd.ferr = d_ferr; dd.weight = d_weight;
+    dd.ndiv = d_ndiv; dd.sub_off = d_sub_off; dd.n = (int)n; dd.S = S; dd.middle = middle; dd.range = range;
+
+    // ---------------- launches
+    hipEvent_t *ev = ctx->ev;
+    HIPCHK(hipEventRecord(ev[0], s));
+    launch_faces(s, d_grids, d_rings, nring, nhalf_total, fa, d_cand, d_cand_count, d_status);
+    HIPCHK(hipEventRecord(ev[1], s));
+    launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, d_status);
+    HIPCHK(hipEventRecord(ev[2], s));
+    launch_continuum(s, d_grids, ngrid, nface_total, fa);
+    HIPCHK(hipEventRecord(ev[3], s));
+    if (any_route) {
+        launch_point_sums(s, d_grids, d_insts, (int)M, fa, d_gsums);
+        launch_gint_scales(s, d_insts, (int)M, d_gsums);
+    }
+    launch_flat_build(s, d_grids, ngrid, fa, fl, fp64 ? 1 : 0);
+    launch_phases(s, d_insts, (int)M, dd, pa);
+    if (any_route) launch_route(s, d_insts, (int)M, S, pa, d_lists, d_counts);
+    HIPCHK(hipMemsetAsync(d_out, 0, SM * 2 * sizeof(double), s));
+    HIPCHK(hipEventRecord(ev[4], s));
+    for (int v = 0; v < 8; v++)
+        if (var_range[v].second)
+            launch_flux(s, v, fp64 ? 1 : 0, R, d_tasks + var_range[v].first, var_range[v].second, d_batches, d_grids,
+                        fl, pa, S, d_lists, d_counts, d_partial);
+    HIPCHK(hipEventRecord(ev[5], s));
+    launch_flux_reduce(s, d_batches, (int)batches.size(), S, TILE, d_lists, d_counts, d_partial, d_out);
+    launch_scatter(s, d_insts, (int)M, dd, pa, d_out, d_calc, d_scal);
+    if (!diag_idx.empty()) launch_diag(s, d_grids, d_diag_idx, (int)diag_idx.size(), fa, d_diag);
+    HIPCHK(hipGetLastError());
+
+    // ---------------- downloads
+    char *pb = ctx->pin.base;
+    double *h_calc = reinterpret_cast<double *>(pb);
+    double *h_scal = h_calc + M * n;
+    double *h_diag = h_scal + M * 8;
+    InstDev *h_insts = reinterpret_cast<InstDev *>(h_diag + diag_idx.size() * 8);
+    int *h_status = reinterpret_cast<int *>(h_insts + M);
+    HIPCHK(hipMemcpyAsync(h_calc, d_calc, M * n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h_scal, d_scal, M * 8 * 8, hipMemcpyDeviceToHost, s));
+    if (!diag_idx.empty())
+        HIPCHK(hipMemcpyAsync(h_diag, d_diag, diag_idx.size() * 8 * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h_insts, d_insts, M * sizeof(InstDev), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h_status, d_status, M * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(ev[6], s));
+    HIPCHK(hipStreamSynchronize(s));
+
+    ctx->prof.grid_ms_total += elapsed(ev[0], ev[1]);
+    ctx->prof.eclipse_ms_total += elapsed(ev[1], ev[2]);
+    ctx->prof.cont_ms_total += elapsed(ev[2], ev[3]);
+    ctx->prof.other_ms_total += elapsed(ev[3], ev[4]) + elapsed(ev[5], ev[6]);
+    ctx->prof.flux_ms_total += elapsed(ev[4], ev[5]);
+    for (int v = 0; v < 8; v++) ctx->prof.flux_launches += var_range[v].second ? 1 : 0;
+    ctx->prof.flux_elements += alg_elems;
+    ctx->prof.flux_flops += alg_flops;
+    ctx->prof.flux_bytes += alg_bytes;
+
+    // ---------------- finalise per instance
+    for (size_t i = 0; i < M; i++) {
+        int st = ip[i].status;
+        if (st == LC_OK && h_status[i]) st = h_status[i];
+        rq.status[i] = st;
+        if (st != LC_OK) {
+            if (ip[i].msg.empty()) ip[i].msg = status_msg(st);
+            if (M == 1) ctx->err = ip[i].msg;
+            continue;
+        }
+        ctx->prof.evaluations++;
+        std::memcpy(rq.calc + i * n, h_calc + i * n, n * 8);
+        lc_scalars &io = rq.io[i];
+        const double *sc = h_scal + i * 8;
+        if (o.scale) {
+            for (int k = 0; k < 5; k++) io.sfac[k] = sc[0];
+            io.chisq = sc[1];
+            io.wnok = sc[2];
+        } else if (o.rdata) {
+            io.chisq = sc[1];
+            io.wnok = sc[2];
+        }
+        const lc_model &m = rq.models[i];
+        if (o.diagnostics) {
+            const InstDev &I = h_insts[i];
+            double pn = 0.5;
+            double s1 = 1.;
+            if (!(pn <= I.phase1 || pn >= 1. - I.phase1))
+                s1 = (I.scale11 * (1. - I.phase1 - pn) + I.scale12 * (pn - I.phase1)) / (1 - 2. * I.phase1);
+            double wd = (0.0 + 1.0 * s1 * h_diag[diag_of[i][2] * 8 + 4]) / 1;
+            io.wdwarf = wd * (o.scale ? sc[0] : io.sfac[0]);
+            // comp_gravity1/2 (comp_gravity.cpp) and comp_radius1/2
+            double gm = std::pow(1000. * m.velocity_scale.value, 3.0) * m.tperiod * lcx::DAY / TWOPI;
+            double a = std::pow(gm / lcx::sqr(TWOPI / lcx::DAY / m.tperiod), 1. / 3.);
+            double gscale = 100 * gm / lcx::sqr(a);
+            double r1, r2;
+            get_r1r2(m, r1, r2);
+            double rl2 = 1. - l1(2, m.q.value, m.spin2.value);
+            if (m.r2.value < 0) r2 = rl2;
+            for (int star = 0; star < 2; star++) {
+                double gref;
+                const double ACC = m.delta_phase / 10.;
+                if (star == 0 ? m.roche1 : m.roche2) {
+                    double ffac, rref, pref;
+                    if (star == 0) {
+                        double rl1 = l1(2, m.q.value, m.spin1.value);  // the reference uses xl12 here
+                        ffac = r1 / rl1;
+                    } else {
+                        ffac = r2 / rl2;
+                    }
+                    lcx::ref_sphere(m.q.value, star, star == 0 ? m.spin1.value : m.spin2.value, ffac, rref, pref);
+                    V3 pv, dv;
+                    double rad;
+                    int fst = lcx::face_scalar(m.q.value, star, star == 0 ? m.spin1.value : m.spin2.value,
+                                               lcx::v3(star == 0 ? -1. : 1., 0., 0.), rref, pref, ACC, pv, dv, rad,
+                                               gref);
+                    if (fst) gref = 0.;
+                    gref *= gscale;
+                } else {
+                    gref = star == 0 ? gscale / (1 + m.q.value) / lcx::sqr(r1)
+                                     : gscale * m.q.value / (1 + m.q.value) / lcx::sqr(r2);
+                }
+                const double *dg = h_diag + diag_of[i][star] * 8;
+                double lg = ((gref > 0) & (dg[0] > 0) & (dg[1] > 0)) ? std::log10(gref * dg[0] / dg[1]) : 0.;
+                double rv = std::pow(dg[3] / dg[2], 1. / 3.);
+                if (star == 0) {
+                    io.logg1 = lg;
+                    io.rv1 = rv;
+                } else {
+                    io.logg2 = lg;
+                    io.rv2 = rv;
+                }
+            }
+        } else {
+            io.wdwarf = 0.;
+            io.logg1 = io.logg2 = io.rv1 = io.rv2 = 0.;
+        }
+    }
+}
+
+// light_curve_comp_fast: model at max_model_points evenly indexed points,
+// linear interpolation in time, chi**2 over all points
+// (light_curve_comp_fast.cpp:6-84).
+void run_eval(lc_ctx *ctx, const EvalRequest &rq) {
+    const int Mp = rq.opts.max_model_points;
+    if (Mp <= 0 || !(rq.n > (size_t)Mp)) {
+        run_batch(ctx, rq);
+        return;
+    }
+    if (Mp < 2) throw EngineError(LC_E_ARG, "light_curve_comp_fast: max_model_points must be at least 2");
+    const size_t n = rq.n;
+    std::vector<lc_datum> sample;
+    std::vector<size_t> sidx;
+    for (int i = 0; i < Mp; ++i) {
+        size_t idx = (size_t)i * (n - 1) / (size_t)(Mp - 1);
+        sample.push_back(rq.data[idx]);
+        sidx.push_back(idx);
+    }
+    std::vector<double> mcalc(rq.m * sample.size());
+    EvalRequest sub = rq;
+    sub.data = sample.data();
+    sub.n = sample.size();
+    sub.calc = mcalc.data();
+    run_batch(ctx, sub);
+    for (size_t k = 0; k < rq.m; k++) {
+        if (rq.status[k] != LC_OK) continue;
+        const double *mc = mcalc.data() + k * sample.size();
+        double *c = rq.calc + k * n;
+        for (size_t i = 0; i < n; ++i) {
+            auto it = std::lower_bound(sidx.begin(), sidx.end(), i);
+            if (it == sidx.begin()) {
+                c[i] = mc[0];
+            } else if (it == sidx.end()) {
+                c[i] = mc[sample.size() - 1];
+            } else {
+                size_t i2x = (size_t)(it - sidx.begin()), i1x = i2x - 1;
+                size_t i1 = sidx[i1x], i2 = sidx[i2x];
+                double t = (rq.data[i].time - rq.data[i1].time) / (rq.data[i2].time - rq.data[i1].time);
+                c[i] = mc[i1x] * (1.0 - t) + mc[i2x] * t;
+            }
+        }
+        if (rq.opts.rdata) {
+            double wnok = 0., chisq = 0.;
+            for (size_t i = 0; i < n; ++i)
+                if (rq.data[i].weight > 0.) {
+                    wnok += rq.data[i].weight;
+                    chisq += rq.data[i].weight * lcx::sqr((rq.data[i].flux - c[i]) / rq.data[i].ferr);
+                }
+            rq.io[k].wnok = wnok;
+            rq.io[k].chisq = chisq;
+        }
+    }
+}
+
+template <class F> lc_status guarded(lc_ctx *ctx, F f) {
+    try {
+        return (lc_status)f();
+    } catch (const EngineError &e) {
+        if (ctx) ctx->err = e.what();
+        lch::set_thread_error(e.what());
+        return (lc_status)e.code;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->err = e.what();
+        lch::set_thread_error(e.what());
+        return LC_E_DEVICE;
+    }
+}
+
+}  // namespace
+
+// =================================================================== ABI
+extern "C" {
+
+const char *lc_version(void) { return "lcurve_re_amd 0.1 (gfx950)"; }
+
+lc_status lc_ctx_create(int device, lc_ctx **out) {
+    if (!out) return LC_E_ARG;
+    *out = nullptr;
+    return guarded(nullptr, [&]() -> int {
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (ndev <= 0) throw EngineError(LC_E_DEVICE, "no HIP device available");
+        if (device < 0 || device >= ndev) device = 0;
+        HIPCHK(hipSetDevice(device));
+        std::unique_ptr<lc_ctx> c(new lc_ctx());
+        c->device = device;
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+        std::memset(&c->prof, 0, sizeof(c->prof));
+        if (const char *r = std::getenv("LCURVE_AMD_FLUX_R")) c->flux_rphase = std::atoi(r);
+        if (c->flux_rphase != 1 && c->flux_rphase != 2 && c->flux_rphase != 4) c->flux_rphase = 2;
+        if (const char *ch = std::getenv("LCURVE_AMD_FLUX_CHUNK")) c->flux_chunk = std::max(32, std::atoi(ch));
+        *out = c.release();
+        return LC_OK;
+    });
+}
+
+void lc_ctx_destroy(lc_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *lc_last_error(const lc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+lc_status lc_ctx_get_profile(const lc_ctx *ctx, lc_profile *out) {
+    if (!ctx || !out) return LC_E_ARG;
+    *out = ctx->prof;
+    return LC_OK;
+}
+
+lc_status lc_ctx_reset_profile(lc_ctx *ctx) {
+    if (!ctx) return LC_E_ARG;
+    std::memset(&ctx->prof, 0, sizeof(ctx->prof));
+    return LC_OK;
+}
+
+lc_status lc_star_grid_size(const lc_model *m, int star, int fine, size_t *nface) {
+    if (!m || !nface || (star != 0 && star != 1)) return LC_E_ARG;
+    return guarded(nullptr, [&]() -> int {
+        GridPlan gp = plan_grid(*m, star, fine != 0);
+        *nface = (size_t)gp.dev.nface;
+        return LC_OK;
+    });
+}
+
+lc_status lc_light_curve_comp_batch(lc_ctx *ctx, const lc_model *models, size_t m, const lc_datum *data, size_t n,
+                                    const lc_eval_opts *opts, double *calc, lc_scalars *io, int32_t *status) {
+    if (!ctx || !models || !data || !opts || !calc || !io || !status) return LC_E_ARG;
+    if (m == 0) return LC_OK;
+    return guarded(ctx, [&]() -> int {
+        if (n == 0) throw EngineError(LC_E_ARG, "light_curve_comp: empty data");
+        HIPCHK(hipSetDevice(ctx->device));
+        EvalRequest rq{models, m, data, n, *opts, calc, io, status};
+        run_eval(ctx, rq);
+        return LC_OK;
+    });
+}
+
+lc_status lc_light_curve_comp(lc_ctx *ctx, const lc_model *model, const lc_datum *data, size_t n,
+                              const lc_eval_opts *opts, double *calc, lc_scalars *io) {
+    if (!ctx || !model || !data || !opts || !calc || !io) return LC_E_ARG;
+    int32_t st = LC_OK;
+    lc_status r = lc_light_curve_comp_batch(ctx, model, 1, data, n, opts, calc, io, &st);
+    if (r != LC_OK) return r;
+    return (lc_status)st;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------- stage entry points
+namespace {
+
+template <class T> T *dev_upload(DevArena &A, const T *src, size_t n, hipStream_t s) {
+    T *d = A.take<T>(std::max<size_t>(n, 1));
+    if (n) HIPCHK(hipMemcpyAsync(d, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return d;
+}
+
+// flat_sum_star{1,2}_multi on caller-supplied FlatGrid / PhaseBatch arrays.
+int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_batch *pb, const lc_ldc *ldc,
+                   double beam, double spin, double VFAC, double XCOFM, int glens1, double rlens1, int precision,
+                   double *out) {
+    if (!g || !pb || !ldc || !out) throw EngineError(LC_E_ARG, "flat_sum: null argument");
+    if (g->n1 != g->n) throw EngineError(LC_E_UNSUPPORTED, "flat_sum: faces with several eclipse ranges");
+    if (g->n0 > g->n1) throw EngineError(LC_E_ARG, "flat_sum: n0 > n1");
+    const size_t nf = g->n, S = pb->n;
+    if (S == 0) return LC_OK;
+    if (nf == 0) {
+        std::fill(out, out + S, 0.0);
+        return LC_OK;
+    }
+    if (nf > (size_t)(1 << 30) || S > (size_t)(1 << 28)) throw EngineError(LC_E_ARG, "flat_sum: arrays too large");
+    const bool fp64 = precision == LC_PREC_FP64;
+    GridDev G;
+    std::memset(&G, 0, sizeof(G));
+    G.star = star;
+    G.nface = (int)nf;
+    G.beam = beam;
+    G.spin_beam = spin;
+    G.VFAC = VFAC;
+    G.XCOFM = XCOFM;
+    G.rlens1 = rlens1;
+    G.glens = (star == 1 && glens1) ? 1 : 0;
+    G.ldc[0] = ldc->c1; G.ldc[1] = ldc->c2; G.ldc[2] = ldc->c3; G.ldc[3] = ldc->c4;
+    G.mucrit = ldc->mucrit;
+    G.ltype = ldc->type;
+    std::vector<float4> r0(nf), r1(nf), r2(nf);
+    std::vector<double2> ec(nf);
+    for (size_t i = 0; i < nf; i++) {
+        r0[i] = make_float4((float)g->dx[i], (float)g->dy[i], (float)g->dz[i], (float)g->flux[i]);
+        double vx = -VFAC * spin * g->py[i];
+        double vy = star == 0 ? VFAC * (spin * g->px[i] - XCOFM) : VFAC * (spin * (g->px[i] - 1.0) + 1.0 - XCOFM);
+        double vn = g->dx[i] * vx + g->dy[i] * vy;
+        r1[i] = make_float4((float)vx, (float)vy, (float)vn, 0.f);
+        r2[i] = make_float4((float)g->px[i], (float)g->py[i], (float)g->pz[i], 0.f);
+        ec[i] = i >= g->n0 ? make_double2(g->in1[i - g->n0], g->out1[i - g->n0]) : make_double2(0., 0.);
+    }
+    const int R = ctx->flux_rphase, TILE = 256 * R;
+    const int chunk = fp64 ? 1024 : ctx->flux_chunk;
+    BatchDev B;
+    B.inst = 0; B.star = star; B.grid = 0; B.list_off = -1; B.count_idx = -1;
+    B.chunk = chunk;
+    B.nchunk = (int)((nf + chunk - 1) / chunk);
+    B.ntile = (int)((S + TILE - 1) / TILE);
+    B.part_off = 0;
+    std::vector<TaskDev> tasks;
+    for (int t = 0; t < B.ntile; t++)
+        for (int c = 0; c < B.nchunk; c++) tasks.push_back(TaskDev{0, t, c, 0});
+    const size_t part = (size_t)B.nchunk * B.ntile * TILE;
+    ctx->dev.reserve(nf * (3 * 16 + 16 + 7 * 8 + 8) + S * 8 * 10 + part * 8 + tasks.size() * 16 + 65536);
+    ctx->dev.reset();
+    DevArena &A = ctx->dev;
+    hipStream_t s = ctx->stream;
+    FlatArrays fl;
+    std::memset(&fl, 0, sizeof(fl));
+    fl.rec0 = dev_upload(A, r0.data(), nf, s);
+    fl.rec1 = dev_upload(A, r1.data(), nf, s);
+    fl.rec2 = dev_upload(A, r2.data(), nf, s);
+    fl.ecl = dev_upload(A, ec.data(), nf, s);
+    if (fp64) {
+        fl.fdx = dev_upload(A, g->dx, nf, s); fl.fdy = dev_upload(A, g->dy, nf, s); fl.fdz = dev_upload(A, g->dz, nf, s);
+        fl.fpx = dev_upload(A, g->px, nf, s); fl.fpy = dev_upload(A, g->py, nf, s); fl.fpz = dev_upload(A, g->pz, nf, s);
+        fl.fflux = dev_upload(A, g->flux, nf, s);
+    }
+    int n0 = (int)g->n0;
+    fl.n0 = dev_upload(A, &n0, 1, s);
+    PhaseArrays pa;
+    std::memset(&pa, 0, sizeof(pa));
+    pa.ex = dev_upload(A, pb->ex, S, s);
+    pa.ey = dev_upload(A, pb->ey, S, s);
+    pa.ez = dev_upload(A, pb->ez, S, s);
+    pa.phin = dev_upload(A, pb->phin, S, s);
+    GridDev *d_grid = dev_upload(A, &G, 1, s);
+    BatchDev *d_batch = dev_upload(A, &B, 1, s);
+    TaskDev *d_tasks = dev_upload(A, tasks.data(), tasks.size(), s);
+    double *d_partial = A.take<double>(part);
+    double *d_out = A.take<double>(2 * S);
+    HIPCHK(hipMemsetAsync(d_out, 0, 2 * S * sizeof(double), s));
+    HIPCHK(hipEventRecord(ctx->ev[4], s));
+    launch_flux(s, flux_variant(G), fp64 ? 1 : 0, R, d_tasks, (int)tasks.size(), d_batch, d_grid, fl, pa, (int)S,
+                nullptr, nullptr, d_partial);
+    HIPCHK(hipEventRecord(ctx->ev[5], s));
+    launch_flux_reduce(s, d_batch, 1, (int)S, TILE, nullptr, nullptr, d_partial, d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, d_out + (size_t)star * S, S * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->prof.flux_ms_total += elapsed(ctx->ev[4], ctx->ev[5]);
+    ctx->prof.flux_launches += 1;
+    ctx->prof.flux_elements += (double)nf * S;
+    ctx->prof.flux_flops += (double)nf * S * elem_flops(G);
+    ctx->prof.flux_bytes += (double)nf * 40. + (double)S * 32.;
+    return LC_OK;
+}
+
+// Uploads a set of grids' face tables and runs faces + eclipse search.
+int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_faces *out) {
+    if (!m || !out || (star != 0 && star != 1)) throw EngineError(LC_E_ARG, "set_star_grid: bad argument");
+    GridPlan gp = plan_grid(*m, star, fine != 0);
+    fill_physics(*m, gp.dev);
+    GridDev G = gp.dev;
+    if ((size_t)G.nface != out->n) throw EngineError(LC_E_ARG, "set_star_grid: output size differs from the grid size");
+    const int nf = G.nface;
+    std::vector<RingDev> rings;
+    int fo = 0, ho = 0;
+    for (auto &R : gp.rings) {
+        RingDev d;
+        d.sint = R.sint; d.cost = R.cost; d.phi1 = R.phi1; d.dphi_span = R.dphi_span; d.tspan_nl = R.tspan_nl;
+        d.nphi = R.nphi; d.face_off = fo; d.half_off = ho; d.grid = 0;
+        rings.push_back(d);
+        fo += R.nphi;
+        ho += (R.nphi + 1) / 2;
+    }
+    G.face_off = 0; G.ring_off = 0; G.nring = (int)rings.size(); G.half_off = 0; G.nhalf = ho; G.inst = 0;
+    if (nf == 0) return LC_OK;
+    const size_t FN = (size_t)nf;
+    ctx->dev.reserve(FN * (6 * 8 + 3 * 4 + 4 + 16 + 4) + rings.size() * sizeof(RingDev) + 65536);
+    ctx->dev.reset();
+    DevArena &A = ctx->dev;
+    hipStream_t s = ctx->stream;
+    FaceArrays fa;
+    fa.px = A.take<double>(FN); fa.py = A.take<double>(FN); fa.pz = A.take<double>(FN);
+    fa.dx = A.take<double>(FN); fa.dy = A.take<double>(FN); fa.dz = A.take<double>(FN);
+    fa.area = A.take<float>(FN); fa.grav = A.take<float>(FN); fa.flux = A.take<float>(FN);
+    fa.ecl = A.take<int>(FN); fa.ein = A.take<double>(FN); fa.eout = A.take<double>(FN);
+    int *d_cand = A.take<int>(FN);
+    int *d_count = A.take<int>(1);
+    int *d_status = A.take<int>(1);
+    GridDev *d_grid = dev_upload(A, &G, 1, s);
+    RingDev *d_rings = dev_upload(A, rings.data(), rings.size(), s);
+    HIPCHK(hipMemsetAsync(d_count, 0, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(d_status, 0, sizeof(int), s));
+    if (gp.ring_faces < nf) {
+        int gap = nf - gp.ring_faces, g0 = gp.ring_faces;
+        for (double *p : {fa.px, fa.py, fa.pz, fa.dx, fa.dy, fa.dz, fa.ein, fa.eout})
+            HIPCHK(hipMemsetAsync(p + g0, 0, gap * 8, s));
+        HIPCHK(hipMemsetAsync(fa.area + g0, 0, gap * 4, s));
+        HIPCHK(hipMemsetAsync(fa.ecl + g0, 0, gap * 4, s));
+        std::vector<float> ones(gap, 1.f);
+        HIPCHK(hipMemcpyAsync(fa.grav + g0, ones.data(), gap * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    HIPCHK(hipEventRecord(ctx->ev[0], s));
+    launch_faces(s, d_grid, d_rings, (int)rings.size(), ho, fa, d_cand, d_count, d_status);
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    launch_eclipse(s, d_grid, 1, fa, d_cand, d_count, nf, d_status);
+    HIPCHK(hipEventRecord(ctx->ev[2], s));
+    HIPCHK(hipGetLastError());
+    int st = 0;
+    HIPCHK(hipMemcpyAsync(&st, d_status, sizeof(int), hipMemcpyDeviceToHost, s));
+    struct Dn { void *dst; const void *src; size_t b; };
+    Dn dns[] = {{out->px, fa.px, FN * 8}, {out->py, fa.py, FN * 8}, {out->pz, fa.pz, FN * 8},
+                {out->dx, fa.dx, FN * 8}, {out->dy, fa.dy, FN * 8}, {out->dz, fa.dz, FN * 8},
+                {out->area, fa.area, FN * 4}, {out->gravity, fa.grav, FN * 4}, {out->necl, fa.ecl, FN * 4},
+                {out->ingress, fa.ein, FN * 8}, {out->egress, fa.eout, FN * 8}};
+    for (auto &d : dns)
+        if (d.dst) HIPCHK(hipMemcpyAsync(d.dst, d.src, d.b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (out->flux) std::fill(out->flux, out->flux + FN, 0.f);
+    ctx->prof.grid_ms_total += elapsed(ctx->ev[0], ctx->ev[1]);
+    ctx->prof.eclipse_ms_total += elapsed(ctx->ev[1], ctx->ev[2]);
+    if (st) throw EngineError(st, status_msg(st));
+    return LC_OK;
+}
+
+int continuum_stage(lc_ctx *ctx, const lc_model *m, lc_star_faces *s1, lc_star_faces *s2) {
+    if (!m || !s1 || !s2) throw EngineError(LC_E_ARG, "set_star_continuum: null argument");
+    GridDev G[2];
+    for (int k = 0; k < 2; k++) {
+        std::memset(&G[k], 0, sizeof(GridDev));
+        G[k].star = k;
+        fill_physics(*m, G[k]);
+    }
+    G[0].nface = (int)s1->n;
+    G[0].face_off = 0;
+    G[1].nface = (int)s2->n;
+    G[1].face_off = (int)s1->n;
+    const size_t FN = s1->n + s2->n;
+    if (FN == 0) return LC_OK;
+    ctx->dev.reserve(FN * (6 * 8 + 3 * 4) + 65536);
+    ctx->dev.reset();
+    DevArena &A = ctx->dev;
+    hipStream_t s = ctx->stream;
+    FaceArrays fa;
+    std::memset(&fa, 0, sizeof(fa));
+    double **dp[6] = {&fa.px, &fa.py, &fa.pz, &fa.dx, &fa.dy, &fa.dz};
+    double *src1[6] = {s1->px, s1->py, s1->pz, s1->dx, s1->dy, s1->dz};
+    double *src2[6] = {s2->px, s2->py, s2->pz, s2->dx, s2->dy, s2->dz};
+    for (int k = 0; k < 6; k++) {
+        *dp[k] = A.take<double>(FN);
+        if (s1->n) HIPCHK(hipMemcpyAsync(*dp[k], src1[k], s1->n * 8, hipMemcpyHostToDevice, s));
+        if (s2->n) HIPCHK(hipMemcpyAsync(*dp[k] + s1->n, src2[k], s2->n * 8, hipMemcpyHostToDevice, s));
+    }
+    fa.area = A.take<float>(FN);
+    fa.grav = A.take<float>(FN);
+    fa.flux = A.take<float>(FN);
+    if (s1->n) {
+        HIPCHK(hipMemcpyAsync(fa.area, s1->area, s1->n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(fa.grav, s1->gravity, s1->n * 4, hipMemcpyHostToDevice, s));
+    }
+    if (s2->n) {
+        HIPCHK(hipMemcpyAsync(fa.area + s1->n, s2->area, s2->n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(fa.grav + s1->n, s2->gravity, s2->n * 4, hipMemcpyHostToDevice, s));
+    }
+    GridDev *d_grids = dev_upload(A, G, 2, s);
+    HIPCHK(hipEventRecord(ctx->ev[2], s));
+    // grids with no faces still take part in the binary search by offset
+    launch_continuum(s, d_grids, s2->n ? 2 : 1, (int)FN, fa);
+    HIPCHK(hipEventRecord(ctx->ev[3], s));
+    HIPCHK(hipGetLastError());
+    if (s1->n) HIPCHK(hipMemcpyAsync(s1->flux, fa.flux, s1->n * 4, hipMemcpyDeviceToHost, s));
+    if (s2->n) HIPCHK(hipMemcpyAsync(s2->flux, fa.flux + s1->n, s2->n * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->prof.cont_ms_total += elapsed(ctx->ev[2], ctx->ev[3]);
+    return LC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lc_status lc_flat_sum_star1_multi(lc_ctx *ctx, const lc_flat_grid *g, const lc_phase_batch *pb, const lc_ldc *ldc,
+                                  double beam, double spin, double VFAC, double XCOFM, int precision, double *out) {
+    if (!ctx) return LC_E_ARG;
+    return guarded(ctx, [&]() -> int {
+        HIPCHK(hipSetDevice(ctx->device));
+        return flat_sum_stage(ctx, 0, g, pb, ldc, beam, spin, VFAC, XCOFM, 0, 0., precision, out);
+    });
+}
+
+lc_status lc_flat_sum_star2_multi(lc_ctx *ctx, const lc_flat_grid *g, const lc_phase_batch *pb, const lc_ldc *ldc,
+                                  double beam, double spin, double VFAC, double XCOFM, int glens1, double rlens1,
+                                  int precision, double *out) {
+    if (!ctx) return LC_E_ARG;
+    return guarded(ctx, [&]() -> int {
+        HIPCHK(hipSetDevice(ctx->device));
+        return flat_sum_stage(ctx, 1, g, pb, ldc, beam, spin, VFAC, XCOFM, glens1, rlens1, precision, out);
+    });
+}
+
+lc_status lc_set_star_grid(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_faces *out) {
+    if (!ctx) return LC_E_ARG;
+    return guarded(ctx, [&]() -> int {
+        HIPCHK(hipSetDevice(ctx->device));
+        return star_grid_stage(ctx, m, star, fine, out);
+    });
+}
+
+lc_status lc_set_star_continuum(lc_ctx *ctx, const lc_model *m, lc_star_faces *star1, lc_star_faces *star2) {
+    if (!ctx) return LC_E_ARG;
+    return guarded(ctx, [&]() -> int {
+        HIPCHK(hipSetDevice(ctx->device));
+        return continuum_stage(ctx, m, star1, star2);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,545 @@
+// k_geom.hip -- face grid, eclipse search, continuum, FlatGrid build,
+// Ginterp and diagnostic reductions (FP64).
+//
+// Face grid: one work item per half-ring face solves the radius (the ring
+// solve of face.cpp:71-161, falling back to the scalar Roche::face when it
+// fails, set_star_grid.cpp:455-471) and writes the face and its mirror
+// partner (set_star_grid.cpp:493-510).  Faces whose line of sight can cross
+// the companion's reference sphere are appended to a candidate list; the
+// eclipse kernel then runs the FP64 ingress/egress search
+// (ingress_egress.cpp:35-97) over that compacted list only.
+#include "lc_kernels.h"
+#include "lc_math.h"
+
+namespace lck {
+using namespace lcx;
+
+namespace {
+
+__device__ __forceinline__ void set_status(int *inst_status, int inst, int st) {
+    if (st) atomicCAS(&inst_status[inst], 0, st);
+}
+
+__device__ __forceinline__ int find_grid(const GridDev *grids, int ngrid, int face) {
+    int lo = 0, hi = ngrid - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (grids[mid].face_off <= face)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Eclipse of one face by the companion (set_star_grid.cpp:372-386).  For a
+// Roche companion only the reference-sphere test runs here: a hit makes the
+// face a candidate for the full search.
+__device__ __forceinline__ void eclipse_pretest(const GridDev &G, const V3 &pos, int face, FaceArrays fa, int *cand,
+                                                int *cand_count) {
+    fa.ecl[face] = 0;
+    fa.ein[face] = 0.;
+    fa.eout[face] = 0.;
+    if (!G.eclipse) return;
+    const V3 cofm_other = G.star == PRIMARY ? v3(1., 0., 0.) : v3(0., 0., 0.);
+    double phi1, phi2, lam1, lam2;
+    if (G.roche_other) {
+        if (sphere_eclipse(G.cosi, G.sini, pos, cofm_other, G.rref_other, phi1, phi2, lam1, lam2)) {
+            int k = atomicAdd(cand_count, 1);
+            cand[k] = face;
+        }
+    } else if (sphere_eclipse(G.cosi, G.sini, pos, cofm_other, G.r_other, phi1, phi2, lam1, lam2)) {
+        fa.ecl[face] = 1;
+        fa.ein[face] = phi1;
+        fa.eout[face] = phi2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_faces(const GridDev *__restrict__ grids, const RingDev *__restrict__ rings,
+                                              int nring, int nhalf_total, FaceArrays fa, int *cand, int *cand_count,
+                                              int *inst_status) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nhalf_total) return;
+    int lo = 0, hi = nring - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (rings[mid].half_off <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const RingDev R = rings[lo];
+    const GridDev &G = grids[R.grid];
+    const int np = t - R.half_off;
+    const int nphi = R.nphi;
+    double phi = R.phi1 + R.dphi_span * (np + 0.5) / nphi;
+    double sinp, cosp;
+    sincos(phi, &sinp, &cosp);
+    V3 dirn = G.npole ? v3(R.sint * cosp, R.sint * sinp, R.cost) : v3(R.cost, R.sint * cosp, R.sint * sinp);
+    const V3 cofm = G.star == PRIMARY ? v3(0., 0., 0.) : v3(1., 0., 0.);
+    V3 posn, dvec;
+    double rad, gravity;
+    if (G.roche_self) {
+        RayPot rp = make_raypot(G.q, G.star, G.spin_self, dirn, false);
+        int fail = solve_radius(rp, G.rref_self, G.pref_self, G.acc, rad);
+        if (fail) {
+            int st = face_scalar(G.q, G.star, G.spin_self, dirn, G.rref_self, G.pref_self, G.acc, posn, dvec, rad,
+                                 gravity);
+            if (st) {
+                set_status(inst_status, G.inst, ST_FACE);
+                posn = add(cofm, scl(G.rref_self, dirn));
+                dvec = dirn;
+                rad = G.rref_self;
+                gravity = 1.;
+            }
+        } else {
+            posn = add(cofm, scl(rad, dirn));
+            dvec = drpot_star(G.star, G.q, G.spin_self, posn);
+            gravity = len(dvec);
+            dvec = v3(dvec.x / gravity, dvec.y / gravity, dvec.z / gravity);
+        }
+    } else {
+        rad = G.r_self;
+        posn = add(cofm, scl(rad, dirn));
+        dvec = dirn;
+        gravity = 1.;
+    }
+    double area = ((R.dphi_span / nphi * rad * R.sint) * (R.tspan_nl * rad)) / dot(dirn, dvec);
+    const float farea = (float)area, fgrav = (float)(gravity / G.gref);
+
+    int f = R.face_off + np;
+    fa.px[f] = posn.x; fa.py[f] = posn.y; fa.pz[f] = posn.z;
+    fa.dx[f] = dvec.x; fa.dy[f] = dvec.y; fa.dz[f] = dvec.z;
+    fa.area[f] = farea;
+    fa.grav[f] = fgrav;
+    fa.flux[f] = 0.f;
+    eclipse_pretest(G, posn, f, fa, cand, cand_count);
+
+    int np2 = nphi - 1 - np;
+    if (np2 != np) {
+        V3 posn2 = posn, dvec2 = dvec;
+        if (G.npole) {
+            posn2.y = -posn.y;
+            dvec2.y = -dvec.y;
+        } else {
+            posn2.z = -posn.z;
+            dvec2.z = -dvec.z;
+        }
+        int f2 = R.face_off + np2;
+        fa.px[f2] = posn2.x; fa.py[f2] = posn2.y; fa.pz[f2] = posn2.z;
+        fa.dx[f2] = dvec2.x; fa.dy[f2] = dvec2.y; fa.dz[f2] = dvec2.z;
+        fa.area[f2] = farea;
+        fa.grav[f2] = fgrav;
+        fa.flux[f2] = 0.f;
+        eclipse_pretest(G, posn2, f2, fa, cand, cand_count);
+    }
+}
+
+// Full FP64 ingress/egress search over the compacted candidates.  The
+// candidate count is read on the device, so no host round trip is needed;
+// the grid-stride loop ends for every wave once the list is consumed.
+__global__ __launch_bounds__(128) void k_eclipse(const GridDev *__restrict__ grids, int ngrid, FaceArrays fa,
+                                                const int *__restrict__ cand, const int *__restrict__ cand_count,
+                                                int *inst_status) {
+    const int n = *cand_count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int f = cand[i];
+        const GridDev &G = grids[find_grid(grids, ngrid, f)];
+        const int other = G.star == PRIMARY ? SECONDARY : PRIMARY;
+        double ingress = 0., egress = 0.;
+        bool ecl = false;
+        int st = ingress_egress(G.q, other, G.spin_other, G.cosi, G.sini, G.delta, v3(fa.px[f], fa.py[f], fa.pz[f]),
+                                G.rref_other, G.pref_other, ingress, egress, ecl);
+        if (st) set_status(inst_status, G.inst, st);
+        fa.ecl[f] = (ecl && !st) ? 1 : 0;
+        fa.ein[f] = ecl ? ingress : 0.;
+        fa.eout[f] = ecl ? egress : 0.;
+    }
+}
+
+// Continuum brightness * area of one face (set_star_continuum.cpp:103-296)
+__global__ __launch_bounds__(256) void k_continuum(const GridDev *__restrict__ grids, int ngrid, int nface_total,
+                                                  FaceArrays fa) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface_total) return;
+    const GridDev &G = grids[find_grid(grids, ngrid, f)];
+    const V3 posn = v3(fa.px[f], fa.py[f], fa.pz[f]);
+    const V3 dirn = v3(fa.dx[f], fa.dy[f], fa.dz[f]);
+    const V3 cofm_other = G.star == PRIMARY ? v3(1., 0., 0.) : v3(0., 0., 0.);
+    V3 vec = sub(cofm_other, posn);
+    double r = len(vec);
+    double mu = dot(dirn, vec) / r;
+    double t = G.t_base;
+    for (int k = 0; k < G.nspot; k++) {
+        V3 sd = v3(G.spot_dir[k][0], G.spot_dir[k][1], G.spot_dir[k][2]);
+        double dist;
+        if (G.star == PRIMARY) {
+            dist = rad2deg(acos(dot(posn, sd) / len(posn)));
+        } else {
+            V3 off = sub(posn, v3(1., 0., 0.));
+            dist = rad2deg(acos(dot(off, sd) / len(off)));
+        }
+        const double EFAC = 2.3548200450309493;  // sqrt(8 ln 2), constants.h:96
+        double tref = G.spot_running ? t : G.t_base;
+        t += (G.spot_tcen[k] - tref) * exp(-sqr(dist / (G.spot_fwhm[k] / EFAC)) / 2.);
+    }
+    if (G.uespot) {
+        V3 evec = v3(posn.x, posn.y, 0);
+        double evecl = len(evec);
+        if (evecl > 0.) {
+            double rd = len(posn);
+            double clat = dot(posn, evec) / evecl / rd;
+            clat = dmin(1.0, dmax(-1., clat));
+            double dlat = rad2deg(acos(clat));
+            V3 ue = v3(G.ue_dir[0], G.ue_dir[1], G.ue_dir[2]);
+            double clong = dot(ue, evec) / evecl;
+            clong = dmin(1.0, dmax(-1., clong));
+            double dlong = rad2deg(acos(clong));
+            if (dlat <= G.ue_lathw && dlong <= G.ue_longhw)
+                t = G.ue_temp;
+            else if (dlat <= G.ue_lathw)
+                t += (G.ue_temp - G.t_base) * exp(-(dlong - G.ue_longhw) / G.ue_taper);
+            else if (dlong <= G.ue_longhw)
+                t += (G.ue_temp - G.t_base) * exp(-(dlat - G.ue_lathw) / G.ue_taper);
+            else
+                t += (G.ue_temp - G.t_base) * exp(-(dlat - G.ue_lathw) / G.ue_taper) *
+                     exp(-(dlong - G.ue_longhw) / G.ue_taper);
+        }
+    }
+    const double ro = G.r_cont_other;
+    double temp, geom;
+    const double g = (double)fa.grav[f];
+    if (mu >= ro) {
+        geom = sqr(ro / r) * mu;
+        temp = pow(pow(t * pow(g, G.gdcbol), 4.0) + G.absorb * pow(G.t_irr, 4.0) * geom, 0.25);
+    } else if (mu > -ro) {
+        double x0 = -mu / ro;
+        geom = sqr(ro / r) * ro * (sqrt(1. - x0 * x0) * (2 + x0 * x0) / 3 - x0 * (PI / 2 - asin(x0))) / PI;
+        temp = pow(pow(t * pow(g, G.gdcbol), 4.0) + G.absorb * pow(G.t_irr, 4.0) * geom, 0.25);
+    } else {
+        geom = 0.;
+        temp = t * pow(g, G.gdcbol);
+    }
+    const double area = (double)fa.area[f];
+    float flux = (float)(area * planck(G.wave, temp));
+    if (G.mirror) flux = (float)((double)flux + area * geom * planck(G.wave, G.t_mirror));
+    fa.flux[f] = flux;
+}
+
+// Deterministic block sums (fixed tree order).
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+template <int NT>
+__device__ __forceinline__ int block_scan_excl(int v, int *lds, int &total) {
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 1; s < NT; s <<= 1) {
+        int add = (int)threadIdx.x >= s ? lds[threadIdx.x - s] : 0;
+        __syncthreads();
+        lds[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int incl = lds[threadIdx.x];
+    total = lds[NT - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// Stable partition of every grid into never-eclipsed / eclipsed faces
+// (FlatGrid::build, comp_light_flat.cpp:26-65) and the packed float4
+// records of the mixed-precision flux kernel.  One workgroup per grid.
+__global__ __launch_bounds__(1024) void k_flat_build(const GridDev *__restrict__ grids, FaceArrays fa, FlatArrays fl,
+                                                    int fp64) {
+    __shared__ int lds[1024];
+    const GridDev &G = grids[blockIdx.x];
+    const int n = G.nface, base = G.face_off;
+    int cnt = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) cnt += fa.ecl[base + i] ? 0 : 1;
+    int n0;
+    block_scan_excl<1024>(cnt, lds, n0);
+    if (threadIdx.x == 0) fl.n0[blockIdx.x] = n0;
+    int done0 = 0, done1 = 0;
+    for (int c = 0; c < n; c += 1024) {
+        int i = c + (int)threadIdx.x;
+        int plain = (i < n && !fa.ecl[base + i]) ? 1 : 0;
+        int tot;
+        int ex = block_scan_excl<1024>(plain, lds, tot);
+        if (i < n) {
+            int pos = plain ? done0 + ex : n0 + done1 + ((int)threadIdx.x - ex);
+            int f = base + i, p = base + pos;
+            fl.perm[p] = f;
+            double dx = fa.dx[f], dy = fa.dy[f], dz = fa.dz[f];
+            double px = fa.px[f], py = fa.py[f], pz = fa.pz[f];
+            float flux = fa.flux[f];
+            fl.rec0[p] = make_float4((float)dx, (float)dy, (float)dz, flux);
+            double vx = -G.VFAC * G.spin_beam * py;
+            double vy = G.star == PRIMARY ? G.VFAC * (G.spin_beam * px - G.XCOFM)
+                                          : G.VFAC * (G.spin_beam * (px - 1.0) + 1.0 - G.XCOFM);
+            double vn = dx * vx + dy * vy;
+            fl.rec1[p] = make_float4((float)vx, (float)vy, (float)vn, 0.f);
+            fl.rec2[p] = make_float4((float)px, (float)py, (float)pz, 0.f);
+            fl.ecl[p] = make_double2(fa.ein[f], fa.eout[f]);
+            if (fp64) {
+                fl.fdx[p] = dx; fl.fdy[p] = dy; fl.fdz[p] = dz;
+                fl.fpx[p] = px; fl.fpy[p] = py; fl.fpz[p] = pz;
+                fl.fflux[p] = (double)flux;
+            }
+        }
+        done0 += tot;
+        done1 += min(1024, n - c) - tot;
+    }
+}
+
+// LDC::imu (lcurve.h:216-231)
+__device__ __forceinline__ double ldc_imu(double mu, const GridDev &G) {
+    if (mu <= 0) return 0.;
+    mu = dmin(mu, 1.);
+    double ommu = 1. - mu, im = 1.;
+    if (G.ltype == 0) {
+        im -= ommu * (G.ldc[0] + ommu * (G.ldc[1] + ommu * (G.ldc[2] + ommu * G.ldc[3])));
+    } else {
+        im -= G.ldc[0] + G.ldc[1] + G.ldc[2] + G.ldc[3];
+        double msq = sqrt(mu);
+        im += msq * (G.ldc[0] + msq * (G.ldc[1] + msq * (G.ldc[2] + msq * G.ldc[3])));
+    }
+    return im;
+}
+
+// Point-path element at one phase with spin = 1 for beaming
+// (comp_light.cpp:21-92 as called by comp_star1/comp_star2)
+__device__ double point_elem(const GridDev &G, FaceArrays fa, int f, const V3 &earth, double phi) {
+    double ph = phi - floor(phi);
+    if (fa.ecl[f] && ((ph >= fa.ein[f] && ph <= fa.eout[f]) || ph <= fa.eout[f] - 1.0)) return 0.0;
+    V3 dirn = v3(fa.dx[f], fa.dy[f], fa.dz[f]);
+    double mu = dot(earth, dirn);
+    if (!(mu > G.mucrit)) return 0.0;
+    double flux = (double)fa.flux[f];
+    double magn = 1.0;
+    if (G.star == SECONDARY && G.glens) {
+        V3 s = v3(fa.px[f], fa.py[f], fa.pz[f]);
+        double d = -dot(s, earth);
+        if (d > 0.0) {
+            double p = len(add(s, scl(d, earth)));
+            double hp = 0.5 * p;
+            double rd = G.rlens1 * d;
+            double pd = (hp * hp > 25.0 * rd) ? p + rd / p : hp + sqrt(hp * hp + rd);
+            magn = pd * pd / (pd - hp) / hp / 4.0;
+        }
+    }
+    if (G.beam == 0.0) {
+        if (G.star == PRIMARY) return mu * flux * ldc_imu(mu, G);
+        return mu * magn * flux * ldc_imu(mu, G);
+    }
+    const double spin = 1.0;
+    double vx = -G.VFAC * spin * fa.py[f];
+    double vy = G.star == PRIMARY ? G.VFAC * (spin * fa.px[f] - G.XCOFM)
+                                  : G.VFAC * (spin * (fa.px[f] - 1.0) + 1.0 - G.XCOFM);
+    double vr = -(earth.x * vx + earth.y * vy);
+    double vn = dirn.x * vx + dirn.y * vy;
+    double mud = mu - mu * vr - vn;
+    if (G.star == PRIMARY) return mu * flux * (1.0 - G.beam * vr) * ldc_imu(mud, G);
+    return mu * magn * flux * (1.0 - G.beam * vr) * ldc_imu(mud, G);
+}
+
+__device__ __forceinline__ int gint_type(double phase, double phase1, double phase2) {
+    double pnorm = phase - floor(phase);
+    if (pnorm <= phase1 || pnorm >= 1. - phase1) return 1;
+    if ((pnorm > phase1 && pnorm < phase2) || (pnorm > 1. - phase2 && pnorm < 1. - phase1)) return 2;
+    return 3;
+}
+
+// Phases at which light_curve_comp.cpp:220-254 samples the Point-path sums.
+__device__ __forceinline__ double gint_phase(int k, const InstDev &I) {
+    switch (k) {
+    case 0: return 0.9999999999 * I.phase1;
+    case 1: return 1.0000000001 * I.phase1;
+    case 2: return 1. - 0.9999999999 * I.phase1;
+    case 3: return 1. - 1.0000000001 * I.phase1;
+    case 4: return 1 - 1.0000000001 * I.phase2;
+    case 5: return 1 - 0.9999999999 * I.phase2;
+    case 6: return 1.0000000001 * I.phase2;
+    default: return 0.9999999999 * I.phase2;
+    }
+}
+
+// Raw Point-path sums for the Ginterp scale factors: block (k, inst) sums
+// the grid type(phi) selects.  sums[inst * 8 + k].
+__global__ __launch_bounds__(1024) void k_point_sums(const GridDev *__restrict__ grids,
+                                                    const InstDev *__restrict__ insts, FaceArrays fa, double *sums) {
+    __shared__ double lds[1024];
+    const int k = blockIdx.x, inst = blockIdx.y;
+    const InstDev &I = insts[inst];
+    const bool star1 = k < 4;
+    if (!I.active || (star1 && I.copy1) || (!star1 && I.copy2)) return;
+    double phi = gint_phase(k, I);
+    int ty = gint_type(phi, I.phase1, I.phase2);
+    int gi = star1 ? (ty == 1 ? I.grid1f : I.grid1c) : (ty == 3 ? I.grid2f : I.grid2c);
+    const GridDev &G = grids[gi];
+    V3 earth = set_earth(I.cosi, I.sini, phi);
+    double s = 0.0;
+    for (int i = threadIdx.x; i < G.nface; i += 1024) s += point_elem(G, fa, G.face_off + i, earth, phi);
+    s = block_sum<1024>(s, lds);
+    if (threadIdx.x == 0) sums[inst * 8 + k] = s;
+}
+
+__device__ __forceinline__ double gscale1(const InstDev &I, double phase) {
+    double pnorm = phase - floor(phase);
+    if (pnorm <= I.phase1 || pnorm >= 1. - I.phase1) return 1.;
+    return (I.scale11 * (1. - I.phase1 - pnorm) + I.scale12 * (pnorm - I.phase1)) / (1 - 2. * I.phase1);
+}
+__device__ __forceinline__ double gscale2(const InstDev &I, double phase) {
+    double pnorm = phase - floor(phase);
+    if (pnorm >= I.phase2 && pnorm <= 1. - I.phase2) return 1.;
+    if (pnorm < 0.5) return (I.scale22 * (I.phase2 - pnorm) + I.scale21 * (pnorm + I.phase2)) / (2. * I.phase2);
+    return (I.scale21 * (1. + I.phase2 - pnorm) + I.scale22 * (pnorm - 1. + I.phase2)) / (2. * I.phase2);
+}
+
+// Ginterp scale factors in the reference's update order
+// (light_curve_comp.cpp:214-254): each comp_star value is
+// (0 + 1 * scale(phi) * s) / 1 with the scales updated so far.
+__global__ void k_gint_scales(InstDev *insts, int ninst, const double *sums) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
+    InstDev I = insts[i];
+    if (!I.active) return;
+    I.scale11 = I.scale12 = I.scale21 = I.scale22 = 1.;
+    const double *s = sums + i * 8;
+    if (!I.copy1) {
+        double ff = (0.0 + 1.0 * gscale1(I, gint_phase(0, I)) * s[0]) / 1;
+        double fc = (0.0 + 1.0 * gscale1(I, gint_phase(1, I)) * s[1]) / 1;
+        I.scale11 = ff / fc;
+        ff = (0.0 + 1.0 * gscale1(I, gint_phase(2, I)) * s[2]) / 1;
+        fc = (0.0 + 1.0 * gscale1(I, gint_phase(3, I)) * s[3]) / 1;
+        I.scale12 = ff / fc;
+    }
+    if (!I.copy2) {
+        double ff = (0.0 + 1.0 * gscale2(I, gint_phase(4, I)) * s[4]) / 1;
+        double fc = (0.0 + 1.0 * gscale2(I, gint_phase(5, I)) * s[5]) / 1;
+        I.scale21 = ff / fc;
+        ff = (0.0 + 1.0 * gscale2(I, gint_phase(6, I)) * s[6]) / 1;
+        fc = (0.0 + 1.0 * gscale2(I, gint_phase(7, I)) * s[7]) / 1;
+        I.scale22 = ff / fc;
+    }
+    insts[i] = I;
+}
+
+// Diagnostic sums of one grid: [0] sum flux*gravity (float product,
+// comp_gravity.cpp), [1] sum flux, [2] sum area*rcosa/r^3 and [3] sum
+// area*rcosa (comp_radius.cpp), [4] the phase-0.5 flat sum with spin = 1
+// for the white-dwarf contribution (comp_light_flat.cpp:543-571).
+__global__ __launch_bounds__(1024) void k_diag(const GridDev *__restrict__ grids, const int *__restrict__ gidx,
+                                              FaceArrays fa, double *sums) {
+    __shared__ double lds[1024];
+    const GridDev &G = grids[gidx[blockIdx.x]];
+    const V3 cofm = G.star == PRIMARY ? v3(0., 0., 0.) : v3(1., 0., 0.);
+    const V3 earth = set_earth(G.cosi, G.sini, 0.5);
+    double s[5] = {0., 0., 0., 0., 0.};
+    for (int i = threadIdx.x; i < G.nface; i += 1024) {
+        const int f = G.face_off + i;
+        float fl = fa.flux[f];
+        s[0] += (double)(fl * fa.grav[f]);
+        s[1] += (double)fl;
+        V3 vec = sub(v3(fa.px[f], fa.py[f], fa.pz[f]), cofm);
+        double r = len(vec);
+        double rcosa = dot(v3(fa.dx[f], fa.dy[f], fa.dz[f]), vec);
+        s[2] += fa.area[f] * rcosa / pow(r, 3.0);
+        s[3] += fa.area[f] * rcosa;
+        if (G.star == PRIMARY) {
+            // flat element (comp_light_flat.cpp:110-127), eclipse by range
+            double mu = earth.x * fa.dx[f] + earth.y * fa.dy[f] + earth.z * fa.dz[f];
+            bool hidden = fa.ecl[f] && ((0.5 >= fa.ein[f] && 0.5 <= fa.eout[f]) || 0.5 <= fa.eout[f] - 1.0);
+            if (mu > G.mucrit && !hidden) {
+                double m = dmin(mu, 1.0), im;
+                if (G.ltype == 0) {
+                    double om = 1.0 - m;
+                    im = 1.0 - om * (G.ldc[0] + om * (G.ldc[1] + om * (G.ldc[2] + om * G.ldc[3])));
+                } else {
+                    double base = 1.0 - (G.ldc[0] + G.ldc[1] + G.ldc[2] + G.ldc[3]);
+                    double msq = sqrt(dmax(m, 0.0));
+                    im = base + msq * (G.ldc[0] + msq * (G.ldc[1] + msq * (G.ldc[2] + msq * G.ldc[3])));
+                }
+                double val;
+                if (G.beam == 0.0) {
+                    val = mu * (double)fl * (mu > 0.0 ? im : 0.0);
+                } else {
+                    double vx = -G.VFAC * 1.0 * fa.py[f];
+                    double vy = G.VFAC * (1.0 * fa.px[f] - G.XCOFM);
+                    double vr = -(earth.x * vx + earth.y * vy);
+                    double vn = fa.dx[f] * vx + fa.dy[f] * vy;
+                    double mud = mu - mu * vr - vn;
+                    double md = dmin(mud, 1.0), imd;
+                    if (G.ltype == 0) {
+                        double om = 1.0 - md;
+                        imd = 1.0 - om * (G.ldc[0] + om * (G.ldc[1] + om * (G.ldc[2] + om * G.ldc[3])));
+                    } else {
+                        double base = 1.0 - (G.ldc[0] + G.ldc[1] + G.ldc[2] + G.ldc[3]);
+                        double msq = sqrt(dmax(md, 0.0));
+                        imd = base + msq * (G.ldc[0] + msq * (G.ldc[1] + msq * (G.ldc[2] + msq * G.ldc[3])));
+                    }
+                    val = mu * (double)fl * (1.0 - G.beam * vr) * (mud > 0.0 ? imd : 0.0);
+                }
+                s[4] += val;
+            }
+        }
+    }
+    for (int k = 0; k < 5; k++) {
+        double t = block_sum<1024>(s[k], lds);
+        if (threadIdx.x == 0) sums[blockIdx.x * 8 + k] = t;
+    }
+}
+
+}  // namespace
+
+void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int nring, int nhalf_total,
+                  FaceArrays fa, int *cand, int *cand_count, int *inst_status) {
+    if (nhalf_total <= 0) return;
+    hipLaunchKernelGGL(k_faces, dim3((nhalf_total + 255) / 256), dim3(256), 0, s, grids, rings, nring, nhalf_total,
+                       fa, cand, cand_count, inst_status);
+}
+
+void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int *cand,
+                    const int *cand_count, int max_cand, int *inst_status) {
+    if (max_cand <= 0 || ngrid <= 0) return;
+    int blocks = (max_cand + 127) / 128;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_eclipse, dim3(blocks), dim3(128), 0, s, grids, ngrid, fa, cand, cand_count, inst_status);
+}
+
+void launch_continuum(hipStream_t s, const GridDev *grids, int ngrid, int nface_total, FaceArrays fa) {
+    if (nface_total <= 0) return;
+    hipLaunchKernelGGL(k_continuum, dim3((nface_total + 255) / 256), dim3(256), 0, s, grids, ngrid, nface_total, fa);
+}
+
+void launch_flat_build(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, FlatArrays fl, int fp64) {
+    if (ngrid <= 0) return;
+    hipLaunchKernelGGL(k_flat_build, dim3(ngrid), dim3(1024), 0, s, grids, fa, fl, fp64);
+}
+
+void launch_point_sums(hipStream_t s, const GridDev *grids, const InstDev *insts, int ninst, FaceArrays fa,
+                       double *sums) {
+    if (ninst <= 0) return;
+    hipLaunchKernelGGL(k_point_sums, dim3(8, ninst), dim3(1024), 0, s, grids, insts, fa, sums);
+}
+
+void launch_gint_scales(hipStream_t s, InstDev *insts, int ninst, const double *sums) {
+    if (ninst <= 0) return;
+    hipLaunchKernelGGL(k_gint_scales, dim3((ninst + 63) / 64), dim3(64), 0, s, insts, ninst, sums);
+}
+
+void launch_diag(hipStream_t s, const GridDev *grids, const int *gidx, int ng, FaceArrays fa, double *sums) {
+    if (ng <= 0) return;
+    hipLaunchKernelGGL(k_diag, dim3(ng), dim3(1024), 0, s, grids, gidx, fa, sums);
+}
+
+}  // namespace lck

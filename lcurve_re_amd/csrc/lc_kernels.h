@@ -1,0 +1,116 @@
+// lc_kernels.h -- device-side records of the engine and the host launch
+// wrappers of its kernels.  All records are plain structs shared by host
+// planning and the kernels (HBM layouts are structure-of-arrays except the
+// packed float4 flux records).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lck {
+
+// One star grid of one model instance (fine or coarse).
+struct GridDev {
+    int inst, star, fine, nface;
+    int face_off;            // first face in the global face arrays
+    int ring_off, nring;     // rings of this grid in the ring table
+    int half_off, nhalf;     // first / number of half-ring work items
+    int roche_self, roche_other, eclipse, npole;
+    // geometry (set_star_grid.cpp:52-207 derived values)
+    double q, spin_self, spin_other;
+    double rref_self, pref_self, rref_other, pref_other;
+    double r_self, r_other, acc, delta, cosi, sini, gref;
+    // continuum (set_star_continuum.cpp)
+    double t_base;          // T1 for star 1, |T2| for star 2
+    double t_irr;           // temperature of the irradiating star in pow(T, 4)
+    double t_mirror;        // temperature of the mirror term
+    double gdcbol, absorb, wave, r_cont_other;
+    int mirror, nspot, uespot, spot_running;  // spot_running: star 2 offsets use the running T
+    double spot_dir[3][3], spot_fwhm[3], spot_tcen[3];
+    double ue_dir[3], ue_longhw, ue_lathw, ue_taper, ue_temp;
+    // flux sums (comp_light_flat.cpp:110-168)
+    double beam, spin_beam, VFAC, XCOFM, rlens1;
+    double ldc[4], mucrit;
+    int ltype, glens;
+};
+
+// One latitude ring (set_star_grid.cpp:262-320).
+struct RingDev {
+    double sint, cost, phi1, dphi_span, tspan_nl;
+    int nphi, face_off, half_off, grid;
+};
+
+// Per-instance parameters for phases, scatter and scaling.
+struct InstDev {
+    double t0, period, pdot, deltat, slope, quad, cube, third, cosi, sini;
+    double phase1, phase2, scale11, scale12, scale21, scale22;
+    double sfac0;
+    int copy1, copy2, grid1f, grid2f, grid1c, grid2c;
+    int scale, rdata, active, pad;
+};
+
+struct FaceArrays {
+    double *px, *py, *pz, *dx, *dy, *dz;
+    float *area, *grav, *flux;
+    int *ecl;
+    double *ein, *eout;
+};
+
+// Flux-stage copy of every grid, in the reference's FlatGrid order
+// (never-eclipsed faces first, comp_light_flat.cpp:26-65).
+struct FlatArrays {
+    float4 *rec0;   // dx, dy, dz, flux
+    float4 *rec1;   // vx, vy, vn (beaming), 0
+    float4 *rec2;   // px, py, pz (lensing), 0
+    double2 *ecl;   // ingress, egress
+    int *perm;      // flat position -> face index
+    double *fdx, *fdy, *fdz, *fpx, *fpy, *fpz, *fflux;  // FP64 copies
+    int *n0;        // never-eclipsed count per grid
+};
+
+struct DataDev {
+    const double *time, *expose, *flux, *ferr, *weight;
+    const int *ndiv, *sub_off;
+    int n, S;
+    double middle, range;
+};
+
+struct PhaseArrays {
+    double *ex, *ey, *ez, *phin, *w1, *w2;  // [inst * S + j]
+    signed char *type;                      // Ginterp::type(phi)
+};
+
+struct BatchDev {
+    int inst, star, grid;
+    int list_off;    // offset of the sub-phase index list, -1: identity
+    int count_idx;   // index into the device counts, -1: all S sub-phases
+    int nchunk, chunk, ntile;
+    long long part_off;
+};
+
+struct TaskDev {
+    int batch, tile, chunk, pad;
+};
+
+// ------------------------------------------------------------ launches
+void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int nring, int nhalf_total,
+                  FaceArrays fa, int *cand, int *cand_count, int *inst_status);
+void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int *cand,
+                    const int *cand_count, int max_cand, int *inst_status);
+void launch_continuum(hipStream_t s, const GridDev *grids, int ngrid, int nface_total, FaceArrays fa);
+void launch_flat_build(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, FlatArrays fl, int fp64);
+void launch_point_sums(hipStream_t s, const GridDev *grids, const InstDev *insts, int ninst, FaceArrays fa,
+                       double *sums);
+void launch_gint_scales(hipStream_t s, InstDev *insts, int ninst, const double *sums);
+void launch_phases(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa);
+void launch_route(hipStream_t s, const InstDev *insts, int ninst, int S, PhaseArrays pa, int *lists, int *counts);
+// variant = ltype | beaming << 1 | lensing << 2; rphase = sub-phases per lane
+void launch_flux(hipStream_t s, int variant, int fp64, int rphase, const TaskDev *tasks, int ntask,
+                 const BatchDev *batches, const GridDev *grids, FlatArrays fl, PhaseArrays pa, int S,
+                 const int *lists, const int *counts, double *partial);
+void launch_flux_reduce(hipStream_t s, const BatchDev *batches, int nbatch, int S, int threads_per_tile,
+                        const int *lists, const int *counts, const double *partial, double *out);
+void launch_scatter(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa, const double *out,
+                    double *calc, double *scal);
+void launch_diag(hipStream_t s, const GridDev *grids, const int *gidx, int ng, FaceArrays fa, double *sums);
+
+}  // namespace lck

@@ -1,0 +1,275 @@
+"""ctypes bindings of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg to
+check or time the GPU engine; the product never imports this module.
+Also binds oracle/_ref/libref.so (the reference compiled from its own
+sources) when that library has been built.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "_build", "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libref.so")
+
+_DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int32)
+
+
+def build(ref=True):
+    """Compiles the oracle (and, when the reference sources exist, _ref)."""
+    target = "all" if ref else "oracle"
+    subprocess.run(["make", "-C", HERE, "-j8", target], check=True,
+                   stdout=subprocess.DEVNULL)
+
+
+def _abi():
+    from lcurve_re_amd import _abi as abi  # structure layouts only
+    return abi
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is not None:
+        return _oracle
+    if not os.path.exists(ORACLE_LIB):
+        raise RuntimeError(f"oracle library not built: {ORACLE_LIB}")
+    abi = _abi()
+    lib = C.CDLL(ORACLE_LIB)
+    sig = {
+        "orc_last_error": (C.c_char_p, []),
+        "orc_set_threads": (None, [C.c_int]),
+        "orc_xl1": (C.c_double, [C.c_double, _IP]),
+        "orc_xl11": (C.c_double, [C.c_double, C.c_double, _IP]),
+        "orc_xl12": (C.c_double, [C.c_double, C.c_double, _IP]),
+        "orc_ref_sphere": (C.c_int, [C.c_double, C.c_int, C.c_double, C.c_double, _DP, _DP]),
+        "orc_rpot1": (C.c_double, [C.c_double, C.c_double, _DP]),
+        "orc_rpot2": (C.c_double, [C.c_double, C.c_double, _DP]),
+        "orc_drpot1": (None, [C.c_double, C.c_double, _DP, _DP]),
+        "orc_drpot2": (None, [C.c_double, C.c_double, _DP, _DP]),
+        "orc_face": (C.c_int, [C.c_double, C.c_int, C.c_double, _DP, C.c_double, C.c_double, C.c_double,
+                               _DP, _DP, _DP, _DP]),
+        "orc_face_chop_batch": (None, [C.c_double, C.c_int, C.c_double, _DP, _DP, _DP, C.c_int, C.c_double,
+                                       C.c_double, C.c_double, _DP, C.POINTER(C.c_ubyte)]),
+        "orc_sphere_eclipse": (C.c_int, [C.c_double, C.c_double, _DP, _DP, C.c_double, _DP, _DP, _DP, _DP]),
+        "orc_ingress_egress": (C.c_int, [C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         _DP, C.c_double, C.c_double, _DP, _DP]),
+        "orc_fblink": (C.c_int, [C.c_double, C.c_int, C.c_double, C.c_double, _DP, _DP, C.c_double,
+                                 C.c_double]),
+        "orc_numface": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]),
+        "orc_planck": (C.c_double, [C.c_double, C.c_double]),
+        "orc_dlpdlt": (C.c_double, [C.c_double, C.c_double]),
+        "orc_star_grid_size": (C.c_int, [C.POINTER(abi.lc_model), C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
+        "orc_set_star_grid": (C.c_int, [C.POINTER(abi.lc_model), C.c_int, C.c_int,
+                                        C.POINTER(abi.lc_star_faces)]),
+        "orc_set_star_continuum": (C.c_int, [C.POINTER(abi.lc_model), C.POINTER(abi.lc_star_faces),
+                                             C.POINTER(abi.lc_star_faces)]),
+        "orc_flat_sum_multi": (C.c_int, [C.c_int, C.POINTER(abi.lc_flat_grid), C.POINTER(abi.lc_phase_batch),
+                                         C.POINTER(abi.lc_ldc), C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_int, C.c_double, _DP]),
+        "orc_point_sum": (C.c_double, [C.c_int, C.POINTER(abi.lc_star_faces), C.POINTER(abi.lc_ldc),
+                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                       C.c_double]),
+        "orc_re_scale": (C.c_double, [C.POINTER(abi.lc_datum), C.c_size_t, _DP, _DP, _DP]),
+        "orc_light_curve_comp": (C.c_int, [C.POINTER(abi.lc_model), C.POINTER(abi.lc_datum), C.c_size_t,
+                                           C.POINTER(abi.lc_eval_opts), _DP, C.POINTER(abi.lc_scalars)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _oracle = lib
+    return lib
+
+
+def have_ref():
+    return os.path.exists(REF_LIB)
+
+
+def ref():
+    """The reference compiled from its sources (oracle/_ref/libref.so)."""
+    global _ref
+    if _ref is not None:
+        return _ref
+    if not have_ref():
+        raise RuntimeError("oracle/_ref/libref.so not built")
+    lib = C.CDLL(REF_LIB)
+    D, I, Z = C.c_double, C.c_int, C.c_size_t
+    sig = {
+        "ref_last_error": (C.c_char_p, []),
+        "ref_xl1": (D, [D]), "ref_xl11": (D, [D, D]), "ref_xl12": (D, [D, D]),
+        "ref_ref_sphere": (None, [D, I, D, D, _DP, _DP]),
+        "ref_rpot1": (D, [D, D, _DP]), "ref_rpot2": (D, [D, D, _DP]),
+        "ref_face": (I, [D, I, D, _DP, D, D, D, _DP, _DP, _DP, _DP]),
+        "ref_face_chop_batch": (None, [D, I, D, _DP, _DP, _DP, I, D, D, D, _DP, C.POINTER(C.c_ubyte)]),
+        "ref_sphere_eclipse": (I, [D, D, _DP, _DP, D, _DP, _DP, _DP, _DP]),
+        "ref_ingress_egress": (I, [D, I, D, D, D, D, _DP, D, D, _DP, _DP]),
+        "ref_fblink": (I, [D, I, D, D, _DP, _DP, D, D]),
+        "ref_numface": (I, [I, I, D, D, I, I]),
+        "ref_envelope": (None, [D, D, D, _DP, _DP]),
+        "ref_planck": (D, [D, D]), "ref_dlpdlt": (D, [D, D]),
+        "ref_add_faces": (I, [I, _IP, D, D, D, I, I, I, I, D, D, D, D, D, D, I, I, D, D, I, D, D, D, D, D, D,
+                              _DP, _DP, _DP, _DP, _DP, _DP, _FP, _FP, _IP, _DP, _DP]),
+        "ref_flat_sum_multi": (None, [I, Z, Z, Z, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, Z, _DP, _DP,
+                                      _DP, _DP, D, D, D, D, D, I, D, D, D, D, I, D, _DP]),
+        "ref_comp_star": (D, [I, Z, _DP, _DP, _DP, _DP, _DP, _DP, _FP, _IP, _DP, _DP, D, D, D, D, D, I, D, D,
+                              D, I, D, D, C.c_float, I, D, D, D]),
+        "ref_re_scale": (D, [Z, _DP, _DP, _DP, _DP, _DP, _DP, _DP]),
+        "ref_comp_radius": (D, [I, Z, _DP, _DP, _DP, _DP, _DP, _DP, _FP]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _ref = lib
+    return lib
+
+
+# ------------------------------------------------------------------ helpers
+def dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+def fptr(a):
+    return a.ctypes.data_as(_FP)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_IP)
+
+
+class Faces:
+    """SoA star grid (numpy arrays) with an lc_star_faces view."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        for k in ("px", "py", "pz", "dx", "dy", "dz", "ingress", "egress"):
+            setattr(self, k, np.zeros(self.n, np.float64))
+        for k in ("area", "gravity", "flux"):
+            setattr(self, k, np.zeros(self.n, np.float32))
+        self.necl = np.zeros(self.n, np.int32)
+
+    def view(self):
+        abi = _abi()
+        v = abi.lc_star_faces()
+        v.n = self.n
+        for k in ("px", "py", "pz", "dx", "dy", "dz", "ingress", "egress"):
+            setattr(v, k, dptr(getattr(self, k)))
+        for k in ("area", "gravity", "flux"):
+            setattr(v, k, fptr(getattr(self, k)))
+        v.necl = iptr(self.necl)
+        return v
+
+    def copy(self):
+        f = Faces(self.n)
+        for k in ("px", "py", "pz", "dx", "dy", "dz", "ingress", "egress", "area", "gravity", "flux", "necl"):
+            setattr(f, k, getattr(self, k).copy())
+        return f
+
+
+class Flat:
+    """Lcurve::FlatGrid in numpy: stable partition, never-eclipsed first
+    (comp_light_flat.cpp:26-65)."""
+
+    def __init__(self, faces):
+        order = np.concatenate([np.nonzero(faces.necl == 0)[0], np.nonzero(faces.necl == 1)[0]])
+        self.n = faces.n
+        self.n0 = int(np.count_nonzero(faces.necl == 0))
+        self.n1 = self.n
+        for k in ("dx", "dy", "dz", "px", "py", "pz"):
+            setattr(self, k, np.ascontiguousarray(getattr(faces, k)[order]))
+        self.flux = faces.flux[order].astype(np.float64)
+        self.in1 = np.ascontiguousarray(faces.ingress[order][self.n0:])
+        self.out1 = np.ascontiguousarray(faces.egress[order][self.n0:])
+        self.moff = np.zeros(1, np.int32)
+
+    def view(self):
+        abi = _abi()
+        v = abi.lc_flat_grid()
+        v.n, v.n0, v.n1 = self.n, self.n0, self.n1
+        for k in ("dx", "dy", "dz", "px", "py", "pz", "flux", "in1", "out1"):
+            setattr(v, k, dptr(getattr(self, k)))
+        v.moff = iptr(self.moff)
+        return v
+
+
+class Phases:
+    def __init__(self, ex, ey, ez, phin):
+        self.ex, self.ey, self.ez, self.phin = (np.ascontiguousarray(a, np.float64) for a in (ex, ey, ez, phin))
+        self.n = len(self.ex)
+
+    def view(self):
+        abi = _abi()
+        v = abi.lc_phase_batch()
+        v.n = self.n
+        for k in ("ex", "ey", "ez", "phin"):
+            setattr(v, k, dptr(getattr(self, k)))
+        return v
+
+
+def phases_for(iangle_deg, phases):
+    """Earth vectors of set_earth (set_earth.cpp:39-46) for given phases."""
+    lib = oracle()
+    ri = np.pi * iangle_deg / 180.0
+    cosi, sini = np.cos(ri), np.sin(ri)
+    ph = np.asarray(phases, np.float64)
+    rp = 2.0 * np.pi * ph
+    ex = sini * np.cos(rp)
+    ey = -sini * np.sin(rp)
+    ez = np.full_like(ph, cosi)
+    return Phases(ex, ey, ez, ph - np.floor(ph))
+
+
+def star_grid(model, star, fine):
+    lib = oracle()
+    n = C.c_size_t()
+    st = lib.orc_star_grid_size(C.byref(model), star, int(fine), C.byref(n))
+    if st:
+        raise RuntimeError(f"orc_star_grid_size: {st} {lib.orc_last_error().decode()}")
+    f = Faces(n.value)
+    v = f.view()
+    st = lib.orc_set_star_grid(C.byref(model), star, int(fine), C.byref(v))
+    if st:
+        raise RuntimeError(f"orc_set_star_grid: {st} {lib.orc_last_error().decode()}")
+    return f
+
+
+def star_continuum(model, f1, f2):
+    lib = oracle()
+    v1, v2 = f1.view(), f2.view()
+    st = lib.orc_set_star_continuum(C.byref(model), C.byref(v1), C.byref(v2))
+    if st:
+        raise RuntimeError(f"orc_set_star_continuum: {st} {lib.orc_last_error().decode()}")
+
+
+def flat_sum(star, flat, phases, ldc, beam=0.0, spin=1.0, VFAC=0.0, XCOFM=0.0, glens1=0, rlens1=0.0):
+    lib = oracle()
+    out = np.zeros(phases.n, np.float64)
+    g, pb = flat.view(), phases.view()
+    st = lib.orc_flat_sum_multi(star, C.byref(g), C.byref(pb), C.byref(ldc), beam, spin, VFAC, XCOFM,
+                                int(glens1), rlens1, dptr(out))
+    if st:
+        raise RuntimeError(f"orc_flat_sum_multi: {st}")
+    return out
+
+
+def light_curve(model, data, scale=False, rdata=True, diagnostics=False, max_model_points=0, sfac=1.0):
+    """Runs the oracle's light_curve_comp(_fast); returns (status, calc, scalars)."""
+    abi = _abi()
+    lib = oracle()
+    n = len(data)
+    calc = np.zeros(n, np.float64)
+    io = abi.lc_scalars()
+    for k in range(5):
+        io.sfac[k] = sfac
+    o = abi.lc_eval_opts(int(scale), int(rdata), 0, int(diagnostics), int(max_model_points), 0)
+    st = lib.orc_light_curve_comp(C.byref(model), data, n, C.byref(o), dptr(calc), C.byref(io))
+    return st, calc, io
