@@ -37,9 +37,9 @@ def build_workload(name):
     import lcurve_re_amd as L
     from lcurve_re_amd import workloads as W
     if name == "batch_prior":
-        w = W.batch_prior()
+        w = W.batch_prior(m=512 + 64)
         models = [L.Model.from_json(c) for c in w["models"]]
-        cfg = {"workload": "batch_prior", "instances": len(models), "phases": len(w["time"]),
+        cfg = {"workload": "batch_prior", "instances": 512, "phases": len(w["time"]),
                "nlat": "U{40..200}", "seed": w["seed"], "generator": w["generator"]}
     elif name == "semidetached_large":
         w = W.semidetached_large()
@@ -94,9 +94,23 @@ def main():
 
     import lcurve_re_amd as L
     models, data, cfg = build_workload(args.workload)
-    M = len(models)
-    log(f"bench: workload {cfg}")
     eng = L.Engine(0)
+    M = cfg["instances"]
+    if len(models) > M:
+        # draws the engine rejects (a geometry error, as the reference would
+        # throw) are replaced by the next draws of the same generator
+        pool, models = models[M:], models[:M]
+        redrawn = 0
+        while True:
+            _, _, status = eng.light_curve_comp_batch(models, data, precision=args.precision)
+            bad = [i for i in range(M) if status[i] != 0]
+            if not bad:
+                break
+            for i in bad:
+                models[i] = pool.pop(0)
+                redrawn += 1
+        cfg["redrawn"] = redrawn
+    log(f"bench: workload {cfg}")
     calc = None
     for i in range(args.warmup):
         calc, io, status = eng.light_curve_comp_batch(models, data, precision=args.precision)
@@ -118,7 +132,9 @@ def main():
                 "alg_flop_per_launch": prof["flux_flops"] / launches,
                 "stage_ms_per_step": {k: prof[k] / args.steps for k in
                                       ("grid_ms_total", "eclipse_ms_total", "cont_ms_total", "flux_ms_total",
-                                       "other_ms_total")}}
+                                       "other_ms_total", "host_plan_ms_total", "call_ms_total")},
+                "eclipse_candidates_per_step": prof["eclipse_candidates"] / args.steps,
+                "eclipse_hits_per_step": prof["eclipse_hits"] / args.steps}
     cpu, err = (None, None)
     if not args.no_cpu:
         cpu, err = cpu_baseline(models, data, calc, args.cpu_budget)

@@ -154,6 +154,10 @@ typedef struct lc_profile {
     double flux_elements;   /* algorithmic face x sub-phase elements summed      */
     double flux_flops;      /* algorithmic FLOP of those elements                */
     double flux_bytes;      /* algorithmic HBM bytes of the flux kernels         */
+    double host_plan_ms_total; /* host planning + table building (wall)          */
+    double call_ms_total;   /* wall time inside the evaluation entry points      */
+    double eclipse_candidates; /* faces sent to the FP64 eclipse search          */
+    double eclipse_hits;    /* of those, faces found eclipsed                    */
 } lc_profile;
 
 typedef struct lc_ctx lc_ctx;

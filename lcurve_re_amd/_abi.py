@@ -114,7 +114,9 @@ class lc_profile(C.Structure):
                 ("eclipse_ms_total", C.c_double), ("cont_ms_total", C.c_double),
                 ("other_ms_total", C.c_double), ("flux_launches", C.c_int64),
                 ("evaluations", C.c_int64), ("flux_elements", C.c_double),
-                ("flux_flops", C.c_double), ("flux_bytes", C.c_double)]
+                ("flux_flops", C.c_double), ("flux_bytes", C.c_double),
+                ("host_plan_ms_total", C.c_double), ("call_ms_total", C.c_double),
+                ("eclipse_candidates", C.c_double), ("eclipse_hits", C.c_double)]
 
 
 # Every entry point include/lcurve_amd.h declares, with its signature.

@@ -335,10 +335,13 @@ GridPlan plan_grid(const lc_model &m, int star, bool fine) {
             segment(0., PI, 0, 0);
         }
     }
+    // The rings may hold one or two faces more than numface() allots (nlat =
+    // 153, 297, 315, 333, 339, ...): the reference writes those past the end
+    // of its vector, so they are not part of the star; the face kernel
+    // drops every face at or beyond nface.
     gp.ring_faces = 0;
     for (auto &R : gp.rings) gp.ring_faces += R.nphi;
-    if (gp.ring_faces > G.nface)
-        throw EngineError(LC_E_PARAM, "set_star_grid: latitude rings hold more faces than numface() allows");
+    gp.ring_faces = std::min(gp.ring_faces, G.nface);
 
     // geometry scalars of the face and eclipse kernels
     const double ri = lcx::deg2rad(m.iangle.value);
@@ -447,7 +450,9 @@ void fill_physics(const lc_model &m, GridDev &G) {
     }
 }
 
-int flux_variant(const GridDev &G) { return (G.ltype ? 1 : 0) | (G.beam != 0.0 ? 2 : 0) | (G.glens ? 4 : 0); }
+int flux_variant(const GridDev &G) {
+    return (G.ltype ? 1 : 0) | (G.beam != 0.0 ? 2 : 0) | (G.glens ? 4 : 0) | (G.mucrit > 0.0 ? 8 : 0);
+}
 
 struct InstPlan {
     int status = LC_OK;
@@ -470,7 +475,7 @@ struct lc_ctx {
     lc_profile prof;
     hipEvent_t ev[16];
     int flux_rphase = 2;
-    int flux_chunk = 2048;
+    int flux_chunk = 1024;
 };
 
 namespace {
@@ -509,6 +514,7 @@ struct EvalRequest {
 
 // Runs the whole pipeline for a batch of instances sharing the data set.
 void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
+    const auto t_call0 = std::chrono::steady_clock::now();
     const size_t M = rq.m, n = rq.n;
     const lc_eval_opts &o = rq.opts;
     const bool fp64 = o.precision == LC_PREC_FP64;
@@ -650,7 +656,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         return p.status == LC_OK && (!p.copy1 || !p.copy2);
     });
     std::vector<BatchDev> batches;
-    std::vector<std::vector<TaskDev>> tasks_by_var(8);
+    std::vector<std::vector<TaskDev>> tasks_by_var(16);
     long long part_total = 0;
     double alg_flops = 0., alg_elems = 0., alg_bytes = 0.;
     const int ntile = S > 0 ? (S + TILE - 1) / TILE : 0;
@@ -687,8 +693,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         }
     }
     std::vector<TaskDev> tasks;
-    std::vector<std::pair<int, int>> var_range(8, {0, 0});
-    for (int v = 0; v < 8; v++) {
+    std::vector<std::pair<int, int>> var_range(16, {0, 0});
+    for (int v = 0; v < 16; v++) {
         var_range[v] = {(int)tasks.size(), (int)tasks_by_var[v].size()};
         tasks.insert(tasks.end(), tasks_by_var[v].begin(), tasks_by_var[v].end());
     }
@@ -730,6 +736,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<double>(FN);
     sz.add<int>(FN);  // cand
     sz.add<int>(1);
+    sz.add<int>(1);      // eclipse work lists
+    sz.add<int>(FN);
+    for (int k = 0; k < 6; k++) sz.add<double>(FN);
     sz.add<float4>(FN);
     sz.add<float4>(FN);
     sz.add<float4>(FN);
@@ -772,6 +781,13 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     fa.eout = A.take<double>(FN);
     int *d_cand = A.take<int>(FN);
     int *d_cand_count = A.take<int>(1);
+    EclipseWork ew;
+    ew.hit_count = A.take<int>(1);
+    ew.face = A.take<int>(FN);
+    ew.phi = A.take<double>(FN);
+    ew.lim = A.take<double>(2 * FN);
+    ew.acc = A.take<double>(FN);
+    ew.res = A.take<double>(2 * FN);
     FlatArrays fl;
     std::memset(&fl, 0, sizeof(fl));
     fl.rec0 = A.take<float4>(FN);
@@ -848,6 +864,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     const size_t down_bytes = M * n * 8 + M * 8 * 8 + M * sizeof(int) + diag_idx.size() * 8 * 8 + M * sizeof(InstDev);
     ctx->pin.reserve(std::max(up_total, down_bytes) + 4096);
     hipStream_t s = ctx->stream;
+    ctx->prof.host_plan_ms_total +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call0).count();
     {
         size_t off = 0;
         for (auto &u : ups) {
@@ -858,6 +876,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         }
     }
     HIPCHK(hipMemsetAsync(d_cand_count, 0, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(ew.hit_count, 0, sizeof(int), s));
     for (auto &g : gaps) {
         HIPCHK(hipMemsetAsync(fa.px + g.first, 0, g.second * 8, s));
         HIPCHK(hipMemsetAsync(fa.py + g.first, 0, g.second * 8, s));
@@ -882,7 +901,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     HIPCHK(hipEventRecord(ev[0], s));
     launch_faces(s, d_grids, d_rings, nring, nhalf_total, fa, d_cand, d_cand_count, d_status);
     HIPCHK(hipEventRecord(ev[1], s));
-    launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, d_status);
+    launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, ew, d_status);
     HIPCHK(hipEventRecord(ev[2], s));
     launch_continuum(s, d_grids, ngrid, nface_total, fa);
     HIPCHK(hipEventRecord(ev[3], s));
@@ -895,7 +914,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     if (any_route) launch_route(s, d_insts, (int)M, S, pa, d_lists, d_counts);
     HIPCHK(hipMemsetAsync(d_out, 0, SM * 2 * sizeof(double), s));
     HIPCHK(hipEventRecord(ev[4], s));
-    for (int v = 0; v < 8; v++)
+    for (int v = 0; v < 16; v++)
         if (var_range[v].second)
             launch_flux(s, v, fp64 ? 1 : 0, R, d_tasks + var_range[v].first, var_range[v].second, d_batches, d_grids,
                         fl, pa, S, d_lists, d_counts, d_partial);
@@ -918,15 +937,32 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         HIPCHK(hipMemcpyAsync(h_diag, d_diag, diag_idx.size() * 8 * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(h_insts, d_insts, M * sizeof(InstDev), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(h_status, d_status, M * sizeof(int), hipMemcpyDeviceToHost, s));
+    int h_counts[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&h_counts[0], d_cand_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&h_counts[1], ew.hit_count, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(ev[6], s));
     HIPCHK(hipStreamSynchronize(s));
+    ctx->prof.eclipse_candidates += h_counts[0];
+    ctx->prof.eclipse_hits += h_counts[1];
 
+    if (any_route) {  // algorithmic work of the fine/coarse batches
+        std::vector<int> cnt(M * 4);
+        HIPCHK(hipMemcpy(cnt.data(), d_counts, M * 4 * sizeof(int), hipMemcpyDeviceToHost));
+        for (const BatchDev &B : batches) {
+            if (B.count_idx < 0) continue;
+            const GridDev &G = gdev[B.grid];
+            const double c = cnt[B.count_idx];
+            alg_elems += (double)G.nface * c;
+            alg_flops += (double)G.nface * c * elem_flops(G);
+            alg_bytes += (double)G.nface * 40. + c * 32.;
+        }
+    }
     ctx->prof.grid_ms_total += elapsed(ev[0], ev[1]);
     ctx->prof.eclipse_ms_total += elapsed(ev[1], ev[2]);
     ctx->prof.cont_ms_total += elapsed(ev[2], ev[3]);
     ctx->prof.other_ms_total += elapsed(ev[3], ev[4]) + elapsed(ev[5], ev[6]);
     ctx->prof.flux_ms_total += elapsed(ev[4], ev[5]);
-    for (int v = 0; v < 8; v++) ctx->prof.flux_launches += var_range[v].second ? 1 : 0;
+    for (int v = 0; v < 16; v++) ctx->prof.flux_launches += var_range[v].second ? 1 : 0;
     ctx->prof.flux_elements += alg_elems;
     ctx->prof.flux_flops += alg_flops;
     ctx->prof.flux_bytes += alg_bytes;
@@ -1009,6 +1045,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             io.logg1 = io.logg2 = io.rv1 = io.rv2 = 0.;
         }
     }
+    ctx->prof.call_ms_total +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call0).count();
 }
 
 // light_curve_comp_fast: model at max_model_points evenly indexed points,
@@ -1291,7 +1329,7 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     G.face_off = 0; G.ring_off = 0; G.nring = (int)rings.size(); G.half_off = 0; G.nhalf = ho; G.inst = 0;
     if (nf == 0) return LC_OK;
     const size_t FN = (size_t)nf;
-    ctx->dev.reserve(FN * (6 * 8 + 3 * 4 + 4 + 16 + 4) + rings.size() * sizeof(RingDev) + 65536);
+    ctx->dev.reserve(FN * (6 * 8 + 3 * 4 + 4 + 16 + 4 + 4 + 6 * 8) + rings.size() * sizeof(RingDev) + 65536);
     ctx->dev.reset();
     DevArena &A = ctx->dev;
     hipStream_t s = ctx->stream;
@@ -1303,10 +1341,18 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     int *d_cand = A.take<int>(FN);
     int *d_count = A.take<int>(1);
     int *d_status = A.take<int>(1);
+    EclipseWork ew;
+    ew.hit_count = A.take<int>(1);
+    ew.face = A.take<int>(FN);
+    ew.phi = A.take<double>(FN);
+    ew.lim = A.take<double>(2 * FN);
+    ew.acc = A.take<double>(FN);
+    ew.res = A.take<double>(2 * FN);
     GridDev *d_grid = dev_upload(A, &G, 1, s);
     RingDev *d_rings = dev_upload(A, rings.data(), rings.size(), s);
     HIPCHK(hipMemsetAsync(d_count, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(d_status, 0, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(ew.hit_count, 0, sizeof(int), s));
     if (gp.ring_faces < nf) {
         int gap = nf - gp.ring_faces, g0 = gp.ring_faces;
         for (double *p : {fa.px, fa.py, fa.pz, fa.dx, fa.dy, fa.dz, fa.ein, fa.eout})
@@ -1320,7 +1366,7 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     HIPCHK(hipEventRecord(ctx->ev[0], s));
     launch_faces(s, d_grid, d_rings, (int)rings.size(), ho, fa, d_cand, d_count, d_status);
     HIPCHK(hipEventRecord(ctx->ev[1], s));
-    launch_eclipse(s, d_grid, 1, fa, d_cand, d_count, nf, d_status);
+    launch_eclipse(s, d_grid, 1, fa, d_cand, d_count, nf, ew, d_status);
     HIPCHK(hipEventRecord(ctx->ev[2], s));
     HIPCHK(hipGetLastError());
     int st = 0;

@@ -2,27 +2,33 @@
 // (flat_sum_star{1,2}_multi, comp_light_flat.cpp:241-465).
 //
 // Work decomposition: a task is (batch, phase tile, face chunk).  Each lane
-// owns R sub-phases of the tile and keeps their sums in registers while the
-// workgroup streams the chunk's faces; face records are read with uniform
-// (scalar) loads, so every element costs VALU work only.  Partial sums per
-// (chunk, sub-phase) are merged in chunk order by k_flux_reduce, which
-// keeps results independent of scheduling.
+// owns R sub-phases of the tile and keeps their sums in registers.  The
+// workgroup walks its face chunk in stages of 256 faces: every thread loads
+// one face record (coalesced, prefetched one stage ahead), decides once for
+// the whole tile whether the face can contribute, and the contributing
+// faces are compacted -- in ascending face order -- into LDS.  All waves
+// then sweep the compacted records with broadcast LDS reads, so the inner
+// loop is pure VALU work on registers.  Partial sums per (chunk, sub-phase)
+// are merged in chunk order by k_flux_reduce, independent of scheduling.
+//
+// A face is skipped for a tile when it is provably invisible for every
+// sub-phase (mu <= mucrit: with e_c the tile's centre earth vector and R
+// the largest |e - e_c|, mu(e) <= d.e_c + R) or eclipsed at every sub-phase
+// of the tile; skipped elements are exact zeros in the reference sums.
 //
 // Two element precisions:
-//   mixed: FP32 per-face arithmetic, FP32 partial sums over 32 faces, FP64
-//          accumulation of those partials;
+//   mixed: FP32 per-face arithmetic, FP32 partial sums over at most 32
+//          faces, FP64 accumulation of those partials;
 //   fp64 : the reference element arithmetic in FP64 with 1024-face chunks,
 //          faces ascending, chunks merged in order -- the reduction order of
-//          comp_light_flat.cpp:254-307, so sums are bit-identical to it.
-// Both skip whole faces that are provably invisible for every sub-phase of
-// the tile (mu <= mucrit) and eclipse tests that cannot trigger inside the
-// tile's phase range; skipped elements are exact zeros in the reference.
+//          comp_light_flat.cpp:254-307, so the sums are bit-identical.
 #include "lc_kernels.h"
 
 namespace lck {
 namespace {
 
-constexpr int NT = 256;  // threads per workgroup (4 waves)
+constexpr int NT = 256;  // threads per workgroup (4 waves of 64)
+constexpr int FB = 256;  // faces per stage
 
 __device__ __forceinline__ float wave_max(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
@@ -38,8 +44,9 @@ __device__ __forceinline__ double wave_max_d(double v) {
 }
 
 struct TileInfo {
-    float ecx, ecy, ecz, rad;  // centre earth vector and max |e - e_c| over the tile
-    double pmin, pmax;         // phin range of the tile
+    double cx, cy, cz;  // centre earth vector
+    float rad;          // max |e - e_c| over the tile
+    double pmin, pmax;  // phin range of the tile
 };
 
 // Tile statistics for the visibility and eclipse shortcuts.
@@ -79,9 +86,10 @@ __device__ __forceinline__ TileInfo tile_info(const double *ex, const double *ey
         t.pmin = fmin(t.pmin, s_min[k]);
         t.pmax = fmax(t.pmax, s_max[k]);
     }
-    t.ecx = (float)cx;
-    t.ecy = (float)cy;
-    t.ecz = (float)cz;
+    t.cx = cx;
+    t.cy = cy;
+    t.cz = cz;
+    __syncthreads();
     return t;
 }
 
@@ -123,23 +131,126 @@ __device__ __forceinline__ int tile_eclipse_class(const TileInfo &t, double in, 
     return all ? 2 : 1;
 }
 
-template <int LT>
-__device__ __forceinline__ float imu_f(float m, float c1, float c2, float c3, float c4, float base) {
-    // m already clipped to <= 1; caller masks m <= 0
-    if (LT == 0) {
-        float om = 1.f - m;
-        return 1.f - om * (c1 + om * (c2 + om * (c3 + om * c4)));
+// Order-preserving compaction of the active faces of a stage: returns the
+// slot of this thread's face (if active) and the stage's active count.
+__device__ __forceinline__ int compact_slot(bool act, int &total, int *s_cnt) {
+    const unsigned long long m = __ballot(act);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int pre = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) s_cnt[w] = __popcll(m);
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < NT / 64; k++) {
+        base += k < w ? s_cnt[k] : 0;
+        tot += s_cnt[k];
     }
-    float s = sqrtf(fmaxf(m, 0.f));
-    return base + s * (c1 + s * (c2 + s * (c3 + s * c4)));
+    total = tot;
+    return base + pre;
 }
 
-template <int LT, bool BEAM, bool GLENS, int R>
+struct Coef32 {
+    float c1, c2, c3, c4, base, mucrit, beam, rlens;
+};
+
+// FP32 element without the face flux: returns the factor that multiplies
+// the face's brightness*area (comp_light_flat.cpp:110-168).  mu is clipped
+// to [0, 1]; for mucrit <= 0 the clip alone reproduces the reference's
+// masks (mu <= 0 gives zero), otherwise (MUC) mu > mucrit is tested too.
+template <int LT, bool BEAM, bool GLENS, bool MUC>
+__device__ __forceinline__ float elem32(float ex, float ey, float ez, const float4 &a, const float4 &v,
+                                        const float4 &p, const Coef32 &K) {
+#pragma clang fp contract(fast)
+    const float mu = fmaf(ez, a.z, fmaf(ey, a.y, ex * a.x));
+    float val;
+    if (!BEAM) {
+        const float m = __builtin_amdgcn_fmed3f(mu, 0.f, 1.f);
+        float im;
+        if (LT == 0) {
+            const float om = 1.f - m;
+            im = fmaf(-om, fmaf(om, fmaf(om, fmaf(om, K.c4, K.c3), K.c2), K.c1), 1.f);
+        } else {
+            const float s = __builtin_sqrtf(m);
+            im = fmaf(s, fmaf(s, fmaf(s, fmaf(s, K.c4, K.c3), K.c2), K.c1), K.base);
+        }
+        val = m * im;
+        if (MUC) val = mu > K.mucrit ? val : 0.f;
+    } else {
+        // v = (vx, vy, vn): vr_ref = -(ex vx + ey vy)
+        const float vr = fmaf(ey, v.y, ex * v.x);
+        const float mud = fmaf(mu, vr, mu) - v.z;
+        const float md = __builtin_amdgcn_fmed3f(mud, 0.f, 1.f);
+        float im;
+        if (LT == 0) {
+            const float om = 1.f - md;
+            im = fmaf(-om, fmaf(om, fmaf(om, fmaf(om, K.c4, K.c3), K.c2), K.c1), 1.f);
+        } else {
+            const float s = __builtin_sqrtf(md);
+            im = fmaf(s, fmaf(s, fmaf(s, fmaf(s, K.c4, K.c3), K.c2), K.c1), K.base);
+        }
+        const float g = fmaf(K.beam, vr, 1.f);
+        val = (mud > 0.f && mu > K.mucrit) ? mu * g * im : 0.f;
+    }
+    if (GLENS) {
+        const float d = -(p.x * ex + p.y * ey + p.z * ez);
+        const float qx = fmaf(d, ex, p.x), qy = fmaf(d, ey, p.y), qz = fmaf(d, ez, p.z);
+        const float pp = __builtin_sqrtf(qx * qx + qy * qy + qz * qz);
+        const float hp = 0.5f * pp;
+        const float rd = K.rlens * d;
+        const float pd = (hp * hp > 25.f * rd) ? pp + rd / pp : hp + __builtin_sqrtf(hp * hp + rd);
+        const float mg = pd * pd / (pd - hp) / hp / 4.f;
+        val *= d > 0.f ? mg : 1.f;
+    }
+    return val;
+}
+
+template <int LT, bool BEAM, bool GLENS, bool MUC, int R>
+__device__ __forceinline__ void sweep(const float4 *s_a0, const float4 *s_a1, const float4 *s_a2, const double2 *s_io,
+                                      int n, bool eclipsed, const float *ex, const float *ey, const float *ez,
+                                      const double *ph, const Coef32 &K, double *acc) {
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k0 = 0; k0 < n; k0 += 32) {
+        float a32[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) a32[r] = 0.f;
+        const int k1 = min(k0 + 32, n);
+        for (int k = k0; k < k1; k++) {
+            const float4 a = s_a0[k];
+            const float4 v = BEAM ? s_a1[k] : z4;
+            const float4 p = GLENS ? s_a2[k] : z4;
+            if (eclipsed) {
+                const double2 e = s_io[k];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    float val = elem32<LT, BEAM, GLENS, MUC>(ex[r], ey[r], ez[r], a, v, p, K);
+                    val = ecl_range(ph[r], e.x, e.y) ? 0.f : val;
+                    a32[r] = fmaf(a.w, val, a32[r]);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    a32[r] = fmaf(a.w, elem32<LT, BEAM, GLENS, MUC>(ex[r], ey[r], ez[r], a, v, p, K), a32[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r] += (double)a32[r];
+    }
+}
+
+template <int LT, bool BEAM, bool GLENS, bool MUC, int R>
 __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ tasks,
                                                    const BatchDev *__restrict__ batches,
                                                    const GridDev *__restrict__ grids, FlatArrays fl, PhaseArrays pa,
                                                    int S, const int *__restrict__ lists,
                                                    const int *__restrict__ counts, double *__restrict__ partial) {
+    // compacted records: [0, FB) faces seen at every sub-phase of the tile,
+    // [FB, 2FB) faces whose eclipse starts or ends inside the tile
+    __shared__ float4 s_a0[2 * FB];
+    __shared__ float4 s_a1[BEAM ? 2 * FB : 1];
+    __shared__ float4 s_a2[GLENS ? 2 * FB : 1];
+    __shared__ double2 s_io[FB];
+    __shared__ int s_cnt[2][NT / 64];
+
     const TaskDev T = tasks[blockIdx.x];
     const BatchDev B = batches[T.batch];
     const GridDev &G = grids[B.grid];
@@ -160,12 +271,16 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
         ey[r] = (float)dey[r];
         ez[r] = (float)dez[r];
     }
-
-    const float c1 = (float)G.ldc[0], c2 = (float)G.ldc[1], c3 = (float)G.ldc[2], c4 = (float)G.ldc[3];
-    const float base = (float)(1.0 - (G.ldc[0] + G.ldc[1] + G.ldc[2] + G.ldc[3]));
-    const float mucrit = (float)G.mucrit;
-    const float beam = (float)G.beam;
-    const float rlens = (float)G.rlens1;
+    Coef32 K;
+    K.c1 = (float)G.ldc[0];
+    K.c2 = (float)G.ldc[1];
+    K.c3 = (float)G.ldc[2];
+    K.c4 = (float)G.ldc[3];
+    K.base = (float)(1.0 - (G.ldc[0] + G.ldc[1] + G.ldc[2] + G.ldc[3]));
+    K.mucrit = (float)G.mucrit;
+    K.beam = (float)G.beam;
+    K.rlens = (float)G.rlens1;
+    const float ecx = (float)ti.cx, ecy = (float)ti.cy, ecz = (float)ti.cz;
     const float cull = (float)G.mucrit - 1.e-5f - ti.rad;
 
     const int n0 = fl.n0[B.grid];
@@ -175,65 +290,56 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
     const float4 *__restrict__ rec1 = fl.rec1 + G.face_off;
     const float4 *__restrict__ rec2 = fl.rec2 + G.face_off;
     const double2 *__restrict__ eclr = fl.ecl + G.face_off;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
 
     double acc[R];
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.;
 
-    for (int fb = f_lo; fb < f_hi; fb += 32) {
-        float a32[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) a32[r] = 0.f;
-        const int fe = min(fb + 32, f_hi);
-        for (int f = fb; f < fe; f++) {
-            const float4 a = rec0[f];
-            // every sub-phase of the tile sees mu <= mucrit: exact zero
-            if (fmaf(a.z, ti.ecz, fmaf(a.y, ti.ecy, a.x * ti.ecx)) <= cull) continue;
-            int ecls = 0;
-            double ein = 0., eout = 0.;
-            if (f >= n0) {
-                const double2 io = eclr[f];
-                ein = io.x;
-                eout = io.y;
-                ecls = tile_eclipse_class(ti, ein, eout);
-                if (ecls == 2) continue;
-            }
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f), p = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (BEAM) v = rec1[f];
-            if (GLENS) p = rec2[f];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                float mu = fmaf(ez[r], a.z, fmaf(ey[r], a.y, ex[r] * a.x));
-                float magn = 1.f;
-                if (GLENS) {
-                    float d = -(p.x * ex[r] + p.y * ey[r] + p.z * ez[r]);
-                    float qx = p.x + d * ex[r], qy = p.y + d * ey[r], qz = p.z + d * ez[r];
-                    float pp = sqrtf(qx * qx + qy * qy + qz * qz);
-                    float hp = 0.5f * pp;
-                    float rd = rlens * d;
-                    float pd = (hp * hp > 25.f * rd) ? pp + rd / pp : hp + sqrtf(hp * hp + rd);
-                    float mg = pd * pd / (pd - hp) / hp / 4.f;
-                    magn = d > 0.f ? mg : 1.f;
-                }
-                float val;
-                if (!BEAM) {
-                    float m = fminf(mu, 1.f);
-                    float im = imu_f<LT>(m, c1, c2, c3, c4, base);
-                    val = (mu > 0.f) ? mu * magn * a.w * im : 0.f;
-                } else {
-                    float vr = -(ex[r] * v.x + ey[r] * v.y);
-                    float mud = mu - mu * vr - v.z;
-                    float m = fminf(mud, 1.f);
-                    float im = imu_f<LT>(m, c1, c2, c3, c4, base);
-                    val = (mud > 0.f) ? mu * magn * a.w * (1.f - beam * vr) * im : 0.f;
-                }
-                bool vis = mu > mucrit;
-                if (ecls == 1) vis = vis && !ecl_range(ph[r], ein, eout);
-                a32[r] += vis ? val : 0.f;
-            }
+    // stage prefetch registers (clamped index keeps the loads unconditional)
+    int f = f_lo + (int)threadIdx.x;
+    const int flast = f_hi - 1;
+    float4 pa0 = rec0[min(f, flast)];
+    float4 pa1 = BEAM ? rec1[min(f, flast)] : z4;
+    float4 pa2 = GLENS ? rec2[min(f, flast)] : z4;
+    double2 pio = eclr[min(f, flast)];
+
+    for (int fb = f_lo; fb < f_hi; fb += FB) {
+        const float4 a0 = pa0, a1 = pa1, a2 = pa2;
+        const double2 io = pio;
+        const bool in = f < f_hi;
+        const bool ecl = f >= n0;
+        const int fn = f + FB;
+        if (fb + FB < f_hi) {  // prefetch the next stage
+            const int fc = min(fn, flast);
+            pa0 = rec0[fc];
+            if (BEAM) pa1 = rec1[fc];
+            if (GLENS) pa2 = rec2[fc];
+            pio = eclr[fc];
         }
-#pragma unroll
-        for (int r = 0; r < R; r++) acc[r] += (double)a32[r];
+        bool act = in && fmaf(a0.z, ecz, fmaf(a0.y, ecy, a0.x * ecx)) > cull;
+        int cls = 0;
+        if (act && ecl) {
+            cls = tile_eclipse_class(ti, io.x, io.y);
+            act = cls != 2;
+        }
+        int tot0, tot1;
+        const int slot0 = compact_slot(act && cls == 0, tot0, s_cnt[0]);
+        const int slot1 = compact_slot(act && cls == 1, tot1, s_cnt[1]);
+        if (act) {
+            const int slot = cls == 0 ? slot0 : FB + slot1;
+            s_a0[slot] = a0;
+            if (BEAM) s_a1[slot] = a1;
+            if (GLENS) s_a2[slot] = a2;
+            if (cls) s_io[slot1] = io;
+        }
+        __syncthreads();
+        sweep<LT, BEAM, GLENS, MUC, R>(s_a0, s_a1, s_a2, s_io, tot0, false, ex, ey, ez, ph, K, acc);
+        if (tot1)
+            sweep<LT, BEAM, GLENS, MUC, R>(s_a0 + FB, BEAM ? s_a1 + FB : s_a1, GLENS ? s_a2 + FB : s_a2, s_io, tot1,
+                                           true, ex, ey, ez, ph, K, acc);
+        __syncthreads();
+        f = fn;
     }
 
     double *out = partial + B.part_off + (size_t)T.chunk * (size_t)(B.ntile * TILE);
@@ -244,7 +350,8 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
     }
 }
 
-// FP64 element in the reference's operand order (comp_light_flat.cpp:81-168)
+// FP64 element in the reference's operand order (comp_light_flat.cpp:81-168);
+// zero when the reference's masks make it zero.
 template <int LT, bool BEAM, bool GLENS>
 __device__ __forceinline__ double elem64(double ex, double ey, double ez, double dx, double dy, double dz, double px,
                                          double py, double pz, double flux, const GridDev &G, double base) {
@@ -294,6 +401,11 @@ __global__ __launch_bounds__(NT) void k_flux_fp64(const TaskDev *__restrict__ ta
                                                   const GridDev *__restrict__ grids, FlatArrays fl, PhaseArrays pa,
                                                   int S, const int *__restrict__ lists,
                                                   const int *__restrict__ counts, double *__restrict__ partial) {
+    __shared__ double s_d[7][FB];  // dx dy dz px py pz flux
+    __shared__ double2 s_io[FB];
+    __shared__ int s_cls[FB];
+    __shared__ int s_cnt[NT / 64];
+
     const TaskDev T = tasks[blockIdx.x];
     const BatchDev B = batches[T.batch];
     const GridDev &G = grids[B.grid];
@@ -318,25 +430,51 @@ __global__ __launch_bounds__(NT) void k_flux_fp64(const TaskDev *__restrict__ ta
     double acc[R];
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.0;
-    for (int f = f_lo; f < f_hi; f++) {
-        const double dx = fl.fdx[o + f], dy = fl.fdy[o + f], dz = fl.fdz[o + f];
-        if (dx * cx + dy * cy + dz * cz <= cull) continue;
-        int ecls = 0;
-        double ein = 0., eout = 0.;
-        if (f >= n0) {
-            const double2 io = fl.ecl[o + f];
-            ein = io.x;
-            eout = io.y;
-            ecls = tile_eclipse_class(ti, ein, eout);
-            if (ecls == 2) continue;
+    for (int fb = f_lo; fb < f_hi; fb += FB) {
+        const int f = fb + (int)threadIdx.x;
+        bool act = false;
+        int cls = 0;
+        double d[7] = {0., 0., 0., 0., 0., 0., 0.};
+        double2 io = make_double2(0., 0.);
+        if (f < f_hi) {
+            d[0] = fl.fdx[o + f];
+            d[1] = fl.fdy[o + f];
+            d[2] = fl.fdz[o + f];
+            act = d[0] * cx + d[1] * cy + d[2] * cz > cull;
+            if (act && f >= n0) {
+                io = fl.ecl[o + f];
+                cls = tile_eclipse_class(ti, io.x, io.y);
+                act = cls != 2;
+            }
+            if (act) {
+                d[3] = fl.fpx[o + f];
+                d[4] = fl.fpy[o + f];
+                d[5] = fl.fpz[o + f];
+                d[6] = fl.fflux[o + f];
+            }
         }
-        const double px = fl.fpx[o + f], py = fl.fpy[o + f], pz = fl.fpz[o + f], flux = fl.fflux[o + f];
+        int tot;
+        const int slot = compact_slot(act, tot, s_cnt);
+        if (act) {
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            double v = elem64<LT, BEAM, GLENS>(ex[r], ey[r], ez[r], dx, dy, dz, px, py, pz, flux, G, base);
-            if (ecls == 1 && ecl_range(ph[r], ein, eout)) v = 0.0;
-            if (v != 0.0) acc[r] += v;
+            for (int k = 0; k < 7; k++) s_d[k][slot] = d[k];
+            s_io[slot] = io;
+            s_cls[slot] = cls;
         }
+        __syncthreads();
+        for (int k = 0; k < tot; k++) {
+            const double dx = s_d[0][k], dy = s_d[1][k], dz = s_d[2][k];
+            const double px = s_d[3][k], py = s_d[4][k], pz = s_d[5][k], flux = s_d[6][k];
+            const int c = s_cls[k];
+            const double2 e = s_io[k];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                double v = elem64<LT, BEAM, GLENS>(ex[r], ey[r], ez[r], dx, dy, dz, px, py, pz, flux, G, base);
+                if (c == 1 && ecl_range(ph[r], e.x, e.y)) v = 0.0;
+                if (v != 0.0) acc[r] += v;
+            }
+        }
+        __syncthreads();
     }
     double *out = partial + B.part_off + (size_t)T.chunk * (size_t)(B.ntile * TILE);
 #pragma unroll
@@ -371,11 +509,15 @@ void launch_variant(hipStream_t s, int variant, int fp64, const TaskDev *tasks, 
         if (fp64)                                                                                                   \
             hipLaunchKernelGGL((k_flux_fp64<LT, BM, GL, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches, grids,   \
                                fl, pa, S, lists, counts, partial);                                                 \
+        else if (muc)                                                                                               \
+            hipLaunchKernelGGL((k_flux_mixed<LT, BM, GL, true, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches,   \
+                               grids, fl, pa, S, lists, counts, partial);                                          \
         else                                                                                                        \
-            hipLaunchKernelGGL((k_flux_mixed<LT, BM, GL, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches, grids,  \
-                               fl, pa, S, lists, counts, partial);                                                 \
+            hipLaunchKernelGGL((k_flux_mixed<LT, BM, GL, false, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches,  \
+                               grids, fl, pa, S, lists, counts, partial);                                          \
         break;
-    switch (variant) {
+    const bool muc = (variant & 8) != 0;
+    switch (variant & 7) {
         LC_FLUX_CASE(0, 0, false, false)
         LC_FLUX_CASE(1, 1, false, false)
         LC_FLUX_CASE(2, 0, true, false)
@@ -391,7 +533,7 @@ void launch_variant(hipStream_t s, int variant, int fp64, const TaskDev *tasks, 
 
 }  // namespace
 
-// variant = ltype | beam << 1 | glens << 2; rphase in {1, 2, 4}
+// variant = ltype | beam << 1 | glens << 2 | (mucrit > 0) << 3; rphase in {1, 2, 4}
 void launch_flux(hipStream_t s, int variant, int fp64, int rphase, const TaskDev *tasks, int ntask,
                  const BatchDev *batches, const GridDev *grids, FlatArrays fl, PhaseArrays pa, int S,
                  const int *lists, const int *counts, double *partial) {

@@ -8,6 +8,8 @@
 // the companion's reference sphere are appended to a candidate list; the
 // eclipse kernel then runs the FP64 ingress/egress search
 // (ingress_egress.cpp:35-97) over that compacted list only.
+#include <algorithm>
+
 #include "lc_kernels.h"
 #include "lc_math.h"
 
@@ -107,16 +109,20 @@ __global__ __launch_bounds__(256) void k_faces(const GridDev *__restrict__ grids
     double area = ((R.dphi_span / nphi * rad * R.sint) * (R.tspan_nl * rad)) / dot(dirn, dvec);
     const float farea = (float)area, fgrav = (float)(gravity / G.gref);
 
+    // faces past numface() are not part of the star (see plan_grid)
+    const int fend = G.face_off + G.nface;
     int f = R.face_off + np;
-    fa.px[f] = posn.x; fa.py[f] = posn.y; fa.pz[f] = posn.z;
-    fa.dx[f] = dvec.x; fa.dy[f] = dvec.y; fa.dz[f] = dvec.z;
-    fa.area[f] = farea;
-    fa.grav[f] = fgrav;
-    fa.flux[f] = 0.f;
-    eclipse_pretest(G, posn, f, fa, cand, cand_count);
+    if (f < fend) {
+        fa.px[f] = posn.x; fa.py[f] = posn.y; fa.pz[f] = posn.z;
+        fa.dx[f] = dvec.x; fa.dy[f] = dvec.y; fa.dz[f] = dvec.z;
+        fa.area[f] = farea;
+        fa.grav[f] = fgrav;
+        fa.flux[f] = 0.f;
+        eclipse_pretest(G, posn, f, fa, cand, cand_count);
+    }
 
     int np2 = nphi - 1 - np;
-    if (np2 != np) {
+    if (np2 != np && R.face_off + np2 < fend) {
         V3 posn2 = posn, dvec2 = dvec;
         if (G.npole) {
             posn2.y = -posn.y;
@@ -135,25 +141,92 @@ __global__ __launch_bounds__(256) void k_faces(const GridDev *__restrict__ grids
     }
 }
 
-// Full FP64 ingress/egress search over the compacted candidates.  The
-// candidate count is read on the device, so no host round trip is needed;
-// the grid-stride loop ends for every wave once the list is consumed.
-__global__ __launch_bounds__(128) void k_eclipse(const GridDev *__restrict__ grids, int ngrid, FaceArrays fa,
-                                                const int *__restrict__ cand, const int *__restrict__ cand_count,
-                                                int *inst_status) {
+// FP64 ingress/egress search (ingress_egress.cpp:35-97) over the compacted
+// candidates, in three kernels so that each keeps little live state:
+//   k_eclipse_potmin  reference-sphere window + pot_min per candidate; hits
+//                     are appended to a second list;
+//   k_eclipse_bisect  one work item per (hit, side): the ingress or egress
+//                     bisection with fblink at every step;
+//   k_eclipse_final   normalises the phases and marks the face eclipsed.
+// Counts live on the device, so the chain needs no host round trip; every
+// grid-stride loop ends once its list is consumed.
+__global__ __launch_bounds__(128) void k_eclipse_potmin(const GridDev *__restrict__ grids, int ngrid, FaceArrays fa,
+                                                       const int *__restrict__ cand,
+                                                       const int *__restrict__ cand_count, EclipseWork ew,
+                                                       int *inst_status) {
     const int n = *cand_count;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int f = cand[i];
         const GridDev &G = grids[find_grid(grids, ngrid, f)];
         const int other = G.star == PRIMARY ? SECONDARY : PRIMARY;
-        double ingress = 0., egress = 0.;
-        bool ecl = false;
-        int st = ingress_egress(G.q, other, G.spin_other, G.cosi, G.sini, G.delta, v3(fa.px[f], fa.py[f], fa.pz[f]),
-                                G.rref_other, G.pref_other, ingress, egress, ecl);
-        if (st) set_status(inst_status, G.inst, st);
-        fa.ecl[f] = (ecl && !st) ? 1 : 0;
-        fa.ein[f] = ecl ? ingress : 0.;
-        fa.eout[f] = ecl ? egress : 0.;
+        const V3 cofm = other == PRIMARY ? v3(0., 0., 0.) : v3(1., 0., 0.);
+        const V3 r = v3(fa.px[f], fa.py[f], fa.pz[f]);
+        fa.ecl[f] = 0;
+        fa.ein[f] = 0.;
+        fa.eout[f] = 0.;
+        double phi1, phi2, lam1, lam2, phi, lam;
+        if (!sphere_eclipse(G.cosi, G.sini, r, cofm, G.rref_other, phi1, phi2, lam1, lam2)) continue;
+        const double ACC = 2. * sqrt(2. * TWOPI * (lam2 - lam1) * G.delta);
+        bool hit;
+        int st = pot_min(G.q, G.cosi, G.sini, other, G.spin_other, r, phi1, phi2, lam1, lam2, G.rref_other,
+                         G.pref_other, ACC, phi, lam, hit);
+        if (st) {
+            set_status(inst_status, G.inst, st);
+            continue;
+        }
+        if (!hit) continue;
+        const int k = atomicAdd(ew.hit_count, 1);
+        ew.face[k] = f;
+        ew.phi[k] = phi;
+        ew.lim[2 * k] = phi1;
+        ew.lim[2 * k + 1] = phi2;
+        ew.acc[k] = ACC;
+    }
+}
+
+__global__ __launch_bounds__(128) void k_eclipse_bisect(const GridDev *__restrict__ grids, int ngrid,
+                                                       FaceArrays fa, EclipseWork ew, int *inst_status) {
+    const int n = *ew.hit_count * 2;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int k = i >> 1, side = i & 1;
+        const int f = ew.face[k];
+        const GridDev &G = grids[find_grid(grids, ngrid, f)];
+        const int other = G.star == PRIMARY ? SECONDARY : PRIMARY;
+        const V3 r = v3(fa.px[f], fa.py[f], fa.pz[f]);
+        const double ACC = ew.acc[k];
+        double pin = ew.phi[k], pout = ew.lim[2 * k + side];
+        int st = ST_OK;
+        while (fabs(pin - pout) > G.delta) {
+            const double pmid = (pin + pout) / 2.;
+            bool h;
+            st = fblink(G.q, other, G.spin_other, ACC, set_earth(G.cosi, G.sini, pmid), r, G.rref_other,
+                        G.pref_other, h);
+            if (st) break;
+            if (h)
+                pin = pmid;
+            else
+                pout = pmid;
+        }
+        if (st) {
+            set_status(inst_status, G.inst, st);
+            ew.res[i] = 0.;
+            continue;
+        }
+        const double x = (pin + pout) / 2.;
+        ew.res[i] = x - floor(x);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_eclipse_final(FaceArrays fa, EclipseWork ew) {
+    const int n = *ew.hit_count;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int f = ew.face[k];
+        const double ingress = ew.res[2 * k];
+        double egress = ew.res[2 * k + 1];
+        if (egress < ingress) egress++;
+        fa.ecl[f] = 1;
+        fa.ein[f] = ingress;
+        fa.eout[f] = egress;
     }
 }
 
@@ -509,11 +582,14 @@ void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int
 }
 
 void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int *cand,
-                    const int *cand_count, int max_cand, int *inst_status) {
+                    const int *cand_count, int max_cand, EclipseWork ew, int *inst_status) {
     if (max_cand <= 0 || ngrid <= 0) return;
-    int blocks = (max_cand + 127) / 128;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_eclipse, dim3(blocks), dim3(128), 0, s, grids, ngrid, fa, cand, cand_count, inst_status);
+    const int blocks = std::min((max_cand + 127) / 128, 16384);
+    hipLaunchKernelGGL(k_eclipse_potmin, dim3(blocks), dim3(128), 0, s, grids, ngrid, fa, cand, cand_count, ew,
+                       inst_status);
+    const int blocks2 = std::min((2 * max_cand + 127) / 128, 16384);
+    hipLaunchKernelGGL(k_eclipse_bisect, dim3(blocks2), dim3(128), 0, s, grids, ngrid, fa, ew, inst_status);
+    hipLaunchKernelGGL(k_eclipse_final, dim3(std::min((max_cand + 255) / 256, 4096)), dim3(256), 0, s, fa, ew);
 }
 
 void launch_continuum(hipStream_t s, const GridDev *grids, int ngrid, int nface_total, FaceArrays fa) {

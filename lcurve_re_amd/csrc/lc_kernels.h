@@ -67,6 +67,16 @@ struct FlatArrays {
     int *n0;        // never-eclipsed count per grid
 };
 
+// Work lists of the three-kernel eclipse search (k_geom.hip).
+struct EclipseWork {
+    int *hit_count;  // faces pot_min found eclipsed
+    int *face;       // [hit] face index
+    double *phi;     // [hit] phase inside the eclipse (pot_min)
+    double *lim;     // [2*hit] sphere window phi1, phi2
+    double *acc;     // [hit] position accuracy of fblink
+    double *res;     // [2*hit + side] ingress (side 0) / egress (side 1) in [0, 1)
+};
+
 struct DataDev {
     const double *time, *expose, *flux, *ferr, *weight;
     const int *ndiv, *sub_off;
@@ -95,7 +105,7 @@ struct TaskDev {
 void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int nring, int nhalf_total,
                   FaceArrays fa, int *cand, int *cand_count, int *inst_status);
 void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int *cand,
-                    const int *cand_count, int max_cand, int *inst_status);
+                    const int *cand_count, int max_cand, EclipseWork ew, int *inst_status);
 void launch_continuum(hipStream_t s, const GridDev *grids, int ngrid, int nface_total, FaceArrays fa);
 void launch_flat_build(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, FlatArrays fl, int fp64);
 void launch_point_sums(hipStream_t s, const GridDev *grids, const InstDev *insts, int ninst, FaceArrays fa,
@@ -103,7 +113,8 @@ void launch_point_sums(hipStream_t s, const GridDev *grids, const InstDev *insts
 void launch_gint_scales(hipStream_t s, InstDev *insts, int ninst, const double *sums);
 void launch_phases(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa);
 void launch_route(hipStream_t s, const InstDev *insts, int ninst, int S, PhaseArrays pa, int *lists, int *counts);
-// variant = ltype | beaming << 1 | lensing << 2; rphase = sub-phases per lane
+// variant = ltype | beaming << 1 | lensing << 2 | (mucrit > 0) << 3;
+// rphase = sub-phases per lane
 void launch_flux(hipStream_t s, int variant, int fp64, int rphase, const TaskDev *tasks, int ntask,
                  const BatchDev *batches, const GridDev *grids, FlatArrays fl, PhaseArrays pa, int S,
                  const int *lists, const int *counts, double *partial);

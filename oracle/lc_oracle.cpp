@@ -1128,8 +1128,11 @@ void add_faces(const lc_model &m, const GridSetup &s, std::vector<Face> &star, i
         }
         nface += R.nphi;
     }
-    if (nface > (int)star.size())
-        throw OracleError(LC_E_PARAM, "set_star_grid: ring layout exceeds numface() count");
+    // For a few nlat (153, 297, 315, 333, 339, ...) the rings hold one or two
+    // faces more than numface() allots; the reference writes those past the
+    // end of its vector (set_star_grid.cpp:490, 508), so they are never part
+    // of the star.  Faces at or beyond star.size() are dropped here.
+    const int nstar = (int)star.size();
 
     const bool roche_solve = (which == PRIMARY && roche1) || (which == SECONDARY && roche2);
     const double rref_self = which == PRIMARY ? s.rref1 : s.rref2;
@@ -1206,15 +1209,17 @@ void add_faces(const lc_model &m, const GridSetup &s, std::vector<Face> &star, i
                     gravity = 1.;
                 }
                 double area = ((R.phi2 - R.phi1) / nphi * rad * R.sint) * ((thi - tlo) / R.nl * rad) / dot(dirn, dvec);
-                Face &f = star[idx];
-                f.posn = posn;
-                f.dirn = dvec;
-                f.area = (float)area;
-                f.gravity = (float)(gravity / s.gref);
-                f.flux = 0.f;
-                compute_eclipse(posn, f);
+                if (idx < nstar) {
+                    Face &f = star[idx];
+                    f.posn = posn;
+                    f.dirn = dvec;
+                    f.area = (float)area;
+                    f.gravity = (float)(gravity / s.gref);
+                    f.flux = 0.f;
+                    compute_eclipse(posn, f);
+                }
                 int np2 = nphi - 1 - np;
-                if (np2 != np) {
+                if (np2 != np && R.off + np2 < nstar) {
                     Face &g = star[R.off + np2];
                     V3 posn2 = posn, dvec2 = dvec;
                     if (npole) {
@@ -1226,8 +1231,8 @@ void add_faces(const lc_model &m, const GridSetup &s, std::vector<Face> &star, i
                     }
                     g.posn = posn2;
                     g.dirn = dvec2;
-                    g.area = f.area;
-                    g.gravity = f.gravity;
+                    g.area = (float)area;
+                    g.gravity = (float)(gravity / s.gref);
                     g.flux = 0.f;
                     compute_eclipse(posn2, g);
                 }
