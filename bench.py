@@ -76,6 +76,38 @@ def cpu_baseline(models, data, gpu_calc, budget_s=15.0):
                       f"(OpenMP, -O3 -ffp-contract=off), {el:.1f} s"}, (max(errs) if errs else None)
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE runs, scaled by a
+    plain copy's readings), or None when there is none for this kernel."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    want = kernel.replace("True", "true").replace("False", "false")
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch "):
+            name, val = line[len("traffic_bytes_per_launch "):].rsplit("=", 1)
+            if name.strip() == want:
+                return float(val)
+    return None
+
+
+def copy_calibration(mib=256, reps=4):
+    """Plain device-to-device copies of `mib` MiB: their FETCH_SIZE/WRITE_SIZE
+    counter readings against the known byte counts give the factor that turns
+    the counters of the flux kernel into HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": n * 4, "bytes_written": n * 4, "reps": reps,
+                                           "checksum": float(y[::4096].sum().item())}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,11 +118,17 @@ def main():
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is designed, not built", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import lcurve_re_amd as L
     models, data, cfg = build_workload(args.workload)
@@ -123,13 +161,33 @@ def main():
     prof = eng.profile()
     nok = int(np.count_nonzero(status == 0))
     log(f"bench: {args.steps} steps in {el:.3f} s; profile {prof}")
-    flux_s = prof["flux_ms_total"] * 1e-3
-    achieved = prof["flux_flops"] / flux_s / 1e12 if flux_s > 0 else 0.0
-    launches = max(prof["flux_launches"], 1)
+    # dominant flux kernel: the variant (limb-darkening law x beaming x
+    # lensing x mucrit) with the most launch time in the timed region
+    var_ms = prof["flux_var_ms"]
+    v = int(np.argmax(var_ms))
+    launches = max(int(prof["flux_var_launches"][v]), 1)
+    avg_ms = var_ms[v] / launches
+    flop_per_launch = prof["flux_var_flops"][v] / launches
+    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    R = int(os.environ.get("LCURVE_AMD_FLUX_R", "2"))
+    kname = (f"k_flux_{'fp64' if args.precision else 'mixed'}<{v & 1}, {bool(v & 2)}, {bool(v & 4)}"
+             + ("" if args.precision else f", {bool(v & 8)}") + f", {R}>")
+    c_elem = {0: 24, 1: 26, 2: 39, 3: 41}[v & 3]
+    evald = prof["flux_var_evaluated"][v] / launches
+    traffic = measured_traffic(cfg["workload"], kname)
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                "kernel": "k_flux_mixed", "avg_launch_ms": prof["flux_ms_total"] / launches,
-                "alg_flop_per_launch": prof["flux_flops"] / launches,
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                "traffic_source": f"profiles/hbm_traffic_{cfg['workload']}.md" if traffic else None,
+                "hbm_frac": traffic / (avg_ms * 1e-3) / (PEAK_HBM_GBS * 1e9) if traffic and avg_ms > 0 else None,
+                "kernel": kname, "launches_per_step": launches / args.steps, "avg_launch_ms": avg_ms,
+                "alg_flop_per_launch": flop_per_launch, "flop_per_element": c_elem,
+                "alg_elements_per_launch": flop_per_launch / c_elem,
+                # elements left after tile culling (provably zero ones skipped):
+                # the rate the VALUs actually ran at
+                "evaluated_elements_per_launch": evald,
+                "evaluated_tflops": evald * c_elem / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0,
+                "flux_stage_tflops": prof["flux_flops"] / (prof["flux_ms_total"] * 1e-3) / 1e12
+                if prof["flux_ms_total"] > 0 else 0.0,
                 "stage_ms_per_step": {k: prof[k] / args.steps for k in
                                       ("grid_ms_total", "eclipse_ms_total", "cont_ms_total", "flux_ms_total",
                                        "other_ms_total", "host_plan_ms_total", "call_ms_total")},

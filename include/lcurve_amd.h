@@ -158,6 +158,11 @@ typedef struct lc_profile {
     double call_ms_total;   /* wall time inside the evaluation entry points      */
     double eclipse_candidates; /* faces sent to the FP64 eclipse search          */
     double eclipse_hits;    /* of those, faces found eclipsed                    */
+    /* per flux-kernel variant v = ldc_type | beaming<<1 | lensing<<2 | mucrit<<3 */
+    double flux_var_ms[16];     /* summed launch durations of variant v          */
+    double flux_var_flops[16];  /* algorithmic FLOP of variant v                  */
+    int64_t flux_var_launches[16];
+    double flux_var_evaluated[16]; /* elements actually evaluated (after culling) */
 } lc_profile;
 
 typedef struct lc_ctx lc_ctx;

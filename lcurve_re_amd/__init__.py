@@ -253,7 +253,11 @@ class Engine:
     def profile(self):
         p = _abi.lc_profile()
         _lib().lc_ctx_get_profile(self._ctx, C.byref(p))
-        return {k: getattr(p, k) for k, _ in p._fields_}
+        out = {}
+        for k, _ in p._fields_:
+            v = getattr(p, k)
+            out[k] = list(v) if isinstance(v, C.Array) else v
+        return out
 
     def reset_profile(self):
         _lib().lc_ctx_reset_profile(self._ctx)

@@ -116,7 +116,9 @@ class lc_profile(C.Structure):
                 ("evaluations", C.c_int64), ("flux_elements", C.c_double),
                 ("flux_flops", C.c_double), ("flux_bytes", C.c_double),
                 ("host_plan_ms_total", C.c_double), ("call_ms_total", C.c_double),
-                ("eclipse_candidates", C.c_double), ("eclipse_hits", C.c_double)]
+                ("eclipse_candidates", C.c_double), ("eclipse_hits", C.c_double),
+                ("flux_var_ms", C.c_double * 16), ("flux_var_flops", C.c_double * 16),
+                ("flux_var_launches", C.c_int64 * 16), ("flux_var_evaluated", C.c_double * 16)]
 
 
 # Every entry point include/lcurve_amd.h declares, with its signature.

@@ -474,6 +474,7 @@ struct lc_ctx {
     HostPinned pin;
     lc_profile prof;
     hipEvent_t ev[16];
+    hipEvent_t ev_var[17];  // boundaries between flux-variant launches
     int flux_rphase = 2;
     int flux_chunk = 1024;
 };
@@ -659,6 +660,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     std::vector<std::vector<TaskDev>> tasks_by_var(16);
     long long part_total = 0;
     double alg_flops = 0., alg_elems = 0., alg_bytes = 0.;
+    double var_flops[16] = {0.};
     const int ntile = S > 0 ? (S + TILE - 1) / TILE : 0;
     for (size_t i = 0; i < M; i++) {
         if (ip[i].status != LC_OK || S == 0) continue;
@@ -688,6 +690,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
                     alg_elems += (double)G.nface * S;
                     alg_flops += (double)G.nface * S * elem_flops(G);
                     alg_bytes += (double)G.nface * 40. + (double)S * 32.;
+                    var_flops[var] += (double)G.nface * S * elem_flops(G);
                 }
             }
         }
@@ -747,6 +750,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     if (fp64)
         for (int k = 0; k < 7; k++) sz.add<double>(FN);
     sz.add<int>(std::max(ngrid, 1));
+    sz.add<unsigned long long>(16 * 64);
     for (int k = 0; k < 5; k++) sz.add<double>(std::max<size_t>(n, 1));
     sz.add<int>(std::max<size_t>(n, 1));
     sz.add<int>(n + 1);
@@ -801,6 +805,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         fl.fflux = A.take<double>(FN);
     }
     fl.n0 = A.take<int>(std::max(ngrid, 1));
+    fl.evald = A.take<unsigned long long>(16 * 64);
     double *d_time = A.take<double>(std::max<size_t>(n, 1));
     double *d_expose = A.take<double>(std::max<size_t>(n, 1));
     double *d_flux = A.take<double>(std::max<size_t>(n, 1));
@@ -913,11 +918,17 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     launch_phases(s, d_insts, (int)M, dd, pa);
     if (any_route) launch_route(s, d_insts, (int)M, S, pa, d_lists, d_counts);
     HIPCHK(hipMemsetAsync(d_out, 0, SM * 2 * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(fl.evald, 0, 16 * 64 * sizeof(unsigned long long), s));
     HIPCHK(hipEventRecord(ev[4], s));
+    std::vector<int> var_used;
     for (int v = 0; v < 16; v++)
-        if (var_range[v].second)
+        if (var_range[v].second) {
+            if (var_used.empty()) HIPCHK(hipEventRecord(ctx->ev_var[0], s));
             launch_flux(s, v, fp64 ? 1 : 0, R, d_tasks + var_range[v].first, var_range[v].second, d_batches, d_grids,
                         fl, pa, S, d_lists, d_counts, d_partial);
+            var_used.push_back(v);
+            HIPCHK(hipEventRecord(ctx->ev_var[var_used.size()], s));
+        }
     HIPCHK(hipEventRecord(ev[5], s));
     launch_flux_reduce(s, d_batches, (int)batches.size(), S, TILE, d_lists, d_counts, d_partial, d_out);
     launch_scatter(s, d_insts, (int)M, dd, pa, d_out, d_calc, d_scal);
@@ -940,6 +951,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     int h_counts[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(&h_counts[0], d_cand_count, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&h_counts[1], ew.hit_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    unsigned long long h_evald[16 * 64];
+    HIPCHK(hipMemcpyAsync(h_evald, fl.evald, sizeof(h_evald), hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(ev[6], s));
     HIPCHK(hipStreamSynchronize(s));
     ctx->prof.eclipse_candidates += h_counts[0];
@@ -955,6 +968,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             alg_elems += (double)G.nface * c;
             alg_flops += (double)G.nface * c * elem_flops(G);
             alg_bytes += (double)G.nface * 40. + c * 32.;
+            var_flops[flux_variant(G)] += (double)G.nface * c * elem_flops(G);
         }
     }
     ctx->prof.grid_ms_total += elapsed(ev[0], ev[1]);
@@ -963,6 +977,13 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     ctx->prof.other_ms_total += elapsed(ev[3], ev[4]) + elapsed(ev[5], ev[6]);
     ctx->prof.flux_ms_total += elapsed(ev[4], ev[5]);
     for (int v = 0; v < 16; v++) ctx->prof.flux_launches += var_range[v].second ? 1 : 0;
+    for (size_t k = 0; k < var_used.size(); k++) {
+        const int v = var_used[k];
+        ctx->prof.flux_var_ms[v] += elapsed(ctx->ev_var[k], ctx->ev_var[k + 1]);
+        ctx->prof.flux_var_flops[v] += var_flops[v];
+        ctx->prof.flux_var_launches[v] += 1;
+    }
+    for (int k = 0; k < 16 * 64; k++) ctx->prof.flux_var_evaluated[k / 64] += (double)h_evald[k];
     ctx->prof.flux_elements += alg_elems;
     ctx->prof.flux_flops += alg_flops;
     ctx->prof.flux_bytes += alg_bytes;
@@ -1137,6 +1158,7 @@ lc_status lc_ctx_create(int device, lc_ctx **out) {
         c->device = device;
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+        for (auto &e : c->ev_var) HIPCHK(hipEventCreate(&e));
         std::memset(&c->prof, 0, sizeof(c->prof));
         if (const char *r = std::getenv("LCURVE_AMD_FLUX_R")) c->flux_rphase = std::atoi(r);
         if (c->flux_rphase != 1 && c->flux_rphase != 2 && c->flux_rphase != 4) c->flux_rphase = 2;
@@ -1151,6 +1173,8 @@ void lc_ctx_destroy(lc_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->ev_var)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1280,6 +1304,8 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
     }
     int n0 = (int)g->n0;
     fl.n0 = dev_upload(A, &n0, 1, s);
+    fl.evald = A.take<unsigned long long>(16 * 64);
+    HIPCHK(hipMemsetAsync(fl.evald, 0, 16 * 64 * sizeof(unsigned long long), s));
     PhaseArrays pa;
     std::memset(&pa, 0, sizeof(pa));
     pa.ex = dev_upload(A, pb->ex, S, s);
@@ -1299,9 +1325,17 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
     launch_flux_reduce(s, d_batch, 1, (int)S, TILE, nullptr, nullptr, d_partial, d_out);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, d_out + (size_t)star * S, S * sizeof(double), hipMemcpyDeviceToHost, s));
+    unsigned long long h_evald[16 * 64];
+    HIPCHK(hipMemcpyAsync(h_evald, fl.evald, sizeof(h_evald), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    ctx->prof.flux_ms_total += elapsed(ctx->ev[4], ctx->ev[5]);
+    for (int k = 0; k < 16 * 64; k++) ctx->prof.flux_var_evaluated[k / 64] += (double)h_evald[k];
+    const double ms = elapsed(ctx->ev[4], ctx->ev[5]);
+    const int v = flux_variant(G);
+    ctx->prof.flux_ms_total += ms;
     ctx->prof.flux_launches += 1;
+    ctx->prof.flux_var_ms[v] += ms;
+    ctx->prof.flux_var_flops[v] += (double)nf * S * elem_flops(G);
+    ctx->prof.flux_var_launches[v] += 1;
     ctx->prof.flux_elements += (double)nf * S;
     ctx->prof.flux_flops += (double)nf * S * elem_flops(G);
     ctx->prof.flux_bytes += (double)nf * 40. + (double)S * 32.;

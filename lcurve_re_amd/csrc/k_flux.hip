@@ -157,6 +157,10 @@ struct Coef32 {
 // the face's brightness*area (comp_light_flat.cpp:110-168).  mu is clipped
 // to [0, 1]; for mucrit <= 0 the clip alone reproduces the reference's
 // masks (mu <= 0 gives zero), otherwise (MUC) mu > mucrit is tested too.
+// Square roots and reciprocals use the hardware instructions (v_sqrt_f32,
+// v_rcp_f32: about 1 ulp) instead of the correctly rounded expansions,
+// which cost ~15 extra VALU operations each; the element error stays far
+// inside the 1e-5 budget of this mode.
 template <int LT, bool BEAM, bool GLENS, bool MUC>
 __device__ __forceinline__ float elem32(float ex, float ey, float ez, const float4 &a, const float4 &v,
                                         const float4 &p, const Coef32 &K) {
@@ -170,7 +174,7 @@ __device__ __forceinline__ float elem32(float ex, float ey, float ez, const floa
             const float om = 1.f - m;
             im = fmaf(-om, fmaf(om, fmaf(om, fmaf(om, K.c4, K.c3), K.c2), K.c1), 1.f);
         } else {
-            const float s = __builtin_sqrtf(m);
+            const float s = __builtin_amdgcn_sqrtf(m);
             im = fmaf(s, fmaf(s, fmaf(s, fmaf(s, K.c4, K.c3), K.c2), K.c1), K.base);
         }
         val = m * im;
@@ -185,7 +189,7 @@ __device__ __forceinline__ float elem32(float ex, float ey, float ez, const floa
             const float om = 1.f - md;
             im = fmaf(-om, fmaf(om, fmaf(om, fmaf(om, K.c4, K.c3), K.c2), K.c1), 1.f);
         } else {
-            const float s = __builtin_sqrtf(md);
+            const float s = __builtin_amdgcn_sqrtf(md);
             im = fmaf(s, fmaf(s, fmaf(s, fmaf(s, K.c4, K.c3), K.c2), K.c1), K.base);
         }
         const float g = fmaf(K.beam, vr, 1.f);
@@ -194,44 +198,63 @@ __device__ __forceinline__ float elem32(float ex, float ey, float ez, const floa
     if (GLENS) {
         const float d = -(p.x * ex + p.y * ey + p.z * ez);
         const float qx = fmaf(d, ex, p.x), qy = fmaf(d, ey, p.y), qz = fmaf(d, ez, p.z);
-        const float pp = __builtin_sqrtf(qx * qx + qy * qy + qz * qz);
+        const float pp = __builtin_amdgcn_sqrtf(qx * qx + qy * qy + qz * qz);
         const float hp = 0.5f * pp;
         const float rd = K.rlens * d;
-        const float pd = (hp * hp > 25.f * rd) ? pp + rd / pp : hp + __builtin_sqrtf(hp * hp + rd);
-        const float mg = pd * pd / (pd - hp) / hp / 4.f;
+        const float pd = (hp * hp > 25.f * rd) ? pp + rd * __builtin_amdgcn_rcpf(pp)
+                                               : hp + __builtin_amdgcn_sqrtf(hp * hp + rd);
+        const float mg = 0.25f * pd * pd * __builtin_amdgcn_rcpf((pd - hp) * hp);
         val *= d > 0.f ? mg : 1.f;
     }
     return val;
 }
 
-template <int LT, bool BEAM, bool GLENS, bool MUC, int R>
-__device__ __forceinline__ void sweep(const float4 *s_a0, const float4 *s_a1, const float4 *s_a2, const double2 *s_io,
-                                      int n, bool eclipsed, const float *ex, const float *ey, const float *ez,
-                                      const double *ph, const Coef32 &K, double *acc) {
+// One compacted face record against this lane's R sub-phases.
+template <int LT, bool BEAM, bool GLENS, bool MUC, bool ECL, int R>
+__device__ __forceinline__ void face_elems(const float4 *s_a0, const float4 *s_a1, const float4 *s_a2,
+                                           const double2 *s_io, int k, const float *ex, const float *ey,
+                                           const float *ez, const double *ph, const Coef32 &K, float *a32) {
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 a = s_a0[k];
+    const float4 v = BEAM ? s_a1[k] : z4;
+    const float4 p = GLENS ? s_a2[k] : z4;
+    if (ECL) {
+        const double2 e = s_io[k];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            float val = elem32<LT, BEAM, GLENS, MUC>(ex[r], ey[r], ez[r], a, v, p, K);
+            val = ecl_range(ph[r], e.x, e.y) ? 0.f : val;
+            a32[r] = fmaf(a.w, val, a32[r]);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            a32[r] = fmaf(a.w, elem32<LT, BEAM, GLENS, MUC>(ex[r], ey[r], ez[r], a, v, p, K), a32[r]);
+    }
+}
+
+// Sweeps n compacted records; FP32 partials over 32 faces are added into
+// the FP64 accumulators.  n is uniform across the workgroup (it comes from
+// the LDS counters), so the loop bounds live in scalar registers and the
+// face loop is unrolled by 4 to batch the LDS reads.
+template <int LT, bool BEAM, bool GLENS, bool MUC, bool ECL, int R>
+__device__ __forceinline__ void sweep(const float4 *s_a0, const float4 *s_a1, const float4 *s_a2, const double2 *s_io,
+                                      int n_, const float *ex, const float *ey, const float *ez, const double *ph,
+                                      const Coef32 &K, double *acc) {
+    const int n = __builtin_amdgcn_readfirstlane(n_);
     for (int k0 = 0; k0 < n; k0 += 32) {
         float a32[R];
 #pragma unroll
         for (int r = 0; r < R; r++) a32[r] = 0.f;
         const int k1 = min(k0 + 32, n);
-        for (int k = k0; k < k1; k++) {
-            const float4 a = s_a0[k];
-            const float4 v = BEAM ? s_a1[k] : z4;
-            const float4 p = GLENS ? s_a2[k] : z4;
-            if (eclipsed) {
-                const double2 e = s_io[k];
+        int k = k0;
+        for (; k + 4 <= k1; k += 4) {
 #pragma unroll
-                for (int r = 0; r < R; r++) {
-                    float val = elem32<LT, BEAM, GLENS, MUC>(ex[r], ey[r], ez[r], a, v, p, K);
-                    val = ecl_range(ph[r], e.x, e.y) ? 0.f : val;
-                    a32[r] = fmaf(a.w, val, a32[r]);
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < R; r++)
-                    a32[r] = fmaf(a.w, elem32<LT, BEAM, GLENS, MUC>(ex[r], ey[r], ez[r], a, v, p, K), a32[r]);
-            }
+            for (int u = 0; u < 4; u++)
+                face_elems<LT, BEAM, GLENS, MUC, ECL, R>(s_a0, s_a1, s_a2, s_io, k + u, ex, ey, ez, ph, K, a32);
         }
+        for (; k < k1; k++)
+            face_elems<LT, BEAM, GLENS, MUC, ECL, R>(s_a0, s_a1, s_a2, s_io, k, ex, ey, ez, ph, K, a32);
 #pragma unroll
         for (int r = 0; r < R; r++) acc[r] += (double)a32[r];
     }
@@ -258,6 +281,7 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
     const int TILE = R * NT;
     const int tile0 = T.tile * TILE;
     if (tile0 >= count) return;
+    const int live = min(count - tile0, TILE);
 
     double dex[R], dey[R], dez[R], ph[R];
     bool ok[R];
@@ -295,6 +319,7 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
     double acc[R];
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.;
+    unsigned long long evc = 0;  // elements evaluated by this workgroup
 
     // stage prefetch registers (clamped index keeps the loads unconditional)
     int f = f_lo + (int)threadIdx.x;
@@ -334,10 +359,11 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
             if (cls) s_io[slot1] = io;
         }
         __syncthreads();
-        sweep<LT, BEAM, GLENS, MUC, R>(s_a0, s_a1, s_a2, s_io, tot0, false, ex, ey, ez, ph, K, acc);
+        evc += (unsigned long long)(tot0 + tot1) * (unsigned long long)live;
+        sweep<LT, BEAM, GLENS, MUC, false, R>(s_a0, s_a1, s_a2, s_io, tot0, ex, ey, ez, ph, K, acc);
         if (tot1)
-            sweep<LT, BEAM, GLENS, MUC, R>(s_a0 + FB, BEAM ? s_a1 + FB : s_a1, GLENS ? s_a2 + FB : s_a2, s_io, tot1,
-                                           true, ex, ey, ez, ph, K, acc);
+            sweep<LT, BEAM, GLENS, MUC, true, R>(s_a0 + FB, BEAM ? s_a1 + FB : s_a1, GLENS ? s_a2 + FB : s_a2, s_io,
+                                                 tot1, ex, ey, ez, ph, K, acc);
         __syncthreads();
         f = fn;
     }
@@ -348,6 +374,9 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
         int e = tile0 + r * NT + (int)threadIdx.x;
         if (ok[r]) out[e] = acc[r];
     }
+    // one counter update per workgroup, spread over 64 slots per variant
+    if (threadIdx.x == 0 && evc)
+        atomicAdd(fl.evald + (LT | (BEAM ? 2 : 0) | (GLENS ? 4 : 0) | (MUC ? 8 : 0)) * 64 + (blockIdx.x & 63), evc);
 }
 
 // FP64 element in the reference's operand order (comp_light_flat.cpp:81-168);

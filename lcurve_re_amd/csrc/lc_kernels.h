@@ -65,6 +65,7 @@ struct FlatArrays {
     int *perm;      // flat position -> face index
     double *fdx, *fdy, *fdz, *fpx, *fpy, *fpz, *fflux;  // FP64 copies
     int *n0;        // never-eclipsed count per grid
+    unsigned long long *evald;  // [16 x 64] face x sub-phase elements evaluated, per variant
 };
 
 // Work lists of the three-kernel eclipse search (k_geom.hip).
