@@ -49,6 +49,10 @@ enum { LC_LDC_POLY = 0, LC_LDC_CLARET = 1 }; /* LDC::LDCtype (lcurve.h:204)     
 typedef struct lc_pparam {
     double value, range, dstep;
     int32_t vary, defined;
+#ifdef __cplusplus
+    /* Pparam's implicit conversion to its value (lcurve.h:179) */
+    operator double() const { return value; }
+#endif
 } lc_pparam;
 
 /* Lcurve::Model (lcurve.h:317-421), field for field. */

@@ -26,7 +26,7 @@ from . import _abi
 from ._abi import (LC_OK, LC_PREC_FP64, LC_PREC_MIXED, LC_LDC_CLARET, LC_LDC_POLY,
                    lc_datum, lc_eval_opts, lc_ldc, lc_model, lc_scalars)
 
-__all__ = ["Model", "Engine", "LcurveError", "make_data", "lc_model", "lc_datum", "lc_ldc",
+__all__ = ["Model", "Engine", "LcurveError", "make_data", "read_data", "write_data", "lc_model", "lc_datum", "lc_ldc",
            "LC_PREC_MIXED", "LC_PREC_FP64", "LC_LDC_POLY", "LC_LDC_CLARET"]
 
 
@@ -261,3 +261,6 @@ class Engine:
 
     def reset_profile(self):
         _lib().lc_ctx_reset_profile(self._ctx)
+
+
+from .data_io import read_data, write_data  # noqa: E402
