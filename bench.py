@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector peak
 PEAK_HBM_GBS = 8000.0
 
 
@@ -114,8 +115,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="batch_prior",
-                    choices=["batch_prior", "semidetached_large", "detached_small"])
-    ap.add_argument("--precision", type=int, default=0)
+                    choices=["batch_prior", "semidetached_large", "detached_small", "fit_batch"])
+    ap.add_argument("--precision", type=int, default=None,
+                    help="0: FP32 per face (default), 1: FP64 per face (default for fit_batch)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -131,8 +133,59 @@ def main():
         return
 
     import lcurve_re_amd as L
-    models, data, cfg = build_workload(args.workload)
     eng = L.Engine(0)
+    eng.set_stage_cache(0)   # cold: no grid kept from an earlier step
+    if args.workload == "fit_batch":
+        out = run_fit_batch(L, eng, args)
+    else:
+        out = run_batch_workload(L, eng, args)
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+def roofline_from_profile(prof, steps, precision, workload):
+    """Roofline of the dominant flux kernel: the variant (limb-darkening law
+    x beaming x lensing x mucrit) with the most launch time."""
+    var_ms = prof["flux_var_ms"]
+    v = int(np.argmax(var_ms))
+    launches = max(int(prof["flux_var_launches"][v]), 1)
+    avg_ms = var_ms[v] / launches
+    flop_per_launch = prof["flux_var_flops"][v] / launches
+    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    R = int(os.environ.get("LCURVE_AMD_FLUX_R", "2"))
+    kname = (f"k_flux_{'fp64' if precision else 'mixed'}<{v & 1}, {bool(v & 2)}, {bool(v & 4)}"
+             + ("" if precision else f", {bool(v & 8)}") + f", {R}>")
+    peak = PEAK_FP64_TFLOPS if precision else PEAK_FP32_TFLOPS
+    c_elem = {0: 24, 1: 26, 2: 39, 3: 41}[v & 3]
+    evald = prof["flux_var_evaluated"][v] / launches
+    traffic = measured_traffic(workload, kname)
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": traffic,
+            "traffic_source": f"profiles/hbm_traffic_{workload}.md" if traffic else None,
+            "hbm_frac": traffic / (avg_ms * 1e-3) / (PEAK_HBM_GBS * 1e9) if traffic and avg_ms > 0 else None,
+            "kernel": kname, "launches_per_step": launches / steps, "avg_launch_ms": avg_ms,
+            "alg_flop_per_launch": flop_per_launch, "flop_per_element": c_elem,
+            "alg_elements_per_launch": flop_per_launch / c_elem,
+            # elements left after tile culling (provably zero ones skipped):
+            # the rate the VALUs actually ran at (mixed kernels count them)
+            "evaluated_elements_per_launch": evald if not precision else None,
+            "evaluated_tflops": (evald * c_elem / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0)
+            if not precision else None,
+            "flux_stage_tflops": prof["flux_flops"] / (prof["flux_ms_total"] * 1e-3) / 1e12
+            if prof["flux_ms_total"] > 0 else 0.0,
+            "stage_ms_per_step": {k: prof[k] / steps for k in
+                                  ("grid_ms_total", "eclipse_ms_total", "cont_ms_total", "flux_ms_total",
+                                   "other_ms_total", "host_plan_ms_total", "call_ms_total")},
+            "eclipse_candidates_per_step": prof["eclipse_candidates"] / steps,
+            "eclipse_hits_per_step": prof["eclipse_hits"] / steps,
+            "grids_computed_per_step": prof["grids_computed"] / steps,
+            "grids_reused_per_step": prof["grids_reused"] / steps}
+
+
+def run_batch_workload(L, eng, args):
+    if args.precision is None:
+        args.precision = L.LC_PREC_MIXED
+    models, data, cfg = build_workload(args.workload)
     M = cfg["instances"]
     if len(models) > M:
         # draws the engine rejects (a geometry error, as the reference would
@@ -161,49 +214,94 @@ def main():
     prof = eng.profile()
     nok = int(np.count_nonzero(status == 0))
     log(f"bench: {args.steps} steps in {el:.3f} s; profile {prof}")
-    # dominant flux kernel: the variant (limb-darkening law x beaming x
-    # lensing x mucrit) with the most launch time in the timed region
-    var_ms = prof["flux_var_ms"]
-    v = int(np.argmax(var_ms))
-    launches = max(int(prof["flux_var_launches"][v]), 1)
-    avg_ms = var_ms[v] / launches
-    flop_per_launch = prof["flux_var_flops"][v] / launches
-    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
-    R = int(os.environ.get("LCURVE_AMD_FLUX_R", "2"))
-    kname = (f"k_flux_{'fp64' if args.precision else 'mixed'}<{v & 1}, {bool(v & 2)}, {bool(v & 4)}"
-             + ("" if args.precision else f", {bool(v & 8)}") + f", {R}>")
-    c_elem = {0: 24, 1: 26, 2: 39, 3: 41}[v & 3]
-    evald = prof["flux_var_evaluated"][v] / launches
-    traffic = measured_traffic(cfg["workload"], kname)
-    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                "traffic_source": f"profiles/hbm_traffic_{cfg['workload']}.md" if traffic else None,
-                "hbm_frac": traffic / (avg_ms * 1e-3) / (PEAK_HBM_GBS * 1e9) if traffic and avg_ms > 0 else None,
-                "kernel": kname, "launches_per_step": launches / args.steps, "avg_launch_ms": avg_ms,
-                "alg_flop_per_launch": flop_per_launch, "flop_per_element": c_elem,
-                "alg_elements_per_launch": flop_per_launch / c_elem,
-                # elements left after tile culling (provably zero ones skipped):
-                # the rate the VALUs actually ran at
-                "evaluated_elements_per_launch": evald,
-                "evaluated_tflops": evald * c_elem / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0,
-                "flux_stage_tflops": prof["flux_flops"] / (prof["flux_ms_total"] * 1e-3) / 1e12
-                if prof["flux_ms_total"] > 0 else 0.0,
-                "stage_ms_per_step": {k: prof[k] / args.steps for k in
-                                      ("grid_ms_total", "eclipse_ms_total", "cont_ms_total", "flux_ms_total",
-                                       "other_ms_total", "host_plan_ms_total", "call_ms_total")},
-                "eclipse_candidates_per_step": prof["eclipse_candidates"] / args.steps,
-                "eclipse_hits_per_step": prof["eclipse_hits"] / args.steps}
+    roofline = roofline_from_profile(prof, args.steps, args.precision, cfg["workload"])
     cpu, err = (None, None)
     if not args.no_cpu:
         cpu, err = cpu_baseline(models, data, calc, args.cpu_budget)
-    out = {"metric": "light curves/sec on 1 MI355X (batched engine, <=1e-5 rel err vs FP64 oracle)",
-           "value": M * args.steps / el, "unit": "light curves/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "fp32 per face, fp64 geometry/accumulation",
-           "data": "synthetic", "config": cfg, "instances_ok": nok, "roofline": roofline,
-           "cpu_baseline": cpu, "max_rel_err": err}
-    eng.close()
-    print(json.dumps(out), flush=True)
+    return {"metric": "light curves/sec on 1 MI355X (batched engine, <=1e-5 rel err vs FP64 oracle)",
+            "value": M * args.steps / el, "unit": "light curves/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp64 per face and accumulation" if args.precision else
+            "fp32 per face, fp64 geometry/accumulation",
+            "data": "synthetic", "config": cfg, "instances_ok": nok, "roofline": roofline,
+            "cpu_baseline": cpu, "max_rel_err": err}
+
+
+def run_fit_batch(L, eng, args):
+    """One LM Jacobian (base + 2 x 12 central differences) and 64 MCMC
+    walkers per step, all in one batched evaluation of 8000 phases."""
+    from lcurve_re_amd import fit as F
+    from lcurve_re_amd import workloads as W
+    w = W.fit_batch()
+    model = L.Model.from_json(w["model"])
+    prec = L.LC_PREC_FP64 if args.precision is None else args.precision
+    plain = L.make_data(w["time"], ferr=w["sigma"], weight=1.0)
+    curve, _ = eng.light_curve_comp(model, plain, scale=False, rdata=False, diagnostics=False,
+                                    precision=L.LC_PREC_FP64)
+    data = L.make_data(w["time"], ferr=w["sigma"], weight=1.0,
+                       flux=W.make_fit_flux(curve, w["sigma"], w["noise_seed"]))
+    walkers = F.walkers_around(model, w["nwalk"], w["walker_seed"])
+    fb = F.FitBatch(model, data, F.engine_evaluator(eng), scale=True, precision=prec)
+    cfg = {"workload": "fit_batch", "phases": len(w["time"]),
+           "free_parameters": [model.get_name(i) for i in range(model.nvary())], "walkers": len(walkers),
+           "nlat": 40, "sigma": w["sigma"], "noise_seed": w["noise_seed"], "walker_seed": w["walker_seed"],
+           "autoscale": True, "precision": "fp64" if prec == L.LC_PREC_FP64 else "mixed"}
+    log(f"bench: workload {cfg}")
+    out = fb.jacobian(walkers=walkers)
+    # 1 base point + 2 per central column + 1 per one-sided column + walkers
+    cfg["instances"] = out["instances_per_call"][0]
+    cfg["difference_kinds"] = dict(zip(cfg["free_parameters"], out["kinds"]))
+    for i in range(args.warmup):
+        out = fb.jacobian(walkers=walkers)
+    eng.reset_profile()
+    t0 = time.perf_counter()
+    ninst = 0
+    for i in range(args.steps):
+        out = fb.jacobian(walkers=walkers)
+        ninst += sum(out["instances_per_call"])
+    el = time.perf_counter() - t0
+    prof = eng.profile()
+    log(f"bench: {args.steps} steps in {el:.3f} s; profile {prof}")
+    roofline = roofline_from_profile(prof, args.steps, prec, "fit_batch")
+    # the same step with grid reuse inside the batch switched off
+    os.environ["LCURVE_AMD_DEDUPE"] = "0"
+    e2 = L.Engine(0)
+    e2.set_stage_cache(0)
+    fb2 = F.FitBatch(model, data, F.engine_evaluator(e2), scale=True, precision=prec)
+    fb2.jacobian(walkers=walkers)
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        fb2.jacobian(walkers=walkers)
+    el2 = time.perf_counter() - t1
+    e2.close()
+    del os.environ["LCURVE_AMD_DEDUPE"]
+    cpu, jerr = None, None
+    if not args.no_cpu:
+        from oracle import pyoracle as P
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        P.oracle().orc_set_threads(threads)
+        tc = time.perf_counter()
+        ref = F.FitBatch(model, data, P.evaluator(), scale=True).jacobian(walkers=walkers)
+        elc = time.perf_counter() - tc
+        cpu = {"value": sum(ref["instances_per_call"]) / elc, "unit": "light curves/s", "cores": threads,
+               "kind": "port", "sample": f"one full step ({sum(ref['instances_per_call'])} instances x "
+                                         f"{len(w['time'])} phases), FP64 oracle, {elc:.1f} s"}
+        col = [float(np.linalg.norm(out["J"][:, j] - ref["J"][:, j]) / max(np.linalg.norm(ref["J"][:, j]), 1e-300))
+               for j in range(model.nvary())]
+        jerr = {"jacobian_col_rel_err_max": max(col), "jacobian_col_rel_err": dict(zip(cfg["free_parameters"], col)),
+                "chisq0_rel_err": abs(out["chisq0"] / ref["chisq0"] - 1.0),
+                "walker_chisq_rel_err_max": float(np.nanmax(np.abs(out["walker_chisq"] / ref["walker_chisq"] - 1.0))),
+                "chisq0": out["chisq0"], "chisq0_oracle": ref["chisq0"]}
+    return {"metric": "light curves/sec on 1 MI355X (LM Jacobian + 64 walkers per batch, vs FP64 oracle)",
+            "value": ninst / el, "unit": "light curves/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp64 per face and accumulation" if prec == L.LC_PREC_FP64 else
+            "fp32 per face, fp64 geometry/accumulation",
+            "data": "synthetic", "config": cfg, "roofline": roofline,
+            "no_grid_reuse": {"value": ninst / el2 if el2 > 0 else None, "ms_per_step": el2 / args.steps * 1e3},
+            "cpu_baseline": cpu, "parity": jerr}
 
 
 if __name__ == "__main__":

@@ -167,6 +167,9 @@ typedef struct lc_profile {
     double flux_var_flops[16];  /* algorithmic FLOP of variant v                  */
     int64_t flux_var_launches[16];
     double flux_var_evaluated[16]; /* elements actually evaluated (after culling) */
+    int64_t grids_computed; /* star grids built by the face + eclipse kernels     */
+    int64_t grids_reused;   /* star grids copied (same geometry earlier in the
+                               batch, or kept from an earlier call)              */
 } lc_profile;
 
 typedef struct lc_ctx lc_ctx;
@@ -180,6 +183,13 @@ void lc_ctx_destroy(lc_ctx *ctx);
 const char *lc_last_error(const lc_ctx *ctx);
 lc_status lc_ctx_get_profile(const lc_ctx *ctx, lc_profile *out);
 lc_status lc_ctx_reset_profile(lc_ctx *ctx);
+/* Stage reuse across calls (the reference's per-thread grid cache,
+ * light_curve_comp.cpp:34-117, 156-207): keep up to max_faces faces of
+ * recently built star grids on the device and copy, instead of rebuilding,
+ * any grid whose geometry inputs match exactly.  0 turns reuse across calls
+ * off (grids equal within one batch are still built once).  Default 4M
+ * faces, or 0 when LCURVE_NO_GRID_CACHE is set. */
+lc_status lc_ctx_set_stage_cache(lc_ctx *ctx, size_t max_faces);
 /* Library version string; usable without a GPU. */
 const char *lc_version(void);
 /* Message for the last failure of a ctx-free call on this thread. */

@@ -255,6 +255,7 @@ inline void eval(const Model &mdl, const Data &data, bool scale, bool rdata, boo
                  std::vector<double> &calc, double &wdwarf, double &chisq, double &wnok, double &logg1,
                  double &logg2, double &rv1, double &rv2, int max_model_points, bool diagnostics,
                  int precision, const char *where) {
+    if (data.empty()) fail(where, "no data points");
     calc.resize(data.size());
     lc_scalars io{};
     if (!scale) {
@@ -326,6 +327,7 @@ struct BatchResult {
 inline BatchResult light_curve_comp_batch(const std::vector<Model> &models, const Data &data, bool scale, bool rdata,
                                           double sfac = 1.0, int max_model_points = 0, bool diagnostics = false,
                                           int precision = LC_PREC_MIXED) {
+    if (data.empty()) detail::fail("Lcurve::light_curve_comp_batch", "no data points");
     BatchResult r;
     r.n = data.size();
     const size_t m = models.size();

@@ -262,5 +262,11 @@ class Engine:
     def reset_profile(self):
         _lib().lc_ctx_reset_profile(self._ctx)
 
+    def set_stage_cache(self, max_faces):
+        """Faces of star grids kept on the device for reuse across calls (0: off)."""
+        st = _lib().lc_ctx_set_stage_cache(self._ctx, int(max_faces))
+        if st != LC_OK:
+            self._err(st)
+
 
 from .data_io import read_data, write_data  # noqa: E402

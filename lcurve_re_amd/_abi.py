@@ -118,7 +118,8 @@ class lc_profile(C.Structure):
                 ("host_plan_ms_total", C.c_double), ("call_ms_total", C.c_double),
                 ("eclipse_candidates", C.c_double), ("eclipse_hits", C.c_double),
                 ("flux_var_ms", C.c_double * 16), ("flux_var_flops", C.c_double * 16),
-                ("flux_var_launches", C.c_int64 * 16), ("flux_var_evaluated", C.c_double * 16)]
+                ("flux_var_launches", C.c_int64 * 16), ("flux_var_evaluated", C.c_double * 16),
+                ("grids_computed", C.c_int64), ("grids_reused", C.c_int64)]
 
 
 # Every entry point include/lcurve_amd.h declares, with its signature.
@@ -129,6 +130,7 @@ SIGNATURES = {
     "lc_last_error": (_c.c_char_p, [_c.c_void_p]),
     "lc_ctx_get_profile": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_profile)]),
     "lc_ctx_reset_profile": (_c.c_int, [_c.c_void_p]),
+    "lc_ctx_set_stage_cache": (_c.c_int, [_c.c_void_p, _c.c_size_t]),
     "lc_version": (_c.c_char_p, []),
     "lc_thread_error": (_c.c_char_p, []),
     "lc_sizeof": (_c.c_size_t, [_c.c_char_p]),

@@ -21,6 +21,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/lcurve_amd.h"
@@ -177,6 +178,30 @@ struct GridPlan {
     std::vector<RingHost> rings;
     int ring_faces;  // faces laid out by the rings (<= nface)
 };
+
+// Exact-match key of a grid's geometry: every GridDev field the face and
+// eclipse kernels read, plus the ring table.  Equal keys give bit-identical
+// faces and eclipse ranges, so a grid with a known key is copied instead of
+// recomputed.  (The reference keys its cache on the Model fields that feed
+// set_star_grid, light_curve_comp.cpp:34-41; keying on the derived values
+// also catches parameter changes that leave a grid unchanged.)
+std::string geom_key(const GridPlan &gp) {
+    const GridDev &G = gp.dev;
+    std::string k;
+    auto put = [&k](const void *p, size_t n) { k.append(static_cast<const char *>(p), n); };
+    const int iv[8] = {G.star, G.fine, G.nface, G.roche_self, G.roche_other, G.eclipse, G.npole, gp.ring_faces};
+    const double dv[15] = {G.q,          G.spin_self, G.spin_other, G.rref_self, G.pref_self,
+                           G.rref_other, G.pref_other, G.r_self,    G.r_other,   G.acc,
+                           G.delta,      G.cosi,      G.sini,       G.gref,      0.0};
+    put(iv, sizeof(iv));
+    put(dv, sizeof(dv));
+    for (const auto &R : gp.rings) {
+        const double rv[5] = {R.sint, R.cost, R.phi1, R.dphi_span, R.tspan_nl};
+        put(rv, sizeof(rv));
+        put(&R.nphi, sizeof(R.nphi));
+    }
+    return k;
+}
 
 // Everything set_star_grid derives before add_faces, plus the ring table
 // of every add_faces segment (set_star_grid.cpp:49-320).
@@ -465,6 +490,66 @@ struct InstPlan {
 
 }  // namespace
 
+// Device copies of grids kept across calls, keyed by every input of the
+// face and eclipse kernels.  A bump allocator over one allocation: when a
+// new grid does not fit, the whole cache is dropped and refilled.
+constexpr size_t DEFAULT_GEO_CACHE_FACES = size_t(1) << 22;  // 4M faces, ~300 MB
+
+struct GeoCache {
+    struct Entry {
+        int off, n;
+    };
+    size_t cap = 0;  // faces (0: no reuse across calls)
+    size_t used = 0;
+    void *mem = nullptr;
+    FaceArrays pool{};
+    std::unordered_map<std::string, Entry> map;
+    void clear() {
+        map.clear();
+        used = 0;
+    }
+    void release() {
+        clear();
+        if (mem) (void)hipFree(mem);
+        mem = nullptr;
+        std::memset(&pool, 0, sizeof(pool));
+    }
+    // bytes per cached face: positions and normals (6 x 8), area, gravity,
+    // eclipse flag (3 x 4), ingress and egress (2 x 8)
+    static constexpr size_t BYTES_PER_FACE = 6 * 8 + 3 * 4 + 2 * 8;
+    // false when the pool cannot be allocated (reuse across calls is then off)
+    bool ensure_pool() {
+        if (mem) return true;
+        if (cap == 0) return false;
+        if (hipMalloc(&mem, cap * BYTES_PER_FACE + 16 * 256) != hipSuccess) {
+            (void)hipGetLastError();
+            mem = nullptr;
+            cap = 0;
+            clear();
+            return false;
+        }
+        char *p = static_cast<char *>(mem);
+        auto take = [&](size_t bytes) {
+            char *r = p;
+            p += (bytes + 255) & ~(size_t)255;
+            return r;
+        };
+        pool.px = reinterpret_cast<double *>(take(cap * 8));
+        pool.py = reinterpret_cast<double *>(take(cap * 8));
+        pool.pz = reinterpret_cast<double *>(take(cap * 8));
+        pool.dx = reinterpret_cast<double *>(take(cap * 8));
+        pool.dy = reinterpret_cast<double *>(take(cap * 8));
+        pool.dz = reinterpret_cast<double *>(take(cap * 8));
+        pool.area = reinterpret_cast<float *>(take(cap * 4));
+        pool.grav = reinterpret_cast<float *>(take(cap * 4));
+        pool.ecl = reinterpret_cast<int *>(take(cap * 4));
+        pool.ein = reinterpret_cast<double *>(take(cap * 8));
+        pool.eout = reinterpret_cast<double *>(take(cap * 8));
+        pool.flux = nullptr;
+        return true;
+    }
+};
+
 // ================================================================== ctx
 struct lc_ctx {
     int device = 0;
@@ -477,6 +562,8 @@ struct lc_ctx {
     hipEvent_t ev_var[17];  // boundaries between flux-variant launches
     int flux_rphase = 2;
     int flux_chunk = 1024;
+    bool dedupe = true;  // build grids equal within one batch once
+    GeoCache geo;
 };
 
 namespace {
@@ -608,8 +695,29 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         }
     }
 
-    // ---------------- global tables
+    // ---------------- stage reuse: grids equal to an earlier grid of the
+    // batch (dup_of) or to a grid kept from an earlier call (hit_off) are
+    // copied; the rest are computed (and offered to the cache).
     const int ngrid = (int)grids.size();
+    std::vector<int> dup_of(ngrid, -1), hit_off(ngrid, -1);
+    std::vector<std::string> keys(ngrid);
+    {
+        std::unordered_map<std::string, int> first;
+        for (int g = 0; g < ngrid; g++) {
+            keys[g] = geom_key(grids[g]);
+            auto it = ctx->dedupe ? first.find(keys[g]) : first.end();
+            if (it != first.end()) {
+                dup_of[g] = it->second;
+                continue;
+            }
+            first.emplace(keys[g], g);
+            auto h = ctx->geo.map.find(keys[g]);
+            if (h != ctx->geo.map.end() && h->second.n == grids[g].dev.nface) hit_off[g] = h->second.off;
+        }
+    }
+    auto computed = [&](int g) { return dup_of[g] < 0 && hit_off[g] < 0; };
+
+    // ---------------- global tables
     std::vector<GridDev> gdev(ngrid);
     std::vector<RingDev> rdev;
     int face_cur = 0, half_cur = 0;
@@ -618,9 +726,10 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         GridDev &G = gp.dev;
         G.face_off = face_cur;
         G.ring_off = (int)rdev.size();
-        G.nring = (int)gp.rings.size();
+        G.nring = computed(g) ? (int)gp.rings.size() : 0;
         G.half_off = half_cur;
         int fo = face_cur;
+        if (computed(g))
         for (auto &R : gp.rings) {
             RingDev d;
             d.sint = R.sint;
@@ -643,10 +752,36 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     const int nface_total = face_cur, nhalf_total = half_cur, nring = (int)rdev.size();
     if ((long long)face_cur > (1LL << 30)) throw EngineError(LC_E_ARG, "batch too large: more than 2^30 faces");
 
+    // copies of reused grids, and computed grids offered to the cache
+    std::vector<FaceCopy> copy_hit, copy_dup, copy_out;
+    std::vector<std::pair<int, GeoCache::Entry>> inserts;  // grid, pool slot
+    for (int g = 0; g < ngrid; g++) {
+        if (hit_off[g] >= 0) copy_hit.push_back(FaceCopy{hit_off[g], gdev[g].face_off, gdev[g].nface, 0});
+        if (dup_of[g] >= 0) copy_dup.push_back(FaceCopy{gdev[dup_of[g]].face_off, gdev[g].face_off, gdev[g].nface, 0});
+    }
+    if (ctx->geo.cap > 0) {
+        size_t need = 0;
+        for (int g = 0; g < ngrid; g++)
+            if (computed(g) && (size_t)gdev[g].nface <= ctx->geo.cap) need += gdev[g].nface;
+        if (need && ctx->geo.used + need > ctx->geo.cap) ctx->geo.clear();
+        for (int g = 0; g < ngrid; g++) {
+            if (!computed(g) || gdev[g].nface == 0) continue;
+            if (ctx->geo.used + (size_t)gdev[g].nface > ctx->geo.cap) continue;
+            GeoCache::Entry e{(int)ctx->geo.used, gdev[g].nface};
+            ctx->geo.used += gdev[g].nface;
+            inserts.push_back({g, e});
+            copy_out.push_back(FaceCopy{gdev[g].face_off, e.off, e.n, 0});
+        }
+        if (!copy_out.empty() && !ctx->geo.ensure_pool()) {
+            copy_out.clear();
+            inserts.clear();
+        }
+    }
+
     // gaps between the ring layout and numface() hold default faces
     std::vector<std::pair<int, int>> gaps;
     for (int g = 0; g < ngrid; g++)
-        if (grids[g].ring_faces < gdev[g].nface)
+        if (computed(g) && grids[g].ring_faces < gdev[g].nface)
             gaps.push_back({gdev[g].face_off + grids[g].ring_faces, gdev[g].nface - grids[g].ring_faces});
 
     // ---------------- flux batches and tasks
@@ -751,6 +886,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         for (int k = 0; k < 7; k++) sz.add<double>(FN);
     sz.add<int>(std::max(ngrid, 1));
     sz.add<unsigned long long>(16 * 64);
+    sz.add<FaceCopy>(std::max<size_t>(copy_hit.size(), 1));
+    sz.add<FaceCopy>(std::max<size_t>(copy_dup.size(), 1));
+    sz.add<FaceCopy>(std::max<size_t>(copy_out.size(), 1));
     for (int k = 0; k < 5; k++) sz.add<double>(std::max<size_t>(n, 1));
     sz.add<int>(std::max<size_t>(n, 1));
     sz.add<int>(n + 1);
@@ -806,6 +944,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     }
     fl.n0 = A.take<int>(std::max(ngrid, 1));
     fl.evald = A.take<unsigned long long>(16 * 64);
+    FaceCopy *d_copy_hit = A.take<FaceCopy>(std::max<size_t>(copy_hit.size(), 1));
+    FaceCopy *d_copy_dup = A.take<FaceCopy>(std::max<size_t>(copy_dup.size(), 1));
+    FaceCopy *d_copy_out = A.take<FaceCopy>(std::max<size_t>(copy_out.size(), 1));
     double *d_time = A.take<double>(std::max<size_t>(n, 1));
     double *d_expose = A.take<double>(std::max<size_t>(n, 1));
     double *d_flux = A.take<double>(std::max<size_t>(n, 1));
@@ -863,6 +1004,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         {d_batches, batches.data(), batches.size() * sizeof(BatchDev)},
         {d_tasks, tasks.data(), tasks.size() * sizeof(TaskDev)},
         {d_diag_idx, diag_idx.data(), diag_idx.size() * sizeof(int)},
+        {d_copy_hit, copy_hit.data(), copy_hit.size() * sizeof(FaceCopy)},
+        {d_copy_dup, copy_dup.data(), copy_dup.size() * sizeof(FaceCopy)},
+        {d_copy_out, copy_out.data(), copy_out.size() * sizeof(FaceCopy)},
     };
     size_t up_total = 0;
     for (auto &u : ups) up_total += (u.bytes + 255) & ~(size_t)255;
@@ -907,6 +1051,11 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     launch_faces(s, d_grids, d_rings, nring, nhalf_total, fa, d_cand, d_cand_count, d_status);
     HIPCHK(hipEventRecord(ev[1], s));
     launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, ew, d_status);
+    // reused grids: cache hits first (a duplicate may copy from one), then
+    // duplicates within the batch, then the new grids into the cache
+    if (!copy_hit.empty()) launch_copy_faces(s, d_copy_hit, (int)copy_hit.size(), ctx->geo.pool, fa);
+    if (!copy_dup.empty()) launch_copy_faces(s, d_copy_dup, (int)copy_dup.size(), fa, fa);
+    if (!copy_out.empty()) launch_copy_faces(s, d_copy_out, (int)copy_out.size(), fa, ctx->geo.pool);
     HIPCHK(hipEventRecord(ev[2], s));
     launch_continuum(s, d_grids, ngrid, nface_total, fa);
     HIPCHK(hipEventRecord(ev[3], s));
@@ -987,6 +1136,23 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     ctx->prof.flux_elements += alg_elems;
     ctx->prof.flux_flops += alg_flops;
     ctx->prof.flux_bytes += alg_bytes;
+
+    // ---------------- stage reuse bookkeeping
+    for (int g = 0; g < ngrid; g++) {  // a copy inherits its source's failure
+        if (dup_of[g] < 0) continue;
+        const int si = gdev[dup_of[g]].inst, di = gdev[g].inst;
+        if (h_status[si] && !h_status[di]) h_status[di] = h_status[si];
+    }
+    for (const auto &ins : inserts) {
+        const int inst = gdev[ins.first].inst;
+        if (ip[inst].status == LC_OK && !h_status[inst]) ctx->geo.map[keys[ins.first]] = ins.second;
+    }
+    for (int g = 0; g < ngrid; g++) {
+        if (computed(g))
+            ctx->prof.grids_computed += 1;
+        else
+            ctx->prof.grids_reused += 1;
+    }
 
     // ---------------- finalise per instance
     for (size_t i = 0; i < M; i++) {
@@ -1163,6 +1329,10 @@ lc_status lc_ctx_create(int device, lc_ctx **out) {
         if (const char *r = std::getenv("LCURVE_AMD_FLUX_R")) c->flux_rphase = std::atoi(r);
         if (c->flux_rphase != 1 && c->flux_rphase != 2 && c->flux_rphase != 4) c->flux_rphase = 2;
         if (const char *ch = std::getenv("LCURVE_AMD_FLUX_CHUNK")) c->flux_chunk = std::max(32, std::atoi(ch));
+        // grids kept across calls, as the reference's per-thread cache
+        // (disabled by the same environment variable)
+        c->geo.cap = std::getenv("LCURVE_NO_GRID_CACHE") ? 0 : DEFAULT_GEO_CACHE_FACES;
+        if (const char *d = std::getenv("LCURVE_AMD_DEDUPE")) c->dedupe = std::atoi(d) != 0;
         *out = c.release();
         return LC_OK;
     });
@@ -1176,6 +1346,7 @@ void lc_ctx_destroy(lc_ctx *ctx) {
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->ev_var)
         if (e) (void)hipEventDestroy(e);
+    ctx->geo.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1194,6 +1365,17 @@ lc_status lc_ctx_reset_profile(lc_ctx *ctx) {
     return LC_OK;
 }
 
+lc_status lc_ctx_set_stage_cache(lc_ctx *ctx, size_t max_faces) {
+    if (!ctx) return LC_E_ARG;
+    return guarded(ctx, [&]() -> int {
+        HIPCHK(hipSetDevice(ctx->device));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->geo.release();
+        ctx->geo.cap = max_faces;
+        return LC_OK;
+    });
+}
+
 lc_status lc_star_grid_size(const lc_model *m, int star, int fine, size_t *nface) {
     if (!m || !nface || (star != 0 && star != 1)) return LC_E_ARG;
     return guarded(nullptr, [&]() -> int {
@@ -1205,7 +1387,11 @@ lc_status lc_star_grid_size(const lc_model *m, int star, int fine, size_t *nface
 
 lc_status lc_light_curve_comp_batch(lc_ctx *ctx, const lc_model *models, size_t m, const lc_datum *data, size_t n,
                                     const lc_eval_opts *opts, double *calc, lc_scalars *io, int32_t *status) {
-    if (!ctx || !models || !data || !opts || !calc || !io || !status) return LC_E_ARG;
+    if (!ctx) return LC_E_ARG;
+    if (!models || !data || !opts || !calc || !io || !status) {
+        ctx->err = "light_curve_comp: null argument (an empty data set has no data pointer)";
+        return LC_E_ARG;
+    }
     if (m == 0) return LC_OK;
     return guarded(ctx, [&]() -> int {
         if (n == 0) throw EngineError(LC_E_ARG, "light_curve_comp: empty data");
@@ -1218,7 +1404,11 @@ lc_status lc_light_curve_comp_batch(lc_ctx *ctx, const lc_model *models, size_t 
 
 lc_status lc_light_curve_comp(lc_ctx *ctx, const lc_model *model, const lc_datum *data, size_t n,
                               const lc_eval_opts *opts, double *calc, lc_scalars *io) {
-    if (!ctx || !model || !data || !opts || !calc || !io) return LC_E_ARG;
+    if (!ctx) return LC_E_ARG;
+    if (!model || !data || !opts || !calc || !io) {
+        ctx->err = "light_curve_comp: null argument (an empty data set has no data pointer)";
+        return LC_E_ARG;
+    }
     int32_t st = LC_OK;
     lc_status r = lc_light_curve_comp_batch(ctx, model, 1, data, n, opts, calc, io, &st);
     if (r != LC_OK) return r;

@@ -230,6 +230,29 @@ __global__ __launch_bounds__(256) void k_eclipse_final(FaceArrays fa, EclipseWor
     }
 }
 
+// Stage reuse: a grid whose geometry inputs equal those of another grid
+// of the batch, or of a grid kept from an earlier call, takes a copy of its
+// faces instead of recomputing them (the reference's exact-match cache,
+// light_curve_comp.cpp:34-117, 156-207).  blockIdx.y selects the copy.
+__global__ __launch_bounds__(256) void k_copy_faces(const FaceCopy *__restrict__ items, FaceArrays src,
+                                                    FaceArrays dst) {
+    const FaceCopy c = items[blockIdx.y];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += gridDim.x * blockDim.x) {
+        const int a = c.src + i, b = c.dst + i;
+        dst.px[b] = src.px[a];
+        dst.py[b] = src.py[a];
+        dst.pz[b] = src.pz[a];
+        dst.dx[b] = src.dx[a];
+        dst.dy[b] = src.dy[a];
+        dst.dz[b] = src.dz[a];
+        dst.area[b] = src.area[a];
+        dst.grav[b] = src.grav[a];
+        dst.ecl[b] = src.ecl[a];
+        dst.ein[b] = src.ein[a];
+        dst.eout[b] = src.eout[a];
+    }
+}
+
 // Continuum brightness * area of one face (set_star_continuum.cpp:103-296)
 __global__ __launch_bounds__(256) void k_continuum(const GridDev *__restrict__ grids, int ngrid, int nface_total,
                                                   FaceArrays fa) {
@@ -590,6 +613,11 @@ void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays f
     const int blocks2 = std::min((2 * max_cand + 127) / 128, 16384);
     hipLaunchKernelGGL(k_eclipse_bisect, dim3(blocks2), dim3(128), 0, s, grids, ngrid, fa, ew, inst_status);
     hipLaunchKernelGGL(k_eclipse_final, dim3(std::min((max_cand + 255) / 256, 4096)), dim3(256), 0, s, fa, ew);
+}
+
+void launch_copy_faces(hipStream_t s, const FaceCopy *items, int nitem, FaceArrays src, FaceArrays dst) {
+    for (int k = 0; k < nitem; k += 65535)
+        hipLaunchKernelGGL(k_copy_faces, dim3(32, std::min(65535, nitem - k)), dim3(256), 0, s, items + k, src, dst);
 }
 
 void launch_continuum(hipStream_t s, const GridDev *grids, int ngrid, int nface_total, FaceArrays fa) {

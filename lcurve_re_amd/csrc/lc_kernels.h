@@ -102,11 +102,19 @@ struct TaskDev {
     int batch, tile, chunk, pad;
 };
 
+// One face-range copy of the geometry arrays (stage reuse of a grid).
+struct FaceCopy {
+    int src, dst, n, pad;
+};
+
 // ------------------------------------------------------------ launches
 void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int nring, int nhalf_total,
                   FaceArrays fa, int *cand, int *cand_count, int *inst_status);
 void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int *cand,
                     const int *cand_count, int max_cand, EclipseWork ew, int *inst_status);
+// Copies positions, normals, areas, gravities and eclipse ranges of whole
+// grids (not the fluxes, which the continuum kernel recomputes).
+void launch_copy_faces(hipStream_t s, const FaceCopy *items, int nitem, FaceArrays src, FaceArrays dst);
 void launch_continuum(hipStream_t s, const GridDev *grids, int ngrid, int nface_total, FaceArrays fa);
 void launch_flat_build(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, FlatArrays fl, int fp64);
 void launch_point_sums(hipStream_t s, const GridDev *grids, const InstDev *insts, int ninst, FaceArrays fa,

@@ -1,0 +1,222 @@
+"""Fit-loop batches: one Levenberg-Marquardt Jacobian plus MCMC walkers in
+one batched evaluation.
+
+The reference's LM solver differentiates the residuals
+r_i = sqrt(w_i) (f_i - m_i) / sigma_i (levmarq_solver.cpp:670-700; a
+non-positive sigma counts as 1e-10, a non-positive weight as 1) with
+central differences of step h_j = fd_step_rel |p_j| + fd_step_min
+(fd_step_rel = eps**(1/4), fd_step_min = 1e-10, at most half the span of
+finite limits; levmarq_solver.cpp:427-441, 766-778).  A central difference
+is used only when p_j +- h_j stay inside Model::get_limit(); otherwise a
+one-sided difference (forward, or backward past the upper limit); a
+failed evaluation halves the step, up to 4 times, and a central difference
+that still fails falls back to the one-sided scheme
+(levmarq_solver.cpp:800-880).  The reference
+evaluates every column with its own light_curve_comp_fast call; here the
+base point, all 2 x npar perturbations and the MCMC walkers go through one
+light_curve_comp_batch call, and only columns whose evaluations failed are
+retried (again batched) with halved steps.
+
+Within a batch, perturbations that leave a star grid unchanged (t0, the
+temperatures, limb darkening, gravity darkening) reuse the base point's
+grids instead of rebuilding them (the engine's stage reuse).
+
+`evaluate` is any callable (models, data, opts) -> (calc[m, n], status[m]);
+engine_evaluator(engine) gives the GPU one.
+"""
+import math
+
+import numpy as np
+
+from . import LC_PREC_FP64, Model, data_view
+
+FD_STEP_REL = math.sqrt(math.sqrt(np.finfo(np.float64).eps))
+FD_STEP_MIN = 1e-10
+
+
+def fd_steps(pars, limits, shrink=1.0):
+    """h_j of levmarq_solver.cpp:766-778."""
+    h = np.empty(len(pars))
+    for j, p in enumerate(pars):
+        hj = shrink * (FD_STEP_REL * abs(p) + FD_STEP_MIN)
+        span = limits[j][1] - limits[j][0]
+        if math.isfinite(span) and span > 0.0:
+            hj = min(hj, 0.5 * span)
+        h[j] = max(hj, FD_STEP_MIN)
+    return h
+
+
+def residuals(data, calc):
+    """r_i = sqrt(w) (f - m) / sigma (levmarq_solver.cpp:690-700)."""
+    v = data_view(data)
+    sigma = np.where(v["ferr"] > 0.0, v["ferr"], 1e-10)
+    w = np.where(v["weight"] > 0.0, v["weight"], 1.0)
+    return np.sqrt(w) * (v["flux"] - calc) / sigma
+
+
+class FitBatch:
+    """Jacobian + walker evaluation of one model against one data set.
+
+    scale: fit the flux scale factor in every evaluation (re_scale), as the
+    solvers do with autoscale on; max_model_points > 0 evaluates through
+    light_curve_comp_fast.  precision defaults to FP64 per face: central
+    differences of step ~1e-4 |p| (1e-10 for a parameter at 0, such as t0)
+    resolve changes below the FP32 element resolution.
+    """
+
+    def __init__(self, model, data, evaluate, scale=True, max_model_points=0, precision=LC_PREC_FP64):
+        self.model = model
+        self.data = data
+        self.evaluate = evaluate
+        self.opts = dict(scale=scale, rdata=True, max_model_points=max_model_points, precision=precision)
+        self.limits = [tuple(x) for x in model.get_limit()]
+        self.npar = model.nvary()
+
+    def _models(self, vectors):
+        out = []
+        for v in vectors:
+            m = self.model.copy()
+            m.set_param(v)
+            out.append(m)
+        return out
+
+    def _run(self, vectors):
+        calc, status = self.evaluate(self._models(vectors), self.data, self.opts)
+        return calc, np.asarray(status)
+
+    def plan(self, pars, h):
+        """(kind, vectors) per column: kind 'c' central (+h, -h), 'f'/'b'
+        one-sided forward/backward (one vector), None when no step fits."""
+        cols = []
+        for j in range(self.npar):
+            lo, hi = self.limits[j]
+            p = pars[j]
+            if p + h[j] <= hi and p - h[j] >= lo:
+                cols.append(("c", h[j]))
+                continue
+            hj, sign = h[j], 1.0
+            if p + hj > hi:
+                sign = -1.0
+                if p - hj < lo:
+                    hj, sign = 0.5 * (hi - lo), 1.0
+                    if hj < 1e-15:
+                        cols.append((None, 0.0))
+                        continue
+            cols.append(("f" if sign > 0 else "b", hj))
+        return cols
+
+    def _vectors(self, pars, cols, which):
+        vecs, index = [], []
+        for j in which:
+            kind, hj = cols[j]
+            if kind == "c":
+                for s in (1.0, -1.0):
+                    v = np.array(pars, float)
+                    v[j] = pars[j] + s * hj
+                    vecs.append(v)
+                index.append((j, len(vecs) - 2, len(vecs) - 1))
+            elif kind in ("f", "b"):
+                v = np.array(pars, float)
+                v[j] = pars[j] + (hj if kind == "f" else -hj)
+                vecs.append(v)
+                index.append((j, len(vecs) - 1, None))
+        return vecs, index
+
+    def jacobian(self, pars=None, walkers=()):
+        """One batched call: base point, Jacobian columns, walkers.
+
+        Returns dict with J (n x npar), resid0, chisq0, calc0, steps,
+        walker_chisq (nan where a walker failed), walker_status, and the
+        number of instances evaluated per call.
+        """
+        pars = np.array(self.model.get_param() if pars is None else pars, float)
+        h = fd_steps(pars, self.limits)
+        cols = self.plan(pars, h)
+        vecs, index = self._vectors(pars, cols, range(self.npar))
+        walkers = [np.asarray(w, float) for w in walkers]
+        batch = [pars] + vecs + walkers
+        calc, status = self._run(batch)
+        n = calc.shape[1]
+        if status[0] != 0:
+            raise RuntimeError(f"base evaluation failed with status {int(status[0])}")
+        r0 = residuals(self.data, calc[0])
+        J = np.zeros((n, self.npar))
+        calls = [len(batch)]
+        pending = []
+        for (j, a, b) in index:
+            kind, hj = cols[j]
+            ok = status[1 + a] == 0 and (b is None or status[1 + b] == 0)
+            if ok:
+                J[:, j] = self._column(kind, hj, calc[1 + a], None if b is None else calc[1 + b], r0)
+            else:
+                pending.append(j)
+        # failed columns: halve the step and retry, batched; a central
+        # difference that keeps failing falls back to a one-sided one
+        tries = {j: 0 for j in pending}
+        first = {j: cols[j][1] for j in pending}
+        while pending:
+            for j in pending:
+                kind, hj = cols[j]
+                cols[j] = (kind, 0.5 * hj)
+                tries[j] += 1
+            vecs, index = self._vectors(pars, cols, pending)
+            c2, s2 = self._run(vecs)
+            calls.append(len(vecs))
+            nxt = []
+            for (j, a, b) in index:
+                kind, hj = cols[j]
+                if s2[a] == 0 and (b is None or s2[b] == 0):
+                    J[:, j] = self._column(kind, hj, c2[a], None if b is None else c2[b], r0)
+                elif tries[j] < 4:
+                    nxt.append(j)
+                elif kind == "c":
+                    cols[j] = ("f", 2.0 * first[j])   # halved again before the next try
+                    tries[j] = 0
+                    nxt.append(j)
+            pending = nxt
+        nw = len(walkers)
+        wst = status[len(batch) - nw:] if nw else np.zeros(0, int)
+        wchi = np.full(nw, np.nan)
+        for k in range(nw):
+            if wst[k] == 0:
+                wchi[k] = float(np.sum(residuals(self.data, calc[len(batch) - nw + k]) ** 2))
+        steps = np.array([hj for _, hj in cols])
+        return {"J": J, "resid0": r0, "chisq0": float(np.sum(r0 ** 2)), "calc0": calc[0], "steps": steps,
+                "kinds": [k for k, _ in cols], "walker_chisq": wchi, "walker_status": wst,
+                "instances_per_call": calls}
+
+    def _column(self, kind, hj, ca, cb, r0):
+        ra = residuals(self.data, ca)
+        if kind == "c":
+            d = (ra - residuals(self.data, cb)) / (2.0 * hj)
+        elif kind == "f":
+            d = (ra - r0) / hj
+        else:
+            d = (r0 - ra) / hj
+        return np.where(np.isfinite(d), d, 0.0)
+
+
+def walkers_around(model, nwalk, seed, rel=1e-3):
+    """nwalk legal parameter vectors p (1 + rel N(0,1)) (+ rel N(0,1) for
+    parameters at 0), from a numpy PCG64 generator seeded with `seed`."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = model.get_param()
+    out = []
+    while len(out) < nwalk:
+        v = p * (1.0 + rel * rng.standard_normal(len(p))) + np.where(p == 0.0, rel * rng.standard_normal(len(p)), 0)
+        if not model.is_not_legal(v):
+            out.append(v)
+    return out
+
+
+def engine_evaluator(engine):
+    """evaluate(models, data, opts) over Engine.light_curve_comp_batch."""
+    def evaluate(models, data, opts):
+        calc, _, status = engine.light_curve_comp_batch(models, data, scale=opts["scale"], rdata=opts["rdata"],
+                                                        precision=opts["precision"],
+                                                        max_model_points=opts["max_model_points"])
+        return calc, status
+    return evaluate
+
+
+__all__ = ["FitBatch", "fd_steps", "residuals", "walkers_around", "engine_evaluator", "Model"]

@@ -144,5 +144,24 @@ FIT_PARAMS = ["q", "iangle", "r1", "r2", "t1", "t2", "ldc1_1", "ldc1_2", "ldc1_3
               "gravity_dark2"]
 
 
+def fit_batch(nphase=8000, nwalk=64, sigma=1.5e-3, noise_seed=20261204, walker_seed=20261205):
+    """The detached binary with the 12 LM parameters free, evaluated at
+    nphase phases k/nphase; the data are its own light curve normalised to
+    mean 1 plus N(0, sigma^2) noise (make_fit_flux), fitted with autoscale.
+    One batch = 1 base point + 2 x 12 central differences + nwalk walkers."""
+    cfg = model_json(vary=FIT_PARAMS)
+    t = np.arange(nphase, dtype=np.float64) / nphase
+    return {"name": "fit_batch", "model": cfg, "time": t, "expose": 0.0, "ndiv": 1, "ferr": sigma,
+            "weight": 1.0, "sigma": sigma, "noise_seed": noise_seed, "walker_seed": walker_seed, "nwalk": nwalk,
+            "generator": "numpy PCG64"}
+
+
+def make_fit_flux(curve, sigma=1.5e-3, seed=20261204):
+    """curve / mean(curve) + N(0, sigma^2) from a PCG64 generator."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    c = np.asarray(curve, np.float64)
+    return c / np.mean(c) + sigma * rng.standard_normal(c.size)
+
+
 def to_json_text(cfg):
     return json.dumps(cfg)

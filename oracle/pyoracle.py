@@ -273,3 +273,20 @@ def light_curve(model, data, scale=False, rdata=True, diagnostics=False, max_mod
     o = abi.lc_eval_opts(int(scale), int(rdata), 0, int(diagnostics), int(max_model_points), 0)
     st = lib.orc_light_curve_comp(C.byref(model), data, n, C.byref(o), dptr(calc), C.byref(io))
     return st, calc, io
+
+
+def evaluator():
+    """evaluate(models, data, opts) -> (calc[m, n], status[m]) over the FP64
+    oracle, one light_curve_comp(_fast) per model (the FitBatch interface)."""
+    def evaluate(models, data, opts):
+        n = len(data)
+        calc = np.zeros((len(models), n), np.float64)
+        status = np.zeros(len(models), np.int32)
+        for i, m in enumerate(models):
+            st, c, _ = light_curve(m.pod, data, scale=opts["scale"], rdata=opts["rdata"], diagnostics=False,
+                                   max_model_points=opts["max_model_points"])
+            status[i] = st
+            if st == 0:
+                calc[i] = c
+        return calc, status
+    return evaluate

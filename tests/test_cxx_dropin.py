@@ -91,6 +91,17 @@ def test_data_ascii_round_trip(tmp_path):
     assert (tmp_path / "py.dat").read_text() == dst.read_text()
 
 
+def test_data_written_by_wrasc_reads_back_empty(tmp_path):
+    # the reference's own quirk, kept: wrasc pads the time to 17 columns and
+    # rasc skips lines starting with a space (lcurve.cpp:35-44, 66-69)
+    src, dst = tmp_path / "in.dat", tmp_path / "out.dat"
+    src.write_text(DATA_TEXT)
+    assert run("data", str(src), str(dst))[0] == 0
+    rc, out = run("data", str(dst), str(tmp_path / "again.dat"))
+    assert rc == 0 and out.strip() == "n 0"
+    assert len(L.read_data(str(dst))) == 0
+
+
 def test_data_malformed_line_raises(tmp_path):
     src = tmp_path / "bad.dat"
     src.write_text("0.1 0 1 0.01 1 1\n0.2 0 x 0.01 1 1\n")
@@ -125,8 +136,16 @@ def test_light_curve_through_cxx_vs_oracle(tmp_path):
     cfg.write_text(w["model"] if isinstance(w["model"], str) else json.dumps(w["model"]))
     data = L.make_data(w["time"], expose=w["expose"], ndiv=w["ndiv"], ferr=w["ferr"], weight=w["weight"],
                        flux=w["flux"])
-    L.write_data(str(tmp_path / "in.dat"), data)
-    data = L.read_data(str(tmp_path / "in.dat"))   # what the program reads, to the printed precision
+    # Data::wrasc right-aligns the time in 17 columns, and Data::rasc skips
+    # lines that start with a space (lcurve.cpp:35-44, 66-69), so the input
+    # is written without leading blanks
+    v = L.data_view(data)
+    with open(tmp_path / "in.dat", "w") as f:
+        for r in v:
+            f.write(" ".join(repr(float(r[k])) for k in ("time", "expose", "flux", "ferr", "weight"))
+                    + f" {int(r['ndiv'])}\n")
+    data = L.read_data(str(tmp_path / "in.dat"))
+    assert len(data) == 400
     print("cxx dropin eval detached_small nphase=400", flush=True)
     rc, out = run("eval", str(cfg), str(tmp_path / "in.dat"), str(tmp_path / "out.txt"))
     assert rc == 0, out
