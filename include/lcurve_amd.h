@@ -259,6 +259,49 @@ lc_status lc_set_star_grid(lc_ctx *ctx, const lc_model *m, int star, int fine,
 lc_status lc_set_star_continuum(lc_ctx *ctx, const lc_model *m,
                                 lc_star_faces *star1, lc_star_faces *star2);
 
+/* ------------------------------------------------ learned surrogate (MLP) */
+/* No reference counterpart: a BF16 multilayer perceptron (SiLU hidden
+ * layers, linear output) trained on light curves from this engine, for the
+ * delayed-acceptance screen of an MCMC sampler (the sampler's cheap first
+ * stage; every accepted step is judged by lc_light_curve_comp_batch).
+ * widths = {inputs, hidden..., outputs}; parameters are laid out layer by
+ * layer as W_l (fan_in x fan_out, row-major) followed by b_l.  Host arrays
+ * are row-major float32; a lc_mlp is not thread-safe. */
+typedef struct lc_mlp lc_mlp;
+lc_status lc_mlp_create(int device, const int32_t *widths, int nwidth, int max_batch, lc_mlp **out);
+void lc_mlp_destroy(lc_mlp *mlp);
+const char *lc_mlp_last_error(const lc_mlp *mlp);
+size_t lc_mlp_num_params(const lc_mlp *mlp);
+lc_status lc_mlp_set_params(lc_mlp *mlp, const float *params, size_t n);  /* also resets Adam */
+lc_status lc_mlp_get_params(lc_mlp *mlp, float *params, size_t n);
+/* y (batch x outputs) = network(x (batch x inputs)) */
+lc_status lc_mlp_forward(lc_mlp *mlp, const float *x, size_t batch, float *y);
+/* gradient of the mean squared error over batch <= max_batch rows */
+lc_status lc_mlp_grad(lc_mlp *mlp, const float *x, const float *target, size_t batch, float *grad, double *loss);
+/* device-resident training set: allocate nrows, then load row ranges */
+lc_status lc_mlp_set_dataset(lc_mlp *mlp, size_t nrows);
+lc_status lc_mlp_load_dataset(lc_mlp *mlp, size_t row0, const float *x, const float *target, size_t count);
+/* steps of Adam (0.9, 0.999, 1e-8) on minibatches drawn from the data set */
+lc_status lc_mlp_train(lc_mlp *mlp, int steps, int batch, float lr, uint64_t seed, double *last_loss);
+/* mean squared error of the network over data-set rows [row0, row0+count) */
+lc_status lc_mlp_eval_dataset(lc_mlp *mlp, size_t row0, size_t count, double *mse);
+/* Optional FP32 linear skip S (inputs x outputs, row-major) added to the
+ * output of lc_mlp_forward / lc_mlp_chisq: y = network(x) + x S.  Training
+ * (lc_mlp_grad, lc_mlp_train, lc_mlp_eval_dataset) leaves it out, so the
+ * network learns what the skip does not explain.  S = NULL removes it. */
+lc_status lc_mlp_set_skip(lc_mlp *mlp, const float *S, size_t n);
+/* Likelihood of predicted curves (outputs = standardised curve at phases
+ * k / outputs: v = y * scale + mu[k] is log F when log_output, else F)
+ * against data: weight = w / ferr^2, zero for ignored points.  lc_mlp_chisq
+ * then gives, for each of m input rows and its t0, chi**2 with the optimal
+ * scale factor, the model flux at a data time being the linear
+ * interpolation of F at phase (t - t0) / period. */
+lc_status lc_mlp_set_likelihood(lc_mlp *mlp, const double *time, const double *flux, const double *weight, size_t n,
+                                const float *mu, float scale, double period, int log_output);
+lc_status lc_mlp_chisq(lc_mlp *mlp, const float *x, const double *t0, size_t m, double *chisq);
+/* GEMM FLOP issued so far and wall time spent in lc_mlp_train */
+lc_status lc_mlp_stats(const lc_mlp *mlp, double *gemm_flop, double *train_ms);
+
 #ifdef __cplusplus
 }
 #endif

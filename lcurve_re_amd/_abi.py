@@ -159,6 +159,23 @@ SIGNATURES = {
                                     _c.POINTER(lc_star_faces)]),
     "lc_set_star_continuum": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_model), _c.POINTER(lc_star_faces),
                                          _c.POINTER(lc_star_faces)]),
+    "lc_mlp_create": (_c.c_int, [_c.c_int, _IP, _c.c_int, _c.c_int, _c.POINTER(_c.c_void_p)]),
+    "lc_mlp_destroy": (None, [_c.c_void_p]),
+    "lc_mlp_last_error": (_c.c_char_p, [_c.c_void_p]),
+    "lc_mlp_num_params": (_c.c_size_t, [_c.c_void_p]),
+    "lc_mlp_set_params": (_c.c_int, [_c.c_void_p, _FP, _c.c_size_t]),
+    "lc_mlp_get_params": (_c.c_int, [_c.c_void_p, _FP, _c.c_size_t]),
+    "lc_mlp_forward": (_c.c_int, [_c.c_void_p, _FP, _c.c_size_t, _FP]),
+    "lc_mlp_grad": (_c.c_int, [_c.c_void_p, _FP, _FP, _c.c_size_t, _FP, _DP]),
+    "lc_mlp_set_dataset": (_c.c_int, [_c.c_void_p, _c.c_size_t]),
+    "lc_mlp_load_dataset": (_c.c_int, [_c.c_void_p, _c.c_size_t, _FP, _FP, _c.c_size_t]),
+    "lc_mlp_train": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_float, _c.c_uint64, _DP]),
+    "lc_mlp_eval_dataset": (_c.c_int, [_c.c_void_p, _c.c_size_t, _c.c_size_t, _DP]),
+    "lc_mlp_stats": (_c.c_int, [_c.c_void_p, _DP, _DP]),
+    "lc_mlp_set_likelihood": (_c.c_int, [_c.c_void_p, _DP, _DP, _DP, _c.c_size_t, _FP, _c.c_float, _c.c_double,
+                                         _c.c_int]),
+    "lc_mlp_chisq": (_c.c_int, [_c.c_void_p, _FP, _DP, _c.c_size_t, _DP]),
+    "lc_mlp_set_skip": (_c.c_int, [_c.c_void_p, _FP, _c.c_size_t]),
 }
 
 _lib = None

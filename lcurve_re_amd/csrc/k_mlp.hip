@@ -1,0 +1,732 @@
+// k_mlp.hip -- BF16 MLP surrogate of the light curve: training and inference
+// on the matrix cores of gfx950.
+//
+// Layers: y = W_L silu(... silu(W_1 x + b_1) ...) + b_L, weights row-major
+// [fan_in x fan_out].  Master weights and Adam moments are FP32; the GEMMs
+// read BF16 copies and accumulate in FP32 with v_mfma_f32_32x32x16_bf16.
+//
+// One GEMM kernel serves the forward pass (X W), the weight gradients
+// (X^T G) and the input gradients (G W^T): the operands can be read
+// transposed while they are staged in LDS, and the epilogue adds the bias,
+// keeps the pre-activation and applies SiLU.  A training step is
+// gather -> forward -> MSE gradient -> backward -> Adam, all on one stream.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lcurve_amd.h"
+#include "lc_host.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct MlpError : std::runtime_error {
+    int code;
+    MlpError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define MLPCHK(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) throw MlpError(LC_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------ numerics
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// round to nearest even (NaN kept quiet)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
+__device__ __forceinline__ float silu_grad(float z) {
+    const float s = sigmoid(z);
+    return s * (1.f + z * (1.f - s));
+}
+
+// ------------------------------------------------------------ GEMM
+// C[M,N] = op(A)[M,K] op(B)[K,N].  A is stored [M,K] (TA false) or [K,M]
+// (TA true) with leading dimension lda; B is stored [K,N] (TB false) or
+// [N,K] (TB true).  Workgroup tile 64x64, four waves of 32x32, K staged in
+// steps of 32 through LDS as A[m][k] and B[n][k] so that every MFMA operand
+// is one 16-byte LDS read.  Out-of-range elements are staged as zeros.
+constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;
+
+enum { EPI_BIAS = 1, EPI_SILU = 2 };
+
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
+                                              const uint16_t *__restrict__ B, int ldb, const float *__restrict__ bias,
+                                              float *__restrict__ Z, float *__restrict__ Cf,
+                                              uint16_t *__restrict__ Cb, int ldc) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[TM * LDK];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[TN * LDK];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0.f;
+
+    for (int k0 = 0; k0 < K; k0 += TK) {
+        if (!TA) {  // row m0 + r, columns k0 + c .. + 8
+            const int r = t >> 2, c = (t & 3) * 8, m = m0 + r;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int k = k0 + c + i;
+                const uint16_t v = (m < M && k < K) ? A[(size_t)m * lda + k] : (uint16_t)0;
+                As[r * LDK + c + i] = __builtin_bit_cast(__bf16, v);
+            }
+        } else {    // stored [K][M]: row k0 + kr, columns m0 + c .. + 8
+            const int kr = t >> 3, c = (t & 7) * 8, k = k0 + kr;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int m = m0 + c + i;
+                const uint16_t v = (m < M && k < K) ? A[(size_t)k * lda + m] : (uint16_t)0;
+                As[(c + i) * LDK + kr] = __builtin_bit_cast(__bf16, v);
+            }
+        }
+        if (!TB) {  // stored [K][N]: row k0 + kr, columns n0 + c .. + 8
+            const int kr = t >> 3, c = (t & 7) * 8, k = k0 + kr;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int n = n0 + c + i;
+                const uint16_t v = (n < N && k < K) ? B[(size_t)k * ldb + n] : (uint16_t)0;
+                Bs[(c + i) * LDK + kr] = __builtin_bit_cast(__bf16, v);
+            }
+        } else {    // stored [N][K]: row n0 + r, columns k0 + c .. + 8
+            const int r = t >> 2, c = (t & 3) * 8, n = n0 + r;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int k = k0 + c + i;
+                const uint16_t v = (n < N && k < K) ? B[(size_t)n * ldb + k] : (uint16_t)0;
+                Bs[r * LDK + c + i] = __builtin_bit_cast(__bf16, v);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TK; kk += 16) {
+            // lane l: row (l % 32), k block 8 * (l / 32) of this 16-wide step
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[(wm + (lane & 31)) * LDK + kk + 8 * (lane >> 5)]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Bs[(wn + (lane & 31)) * LDK + kk + 8 * (lane >> 5)]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // accumulator r of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
+    const int j = n0 + wn + (lane & 31);
+    if (j >= N) return;
+    const float bj = (EPI & EPI_BIAS) ? bias[j] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int i = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (i >= M) continue;
+        const float z = acc[r] + bj;
+        const size_t o = (size_t)i * ldc + j;
+        if (Z) Z[o] = z;
+        const float a = (EPI & EPI_SILU) ? silu(z) : z;
+        if (Cf) Cf[o] = a;
+        if (Cb) Cb[o] = f2bf(a);
+    }
+}
+
+// ------------------------------------------------------------ elementwise
+__global__ void k_cast_bf16(size_t n, const float *__restrict__ x, uint16_t *__restrict__ y) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] = f2bf(x[i]);
+}
+
+// dZ = dA * silu'(Z)
+__global__ void k_silu_bwd(size_t n, const float *__restrict__ dA, const float *__restrict__ Z, float *__restrict__ dZ,
+                           uint16_t *__restrict__ dZb) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float g = dA[i] * silu_grad(Z[i]);
+        dZ[i] = g;
+        dZb[i] = f2bf(g);
+    }
+}
+
+// db[j] = sum_i G[i][j]
+__global__ void k_colsum(int M, int N, const float *__restrict__ G, float *__restrict__ db) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    float s = 0.f;
+    for (int i = 0; i < M; i++) s += G[(size_t)i * N + j];
+    db[j] = s;
+}
+
+// MSE over M x N: dY = 2 (Y - T) / (M N); sum of squares into *loss
+__global__ __launch_bounds__(256) void k_mse(size_t n, float scale, const float *__restrict__ Y,
+                                             const float *__restrict__ T, float *__restrict__ dY,
+                                             uint16_t *__restrict__ dYb, double *__restrict__ loss) {
+    __shared__ double part[256];
+    double s = 0.;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float d = Y[i] - T[i];
+        s += (double)d * d;
+        if (dY) {
+            const float g = scale * d;
+            dY[i] = g;
+            dYb[i] = f2bf(g);
+        }
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, part[0]);
+}
+
+// Adam (bias-corrected) on FP32 masters, refreshing the BF16 copies
+__global__ void k_adam(size_t n, float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, uint16_t *__restrict__ wb, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        if (wb) wb[i] = f2bf(wi);
+    }
+}
+
+// Minibatch rows of the device data set: idx from a stateless hash of
+// (seed, step, row), so no host round trip per step.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+__global__ void k_gather(int B, int nin, int nout, size_t nrows, uint64_t seed, uint64_t step,
+                         const float *__restrict__ X, const float *__restrict__ T, uint16_t *__restrict__ Xb,
+                         float *__restrict__ Tb) {
+    const int r = blockIdx.x;
+    if (r >= B) return;
+    const size_t src = (size_t)(mix64(seed ^ mix64(step * 0x100000001b3ull + (uint64_t)r)) % nrows);
+    for (int c = threadIdx.x; c < nin; c += blockDim.x) Xb[(size_t)r * nin + c] = f2bf(X[src * nin + c]);
+    for (int c = threadIdx.x; c < nout; c += blockDim.x) Tb[(size_t)r * nout + c] = T[src * nout + c];
+}
+
+// chi**2 of each row's predicted curve against the data, with the optimal
+// scale factor s = sum w f m / sum w m^2 (re_scale.cpp:16-43):
+// chi**2 = sum w f^2 - (sum w f m)^2 / sum w m^2.  The network's output row
+// is the standardised curve at nbin phases k / nbin: v = y * scale + mu is
+// the log flux (log_output) or the relative flux itself; the model flux at
+// a data time is the linear interpolation of the flux at the phase
+// (t - t0) / period (t0 per row: an exact shift).  One workgroup per row.
+__global__ __launch_bounds__(256) void k_sur_chisq(int nbin, int log_output, const float *__restrict__ y,
+                                                   const float *__restrict__ mu,
+                                                   float scale, const double *__restrict__ t0, double period, int n,
+                                                   const double *__restrict__ time, const double *__restrict__ flux,
+                                                   const double *__restrict__ w, double *__restrict__ chisq) {
+    __shared__ double red[3][256];
+    const int row = blockIdx.x;
+    const float *yr = y + (size_t)row * nbin;
+    const double tz = t0[row];
+    double sfm = 0., smm = 0., sff = 0.;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        double ph = (time[k] - tz) / period;
+        ph -= floor(ph);
+        const double x = ph * nbin;
+        int i0 = (int)floor(x);
+        const double f = x - i0;
+        i0 = i0 >= nbin ? i0 - nbin : i0;
+        const int i1 = i0 + 1 == nbin ? 0 : i0 + 1;
+        const double v0 = (double)yr[i0] * scale + mu[i0], v1 = (double)yr[i1] * scale + mu[i1];
+        const double m0 = log_output ? exp(v0) : v0, m1 = log_output ? exp(v1) : v1;
+        const double mk = m0 * (1.0 - f) + m1 * f;
+        const double wk = w[k], fk = flux[k];
+        sfm += wk * fk * mk;
+        smm += wk * mk * mk;
+        sff += wk * fk * fk;
+    }
+    red[0][threadIdx.x] = sfm;
+    red[1][threadIdx.x] = smm;
+    red[2][threadIdx.x] = sff;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) chisq[row] = red[1][0] > 0. ? red[2][0] - red[0][0] * red[0][0] / red[1][0] : INFINITY;
+}
+
+// y[i][j] += sum_k x[i][k] S[k][j]: the FP32 linear skip of inference
+__global__ void k_skip_add(int B, int nin, int nout, const float *__restrict__ x, const float *__restrict__ S,
+                           float *__restrict__ y) {
+    const size_t n = (size_t)B * nout;
+    for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n; o += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = o / nout, j = o % nout;
+        float a = y[o];
+        for (int k = 0; k < nin; k++) a = fmaf(x[i * nin + k], S[(size_t)k * nout + j], a);
+        y[o] = a;
+    }
+}
+
+inline int nblocks(size_t n, int bs = 256, int cap = 4096) {
+    return (int)std::max<size_t>(1, std::min<size_t>((n + bs - 1) / bs, (size_t)cap));
+}
+
+template <bool TA, bool TB, int EPI>
+void gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
+          const float *bias, float *Z, float *Cf, uint16_t *Cb, int ldc) {
+    if (M <= 0 || N <= 0) return;
+    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, bias, Z, Cf, Cb, ldc);
+}
+
+}  // namespace
+
+// ================================================================== MLP
+struct lc_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<int> width;     // width[0] inputs ... width[L] outputs
+    int L = 0, max_batch = 0;
+    size_t nparam = 0;
+    std::vector<size_t> woff, boff;  // offsets of W_l and b_l in the parameter vector
+    // device: parameters and optimiser state (one block each, same layout)
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    uint16_t *pb = nullptr;     // BF16 copy of the parameters
+    // activations for max_batch rows
+    uint16_t *xb = nullptr;                  // inputs, BF16
+    float *x32 = nullptr;                    // inputs, FP32 (for the skip)
+    float *skip = nullptr;                   // optional FP32 linear skip [inputs x outputs]
+    std::vector<float *> z;                  // pre-activations per layer (1..L)
+    std::vector<uint16_t *> ab;              // BF16 activations per layer (1..L-1)
+    float *y = nullptr, *t = nullptr;        // output and target
+    float *ga = nullptr, *gz = nullptr;      // gradient scratch (fp32)
+    uint16_t *gzb = nullptr;                 // gradient, BF16
+    double *loss = nullptr;
+    // data set on the device
+    float *dx = nullptr, *dt = nullptr;
+    // likelihood for lc_mlp_chisq: data and output de-standardisation
+    double *lt = nullptr, *lf = nullptr, *lw = nullptr, *lt0 = nullptr, *lchi = nullptr;
+    float *lmu = nullptr;
+    float lscale = 1.f;
+    double lperiod = 1.;
+    int llog = 1;
+    int ln = 0;
+    std::vector<void *> lallocs;
+    size_t nrows = 0;
+    long long adam_t = 0;
+    double gemm_flop = 0., train_ms = 0.;
+    std::vector<void *> allocs;
+
+    template <class T> T *alloc(size_t n) {
+        void *q = nullptr;
+        MLPCHK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+        allocs.push_back(q);
+        return static_cast<T *>(q);
+    }
+    ~lc_mlp() {
+        for (void *q : allocs) (void)hipFree(q);
+        for (void *q : lallocs) (void)hipFree(q);
+        if (skip) (void)hipFree(skip);
+        if (dx) (void)hipFree(dx);
+        if (dt) (void)hipFree(dt);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    float *W(int l) { return p + woff[l]; }
+    uint16_t *Wb(int l) { return pb + woff[l]; }
+    float *bvec(int l) { return p + boff[l]; }
+
+    // forward of B rows whose BF16 inputs are in xb; output in y.  with_skip:
+    // add the linear skip of the FP32 inputs in x32 (inference only; the
+    // network is trained on what the skip leaves)
+    void forward(int B, bool with_skip = false) {
+        hipStream_t s = stream;
+        const uint16_t *in = xb;
+        for (int l = 1; l <= L; l++) {
+            const int K = width[l - 1], N = width[l];
+            if (l < L)
+                gemm<false, false, EPI_BIAS | EPI_SILU>(s, B, N, K, in, K, Wb(l), N, bvec(l), z[l], nullptr, ab[l], N);
+            else
+                gemm<false, false, EPI_BIAS>(s, B, N, K, in, K, Wb(l), N, bvec(l), nullptr, y, nullptr, N);
+            gemm_flop += 2.0 * B * N * K;
+            in = ab[l];
+        }
+        if (with_skip && skip) {
+            const size_t n = (size_t)B * width[L];
+            hipLaunchKernelGGL(k_skip_add, dim3(nblocks(n)), dim3(256), 0, s, B, width[0], width[L], x32, skip, y);
+        }
+    }
+
+    // gradients of mean squared error over B rows (targets in t) into g
+    void backward(int B) {
+        hipStream_t s = stream;
+        const int NL = width[L];
+        const size_t n = (size_t)B * NL;
+        MLPCHK(hipMemsetAsync(loss, 0, sizeof(double), s));
+        hipLaunchKernelGGL(k_mse, dim3(nblocks(n)), dim3(256), 0, s, n, 2.f / (float)n, y, t, gz, gzb, loss);
+        for (int l = L; l >= 1; l--) {
+            const int K = width[l - 1], N = width[l];
+            const uint16_t *in = l == 1 ? xb : ab[l - 1];
+            // dW_l = A_{l-1}^T G_l   (M = K, N = N, K = B)
+            gemm<true, false, 0>(s, K, N, B, in, K, gzb, N, nullptr, nullptr, g + woff[l], nullptr, N);
+            hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256), dim3(256), 0, s, B, N, gz, g + boff[l]);
+            gemm_flop += 2.0 * B * N * K;
+            if (l > 1) {
+                // dA_{l-1} = G_l W_l^T   (M = B, N = K, K = N), then through SiLU
+                gemm<false, true, 0>(s, B, K, N, gzb, N, Wb(l), N, nullptr, nullptr, ga, nullptr, K);
+                gemm_flop += 2.0 * B * N * K;
+                const size_t nk = (size_t)B * K;
+                hipLaunchKernelGGL(k_silu_bwd, dim3(nblocks(nk)), dim3(256), 0, s, nk, ga, z[l - 1], gz, gzb);
+            }
+        }
+    }
+
+    void adam(float lr) {
+        adam_t++;
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        const float c1 = 1.f - (float)std::pow(0.9, (double)adam_t), c2 = 1.f - (float)std::pow(0.999, (double)adam_t);
+        hipLaunchKernelGGL(k_adam, dim3(nblocks(nparam)), dim3(256), 0, stream, nparam, p, g, m, v, pb, lr, b1, b2,
+                           eps, c1, c2);
+    }
+
+    void upload_inputs(const float *x, int B) {
+        const size_t n = (size_t)B * width[0];
+        MLPCHK(hipMemcpyAsync(x32, x, n * sizeof(float), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_cast_bf16, dim3(nblocks(n)), dim3(256), 0, stream, n, x32, xb);
+    }
+};
+
+namespace {
+
+template <class F> lc_status mlp_guarded(lc_mlp *mm, F f) {
+    try {
+        return (lc_status)f();
+    } catch (const MlpError &e) {
+        if (mm) mm->err = e.what();
+        lch::set_thread_error(e.what());
+        return (lc_status)e.code;
+    } catch (const std::exception &e) {
+        if (mm) mm->err = e.what();
+        lch::set_thread_error(e.what());
+        return LC_E_DEVICE;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+lc_status lc_mlp_create(int device, const int32_t *widths, int nwidth, int max_batch, lc_mlp **out) {
+    if (!out || !widths || nwidth < 2 || max_batch <= 0) return LC_E_ARG;
+    *out = nullptr;
+    return mlp_guarded(nullptr, [&]() -> int {
+        for (int i = 0; i < nwidth; i++)
+            if (widths[i] <= 0 || widths[i] > 65536) throw MlpError(LC_E_ARG, "lc_mlp_create: bad layer width");
+        int ndev = 0;
+        MLPCHK(hipGetDeviceCount(&ndev));
+        if (ndev <= 0) throw MlpError(LC_E_DEVICE, "no HIP device available");
+        if (device < 0 || device >= ndev) device = 0;
+        MLPCHK(hipSetDevice(device));
+        std::unique_ptr<lc_mlp> mm(new lc_mlp());
+        mm->device = device;
+        MLPCHK(hipStreamCreateWithFlags(&mm->stream, hipStreamNonBlocking));
+        mm->width.assign(widths, widths + nwidth);
+        mm->L = nwidth - 1;
+        mm->max_batch = max_batch;
+        mm->woff.assign(nwidth, 0);
+        mm->boff.assign(nwidth, 0);
+        size_t off = 0;
+        int wmax = 0;
+        for (int l = 1; l <= mm->L; l++) {
+            mm->woff[l] = off;
+            off += (size_t)widths[l - 1] * widths[l];
+            mm->boff[l] = off;
+            off += widths[l];
+        }
+        for (int i = 0; i < nwidth; i++) wmax = std::max(wmax, widths[i]);
+        mm->nparam = off;
+        mm->p = mm->alloc<float>(off);
+        mm->g = mm->alloc<float>(off);
+        mm->m = mm->alloc<float>(off);
+        mm->v = mm->alloc<float>(off);
+        mm->pb = mm->alloc<uint16_t>(off);
+        MLPCHK(hipMemset(mm->p, 0, off * sizeof(float)));
+        MLPCHK(hipMemset(mm->g, 0, off * sizeof(float)));
+        MLPCHK(hipMemset(mm->m, 0, off * sizeof(float)));
+        MLPCHK(hipMemset(mm->v, 0, off * sizeof(float)));
+        MLPCHK(hipMemset(mm->pb, 0, off * sizeof(uint16_t)));
+        const size_t B = (size_t)max_batch;
+        mm->xb = mm->alloc<uint16_t>(B * std::max(widths[0], 1));
+        mm->x32 = mm->alloc<float>(B * std::max(widths[0], 1));
+        mm->z.assign(nwidth, nullptr);
+        mm->ab.assign(nwidth, nullptr);
+        for (int l = 1; l <= mm->L; l++) {
+            mm->z[l] = mm->alloc<float>(B * widths[l]);
+            if (l < mm->L) mm->ab[l] = mm->alloc<uint16_t>(B * widths[l]);
+        }
+        mm->y = mm->alloc<float>(B * widths[mm->L]);
+        mm->t = mm->alloc<float>(B * widths[mm->L]);
+        mm->ga = mm->alloc<float>(B * wmax);
+        mm->gz = mm->alloc<float>(B * wmax);
+        mm->gzb = mm->alloc<uint16_t>(B * wmax);
+        mm->loss = mm->alloc<double>(1);
+        *out = mm.release();
+        return LC_OK;
+    });
+}
+
+void lc_mlp_destroy(lc_mlp *mm) {
+    if (!mm) return;
+    (void)hipSetDevice(mm->device);
+    if (mm->stream) (void)hipStreamSynchronize(mm->stream);
+    delete mm;
+}
+
+const char *lc_mlp_last_error(const lc_mlp *mm) { return mm ? mm->err.c_str() : "null mlp"; }
+
+size_t lc_mlp_num_params(const lc_mlp *mm) { return mm ? mm->nparam : 0; }
+
+lc_status lc_mlp_set_params(lc_mlp *mm, const float *params, size_t n) {
+    if (!mm || !params) return LC_E_ARG;
+    return mlp_guarded(mm, [&]() -> int {
+        if (n != mm->nparam) throw MlpError(LC_E_ARG, "lc_mlp_set_params: wrong parameter count");
+        MLPCHK(hipSetDevice(mm->device));
+        MLPCHK(hipMemcpyAsync(mm->p, params, n * sizeof(float), hipMemcpyHostToDevice, mm->stream));
+        MLPCHK(hipMemsetAsync(mm->m, 0, n * sizeof(float), mm->stream));
+        MLPCHK(hipMemsetAsync(mm->v, 0, n * sizeof(float), mm->stream));
+        hipLaunchKernelGGL(k_cast_bf16, dim3(nblocks(n)), dim3(256), 0, mm->stream, n, mm->p, mm->pb);
+        MLPCHK(hipStreamSynchronize(mm->stream));
+        mm->adam_t = 0;
+        return LC_OK;
+    });
+}
+
+lc_status lc_mlp_get_params(lc_mlp *mm, float *params, size_t n) {
+    if (!mm || !params) return LC_E_ARG;
+    return mlp_guarded(mm, [&]() -> int {
+        if (n != mm->nparam) throw MlpError(LC_E_ARG, "lc_mlp_get_params: wrong parameter count");
+        MLPCHK(hipSetDevice(mm->device));
+        MLPCHK(hipMemcpyAsync(params, mm->p, n * sizeof(float), hipMemcpyDeviceToHost, mm->stream));
+        MLPCHK(hipStreamSynchronize(mm->stream));
+        return LC_OK;
+    });
+}
+
+lc_status lc_mlp_forward(lc_mlp *mm, const float *x, size_t batch, float *y) {
+    if (!mm || !x || !y) return LC_E_ARG;
+    return mlp_guarded(mm, [&]() -> int {
+        MLPCHK(hipSetDevice(mm->device));
+        const int nin = mm->width[0], nout = mm->width[mm->L];
+        for (size_t r0 = 0; r0 < batch; r0 += (size_t)mm->max_batch) {
+            const int B = (int)std::min<size_t>(mm->max_batch, batch - r0);
+            mm->upload_inputs(x + r0 * nin, B);
+            mm->forward(B, true);
+            MLPCHK(hipMemcpyAsync(y + r0 * nout, mm->y, (size_t)B * nout * sizeof(float), hipMemcpyDeviceToHost,
+                                  mm->stream));
+        }
+        MLPCHK(hipGetLastError());
+        MLPCHK(hipStreamSynchronize(mm->stream));
+        return LC_OK;
+    });
+}
+
+lc_status lc_mlp_grad(lc_mlp *mm, const float *x, const float *target, size_t batch, float *grad, double *loss) {
+    if (!mm || !x || !target || !grad || !loss) return LC_E_ARG;
+    return mlp_guarded(mm, [&]() -> int {
+        if (batch == 0 || batch > (size_t)mm->max_batch) throw MlpError(LC_E_ARG, "lc_mlp_grad: batch out of range");
+        MLPCHK(hipSetDevice(mm->device));
+        const int B = (int)batch, nout = mm->width[mm->L];
+        mm->upload_inputs(x, B);
+        MLPCHK(hipMemcpyAsync(mm->t, target, (size_t)B * nout * sizeof(float), hipMemcpyHostToDevice, mm->stream));
+        mm->forward(B);
+        mm->backward(B);
+        MLPCHK(hipGetLastError());
+        MLPCHK(hipMemcpyAsync(grad, mm->g, mm->nparam * sizeof(float), hipMemcpyDeviceToHost, mm->stream));
+        MLPCHK(hipMemcpyAsync(loss, mm->loss, sizeof(double), hipMemcpyDeviceToHost, mm->stream));
+        MLPCHK(hipStreamSynchronize(mm->stream));
+        *loss /= (double)B * nout;
+        return LC_OK;
+    });
+}
+
+lc_status lc_mlp_set_dataset(lc_mlp *mm, size_t nrows) {
+    if (!mm) return LC_E_ARG;
+    return mlp_guarded(mm, [&]() -> int {
+        MLPCHK(hipSetDevice(mm->device));
+        MLPCHK(hipStreamSynchronize(mm->stream));
+        if (mm->dx) MLPCHK(hipFree(mm->dx));
+        if (mm->dt) MLPCHK(hipFree(mm->dt));
+        mm->dx = mm->dt = nullptr;
+        mm->nrows = 0;
+        if (nrows) {
+            MLPCHK(hipMalloc(&mm->dx, nrows * mm->width[0] * sizeof(float)));
+            MLPCHK(hipMalloc(&mm->dt, nrows * mm->width[mm->L] * sizeof(float)));
+        }
+        mm->nrows = nrows;
+        return LC_OK;
+    });
+}
+
+lc_status lc_mlp_load_dataset(lc_mlp *mm, size_t row0, const float *x, const float *target, size_t count) {
+    if (!mm || !x || !target) return LC_E_ARG;
+    return mlp_guarded(mm, [&]() -> int {
+        if (row0 + count > mm->nrows) throw MlpError(LC_E_ARG, "lc_mlp_load_dataset: rows beyond the data set");
+        MLPCHK(hipSetDevice(mm->device));
+        const size_t nin = mm->width[0], nout = mm->width[mm->L];
+        MLPCHK(hipMemcpyAsync(mm->dx + row0 * nin, x, count * nin * sizeof(float), hipMemcpyHostToDevice, mm->stream));
+        MLPCHK(hipMemcpyAsync(mm->dt + row0 * nout, target, count * nout * sizeof(float), hipMemcpyHostToDevice,
+                              mm->stream));
+        MLPCHK(hipStreamSynchronize(mm->stream));
+        return LC_OK;
+    });
+}
+
+lc_status lc_mlp_train(lc_mlp *mm, int steps, int batch, float lr, uint64_t seed, double *last_loss) {
+    if (!mm || !last_loss) return LC_E_ARG;
+    return mlp_guarded(mm, [&]() -> int {
+        if (!mm->nrows) throw MlpError(LC_E_ARG, "lc_mlp_train: no data set");
+        if (batch <= 0 || batch > mm->max_batch) throw MlpError(LC_E_ARG, "lc_mlp_train: batch out of range");
+        MLPCHK(hipSetDevice(mm->device));
+        const auto t0 = std::chrono::steady_clock::now();
+        const int nin = mm->width[0], nout = mm->width[mm->L];
+        for (int s = 0; s < steps; s++) {
+            hipLaunchKernelGGL(k_gather, dim3(batch), dim3(256), 0, mm->stream, batch, nin, nout, mm->nrows, seed,
+                               (uint64_t)(mm->adam_t + 1), mm->dx, mm->dt, mm->xb, mm->t);
+            mm->forward(batch);
+            mm->backward(batch);
+            mm->adam(lr);
+        }
+        MLPCHK(hipGetLastError());
+        MLPCHK(hipMemcpyAsync(last_loss, mm->loss, sizeof(double), hipMemcpyDeviceToHost, mm->stream));
+        MLPCHK(hipStreamSynchronize(mm->stream));
+        *last_loss /= (double)batch * nout;
+        mm->train_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return LC_OK;
+    });
+}
+
+lc_status lc_mlp_eval_dataset(lc_mlp *mm, size_t row0, size_t count, double *mse) {
+    if (!mm || !mse) return LC_E_ARG;
+    return mlp_guarded(mm, [&]() -> int {
+        if (row0 + count > mm->nrows) throw MlpError(LC_E_ARG, "lc_mlp_eval_dataset: rows beyond the data set");
+        MLPCHK(hipSetDevice(mm->device));
+        const int nin = mm->width[0], nout = mm->width[mm->L];
+        double total = 0.;
+        for (size_t r0 = row0; r0 < row0 + count; r0 += (size_t)mm->max_batch) {
+            const int B = (int)std::min<size_t>(mm->max_batch, row0 + count - r0);
+            const size_t n = (size_t)B * nin;
+            hipLaunchKernelGGL(k_cast_bf16, dim3(nblocks(n)), dim3(256), 0, mm->stream, n, mm->dx + r0 * nin, mm->xb);
+            mm->forward(B);
+            const size_t no = (size_t)B * nout;
+            MLPCHK(hipMemsetAsync(mm->loss, 0, sizeof(double), mm->stream));
+            hipLaunchKernelGGL(k_mse, dim3(nblocks(no)), dim3(256), 0, mm->stream, no, 0.f, mm->y,
+                               mm->dt + r0 * nout, nullptr, nullptr, mm->loss);
+            double l = 0.;
+            MLPCHK(hipMemcpyAsync(&l, mm->loss, sizeof(double), hipMemcpyDeviceToHost, mm->stream));
+            MLPCHK(hipStreamSynchronize(mm->stream));
+            total += l;
+        }
+        *mse = total / ((double)count * nout);
+        return LC_OK;
+    });
+}
+
+lc_status lc_mlp_set_likelihood(lc_mlp *mm, const double *time, const double *flux, const double *weight, size_t n,
+                                const float *mu, float scale, double period, int log_output) {
+    if (!mm || !time || !flux || !weight || !mu || n == 0) return LC_E_ARG;
+    return mlp_guarded(mm, [&]() -> int {
+        MLPCHK(hipSetDevice(mm->device));
+        MLPCHK(hipStreamSynchronize(mm->stream));
+        for (void *q : mm->lallocs) MLPCHK(hipFree(q));
+        mm->lallocs.clear();
+        auto take = [&](size_t bytes) {
+            void *q = nullptr;
+            MLPCHK(hipMalloc(&q, bytes));
+            mm->lallocs.push_back(q);
+            return q;
+        };
+        const int nbin = mm->width[mm->L];
+        mm->lt = static_cast<double *>(take(n * 8));
+        mm->lf = static_cast<double *>(take(n * 8));
+        mm->lw = static_cast<double *>(take(n * 8));
+        mm->lmu = static_cast<float *>(take((size_t)nbin * 4));
+        mm->lt0 = static_cast<double *>(take((size_t)mm->max_batch * 8));
+        mm->lchi = static_cast<double *>(take((size_t)mm->max_batch * 8));
+        MLPCHK(hipMemcpy(mm->lt, time, n * 8, hipMemcpyHostToDevice));
+        MLPCHK(hipMemcpy(mm->lf, flux, n * 8, hipMemcpyHostToDevice));
+        MLPCHK(hipMemcpy(mm->lw, weight, n * 8, hipMemcpyHostToDevice));
+        MLPCHK(hipMemcpy(mm->lmu, mu, (size_t)nbin * 4, hipMemcpyHostToDevice));
+        mm->ln = (int)n;
+        mm->lscale = scale;
+        mm->lperiod = period;
+        mm->llog = log_output ? 1 : 0;
+        return LC_OK;
+    });
+}
+
+lc_status lc_mlp_chisq(lc_mlp *mm, const float *x, const double *t0, size_t m, double *chisq) {
+    if (!mm || !x || !t0 || !chisq) return LC_E_ARG;
+    return mlp_guarded(mm, [&]() -> int {
+        if (!mm->ln) throw MlpError(LC_E_ARG, "lc_mlp_chisq: no likelihood set");
+        MLPCHK(hipSetDevice(mm->device));
+        const int nin = mm->width[0], nbin = mm->width[mm->L];
+        for (size_t r0 = 0; r0 < m; r0 += (size_t)mm->max_batch) {
+            const int B = (int)std::min<size_t>(mm->max_batch, m - r0);
+            mm->upload_inputs(x + r0 * nin, B);
+            MLPCHK(hipMemcpyAsync(mm->lt0, t0 + r0, (size_t)B * 8, hipMemcpyHostToDevice, mm->stream));
+            mm->forward(B, true);
+            hipLaunchKernelGGL(k_sur_chisq, dim3(B), dim3(256), 0, mm->stream, nbin, mm->llog, mm->y, mm->lmu, mm->lscale,
+                               mm->lt0, mm->lperiod, mm->ln, mm->lt, mm->lf, mm->lw, mm->lchi);
+            MLPCHK(hipMemcpyAsync(chisq + r0, mm->lchi, (size_t)B * 8, hipMemcpyDeviceToHost, mm->stream));
+        }
+        MLPCHK(hipGetLastError());
+        MLPCHK(hipStreamSynchronize(mm->stream));
+        return LC_OK;
+    });
+}
+
+lc_status lc_mlp_set_skip(lc_mlp *mm, const float *S, size_t n) {
+    if (!mm) return LC_E_ARG;
+    return mlp_guarded(mm, [&]() -> int {
+        MLPCHK(hipSetDevice(mm->device));
+        MLPCHK(hipStreamSynchronize(mm->stream));
+        if (!S) {
+            if (mm->skip) MLPCHK(hipFree(mm->skip));
+            mm->skip = nullptr;
+            return LC_OK;
+        }
+        const size_t want = (size_t)mm->width[0] * mm->width[mm->L];
+        if (n != want) throw MlpError(LC_E_ARG, "lc_mlp_set_skip: size must be inputs x outputs");
+        if (!mm->skip) MLPCHK(hipMalloc(&mm->skip, want * sizeof(float)));
+        MLPCHK(hipMemcpy(mm->skip, S, want * sizeof(float), hipMemcpyHostToDevice));
+        return LC_OK;
+    });
+}
+
+lc_status lc_mlp_stats(const lc_mlp *mm, double *gemm_flop, double *train_ms) {
+    if (!mm || !gemm_flop || !train_ms) return LC_E_ARG;
+    *gemm_flop = mm->gemm_flop;
+    *train_ms = mm->train_ms;
+    return LC_OK;
+}
+
+}  // extern "C"
