@@ -283,7 +283,16 @@ def run_surrogate_mcmc(L, eng, args):
     sur, linfo = S.train_local_surrogate(eng, base, samples, cov, args.local_curves, args.refine_steps,
                                          seed=31, log=log)
     info.update(linfo)
-    T, cal = S.calibrate_temperature(post, sur, x0, cov, 512, seed=5)
+    T_cal, cal = S.calibrate_temperature(post, sur, x0, cov, 512, seed=5)
+    # stage-1 temperature: short pilot runs (not reported) at a few values,
+    # keeping the one with the most accepted moves per second
+    pilot = {}
+    for Tc in sorted({1.0, 2.0, 4.0, round(T_cal, 3)}):
+        _, stp = S.run_chains(post.loglike, post.legal, x0, cov, 60, seed=41,
+                              sur_loglike=post.surrogate_loglike(sur, Tc), n_sub=1)
+        pilot[Tc] = stp["accepted"] / stp["wall_s"]
+    T = max(pilot, key=pilot.get)
+    cal["pilot_accepted_per_s"] = {str(k): v for k, v in pilot.items()}
     log(f"bench: local surrogate {linfo}; stage-1 temperature {T:.3g} {cal}")
     n_iter = max(args.mcmc_steps // nw, 20)
     burn = n_iter // 10

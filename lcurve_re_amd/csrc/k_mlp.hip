@@ -64,61 +64,96 @@ __device__ __forceinline__ float silu_grad(float z) {
 // (TA true) with leading dimension lda; B is stored [K,N] (TB false) or
 // [N,K] (TB true).  Workgroup tile 64x64, four waves of 32x32, K staged in
 // steps of 32 through LDS as A[m][k] and B[n][k] so that every MFMA operand
-// is one 16-byte LDS read.  Out-of-range elements are staged as zeros.
+// is one 16-byte LDS read.  Each thread moves 8 consecutive elements of A
+// and of B per step: one 16-byte load when they are in range and aligned
+// (vec), element loads with zero fill at the edges.  The next step's
+// elements are loaded into registers while the current step's MFMAs run.
+// blockIdx.z splits K (kchunk per split): split s writes its partial
+// product to Cf + s * M * ldc, summed afterwards by k_sum_splits.
 constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;
 
 enum { EPI_BIAS = 1, EPI_SILU = 2 };
 
+struct Seg {
+    uint4 v;  // 8 bf16
+};
+
+// 8 consecutive elements p[0..7] (valid: how many are in range)
+__device__ __forceinline__ Seg load8(const uint16_t *p, int valid, bool vec) {
+    Seg s;
+    if (vec && valid >= 8) {
+        s.v = *reinterpret_cast<const uint4 *>(p);
+    } else {
+        uint16_t e[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) e[i] = i < valid ? p[i] : (uint16_t)0;
+        s.v.x = e[0] | ((uint32_t)e[1] << 16);
+        s.v.y = e[2] | ((uint32_t)e[3] << 16);
+        s.v.z = e[4] | ((uint32_t)e[5] << 16);
+        s.v.w = e[6] | ((uint32_t)e[7] << 16);
+    }
+    return s;
+}
+
+__device__ __forceinline__ uint16_t seg_at(const Seg &s, int i) {
+    const uint32_t w = i < 2 ? s.v.x : i < 4 ? s.v.y : i < 6 ? s.v.z : s.v.w;
+    return (uint16_t)(w >> (16 * (i & 1)));
+}
+
 template <bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
-                                              const uint16_t *__restrict__ B, int ldb, const float *__restrict__ bias,
-                                              float *__restrict__ Z, float *__restrict__ Cf,
-                                              uint16_t *__restrict__ Cb, int ldc) {
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, const uint16_t *__restrict__ A, int lda,
+                                              bool avec, const uint16_t *__restrict__ B, int ldb, bool bvec,
+                                              const float *__restrict__ bias, float *__restrict__ Z,
+                                              float *__restrict__ Cf, uint16_t *__restrict__ Cb, int ldc) {
     __shared__ __attribute__((aligned(16))) __bf16 As[TM * LDK];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[TN * LDK];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = 0.f;
 
-    for (int k0 = 0; k0 < K; k0 += TK) {
-        if (!TA) {  // row m0 + r, columns k0 + c .. + 8
-            const int r = t >> 2, c = (t & 3) * 8, m = m0 + r;
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const int k = k0 + c + i;
-                const uint16_t v = (m < M && k < K) ? A[(size_t)m * lda + k] : (uint16_t)0;
-                As[r * LDK + c + i] = __builtin_bit_cast(__bf16, v);
-            }
-        } else {    // stored [K][M]: row k0 + kr, columns m0 + c .. + 8
-            const int kr = t >> 3, c = (t & 7) * 8, k = k0 + kr;
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const int m = m0 + c + i;
-                const uint16_t v = (m < M && k < K) ? A[(size_t)k * lda + m] : (uint16_t)0;
-                As[(c + i) * LDK + kr] = __builtin_bit_cast(__bf16, v);
-            }
+    // this thread's segments: (row, first column) in the stored layouts
+    const int ar = TA ? (t >> 3) : (t >> 2), ac = TA ? (t & 7) * 8 : (t & 3) * 8;
+    const int br = TB ? (t >> 2) : (t >> 3), bc = TB ? (t & 3) * 8 : (t & 7) * 8;
+    auto fetch = [&](int k0, Seg &sa, Seg &sb) {
+        if (!TA) {  // row m0 + ar, columns k0 + ac .. + 8
+            const int m = m0 + ar, k = k0 + ac;
+            sa = load8(A + (size_t)m * lda + k, m < M ? min(8, ke - k) : 0, avec);
+        } else {    // row k0 + ar of [K][M], columns m0 + ac .. + 8
+            const int k = k0 + ar, m = m0 + ac;
+            sa = load8(A + (size_t)k * lda + m, k < ke ? min(8, M - m) : 0, avec);
         }
-        if (!TB) {  // stored [K][N]: row k0 + kr, columns n0 + c .. + 8
-            const int kr = t >> 3, c = (t & 7) * 8, k = k0 + kr;
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const int n = n0 + c + i;
-                const uint16_t v = (n < N && k < K) ? B[(size_t)k * ldb + n] : (uint16_t)0;
-                Bs[(c + i) * LDK + kr] = __builtin_bit_cast(__bf16, v);
-            }
-        } else {    // stored [N][K]: row n0 + r, columns k0 + c .. + 8
-            const int r = t >> 2, c = (t & 3) * 8, n = n0 + r;
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const int k = k0 + c + i;
-                const uint16_t v = (n < N && k < K) ? B[(size_t)n * ldb + k] : (uint16_t)0;
-                Bs[r * LDK + c + i] = __builtin_bit_cast(__bf16, v);
-            }
+        if (!TB) {  // row k0 + br of [K][N], columns n0 + bc .. + 8
+            const int k = k0 + br, n = n0 + bc;
+            sb = load8(B + (size_t)k * ldb + n, k < ke ? min(8, N - n) : 0, bvec);
+        } else {    // row n0 + br of [N][K], columns k0 + bc .. + 8
+            const int n = n0 + br, k = k0 + bc;
+            sb = load8(B + (size_t)n * ldb + k, n < N ? min(8, ke - k) : 0, bvec);
         }
+    };
+    auto stage = [&](const Seg &sa, const Seg &sb) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            if (!TA)
+                As[ar * LDK + ac + i] = __builtin_bit_cast(__bf16, seg_at(sa, i));
+            else
+                As[(ac + i) * LDK + ar] = __builtin_bit_cast(__bf16, seg_at(sa, i));
+            if (!TB)
+                Bs[(bc + i) * LDK + br] = __builtin_bit_cast(__bf16, seg_at(sb, i));
+            else
+                Bs[br * LDK + bc + i] = __builtin_bit_cast(__bf16, seg_at(sb, i));
+        }
+    };
+
+    Seg sa, sb;
+    if (kb < ke) fetch(kb, sa, sb);
+    for (int k0 = kb; k0 < ke; k0 += TK) {
+        stage(sa, sb);
         __syncthreads();
+        if (k0 + TK < ke) fetch(k0 + TK, sa, sb);  // in flight during the MFMAs
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 16) {
             // lane l: row (l % 32), k block 8 * (l / 32) of this 16-wide step
@@ -132,6 +167,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_
     const int j = n0 + wn + (lane & 31);
     if (j >= N) return;
     const float bj = (EPI & EPI_BIAS) ? bias[j] : 0.f;
+    float *Cs = Cf ? Cf + (size_t)blockIdx.z * M * ldc : nullptr;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int i = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -140,8 +176,17 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_
         const size_t o = (size_t)i * ldc + j;
         if (Z) Z[o] = z;
         const float a = (EPI & EPI_SILU) ? silu(z) : z;
-        if (Cf) Cf[o] = a;
+        if (Cs) Cs[o] = a;
         if (Cb) Cb[o] = f2bf(a);
+    }
+}
+
+// out[o] = sum over splits of part[s * n + o] (in split order)
+__global__ void k_sum_splits(size_t n, int splits, const float *__restrict__ part, float *__restrict__ out) {
+    for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n; o += (size_t)gridDim.x * blockDim.x) {
+        float a = 0.f;
+        for (int s = 0; s < splits; s++) a += part[(size_t)s * n + o];
+        out[o] = a;
     }
 }
 
@@ -161,13 +206,16 @@ __global__ void k_silu_bwd(size_t n, const float *__restrict__ dA, const float *
     }
 }
 
-// db[j] = sum_i G[i][j]
-__global__ void k_colsum(int M, int N, const float *__restrict__ G, float *__restrict__ db) {
+// db[j] = sum_i G[i][j] in two deterministic stages: CS_PART row chunks
+// (blockIdx.y) of partial column sums, then their sum in chunk order
+constexpr int CS_PART = 64;
+__global__ void k_colsum_part(int M, int N, const float *__restrict__ G, float *__restrict__ part) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
+    const int rows = (M + CS_PART - 1) / CS_PART, i0 = blockIdx.y * rows, i1 = min(M, i0 + rows);
     float s = 0.f;
-    for (int i = 0; i < M; i++) s += G[(size_t)i * N + j];
-    db[j] = s;
+    for (int i = i0; i < i1; i++) s += G[(size_t)i * N + j];
+    part[(size_t)blockIdx.y * N + j] = s;
 }
 
 // MSE over M x N: dY = 2 (Y - T) / (M N); sum of squares into *loss
@@ -290,12 +338,34 @@ inline int nblocks(size_t n, int bs = 256, int cap = 4096) {
     return (int)std::max<size_t>(1, std::min<size_t>((n + bs - 1) / bs, (size_t)cap));
 }
 
+// K per split (a multiple of the K step) and the number of splits it gives
+inline int split_chunk(int K, int splits) {
+    const int c = (K + std::max(splits, 1) - 1) / std::max(splits, 1);
+    return std::max(TK, (c + TK - 1) / TK * TK);
+}
+inline int split_actual(int K, int splits) { return (K + split_chunk(K, splits) - 1) / split_chunk(K, splits); }
+
+// Launches the GEMM; with splits > 1, Cf must hold splits x M x ldc floats
+// (the partial products).  Returns the number of splits used.
 template <bool TA, bool TB, int EPI>
-void gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
-          const float *bias, float *Z, float *Cf, uint16_t *Cb, int ldc) {
-    if (M <= 0 || N <= 0) return;
-    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
-    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, bias, Z, Cf, Cb, ldc);
+int gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
+         const float *bias, float *Z, float *Cf, uint16_t *Cb, int ldc, int splits = 1) {
+    if (M <= 0 || N <= 0 || K <= 0) return 0;
+    const bool avec = (lda % 8 == 0) && ((uintptr_t)A % 16 == 0);
+    const bool bvec = (ldb % 8 == 0) && ((uintptr_t)B % 16 == 0);
+    const int kchunk = split_chunk(K, splits);
+    const int ns = split_actual(K, splits);
+    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, ns);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, kchunk, A, lda, avec, B, ldb, bvec,
+                       bias, Z, Cf, Cb, ldc);
+    return ns;
+}
+
+// splits of K that give the grid about 512 workgroups (at least 256
+// elements of K per split)
+inline int split_count(int M, int N, int K) {
+    const int tiles = ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
+    return std::max(1, std::min(512 / std::max(tiles, 1), K / 256));
 }
 
 }  // namespace
@@ -320,6 +390,8 @@ struct lc_mlp {
     std::vector<uint16_t *> ab;              // BF16 activations per layer (1..L-1)
     float *y = nullptr, *t = nullptr;        // output and target
     float *ga = nullptr, *gz = nullptr;      // gradient scratch (fp32)
+    float *part = nullptr;                   // split-K partial products of the weight gradients
+    float *cpart = nullptr;                  // partial column sums of the bias gradients
     uint16_t *gzb = nullptr;                 // gradient, BF16
     double *loss = nullptr;
     // data set on the device
@@ -386,9 +458,18 @@ struct lc_mlp {
         for (int l = L; l >= 1; l--) {
             const int K = width[l - 1], N = width[l];
             const uint16_t *in = l == 1 ? xb : ab[l - 1];
-            // dW_l = A_{l-1}^T G_l   (M = K, N = N, K = B)
-            gemm<true, false, 0>(s, K, N, B, in, K, gzb, N, nullptr, nullptr, g + woff[l], nullptr, N);
-            hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256), dim3(256), 0, s, B, N, gz, g + boff[l]);
+            // dW_l = A_{l-1}^T G_l   (M = K, N = N, K = B), K split over the batch
+            const int sp = split_count(K, N, B);
+            if (sp > 1) {
+                const int ns = gemm<true, false, 0>(s, K, N, B, in, K, gzb, N, nullptr, nullptr, part, nullptr, N, sp);
+                const size_t nw_ = (size_t)K * N;
+                hipLaunchKernelGGL(k_sum_splits, dim3(nblocks(nw_)), dim3(256), 0, s, nw_, ns, part, g + woff[l]);
+            } else {
+                gemm<true, false, 0>(s, K, N, B, in, K, gzb, N, nullptr, nullptr, g + woff[l], nullptr, N);
+            }
+            hipLaunchKernelGGL(k_colsum_part, dim3((N + 255) / 256, CS_PART), dim3(256), 0, s, B, N, gz, cpart);
+            hipLaunchKernelGGL(k_sum_splits, dim3(nblocks(N)), dim3(256), 0, s, (size_t)N, CS_PART, cpart,
+                               g + boff[l]);
             gemm_flop += 2.0 * B * N * K;
             if (l > 1) {
                 // dA_{l-1} = G_l W_l^T   (M = B, N = K, K = N), then through SiLU
@@ -486,6 +567,13 @@ lc_status lc_mlp_create(int device, const int32_t *widths, int nwidth, int max_b
         mm->y = mm->alloc<float>(B * widths[mm->L]);
         mm->t = mm->alloc<float>(B * widths[mm->L]);
         mm->ga = mm->alloc<float>(B * wmax);
+        size_t part_n = 1;
+        for (int l = 1; l <= mm->L; l++) {
+            const int sp = split_count(widths[l - 1], widths[l], max_batch);
+            if (sp > 1) part_n = std::max(part_n, (size_t)sp * widths[l - 1] * widths[l]);  // splits <= sp
+        }
+        mm->part = mm->alloc<float>(part_n);
+        mm->cpart = mm->alloc<float>((size_t)CS_PART * wmax);
         mm->gz = mm->alloc<float>(B * wmax);
         mm->gzb = mm->alloc<uint16_t>(B * wmax);
         mm->loss = mm->alloc<double>(1);

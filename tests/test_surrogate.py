@@ -226,9 +226,11 @@ def test_chisq_kernel_matches_host():
 @pytest.mark.gpu
 def test_surrogate_screens_engine_posterior(engine):
     """Small end-to-end run: a network trained on engine curves around the
-    fit screens proposals of the detached-binary posterior; the chains keep
-    the same posterior and the engine is called less often per accepted
-    move."""
+    fit screens proposals of the detached-binary posterior; the two-stage
+    chains keep the same posterior, and the first stage saves engine
+    calls.  (Whether that buys effective samples depends on how well the
+    network predicts chi**2 differences between neighbouring points; the
+    bench measures it.)"""
     from lcurve_re_amd import fit as F
     cfg = W.fit_batch(nphase=1000)["model"]
     for key in ("nlat1f", "nlat2f", "nlat1c", "nlat2c"):
@@ -250,11 +252,18 @@ def test_surrogate_screens_engine_posterior(engine):
     sur, info = S.train_local_surrogate(engine, base, samples, cov, 8192, 600, seed=5)
     print(f"surrogate: {info}", flush=True)
     assert info["local_valid_rms_relflux"] < 1e-4, info
-    print("surrogate: plain and surrogate-transition chains, 32 walkers x 150", flush=True)
+    print("surrogate: plain, delayed-acceptance and surrogate-transition chains, 32 walkers x 150", flush=True)
+    T, cal = S.calibrate_temperature(post, sur, x0, cov, 256, seed=6)
+    print(f"surrogate: stage-1 temperature {T} {cal}", flush=True)
+    sl = post.surrogate_loglike(sur, T)
     plain, sp = S.run_chains(post.loglike, post.legal, x0, cov, 150, seed=1)
-    da, sd = S.run_chains(post.loglike, post.legal, x0, cov, 150, seed=2,
-                          sur_loglike=post.surrogate_loglike(sur, 1.0), n_sub=4)
-    assert sd["accepted"] / sd["evaluations"] > sp["accepted"] / sp["evaluations"]
-    sdv = plain[50:].reshape(-1, d).std(axis=0)
-    diff = np.abs(da[50:].reshape(-1, d).mean(axis=0) - plain[50:].reshape(-1, d).mean(axis=0))
-    assert np.all(diff < 1.0 * sdv + 1e-12), diff / sdv
+    da, sd = S.run_chains(post.loglike, post.legal, x0, cov, 150, seed=2, sur_loglike=sl, n_sub=1)
+    st, ss = S.run_chains(post.loglike, post.legal, x0, cov, 150, seed=3, sur_loglike=sl, n_sub=4)
+    print(f"surrogate: plain {sp}\nsurrogate: DA {sd}\nsurrogate: ST {ss}", flush=True)
+    # the first stage turns proposals away without an engine call
+    assert sd["evaluations"] < sp["evaluations"]
+    # and neither two-stage sampler moves the posterior
+    a = plain[50:].reshape(-1, d)
+    for ch in (da, st):
+        diff = np.abs(ch[50:].reshape(-1, d).mean(axis=0) - a.mean(axis=0))
+        assert np.all(diff < 1.0 * a.std(axis=0) + 1e-12), diff / a.std(axis=0)
