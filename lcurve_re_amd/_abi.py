@@ -182,11 +182,12 @@ _lib = None
 
 
 def load(path=None):
-    """Loads the built library (raises if it has not been built)."""
+    """Loads the built library (raises if it has not been built); the
+    environment variable LCURVE_AMD_LIB names another build to load."""
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("LCURVE_AMD_LIB") or LIB_PATH
     if not os.path.exists(p):
         raise RuntimeError(f"lcurve_re_amd: native library not built ({p}); run __graft_entry__.build()")
     lib = C.CDLL(p)

@@ -14,6 +14,10 @@
 // call; results come back in one copy at the end.
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -185,22 +189,30 @@ struct GridPlan {
 // recomputed.  (The reference keys its cache on the Model fields that feed
 // set_star_grid, light_curve_comp.cpp:34-41; keying on the derived values
 // also catches parameter changes that leave a grid unchanged.)
-std::string geom_key(const GridPlan &gp) {
+// The key is split in two: a short signature of the GridDev fields and the
+// ring count (hashed and compared first) and the ring table itself, which
+// is compared only when two signatures are equal.
+std::string geom_sig(const GridPlan &gp) {
     const GridDev &G = gp.dev;
     std::string k;
     auto put = [&k](const void *p, size_t n) { k.append(static_cast<const char *>(p), n); };
-    const int iv[8] = {G.star, G.fine, G.nface, G.roche_self, G.roche_other, G.eclipse, G.npole, gp.ring_faces};
-    const double dv[15] = {G.q,          G.spin_self, G.spin_other, G.rref_self, G.pref_self,
+    const int iv[9] = {G.star, G.fine,    G.nface,        G.roche_self, G.roche_other,
+                       G.eclipse, G.npole, gp.ring_faces, (int)gp.rings.size()};
+    const double dv[14] = {G.q,          G.spin_self, G.spin_other, G.rref_self, G.pref_self,
                            G.rref_other, G.pref_other, G.r_self,    G.r_other,   G.acc,
-                           G.delta,      G.cosi,      G.sini,       G.gref,      0.0};
+                           G.delta,      G.cosi,      G.sini,       G.gref};
     put(iv, sizeof(iv));
     put(dv, sizeof(dv));
-    for (const auto &R : gp.rings) {
-        const double rv[5] = {R.sint, R.cost, R.phi1, R.dphi_span, R.tspan_nl};
-        put(rv, sizeof(rv));
-        put(&R.nphi, sizeof(R.nphi));
-    }
     return k;
+}
+
+bool same_rings(const std::vector<RingHost> &a, const std::vector<RingHost> &b) {
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); i++)
+        if (a[i].sint != b[i].sint || a[i].cost != b[i].cost || a[i].phi1 != b[i].phi1 ||
+            a[i].dphi_span != b[i].dphi_span || a[i].tspan_nl != b[i].tspan_nl || a[i].nphi != b[i].nphi)
+            return false;
+    return true;
 }
 
 // Everything set_star_grid derives before add_faces, plus the ring table
@@ -499,11 +511,15 @@ struct GeoCache {
     struct Entry {
         int off, n;
     };
+    struct Rec {
+        std::vector<RingHost> rings;
+        Entry e;
+    };
     size_t cap = 0;  // faces (0: no reuse across calls)
     size_t used = 0;
     void *mem = nullptr;
     FaceArrays pool{};
-    std::unordered_map<std::string, Entry> map;
+    std::unordered_map<std::string, std::vector<Rec>> map;  // by geom_sig
     void clear() {
         map.clear();
         used = 0;
@@ -589,6 +605,32 @@ double elem_flops(const GridDev &G) {
     return c[(G.ltype ? 1 : 0) + (G.beam != 0.0 ? 2 : 0)];
 }
 
+// Runs f(0..n-1) on up to 16 host threads (work taken one index at a
+// time); the first exception is rethrown after all threads joined.
+template <class F> void parallel_for(size_t n, F f) {
+    const size_t nt = std::min<size_t>({(size_t)std::max(1u, std::thread::hardware_concurrency()), (size_t)16,
+                                        (n + 15) / 16});
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::exception_ptr err;
+    std::mutex mu;
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; t++)
+        th.emplace_back([&] {
+            try {
+                for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!err) err = std::current_exception();
+            }
+        });
+    for (auto &x : th) x.join();
+    if (err) std::rethrow_exception(err);
+}
+
 struct EvalRequest {
     const lc_model *models;
     size_t m;
@@ -619,11 +661,12 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     }
     const double middle = (xmin + xmax) / 2., range = (xmax - xmin) / 2.;
 
-    // ---------------- per instance planning
+    // ---------------- per instance planning (host threads), then the grids
+    // are numbered in instance order
     std::vector<InstPlan> ip(M);
-    std::vector<GridPlan> grids;
     std::vector<InstDev> insts(M);
-    for (size_t i = 0; i < M; i++) {
+    std::vector<std::vector<GridPlan>> local(M);
+    parallel_for(M, [&](size_t i) {
         const lc_model &m = rq.models[i];
         InstPlan &P = ip[i];
         InstDev &I = insts[i];
@@ -639,29 +682,16 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             else if (r2 > rl2)
                 throw EngineError(LC_E_ROCHE_OVERFLOW,
                                   "light_curve_comp: the secondary star is larger than its Roche lobe!");
-            std::vector<GridPlan> local;
-            local.push_back(plan_grid(m, 0, true));
-            local.push_back(plan_grid(m, 1, true));
+            std::vector<GridPlan> &L = local[i];
+            L.push_back(plan_grid(m, 0, true));
+            L.push_back(plan_grid(m, 1, true));
             P.copy1 = (m.nlat1f == m.nlat1c);
             P.copy2 = (m.nlat2f == m.nlat2c) && (!m.npole || r1 >= r2 || (m.nlatfill == 0 && m.nlngfill == 0));
-            if (!P.copy1) local.push_back(plan_grid(m, 0, false));
-            if (!P.copy2) local.push_back(plan_grid(m, 1, false));
-            int k = 0;
-            for (auto &gp : local) {
+            if (!P.copy1) L.push_back(plan_grid(m, 0, false));
+            if (!P.copy2) L.push_back(plan_grid(m, 1, false));
+            for (auto &gp : L) {
                 gp.dev.inst = (int)i;
                 fill_physics(m, gp.dev);
-                int slot = gp.dev.star + (gp.dev.fine ? 0 : 2);
-                P.g[slot] = (int)grids.size() + k;
-                k++;
-            }
-            if (P.copy1) P.g[2] = P.g[0];
-            if (P.copy2) P.g[3] = P.g[1];
-            for (auto &gp : local) grids.push_back(std::move(gp));
-            if (o.info) {
-                std::fprintf(stderr, "Number of points for star 1 (fine) = %d\n", grids[P.g[0]].dev.nface);
-                std::fprintf(stderr, "Number of points for star 2 (fine) = %d\n", grids[P.g[1]].dev.nface);
-                std::fprintf(stderr, "Number of points for star 1 (coarse) = %d\n", grids[P.g[2]].dev.nface);
-                std::fprintf(stderr, "Number of points for star 2 (coarse) = %d\n", grids[P.g[3]].dev.nface);
             }
             // instance parameters
             const double ri = lcx::deg2rad(m.iangle.value);
@@ -681,10 +711,6 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             I.sfac0 = rq.io[i].sfac[0];
             I.copy1 = P.copy1;
             I.copy2 = P.copy2;
-            I.grid1f = P.g[0];
-            I.grid2f = P.g[1];
-            I.grid1c = P.g[2];
-            I.grid2c = P.g[3];
             I.scale = o.scale;
             I.rdata = o.rdata;
             I.active = 1;
@@ -692,6 +718,29 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             P.status = e.code;
             P.msg = e.what();
             I.active = 0;
+            local[i].clear();
+        }
+    });
+    std::vector<GridPlan> grids;
+    for (size_t i = 0; i < M; i++) {
+        if (ip[i].status != LC_OK) continue;
+        InstPlan &P = ip[i];
+        InstDev &I = insts[i];
+        for (auto &gp : local[i]) {
+            P.g[gp.dev.star + (gp.dev.fine ? 0 : 2)] = (int)grids.size();
+            grids.push_back(std::move(gp));
+        }
+        if (P.copy1) P.g[2] = P.g[0];
+        if (P.copy2) P.g[3] = P.g[1];
+        I.grid1f = P.g[0];
+        I.grid2f = P.g[1];
+        I.grid1c = P.g[2];
+        I.grid2c = P.g[3];
+        if (o.info) {
+            std::fprintf(stderr, "Number of points for star 1 (fine) = %d\n", grids[P.g[0]].dev.nface);
+            std::fprintf(stderr, "Number of points for star 2 (fine) = %d\n", grids[P.g[1]].dev.nface);
+            std::fprintf(stderr, "Number of points for star 1 (coarse) = %d\n", grids[P.g[2]].dev.nface);
+            std::fprintf(stderr, "Number of points for star 2 (coarse) = %d\n", grids[P.g[3]].dev.nface);
         }
     }
 
@@ -700,19 +749,29 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     // copied; the rest are computed (and offered to the cache).
     const int ngrid = (int)grids.size();
     std::vector<int> dup_of(ngrid, -1), hit_off(ngrid, -1);
-    std::vector<std::string> keys(ngrid);
+    std::vector<std::string> sigs(ngrid);
     {
-        std::unordered_map<std::string, int> first;
+        std::unordered_map<std::string, std::vector<int>> first;  // computed grids by signature
         for (int g = 0; g < ngrid; g++) {
-            keys[g] = geom_key(grids[g]);
-            auto it = ctx->dedupe ? first.find(keys[g]) : first.end();
-            if (it != first.end()) {
-                dup_of[g] = it->second;
-                continue;
+            sigs[g] = geom_sig(grids[g]);
+            if (ctx->dedupe) {
+                auto it = first.find(sigs[g]);
+                if (it != first.end())
+                    for (int h : it->second)
+                        if (same_rings(grids[h].rings, grids[g].rings)) {
+                            dup_of[g] = h;
+                            break;
+                        }
+                if (dup_of[g] >= 0) continue;
             }
-            first.emplace(keys[g], g);
-            auto h = ctx->geo.map.find(keys[g]);
-            if (h != ctx->geo.map.end() && h->second.n == grids[g].dev.nface) hit_off[g] = h->second.off;
+            first[sigs[g]].push_back(g);
+            auto c = ctx->geo.map.find(sigs[g]);
+            if (c != ctx->geo.map.end())
+                for (const auto &rec : c->second)
+                    if (rec.e.n == grids[g].dev.nface && same_rings(rec.rings, grids[g].rings)) {
+                        hit_off[g] = rec.e.off;
+                        break;
+                    }
         }
     }
     auto computed = [&](int g) { return dup_of[g] < 0 && hit_off[g] < 0; };
@@ -876,6 +935,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<int>(1);
     sz.add<int>(1);      // eclipse work lists
     sz.add<int>(FN);
+    sz.add<int>(FN);
     for (int k = 0; k < 6; k++) sz.add<double>(FN);
     sz.add<float4>(FN);
     sz.add<float4>(FN);
@@ -926,6 +986,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     EclipseWork ew;
     ew.hit_count = A.take<int>(1);
     ew.face = A.take<int>(FN);
+    ew.grid = A.take<int>(FN);
     ew.phi = A.take<double>(FN);
     ew.lim = A.take<double>(2 * FN);
     ew.acc = A.take<double>(FN);
@@ -1145,7 +1206,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     }
     for (const auto &ins : inserts) {
         const int inst = gdev[ins.first].inst;
-        if (ip[inst].status == LC_OK && !h_status[inst]) ctx->geo.map[keys[ins.first]] = ins.second;
+        if (ip[inst].status == LC_OK && !h_status[inst])
+            ctx->geo.map[sigs[ins.first]].push_back(GeoCache::Rec{grids[ins.first].rings, ins.second});
     }
     for (int g = 0; g < ngrid; g++) {
         if (computed(g))
@@ -1553,7 +1615,7 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     G.face_off = 0; G.ring_off = 0; G.nring = (int)rings.size(); G.half_off = 0; G.nhalf = ho; G.inst = 0;
     if (nf == 0) return LC_OK;
     const size_t FN = (size_t)nf;
-    ctx->dev.reserve(FN * (6 * 8 + 3 * 4 + 4 + 16 + 4 + 4 + 6 * 8) + rings.size() * sizeof(RingDev) + 65536);
+    ctx->dev.reserve(FN * (6 * 8 + 3 * 4 + 4 + 16 + 4 + 4 + 4 + 6 * 8) + rings.size() * sizeof(RingDev) + 65536);
     ctx->dev.reset();
     DevArena &A = ctx->dev;
     hipStream_t s = ctx->stream;
@@ -1568,6 +1630,7 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     EclipseWork ew;
     ew.hit_count = A.take<int>(1);
     ew.face = A.take<int>(FN);
+    ew.grid = A.take<int>(FN);
     ew.phi = A.take<double>(FN);
     ew.lim = A.take<double>(2 * FN);
     ew.acc = A.take<double>(FN);

@@ -177,6 +177,7 @@ __global__ __launch_bounds__(128) void k_eclipse_potmin(const GridDev *__restric
         if (!hit) continue;
         const int k = atomicAdd(ew.hit_count, 1);
         ew.face[k] = f;
+        ew.grid[k] = (int)(&G - grids);
         ew.phi[k] = phi;
         ew.lim[2 * k] = phi1;
         ew.lim[2 * k + 1] = phi2;
@@ -184,23 +185,29 @@ __global__ __launch_bounds__(128) void k_eclipse_potmin(const GridDev *__restric
     }
 }
 
-__global__ __launch_bounds__(128) void k_eclipse_bisect(const GridDev *__restrict__ grids, int ngrid,
-                                                       FaceArrays fa, EclipseWork ew, int *inst_status) {
+#ifndef LC_ECL_WAVES
+#define LC_ECL_WAVES 4  // 128 VGPRs: four waves per SIMD without spills
+#endif
+__global__ __launch_bounds__(128, LC_ECL_WAVES) void k_eclipse_bisect(const GridDev *__restrict__ grids, int ngrid,
+                                                                     FaceArrays fa, EclipseWork ew,
+                                                                     int *inst_status) {
     const int n = *ew.hit_count * 2;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int k = i >> 1, side = i & 1;
         const int f = ew.face[k];
-        const GridDev &G = grids[find_grid(grids, ngrid, f)];
+        const GridDev &G = grids[ew.grid[k]];
+        // the grid's constants in registers for the whole search
         const int other = G.star == PRIMARY ? SECONDARY : PRIMARY;
+        const double q = G.q, spin = G.spin_other, cosi = G.cosi, sini = G.sini, delta = G.delta;
+        const double rref = G.rref_other, pref = G.pref_other;
         const V3 r = v3(fa.px[f], fa.py[f], fa.pz[f]);
         const double ACC = ew.acc[k];
         double pin = ew.phi[k], pout = ew.lim[2 * k + side];
         int st = ST_OK;
-        while (fabs(pin - pout) > G.delta) {
+        while (fabs(pin - pout) > delta) {
             const double pmid = (pin + pout) / 2.;
             bool h;
-            st = fblink(G.q, other, G.spin_other, ACC, set_earth(G.cosi, G.sini, pmid), r, G.rref_other,
-                        G.pref_other, h);
+            st = fblink(q, other, spin, ACC, set_earth(cosi, sini, pmid), r, rref, pref, h);
             if (st) break;
             if (h)
                 pin = pmid;

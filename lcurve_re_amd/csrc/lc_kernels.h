@@ -72,6 +72,7 @@ struct FlatArrays {
 struct EclipseWork {
     int *hit_count;  // faces pot_min found eclipsed
     int *face;       // [hit] face index
+    int *grid;       // [hit] grid of that face
     double *phi;     // [hit] phase inside the eclipse (pot_min)
     double *lim;     // [2*hit] sphere window phi1, phi2
     double *acc;     // [hit] position accuracy of fblink
