@@ -126,6 +126,7 @@ def main():
                     help="0: FP32 per face (default), 1: FP64 per face (default for fit_batch)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default flux tile shape")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
@@ -151,7 +152,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def roofline_from_profile(prof, steps, precision, workload):
+def roofline_from_profile(prof, steps, precision, workload, R=None):
     """Roofline of the dominant flux kernel: the variant (limb-darkening law
     x beaming x lensing x mucrit) with the most launch time."""
     var_ms = prof["flux_var_ms"]
@@ -160,7 +161,8 @@ def roofline_from_profile(prof, steps, precision, workload):
     avg_ms = var_ms[v] / launches
     flop_per_launch = prof["flux_var_flops"][v] / launches
     achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
-    R = int(os.environ.get("LCURVE_AMD_FLUX_R", "2"))
+    if R is None:
+        R = int(os.environ.get("LCURVE_AMD_FLUX_R", "2"))
     kname = (f"k_flux_{'fp64' if precision else 'mixed'}<{v & 1}, {bool(v & 2)}, {bool(v & 4)}"
              + ("" if precision else f", {bool(v & 8)}") + f", {R}>")
     peak = PEAK_FP64_TFLOPS if precision else PEAK_FP32_TFLOPS
@@ -209,6 +211,11 @@ def run_batch_workload(L, eng, args):
                 models[i] = pool.pop(0)
                 redrawn += 1
         cfg["redrawn"] = redrawn
+    R = None
+    if not args.precision and not args.no_autotune:
+        # flux tile shape for this workload, timed before the measured steps
+        R, chunk = eng.autotune(models, data, precision=args.precision)
+        cfg["flux_tile"] = {"subphases_per_lane": R, "face_chunk": chunk, "autotuned": True}
     log(f"bench: workload {cfg}")
     calc = None
     for i in range(args.warmup):
@@ -222,7 +229,7 @@ def run_batch_workload(L, eng, args):
     prof = eng.profile()
     nok = int(np.count_nonzero(status == 0))
     log(f"bench: {args.steps} steps in {el:.3f} s; profile {prof}")
-    roofline = roofline_from_profile(prof, args.steps, args.precision, cfg["workload"])
+    roofline = roofline_from_profile(prof, args.steps, args.precision, cfg["workload"], R)
     cpu, err = (None, None)
     if not args.no_cpu:
         cpu, err = cpu_baseline(models, data, calc, args.cpu_budget)

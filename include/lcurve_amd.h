@@ -190,6 +190,13 @@ lc_status lc_ctx_reset_profile(lc_ctx *ctx);
  * off (grids equal within one batch are still built once).  Default 4M
  * faces, or 0 when LCURVE_NO_GRID_CACHE is set. */
 lc_status lc_ctx_set_stage_cache(lc_ctx *ctx, size_t max_faces);
+/* Autotuner of the flux kernel's tile shape: runs the given workload with
+ * every (sub-phases per lane R in {1, 2, 4}) x (face chunk in {512, 1024,
+ * 2048}) and keeps for this ctx the pair with the shortest flux-stage time
+ * (reported through best_rphase / best_chunk when not NULL).  The default is
+ * R = 2, chunk = 1024 (FP64 mode always uses 1024-face chunks). */
+lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const lc_datum *data, size_t n,
+                          const lc_eval_opts *opts, int32_t *best_rphase, int32_t *best_chunk);
 /* Library version string; usable without a GPU. */
 const char *lc_version(void);
 /* Message for the last failure of a ctx-free call on this thread. */

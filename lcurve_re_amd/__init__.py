@@ -262,6 +262,20 @@ class Engine:
     def reset_profile(self):
         _lib().lc_ctx_reset_profile(self._ctx)
 
+    def autotune(self, models, data, precision=LC_PREC_MIXED, max_model_points=0):
+        """Picks the flux tile shape (sub-phases per lane, face chunk) for this
+        workload by timing every candidate; returns (R, chunk)."""
+        m = len(models)
+        pods = (lc_model * m)()
+        for i, md in enumerate(models):
+            C.memmove(C.byref(pods[i]), C.byref(md.pod), C.sizeof(lc_model))
+        o = self._opts(False, True, 0, False, max_model_points, precision)
+        r, c = C.c_int32(), C.c_int32()
+        st = _lib().lc_ctx_autotune(self._ctx, pods, m, data, len(data), C.byref(o), C.byref(r), C.byref(c))
+        if st != LC_OK:
+            self._err(st)
+        return r.value, c.value
+
     def set_stage_cache(self, max_faces):
         """Faces of star grids kept on the device for reuse across calls (0: off)."""
         st = _lib().lc_ctx_set_stage_cache(self._ctx, int(max_faces))

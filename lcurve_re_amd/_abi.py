@@ -131,6 +131,8 @@ SIGNATURES = {
     "lc_ctx_get_profile": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_profile)]),
     "lc_ctx_reset_profile": (_c.c_int, [_c.c_void_p]),
     "lc_ctx_set_stage_cache": (_c.c_int, [_c.c_void_p, _c.c_size_t]),
+    "lc_ctx_autotune": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_model), _c.c_size_t, _c.POINTER(lc_datum), _c.c_size_t,
+                                   _c.POINTER(lc_eval_opts), _IP, _IP]),
     "lc_version": (_c.c_char_p, []),
     "lc_thread_error": (_c.c_char_p, []),
     "lc_sizeof": (_c.c_size_t, [_c.c_char_p]),

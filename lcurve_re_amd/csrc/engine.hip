@@ -1438,6 +1438,58 @@ lc_status lc_ctx_set_stage_cache(lc_ctx *ctx, size_t max_faces) {
     });
 }
 
+lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const lc_datum *data, size_t n,
+                          const lc_eval_opts *opts, int32_t *best_rphase, int32_t *best_chunk) {
+    if (!ctx) return LC_E_ARG;
+    if (!models || !data || !opts || m == 0 || n == 0) {
+        ctx->err = "lc_ctx_autotune: empty or null workload";
+        return LC_E_ARG;
+    }
+    return guarded(ctx, [&]() -> int {
+        HIPCHK(hipSetDevice(ctx->device));
+        std::vector<double> calc(m * n);
+        std::vector<lc_scalars> io(m);
+        std::vector<int32_t> status(m);
+        for (auto &s : io)
+            for (double &f : s.sfac) f = 1.0;
+        EvalRequest rq{models, m, data, n, *opts, calc.data(), io.data(), status.data()};
+        const lc_profile saved = ctx->prof;
+        const int r0 = ctx->flux_rphase, c0 = ctx->flux_chunk;
+        run_eval(ctx, rq);  // warm the arenas once
+        auto time_of = [&](int r, int c) {
+            ctx->flux_rphase = r;
+            ctx->flux_chunk = c;
+            double ms = 0.;
+            for (int rep = 0; rep < 2; rep++) {
+                const double f0 = ctx->prof.flux_ms_total;
+                run_eval(ctx, rq);
+                ms += ctx->prof.flux_ms_total - f0;
+            }
+            return ms;
+        };
+        // a candidate replaces the current shape only when it is more than
+        // 2% faster, so timing noise does not flip between near-equal shapes
+        double best = time_of(r0, c0) / 1.02;
+        int br = r0, bc = c0;
+        for (int r : {1, 2, 4})
+            for (int c : {512, 1024, 2048}) {
+                if (r == r0 && c == c0) continue;
+                const double ms = time_of(r, c);
+                if (ms < best) {
+                    best = ms;
+                    br = r;
+                    bc = c;
+                }
+            }
+        ctx->flux_rphase = br;
+        ctx->flux_chunk = bc;
+        ctx->prof = saved;
+        if (best_rphase) *best_rphase = br;
+        if (best_chunk) *best_chunk = bc;
+        return LC_OK;
+    });
+}
+
 lc_status lc_star_grid_size(const lc_model *m, int star, int fine, size_t *nface) {
     if (!m || !nface || (star != 0 && star != 1)) return LC_E_ARG;
     return guarded(nullptr, [&]() -> int {

@@ -87,3 +87,16 @@ def test_cache_smaller_than_a_grid(cold):
         assert np.array_equal(a, b)
     finally:
         e.close()
+
+
+def test_autotune_picks_a_candidate_and_keeps_results(cold):
+    models = [_model(q=0.3 + 0.1 * k, iangle=80.0 + 2 * k) for k in range(4)]
+    data = _data(3000)
+    ref, _, _ = cold.light_curve_comp_batch(models, data)
+    print("autotune: 4 models x 3000 phases", flush=True)
+    R, chunk = cold.autotune(models, data)
+    assert R in (1, 2, 4) and chunk in (512, 1024, 2048)
+    got, _, st = cold.light_curve_comp_batch(models, data)
+    assert not st.any()
+    # another tile shape changes the FP32 summation grouping only
+    assert np.max(np.abs(got - ref)) <= 1e-6 * np.max(np.abs(ref))
