@@ -99,6 +99,43 @@ LC_HD V3 drpot_star(int star, double q, double spin, const V3 &p) {
     return v3(gx, comp * p.y + mu1 * p.y - ssq * p.y, comp * p.z + mu1 * p.z);
 }
 
+// A potential evaluation that keeps its distances, so that the gradient at
+// the same point needs two divisions more instead of a second evaluation
+// (dbrent asks for f(u) and then, unless it stops, for df(u)).  The
+// operations are those of rpot1/rpot2 and drpot_star term for term:
+// (1-mu)/r1 and mu/r2 are the common subexpressions, so the values are
+// bit-identical to calling the two functions.
+struct PotPoint {
+    V3 r;
+    double r1sq, r2sq, c1, m2;  // c1 = (1-mu)/r1, m2 = mu/r2
+};
+LC_HD double rpot_keep(int star, double q, double spin, const V3 &p, PotPoint &k) {
+    double mu = q / (1 + q);
+    double comp = 1. - mu;
+    double x2y2 = p.x * p.x + p.y * p.y;
+    double r1sq = x2y2 + p.z * p.z;
+    double r1 = sqrt(r1sq);
+    double r2sq = r1sq + 1. - 2. * p.x;
+    double r2 = sqrt(r2sq);
+    k.r = p;
+    k.r1sq = r1sq;
+    k.r2sq = r2sq;
+    k.c1 = comp / r1;
+    k.m2 = mu / r2;
+    if (star == PRIMARY) return (-k.c1 - k.m2 - sqr(spin) * x2y2 / 2. + mu * p.x);
+    return -k.c1 - k.m2 - spin * spin * (0.5 + 0.5 * x2y2 - p.x) - comp * p.x;
+}
+LC_HD V3 drpot_kept(int star, double q, double spin, const PotPoint &k) {
+    const V3 &p = k.r;
+    double mu = q / (1 + q);
+    double mu1 = k.m2 / k.r2sq;
+    double comp = k.c1 / k.r1sq;
+    double ssq = sqr(spin);
+    double gx = star == PRIMARY ? comp * p.x + mu1 * (p.x - 1.) - ssq * p.x + mu
+                                : comp * p.x + mu1 * (p.x - 1.) - ssq * (p.x - 1.) + mu - 1.;
+    return v3(gx, comp * p.y + mu1 * p.y - ssq * p.y, comp * p.z + mu1 * p.z);
+}
+
 // Earth vector at an orbital phase (set_earth.cpp:39-46)
 LC_HD V3 set_earth(double cosi, double sini, double phase) {
     double rp = TWOPI * phase;
@@ -302,19 +339,21 @@ LC_HD bool sphere_cut(const V3 &earth, const V3 &r, const V3 &c, double rs, doub
 // ----------------------------------------------------------------- dbrent
 // Derivative-assisted Brent minimisation with early exit once the value
 // drops below fref (dbrent.cpp:19-126).  Returns ST_ECLIPSE after 100
-// iterations.
+// iterations.  F supplies f(x, keep) and df(keep): the derivative at the
+// point of the last value, from what that evaluation kept.
 template <class F>
 LC_HD int dbrent(double ax, double bx, double cx, const F &fn, double acc, double fref, double &xmin, double &fmin) {
     double a = (ax < cx ? ax : cx);
     double b = (ax > cx ? ax : cx);
     double x = bx, w = bx, v = bx;
-    double fx = fn.f(x), fw = fx, fv = fx;
+    typename F::Keep kp;
+    double fx = fn.f(x, kp), fw = fx, fv = fx;
     if (fx < fref) {
         xmin = x;
         fmin = fx;
         return ST_OK;
     }
-    double dx = fn.df(x), dw = dx, dv = dx;
+    double dx = fn.df(kp), dw = dx, dv = dx;
     double e = 0., d = 0.;
     for (int iter = 0; iter < 100; iter++) {
         double xm = 0.5 * (a + b);
@@ -356,7 +395,7 @@ LC_HD int dbrent(double ax, double bx, double cx, const F &fn, double acc, doubl
         double u, fu;
         if (fabs(d) >= tol1) {
             u = x + d;
-            fu = fn.f(u);
+            fu = fn.f(u, kp);
             if (fu < fref) {
                 xmin = u;
                 fmin = fu;
@@ -364,7 +403,7 @@ LC_HD int dbrent(double ax, double bx, double cx, const F &fn, double acc, doubl
             }
         } else {
             u = x + sign_of(tol1, d);
-            fu = fn.f(u);
+            fu = fn.f(u, kp);
             if (fu < fref) {
                 xmin = u;
                 fmin = fu;
@@ -376,7 +415,7 @@ LC_HD int dbrent(double ax, double bx, double cx, const F &fn, double acc, doubl
                 return ST_OK;
             }
         }
-        double du = fn.df(u);
+        double du = fn.df(kp);
         if (fu <= fx) {
             if (u >= x)
                 a = x;
@@ -416,6 +455,23 @@ struct PhaseLamLine {
         rpot_grad(q, star, spin, set_earth(cosi, sini, phi + dphi * x), p, lam + dlam * x, dp, dl);
         return (dp * dphi + dl * dlam);
     }
+    // the same two values, sharing the earth vector and the distances
+    struct Keep {
+        PotPoint pt;
+        V3 earth;
+        double lam;
+    };
+    LC_HD double f(double x, Keep &k) const {
+        k.earth = set_earth(cosi, sini, phi + dphi * x);
+        k.lam = lam + dlam * x;
+        return rpot_keep(star, q, spin, add(p, scl(k.lam, k.earth)), k.pt);
+    }
+    LC_HD double df(const Keep &k) const {
+        V3 d = drpot_kept(star, q, spin, k.pt);
+        V3 ed = v3(k.earth.y, -k.earth.x, 0.);
+        double dp = TWOPI * k.lam * dot(d, ed), dl = dot(d, k.earth);
+        return (dp * dphi + dl * dlam);
+    }
 };
 
 // Potential along the line of sight (roche.h:328-365, Rlpot / Dlrpot)
@@ -429,6 +485,9 @@ struct SightLine {
         rpot_grad(q, star, spin, earth, p, x, dp, dl);
         return dl;
     }
+    using Keep = PotPoint;
+    LC_HD double f(double x, Keep &k) const { return rpot_keep(star, q, spin, add(p, scl(x, earth)), k); }
+    LC_HD double df(const Keep &k) const { return dot(drpot_kept(star, q, spin, k), earth); }
 };
 
 // Line minimisation in (phase, lambda) space with boundary handling
