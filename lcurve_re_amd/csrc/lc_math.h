@@ -340,14 +340,16 @@ LC_HD bool sphere_cut(const V3 &earth, const V3 &r, const V3 &c, double rs, doub
 // Derivative-assisted Brent minimisation with early exit once the value
 // drops below fref (dbrent.cpp:19-126).  Returns ST_ECLIPSE after 100
 // iterations.  F supplies f(x, keep) and df(keep): the derivative at the
-// point of the last value, from what that evaluation kept.
+// point of the last value, from what that evaluation kept.  dbrent_at
+// starts from a value fx = f(bx) (and its kept distances) that the caller
+// has already computed; dbrent evaluates it first.
 template <class F>
-LC_HD int dbrent(double ax, double bx, double cx, const F &fn, double acc, double fref, double &xmin, double &fmin) {
+LC_HD int dbrent_at(double ax, double bx, double cx, const F &fn, double acc, double fref, double fx,
+                    typename F::Keep &kp, double &xmin, double &fmin) {
     double a = (ax < cx ? ax : cx);
     double b = (ax > cx ? ax : cx);
     double x = bx, w = bx, v = bx;
-    typename F::Keep kp;
-    double fx = fn.f(x, kp), fw = fx, fv = fx;
+    double fw = fx, fv = fx;
     if (fx < fref) {
         xmin = x;
         fmin = fx;
@@ -392,28 +394,20 @@ LC_HD int dbrent(double ax, double bx, double cx, const F &fn, double acc, doubl
             e = (dx >= 0. ? a - x : b - x);
             d = 0.5 * e;
         }
-        double u, fu;
-        if (fabs(d) >= tol1) {
-            u = x + d;
-            fu = fn.f(u, kp);
-            if (fu < fref) {
-                xmin = u;
-                fmin = fu;
-                return ST_OK;
-            }
-        } else {
-            u = x + sign_of(tol1, d);
-            fu = fn.f(u, kp);
-            if (fu < fref) {
-                xmin = u;
-                fmin = fu;
-                return ST_OK;
-            }
-            if (fu > fx) {
-                xmin = x;
-                fmin = fx;
-                return ST_OK;
-            }
+        // the full step or, below tol1, a step of tol1; one evaluation site
+        // for both, so that lanes taking either branch share it
+        const bool full = fabs(d) >= tol1;
+        const double u = full ? x + d : x + sign_of(tol1, d);
+        const double fu = fn.f(u, kp);
+        if (fu < fref) {
+            xmin = u;
+            fmin = fu;
+            return ST_OK;
+        }
+        if (!full && fu > fx) {
+            xmin = x;
+            fmin = fx;
+            return ST_OK;
         }
         double du = fn.df(kp);
         if (fu <= fx) {
@@ -438,6 +432,12 @@ LC_HD int dbrent(double ax, double bx, double cx, const F &fn, double acc, doubl
         }
     }
     return ST_ECLIPSE;
+}
+template <class F>
+LC_HD int dbrent(double ax, double bx, double cx, const F &fn, double acc, double fref, double &xmin, double &fmin) {
+    typename F::Keep kp;
+    const double fx = fn.f(bx, kp);
+    return dbrent_at(ax, bx, cx, fn, acc, fref, fx, kp, xmin, fmin);
 }
 
 // Potential and directional derivative along a line in (phase, lambda)
@@ -714,10 +714,11 @@ LC_HD int fblink(double q, int star, double spin, double acc, const V3 &earth, c
     int nstep = 1;
     double step = lam2 - lam1;
     double f1 = 0., f2 = 0., flam = 1., lam = lam1;
+    SightLine::Keep klam;  // distances at lam, handed to dbrent
     while (step > acc) {
         lam = lam1 + step / 2.;
         for (int n = 0; n < nstep; n++) {
-            flam = fn.f(lam);
+            flam = fn.f(lam, klam);
             if (flam <= pref) {
                 hidden = true;
                 return ST_OK;
@@ -735,7 +736,8 @@ LC_HD int fblink(double q, int star, double spin, double acc, const V3 &earth, c
     }
     if (flam < f1 && flam < f2) {
         double xmin, fmin;
-        int st = dbrent(lam1, lam, lam2, fn, acc, pref, xmin, fmin);
+        // the reference's dbrent evaluates f(lam) again first: same value
+        int st = dbrent_at(lam1, lam, lam2, fn, acc, pref, flam, klam, xmin, fmin);
         if (st) return st;
         hidden = fmin < pref;
     }
