@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <exception>
 #include <mutex>
 #include <thread>
@@ -27,6 +29,8 @@
 #include <string>
 #include <unordered_map>
 #include <vector>
+
+#include <unistd.h>
 
 #include "../../include/lcurve_amd.h"
 #include "lc_host.h"
@@ -607,28 +611,97 @@ double elem_flops(const GridDev &G) {
 
 // Runs f(0..n-1) on up to 16 host threads (work taken one index at a
 // time); the first exception is rethrown after all threads joined.
-template <class F> void parallel_for(size_t n, F f) {
-    const size_t nt = std::min<size_t>({(size_t)std::max(1u, std::thread::hardware_concurrency()), (size_t)16,
-                                        (n + 15) / 16});
-    if (nt <= 1) {
-        for (size_t i = 0; i < n; i++) f(i);
-        return;
+// Host planning workers: one process-wide pool of up to 15 threads that
+// sleep between jobs (starting threads for every call cost up to 0.5 ms).
+// A job runs f(0..n-1) on the pool and the calling thread; one job at a
+// time, so a call that finds the pool busy (another context planning on
+// another thread) runs its loop on its own thread instead.
+class HostPool {
+  public:
+    static HostPool &get() {
+        static HostPool pool;
+        return pool;
     }
-    std::atomic<size_t> next{0};
-    std::exception_ptr err;
-    std::mutex mu;
-    std::vector<std::thread> th;
-    for (size_t t = 0; t < nt; t++)
-        th.emplace_back([&] {
-            try {
-                for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
-            } catch (...) {
-                std::lock_guard<std::mutex> lk(mu);
-                if (!err) err = std::current_exception();
+    bool try_run(size_t n, const std::function<void(size_t)> &f) {
+        // a forked child has the pool object but not its threads
+        if (workers_.empty() || getpid() != pid_) return false;
+        std::unique_lock<std::mutex> job(submit_, std::try_to_lock);
+        if (!job.owns_lock()) return false;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &f;
+            n_ = n;
+            next_.store(0);
+            err_ = nullptr;
+            active_ = (int)workers_.size();
+            gen_++;
+        }
+        wake_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return active_ == 0; });
+        fn_ = nullptr;
+        if (err_) std::rethrow_exception(err_);
+        return true;
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        wake_.notify_all();
+        for (auto &t : workers_) t.join();
+    }
+
+  private:
+    HostPool() : pid_(getpid()) {
+        const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
+        const unsigned nw = std::min(16u, hc) - 1;
+        for (unsigned k = 0; k < nw; k++) workers_.emplace_back([this] { loop(); });
+    }
+    void work() {
+        try {
+            for (size_t i; (i = next_.fetch_add(1)) < n_;) (*fn_)(i);
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (!err_) err_ = std::current_exception();
+        }
+    }
+    void loop() {
+        unsigned long long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                wake_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
             }
-        });
-    for (auto &x : th) x.join();
-    if (err) std::rethrow_exception(err);
+            work();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--active_ == 0) done_.notify_all();
+        }
+    }
+    const pid_t pid_;
+    std::mutex submit_, mu_;
+    std::condition_variable wake_, done_;
+    std::vector<std::thread> workers_;
+    const std::function<void(size_t)> *fn_ = nullptr;
+    size_t n_ = 0;
+    std::atomic<size_t> next_{0};
+    std::exception_ptr err_;
+    int active_ = 0;
+    unsigned long long gen_ = 0;
+    bool stop_ = false;
+};
+
+// Loops of fewer than min_n items run inline: waking the pool costs tens
+// of microseconds, more than a few items of planning.
+template <class F> void parallel_for(size_t n, F f, size_t min_n = 32) {
+    if (n >= std::max<size_t>(min_n, 2)) {
+        const std::function<void(size_t)> fn(std::ref(f));
+        if (HostPool::get().try_run(n, fn)) return;
+    }
+    for (size_t i = 0; i < n; i++) f(i);
 }
 
 struct EvalRequest {
@@ -750,10 +823,10 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     const int ngrid = (int)grids.size();
     std::vector<int> dup_of(ngrid, -1), hit_off(ngrid, -1);
     std::vector<std::string> sigs(ngrid);
+    parallel_for((size_t)ngrid, [&](size_t g) { sigs[g] = geom_sig(grids[g]); });
     {
         std::unordered_map<std::string, std::vector<int>> first;  // computed grids by signature
         for (int g = 0; g < ngrid; g++) {
-            sigs[g] = geom_sig(grids[g]);
             if (ctx->dedupe) {
                 auto it = first.find(sigs[g]);
                 if (it != first.end())
@@ -778,19 +851,30 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
 
     // ---------------- global tables
     std::vector<GridDev> gdev(ngrid);
-    std::vector<RingDev> rdev;
-    int face_cur = 0, half_cur = 0;
+    int face_cur = 0, half_cur = 0, ring_cur = 0;
     for (int g = 0; g < ngrid; g++) {
         GridPlan &gp = grids[g];
         GridDev &G = gp.dev;
         G.face_off = face_cur;
-        G.ring_off = (int)rdev.size();
+        G.ring_off = ring_cur;
         G.nring = computed(g) ? (int)gp.rings.size() : 0;
         G.half_off = half_cur;
-        int fo = face_cur;
-        if (computed(g))
-        for (auto &R : gp.rings) {
-            RingDev d;
+        int nh = 0;
+        for (int k = 0; k < G.nring; k++) nh += (gp.rings[k].nphi + 1) / 2;
+        G.nhalf = nh;
+        half_cur += nh;
+        ring_cur += G.nring;
+        face_cur += G.nface;
+        gdev[g] = G;
+    }
+    // ring table, filled per grid on the planning threads
+    std::unique_ptr<RingDev[]> rdev(new RingDev[std::max(ring_cur, 1)]);  // not zero-filled
+    parallel_for((size_t)ngrid, [&](size_t g) {
+        const GridDev &G = gdev[g];
+        int fo = G.face_off, ho = G.half_off;
+        for (int k = 0; k < G.nring; k++) {
+            const RingHost &R = grids[g].rings[k];
+            RingDev &d = rdev[G.ring_off + k];
             d.sint = R.sint;
             d.cost = R.cost;
             d.phi1 = R.phi1;
@@ -798,17 +882,13 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             d.tspan_nl = R.tspan_nl;
             d.nphi = R.nphi;
             d.face_off = fo;
-            d.half_off = half_cur;
-            d.grid = g;
-            rdev.push_back(d);
+            d.half_off = ho;
+            d.grid = (int)g;
             fo += R.nphi;
-            half_cur += (R.nphi + 1) / 2;
+            ho += (R.nphi + 1) / 2;
         }
-        G.nhalf = half_cur - G.half_off;
-        face_cur += G.nface;
-        gdev[g] = G;
-    }
-    const int nface_total = face_cur, nhalf_total = half_cur, nring = (int)rdev.size();
+    });
+    const int nface_total = face_cur, nhalf_total = half_cur, nring = ring_cur;
     if ((long long)face_cur > (1LL << 30)) throw EngineError(LC_E_ARG, "batch too large: more than 2^30 faces");
 
     // copies of reused grids, and computed grids offered to the cache
@@ -851,7 +931,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         return p.status == LC_OK && (!p.copy1 || !p.copy2);
     });
     std::vector<BatchDev> batches;
-    std::vector<std::vector<TaskDev>> tasks_by_var(16);
+    std::vector<std::vector<int>> batches_by_var(16);
     long long part_total = 0;
     double alg_flops = 0., alg_elems = 0., alg_bytes = 0.;
     double var_flops[16] = {0.};
@@ -878,8 +958,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
                 int bi = (int)batches.size();
                 batches.push_back(B);
                 int var = flux_variant(G);
-                for (int t = 0; t < ntile; t++)
-                    for (int c = 0; c < B.nchunk; c++) tasks_by_var[var].push_back(TaskDev{bi, t, c, 0});
+                batches_by_var[var].push_back(bi);
                 if (copy) {  // algorithmic work: every face x every sub-phase of the grid
                     alg_elems += (double)G.nface * S;
                     alg_flops += (double)G.nface * S * elem_flops(G);
@@ -889,12 +968,21 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             }
         }
     }
-    std::vector<TaskDev> tasks;
+    // the (batch, tile, chunk) task table is expanded on the device from
+    // each batch's first task; a variant's tasks are contiguous
+    std::vector<int> task_first, task_order;
     std::vector<std::pair<int, int>> var_range(16, {0, 0});
+    int ntask = 0;
     for (int v = 0; v < 16; v++) {
-        var_range[v] = {(int)tasks.size(), (int)tasks_by_var[v].size()};
-        tasks.insert(tasks.end(), tasks_by_var[v].begin(), tasks_by_var[v].end());
+        const int v0 = ntask;
+        for (int b : batches_by_var[v]) {
+            task_order.push_back(b);
+            task_first.push_back(ntask);
+            ntask += batches[b].ntile * batches[b].nchunk;
+        }
+        var_range[v] = {v0, ntask - v0};
     }
+    task_first.push_back(ntask);
 
     // diagnostics grids: star 1 fine, star 2 fine, and the wdwarf grid
     std::vector<int> diag_idx;
@@ -961,7 +1049,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<double>(SM * 2);  // out
     sz.add<double>((size_t)std::max<long long>(part_total, 1));
     sz.add<BatchDev>(std::max<size_t>(batches.size(), 1));
-    sz.add<TaskDev>(std::max<size_t>(tasks.size(), 1));
+    sz.add<TaskDev>(std::max(ntask, 1));
+    sz.add<int>(task_first.size());
+    sz.add<int>(std::max<size_t>(task_order.size(), 1));
     sz.add<double>(std::max<size_t>(M, 1) * n);  // calc
     sz.add<double>(std::max<size_t>(M, 1) * 8);  // scalars
     sz.add<double>(std::max<size_t>(M, 1) * 8);  // gint sums
@@ -970,10 +1060,26 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     ctx->dev.reserve(sz.total + 4096);
     ctx->dev.reset();
     DevArena &A = ctx->dev;
+    // everything uploaded from the host first: the staging block has the
+    // same layout and goes up in one copy
     GridDev *d_grids = A.take<GridDev>(std::max(ngrid, 1));
     RingDev *d_rings = A.take<RingDev>(std::max(nring, 1));
     InstDev *d_insts = A.take<InstDev>(std::max<size_t>(M, 1));
     int *d_status = A.take<int>(M);
+    FaceCopy *d_copy_hit = A.take<FaceCopy>(std::max<size_t>(copy_hit.size(), 1));
+    FaceCopy *d_copy_dup = A.take<FaceCopy>(std::max<size_t>(copy_dup.size(), 1));
+    FaceCopy *d_copy_out = A.take<FaceCopy>(std::max<size_t>(copy_out.size(), 1));
+    double *d_time = A.take<double>(std::max<size_t>(n, 1));
+    double *d_expose = A.take<double>(std::max<size_t>(n, 1));
+    double *d_flux = A.take<double>(std::max<size_t>(n, 1));
+    double *d_ferr = A.take<double>(std::max<size_t>(n, 1));
+    double *d_weight = A.take<double>(std::max<size_t>(n, 1));
+    int *d_ndiv = A.take<int>(std::max<size_t>(n, 1));
+    int *d_sub_off = A.take<int>(n + 1);
+    BatchDev *d_batches = A.take<BatchDev>(std::max<size_t>(batches.size(), 1));
+    int *d_task_first = A.take<int>(task_first.size());
+    int *d_task_order = A.take<int>(std::max<size_t>(task_order.size(), 1));
+    int *d_diag_idx = A.take<int>(std::max<size_t>(diag_idx.size(), 1));
     FaceArrays fa;
     fa.px = A.take<double>(FN); fa.py = A.take<double>(FN); fa.pz = A.take<double>(FN);
     fa.dx = A.take<double>(FN); fa.dy = A.take<double>(FN); fa.dz = A.take<double>(FN);
@@ -1005,16 +1111,6 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     }
     fl.n0 = A.take<int>(std::max(ngrid, 1));
     fl.evald = A.take<unsigned long long>(16 * 64);
-    FaceCopy *d_copy_hit = A.take<FaceCopy>(std::max<size_t>(copy_hit.size(), 1));
-    FaceCopy *d_copy_dup = A.take<FaceCopy>(std::max<size_t>(copy_dup.size(), 1));
-    FaceCopy *d_copy_out = A.take<FaceCopy>(std::max<size_t>(copy_out.size(), 1));
-    double *d_time = A.take<double>(std::max<size_t>(n, 1));
-    double *d_expose = A.take<double>(std::max<size_t>(n, 1));
-    double *d_flux = A.take<double>(std::max<size_t>(n, 1));
-    double *d_ferr = A.take<double>(std::max<size_t>(n, 1));
-    double *d_weight = A.take<double>(std::max<size_t>(n, 1));
-    int *d_ndiv = A.take<int>(std::max<size_t>(n, 1));
-    int *d_sub_off = A.take<int>(n + 1);
     PhaseArrays pa;
     pa.ex = A.take<double>(SM); pa.ey = A.take<double>(SM); pa.ez = A.take<double>(SM);
     pa.phin = A.take<double>(SM); pa.w1 = A.take<double>(SM); pa.w2 = A.take<double>(SM);
@@ -1026,12 +1122,10 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     }
     double *d_out = A.take<double>(SM * 2);
     double *d_partial = A.take<double>((size_t)std::max<long long>(part_total, 1));
-    BatchDev *d_batches = A.take<BatchDev>(std::max<size_t>(batches.size(), 1));
-    TaskDev *d_tasks = A.take<TaskDev>(std::max<size_t>(tasks.size(), 1));
+    TaskDev *d_tasks = A.take<TaskDev>(std::max(ntask, 1));
     double *d_calc = A.take<double>(std::max<size_t>(M, 1) * n);
     double *d_scal = A.take<double>(std::max<size_t>(M, 1) * 8);
     double *d_gsums = A.take<double>(std::max<size_t>(M, 1) * 8);
-    int *d_diag_idx = A.take<int>(std::max<size_t>(diag_idx.size(), 1));
     double *d_diag = A.take<double>(std::max<size_t>(diag_idx.size(), 1) * 8);
 
     // ---------------- uploads through one pinned staging block
@@ -1055,7 +1149,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     };
     std::vector<Up> ups = {
         {d_grids, gdev.data(), gdev.size() * sizeof(GridDev)},
-        {d_rings, rdev.data(), rdev.size() * sizeof(RingDev)},
+        {d_rings, rdev.get(), (size_t)nring * sizeof(RingDev)},
         {d_insts, insts.data(), insts.size() * sizeof(InstDev)},
         {d_status, st0.data(), st0.size() * sizeof(int)},
         {d_time, col[0].data(), n * 8}, {d_expose, col[1].data(), n * 8}, {d_flux, col[2].data(), n * 8},
@@ -1063,27 +1157,38 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         {d_ndiv, ndiv.data(), n * sizeof(int)},
         {d_sub_off, sub_off.data(), (n + 1) * sizeof(int)},
         {d_batches, batches.data(), batches.size() * sizeof(BatchDev)},
-        {d_tasks, tasks.data(), tasks.size() * sizeof(TaskDev)},
+        {d_task_first, task_first.data(), task_first.size() * sizeof(int)},
+        {d_task_order, task_order.data(), task_order.size() * sizeof(int)},
         {d_diag_idx, diag_idx.data(), diag_idx.size() * sizeof(int)},
         {d_copy_hit, copy_hit.data(), copy_hit.size() * sizeof(FaceCopy)},
         {d_copy_dup, copy_dup.data(), copy_dup.size() * sizeof(FaceCopy)},
         {d_copy_out, copy_out.data(), copy_out.size() * sizeof(FaceCopy)},
     };
-    size_t up_total = 0;
-    for (auto &u : ups) up_total += (u.bytes + 255) & ~(size_t)255;
+    size_t up_end = 0;  // the uploaded arrays lead the arena
+    for (auto &u : ups)
+        if (u.bytes) up_end = std::max(up_end, (size_t)(static_cast<char *>(u.dst) - A.base) + u.bytes);
     const size_t down_bytes = M * n * 8 + M * 8 * 8 + M * sizeof(int) + diag_idx.size() * 8 * 8 + M * sizeof(InstDev);
-    ctx->pin.reserve(std::max(up_total, down_bytes) + 4096);
+    ctx->pin.reserve(std::max(up_end, down_bytes) + 4096);
     hipStream_t s = ctx->stream;
     ctx->prof.host_plan_ms_total +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call0).count();
-    {
-        size_t off = 0;
-        for (auto &u : ups) {
-            if (!u.bytes) continue;
-            std::memcpy(ctx->pin.base + off, u.src, u.bytes);
-            HIPCHK(hipMemcpyAsync(u.dst, ctx->pin.base + off, u.bytes, hipMemcpyHostToDevice, s));
-            off += (u.bytes + 255) & ~(size_t)255;
-        }
+    if (up_end) {
+        // the staging block mirrors the arena's first up_end bytes; it is
+        // filled in pieces of up to 1 MiB by the planning threads
+        struct Piece {
+            char *dst;
+            const char *src;
+            size_t bytes;
+        };
+        const size_t PIECE = (size_t)1 << 20;
+        std::vector<Piece> pieces;
+        for (auto &u : ups)
+            for (size_t o = 0; o < u.bytes; o += PIECE)
+                pieces.push_back(Piece{ctx->pin.base + (static_cast<char *>(u.dst) - A.base) + o,
+                                       static_cast<const char *>(u.src) + o, std::min(PIECE, u.bytes - o)});
+        parallel_for(
+            pieces.size(), [&](size_t k) { std::memcpy(pieces[k].dst, pieces[k].src, pieces[k].bytes); }, 4);
+        HIPCHK(hipMemcpyAsync(A.base, ctx->pin.base, up_end, hipMemcpyHostToDevice, s));
     }
     HIPCHK(hipMemsetAsync(d_cand_count, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(ew.hit_count, 0, sizeof(int), s));
@@ -1127,6 +1232,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     launch_flat_build(s, d_grids, ngrid, fa, fl, fp64 ? 1 : 0);
     launch_phases(s, d_insts, (int)M, dd, pa);
     if (any_route) launch_route(s, d_insts, (int)M, S, pa, d_lists, d_counts);
+    launch_make_tasks(s, d_task_first, d_task_order, (int)task_order.size(), d_batches, ntask, d_tasks);
     HIPCHK(hipMemsetAsync(d_out, 0, SM * 2 * sizeof(double), s));
     HIPCHK(hipMemsetAsync(fl.evald, 0, 16 * 64 * sizeof(unsigned long long), s));
     HIPCHK(hipEventRecord(ev[4], s));
@@ -1216,17 +1322,15 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             ctx->prof.grids_reused += 1;
     }
 
-    // ---------------- finalise per instance
-    for (size_t i = 0; i < M; i++) {
+    // ---------------- finalise per instance (on the planning threads)
+    parallel_for(M, [&](size_t i) {
         int st = ip[i].status;
         if (st == LC_OK && h_status[i]) st = h_status[i];
         rq.status[i] = st;
         if (st != LC_OK) {
             if (ip[i].msg.empty()) ip[i].msg = status_msg(st);
-            if (M == 1) ctx->err = ip[i].msg;
-            continue;
+            return;
         }
-        ctx->prof.evaluations++;
         std::memcpy(rq.calc + i * n, h_calc + i * n, n * 8);
         lc_scalars &io = rq.io[i];
         const double *sc = h_scal + i * 8;
@@ -1293,6 +1397,12 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             io.wdwarf = 0.;
             io.logg1 = io.logg2 = io.rv1 = io.rv2 = 0.;
         }
+    });
+    for (size_t i = 0; i < M; i++) {
+        if (rq.status[i] == LC_OK)
+            ctx->prof.evaluations++;
+        else if (M == 1)
+            ctx->err = ip[i].msg;
     }
     ctx->prof.call_ms_total +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call0).count();

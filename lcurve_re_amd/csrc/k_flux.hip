@@ -575,6 +575,35 @@ void launch_flux(hipStream_t s, int variant, int fp64, int rphase, const TaskDev
         launch_variant<2>(s, variant, fp64, tasks, ntask, batches, grids, fl, pa, S, lists, counts, partial);
 }
 
+__global__ __launch_bounds__(256) void k_make_tasks(const int *__restrict__ first, const int *__restrict__ order,
+                                                   int nb, const BatchDev *__restrict__ batches, int ntask,
+                                                   TaskDev *__restrict__ tasks) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntask) return;
+    int lo = 0, hi = nb - 1;  // last k with first[k] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int b = order[lo], local = t - first[lo], nchunk = batches[b].nchunk;
+    TaskDev T;
+    T.batch = b;
+    T.tile = local / nchunk;
+    T.chunk = local - T.tile * nchunk;
+    T.pad = 0;
+    tasks[t] = T;
+}
+
+void launch_make_tasks(hipStream_t s, const int *first, const int *order, int nb, const BatchDev *batches,
+                       int ntask, TaskDev *tasks) {
+    if (ntask <= 0 || nb <= 0) return;
+    hipLaunchKernelGGL(k_make_tasks, dim3((ntask + 255) / 256), dim3(256), 0, s, first, order, nb, batches, ntask,
+                       tasks);
+}
+
 void launch_flux_reduce(hipStream_t s, const BatchDev *batches, int nbatch, int S, int tile_len, const int *lists,
                         const int *counts, const double *partial, double *out) {
     if (nbatch <= 0 || S <= 0) return;

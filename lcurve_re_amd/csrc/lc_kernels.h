@@ -128,6 +128,11 @@ void launch_route(hipStream_t s, const InstDev *insts, int ninst, int S, PhaseAr
 void launch_flux(hipStream_t s, int variant, int fp64, int rphase, const TaskDev *tasks, int ntask,
                  const BatchDev *batches, const GridDev *grids, FlatArrays fl, PhaseArrays pa, int S,
                  const int *lists, const int *counts, double *partial);
+// Expands the task table on the device: the tasks of batch order[k] are
+// first[k] .. first[k+1]-1 (first[nb] = ntask), numbered tile-major
+// (tile * nchunk + chunk).
+void launch_make_tasks(hipStream_t s, const int *first, const int *order, int nb, const BatchDev *batches,
+                       int ntask, TaskDev *tasks);
 void launch_flux_reduce(hipStream_t s, const BatchDev *batches, int nbatch, int S, int threads_per_tile,
                         const int *lists, const int *counts, const double *partial, double *out);
 void launch_scatter(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa, const double *out,
