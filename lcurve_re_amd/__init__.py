@@ -26,7 +26,7 @@ from . import _abi
 from ._abi import (LC_OK, LC_PREC_FP64, LC_PREC_MIXED, LC_LDC_CLARET, LC_LDC_POLY,
                    lc_datum, lc_eval_opts, lc_ldc, lc_model, lc_scalars)
 
-__all__ = ["Model", "Engine", "LcurveError", "make_data", "read_data", "write_data", "lc_model", "lc_datum", "lc_ldc",
+__all__ = ["Model", "ModelBatch", "Engine", "LcurveError", "make_data", "read_data", "write_data", "lc_model", "lc_datum", "lc_ldc",
            "LC_PREC_MIXED", "LC_PREC_FP64", "LC_LDC_POLY", "LC_LDC_CLARET"]
 
 
@@ -158,6 +158,27 @@ def data_view(arr):
                                               ("weight", "f8"), ("ndiv", "i4"), ("pad", "i4")]))
 
 
+class ModelBatch:
+    """Models packed once into the C layout (lc_model[m]) so that repeated
+    batch calls on the same models skip the per-call packing."""
+
+    def __init__(self, models):
+        self.m = len(models)
+        self.pods = (lc_model * max(self.m, 1))()
+        for i, md in enumerate(models):
+            C.memmove(C.byref(self.pods[i]), C.byref(md.pod), C.sizeof(lc_model))
+
+    def __len__(self):
+        return self.m
+
+
+def _packed(models):
+    if isinstance(models, ModelBatch):
+        return models.pods, models.m
+    b = ModelBatch(models)
+    return b.pods, b.m
+
+
 class Engine:
     """One lc_ctx: device buffers, stream and profile counters."""
 
@@ -205,18 +226,24 @@ class Engine:
         return self.light_curve_comp(model, data, max_model_points=max_model_points, **kw)
 
     def light_curve_comp_batch(self, models, data, scale=False, rdata=True, diagnostics=False, sfac=1.0,
-                               precision=LC_PREC_MIXED, max_model_points=0):
-        """Evaluates many models against one data set; returns (calc[m, n], scalars, status[m])."""
-        m = len(models)
+                               precision=LC_PREC_MIXED, max_model_points=0, out=None):
+        """Evaluates many models against one data set; returns (calc[m, n], scalars, status[m]).
+
+        `models` is a sequence of Model or a ModelBatch (packed once for
+        repeated calls); `out`, if given, is a C-contiguous float64 [m, n]
+        array that receives calc (rows of failed instances are left as they
+        were)."""
+        pods, m = _packed(models)
         n = len(data)
-        pods = (lc_model * m)()
-        for i, md in enumerate(models):
-            C.memmove(C.byref(pods[i]), C.byref(md.pod), C.sizeof(lc_model))
-        calc = np.zeros((m, n), np.float64)
+        if out is None:
+            calc = np.zeros((m, n), np.float64)
+        else:
+            calc = out
+            if calc.shape != (m, n) or calc.dtype != np.float64 or not calc.flags.c_contiguous:
+                raise ValueError("out must be a C-contiguous float64 array of shape (%d, %d)" % (m, n))
         io = (lc_scalars * m)()
-        for i in range(m):
-            for k in range(5):
-                io[i].sfac[k] = sfac
+        if m:
+            np.ctypeslib.as_array(io)["sfac"][:] = sfac
         status = np.zeros(m, np.int32)
         o = self._opts(scale, rdata, 0, diagnostics, max_model_points, precision)
         st = _lib().lc_light_curve_comp_batch(self._ctx, pods, m, data, n, C.byref(o), _dp(calc), io,
@@ -265,10 +292,7 @@ class Engine:
     def autotune(self, models, data, precision=LC_PREC_MIXED, max_model_points=0):
         """Picks the flux tile shape (sub-phases per lane, face chunk) for this
         workload by timing every candidate; returns (R, chunk)."""
-        m = len(models)
-        pods = (lc_model * m)()
-        for i, md in enumerate(models):
-            C.memmove(C.byref(pods[i]), C.byref(md.pod), C.sizeof(lc_model))
+        pods, m = _packed(models)
         o = self._opts(False, True, 0, False, max_model_points, precision)
         r, c = C.c_int32(), C.c_int32()
         st = _lib().lc_ctx_autotune(self._ctx, pods, m, data, len(data), C.byref(o), C.byref(r), C.byref(c))

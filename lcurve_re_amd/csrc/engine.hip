@@ -917,6 +917,14 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         }
     }
 
+    // FlatGrid build segments
+    std::vector<FlatSeg> fsegs;
+    for (int g = 0; g < ngrid; g++) {
+        const int nf = gdev[g].nface, ns = std::max(1, (nf + FLAT_SEG - 1) / FLAT_SEG), s0 = (int)fsegs.size();
+        for (int k = 0; k < ns; k++)
+            fsegs.push_back(FlatSeg{g, k * FLAT_SEG, std::min(FLAT_SEG, nf - k * FLAT_SEG), s0, ns, {0, 0, 0}});
+    }
+
     // gaps between the ring layout and numface() hold default faces
     std::vector<std::pair<int, int>> gaps;
     for (int g = 0; g < ngrid; g++)
@@ -1033,6 +1041,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     if (fp64)
         for (int k = 0; k < 7; k++) sz.add<double>(FN);
     sz.add<int>(std::max(ngrid, 1));
+    sz.add<FlatSeg>(fsegs.size());
+    sz.add<int>(fsegs.size());
     sz.add<unsigned long long>(16 * 64);
     sz.add<FaceCopy>(std::max<size_t>(copy_hit.size(), 1));
     sz.add<FaceCopy>(std::max<size_t>(copy_dup.size(), 1));
@@ -1055,6 +1065,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<double>(std::max<size_t>(M, 1) * n);  // calc
     sz.add<double>(std::max<size_t>(M, 1) * 8);  // scalars
     sz.add<double>(std::max<size_t>(M, 1) * 8);  // gint sums
+    const size_t scat_part = std::max<size_t>(M, 1) * std::max<size_t>(1, (n + SCAT_PTS - 1) / SCAT_PTS) * 5;
+    sz.add<double>(scat_part);
     sz.add<int>(std::max<size_t>(diag_idx.size(), 1));
     sz.add<double>(std::max<size_t>(diag_idx.size(), 1) * 8);
     ctx->dev.reserve(sz.total + 4096);
@@ -1080,6 +1092,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     int *d_task_first = A.take<int>(task_first.size());
     int *d_task_order = A.take<int>(std::max<size_t>(task_order.size(), 1));
     int *d_diag_idx = A.take<int>(std::max<size_t>(diag_idx.size(), 1));
+    FlatSeg *d_fsegs = A.take<FlatSeg>(fsegs.size());
+    int *d_seg_cnt = A.take<int>(fsegs.size());
     FaceArrays fa;
     fa.px = A.take<double>(FN); fa.py = A.take<double>(FN); fa.pz = A.take<double>(FN);
     fa.dx = A.take<double>(FN); fa.dy = A.take<double>(FN); fa.dz = A.take<double>(FN);
@@ -1126,6 +1140,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     double *d_calc = A.take<double>(std::max<size_t>(M, 1) * n);
     double *d_scal = A.take<double>(std::max<size_t>(M, 1) * 8);
     double *d_gsums = A.take<double>(std::max<size_t>(M, 1) * 8);
+    double *d_scat_part = A.take<double>(scat_part);
     double *d_diag = A.take<double>(std::max<size_t>(diag_idx.size(), 1) * 8);
 
     // ---------------- uploads through one pinned staging block
@@ -1160,6 +1175,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         {d_task_first, task_first.data(), task_first.size() * sizeof(int)},
         {d_task_order, task_order.data(), task_order.size() * sizeof(int)},
         {d_diag_idx, diag_idx.data(), diag_idx.size() * sizeof(int)},
+        {d_fsegs, fsegs.data(), fsegs.size() * sizeof(FlatSeg)},
         {d_copy_hit, copy_hit.data(), copy_hit.size() * sizeof(FaceCopy)},
         {d_copy_dup, copy_dup.data(), copy_dup.size() * sizeof(FaceCopy)},
         {d_copy_out, copy_out.data(), copy_out.size() * sizeof(FaceCopy)},
@@ -1229,7 +1245,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         launch_point_sums(s, d_grids, d_insts, (int)M, fa, d_gsums);
         launch_gint_scales(s, d_insts, (int)M, d_gsums);
     }
-    launch_flat_build(s, d_grids, ngrid, fa, fl, fp64 ? 1 : 0);
+    launch_flat_build(s, d_grids, d_fsegs, (int)fsegs.size(), d_seg_cnt, fa, fl, fp64 ? 1 : 0);
     launch_phases(s, d_insts, (int)M, dd, pa);
     if (any_route) launch_route(s, d_insts, (int)M, S, pa, d_lists, d_counts);
     launch_make_tasks(s, d_task_first, d_task_order, (int)task_order.size(), d_batches, ntask, d_tasks);
@@ -1247,7 +1263,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         }
     HIPCHK(hipEventRecord(ev[5], s));
     launch_flux_reduce(s, d_batches, (int)batches.size(), S, TILE, d_lists, d_counts, d_partial, d_out);
-    launch_scatter(s, d_insts, (int)M, dd, pa, d_out, d_calc, d_scal);
+    launch_scatter(s, d_insts, (int)M, dd, pa, d_out, d_calc, d_scal, d_scat_part);
     if (!diag_idx.empty()) launch_diag(s, d_grids, d_diag_idx, (int)diag_idx.size(), fa, d_diag);
     HIPCHK(hipGetLastError());
 

@@ -343,44 +343,73 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
     return r;
 }
 
+// Exclusive prefix count of a 0/1 flag over the workgroup in thread order
+// (NT a multiple of 64), from wave ballots; the total through `total`.
 template <int NT>
-__device__ __forceinline__ int block_scan_excl(int v, int *lds, int &total) {
-    lds[threadIdx.x] = v;
+__device__ __forceinline__ int block_count_excl(bool flag, int *lds_w, int &total) {
+    const unsigned long long b = __ballot(flag);
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int in_wave = __popcll(b & ((1ull << lane) - 1ull));
+    if (lane == 0) lds_w[w] = __popcll(b);
     __syncthreads();
-    for (int s = 1; s < NT; s <<= 1) {
-        int add = (int)threadIdx.x >= s ? lds[threadIdx.x - s] : 0;
-        __syncthreads();
-        lds[threadIdx.x] += add;
-        __syncthreads();
+    int before = 0, tot = 0;
+    for (int k = 0; k < NT / 64; k++) {
+        const int c = lds_w[k];
+        before += k < w ? c : 0;
+        tot += c;
     }
-    int incl = lds[threadIdx.x];
-    total = lds[NT - 1];
     __syncthreads();
-    return incl - v;
+    total = tot;
+    return before + in_wave;
 }
 
 // Stable partition of every grid into never-eclipsed / eclipsed faces
 // (FlatGrid::build, comp_light_flat.cpp:26-65) and the packed float4
-// records of the mixed-precision flux kernel.  One workgroup per grid.
-__global__ __launch_bounds__(1024) void k_flat_build(const GridDev *__restrict__ grids, FaceArrays fa, FlatArrays fl,
-                                                    int fp64) {
-    __shared__ int lds[1024];
-    const GridDev &G = grids[blockIdx.x];
-    const int n = G.nface, base = G.face_off;
+// records of the mixed-precision flux kernel, in two passes over segments
+// of FLAT_SEG faces: k_flat_count counts each segment's never-eclipsed
+// faces, k_flat_build places a segment's faces after those of the grid's
+// earlier segments.  The result is the single-pass partition's.
+__global__ __launch_bounds__(256) void k_flat_count(const GridDev *__restrict__ grids,
+                                                    const FlatSeg *__restrict__ segs, FaceArrays fa, int *seg_cnt) {
+    __shared__ int lds_w[4];
+    const FlatSeg S = segs[blockIdx.x];
+    const int base = grids[S.grid].face_off + S.start;
     int cnt = 0;
-    for (int i = threadIdx.x; i < n; i += 1024) cnt += fa.ecl[base + i] ? 0 : 1;
-    int n0;
-    block_scan_excl<1024>(cnt, lds, n0);
-    if (threadIdx.x == 0) fl.n0[blockIdx.x] = n0;
+    for (int i = threadIdx.x; i < S.n; i += 256) cnt += fa.ecl[base + i] ? 0 : 1;
+    // per-wave sums, then the workgroup's
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if ((threadIdx.x & 63) == 0) lds_w[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) seg_cnt[blockIdx.x] = lds_w[0] + lds_w[1] + lds_w[2] + lds_w[3];
+}
+
+__global__ __launch_bounds__(1024) void k_flat_build(const GridDev *__restrict__ grids,
+                                                     const FlatSeg *__restrict__ segs,
+                                                     const int *__restrict__ seg_cnt, FaceArrays fa, FlatArrays fl,
+                                                     int fp64) {
+    __shared__ int lds_w[16];
+    const FlatSeg S = segs[blockIdx.x];
+    const GridDev &G = grids[S.grid];
+    // never-eclipsed faces of the whole grid and of its earlier segments
+    int n0 = 0, before0 = 0;
+    for (int k = 0; k < S.nseg; k++) {
+        const int c = seg_cnt[S.seg0 + k];
+        n0 += c;
+        before0 += S.seg0 + k < (int)blockIdx.x ? c : 0;
+    }
+    if ((int)blockIdx.x == S.seg0 && threadIdx.x == 0) fl.n0[S.grid] = n0;
+    const int before1 = S.start - before0;  // eclipsed faces of the earlier segments
+    const int base = G.face_off;
     int done0 = 0, done1 = 0;
-    for (int c = 0; c < n; c += 1024) {
-        int i = c + (int)threadIdx.x;
-        int plain = (i < n && !fa.ecl[base + i]) ? 1 : 0;
+    for (int c = 0; c < S.n; c += 1024) {
+        const bool in = c + (int)threadIdx.x < S.n;
+        const int i = S.start + c + (int)threadIdx.x;
+        const bool plain = in && !fa.ecl[base + i];
         int tot;
-        int ex = block_scan_excl<1024>(plain, lds, tot);
-        if (i < n) {
-            int pos = plain ? done0 + ex : n0 + done1 + ((int)threadIdx.x - ex);
-            int f = base + i, p = base + pos;
+        const int ex = block_count_excl<1024>(plain, lds_w, tot);
+        if (in) {
+            const int pos = plain ? before0 + done0 + ex : n0 + before1 + done1 + ((int)threadIdx.x - ex);
+            const int f = base + i, p = base + pos;
             fl.perm[p] = f;
             double dx = fa.dx[f], dy = fa.dy[f], dz = fa.dz[f];
             double px = fa.px[f], py = fa.py[f], pz = fa.pz[f];
@@ -400,7 +429,7 @@ __global__ __launch_bounds__(1024) void k_flat_build(const GridDev *__restrict__
             }
         }
         done0 += tot;
-        done1 += min(1024, n - c) - tot;
+        done1 += min(1024, S.n - c) - tot;
     }
 }
 
@@ -632,9 +661,11 @@ void launch_continuum(hipStream_t s, const GridDev *grids, int ngrid, int nface_
     hipLaunchKernelGGL(k_continuum, dim3((nface_total + 255) / 256), dim3(256), 0, s, grids, ngrid, nface_total, fa);
 }
 
-void launch_flat_build(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, FlatArrays fl, int fp64) {
-    if (ngrid <= 0) return;
-    hipLaunchKernelGGL(k_flat_build, dim3(ngrid), dim3(1024), 0, s, grids, fa, fl, fp64);
+void launch_flat_build(hipStream_t s, const GridDev *grids, const FlatSeg *segs, int nseg, int *seg_cnt,
+                       FaceArrays fa, FlatArrays fl, int fp64) {
+    if (nseg <= 0) return;
+    hipLaunchKernelGGL(k_flat_count, dim3(nseg), dim3(256), 0, s, grids, segs, fa, seg_cnt);
+    hipLaunchKernelGGL(k_flat_build, dim3(nseg), dim3(1024), 0, s, grids, segs, seg_cnt, fa, fl, fp64);
 }
 
 void launch_point_sums(hipStream_t s, const GridDev *grids, const InstDev *insts, int ninst, FaceArrays fa,

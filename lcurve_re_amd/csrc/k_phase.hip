@@ -137,10 +137,43 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
 // then coarse, star 2 fine then coarse, each ascending; then `third`),
 // followed by re_scale or the fixed scale factor and chi**2.
 // scal[inst*8]: 0 sfac, 1 chisq, 2 wnok, 3 chisq/wnok written.
-__global__ __launch_bounds__(256) void k_scatter(const InstDev *__restrict__ insts, DataDev d, PhaseArrays pa,
-                                                const double *__restrict__ out, double *calc, double *scal) {
+// Scatter of the sub-phase sums onto the data points, third light,
+// optimal scale and chi**2 (light_curve_comp.cpp:476-586, re_scale.cpp:16-43),
+// in three kernels over blocks of SCAT_PTS points so that one large
+// instance fills the GPU:
+//   k_scatter_sum    calc before scaling, and per block the scale sums;
+//   k_scatter_scale  the scale (every block sums the instance's partials in
+//                    block order, so all agree), scaled calc, chi**2 sums;
+//   k_scatter_final  the instance's chi**2 in block order and its scalars.
+__device__ __forceinline__ double point_value(const InstDev &I, const DataDev &d, const PhaseArrays &pa, size_t base,
+                                              const double *o1, const double *o2, int np) {
+    const int j0 = d.sub_off[np], nd = d.ndiv[np];
+    double v = 0.;
+    for (int k = 0; k < nd; k++) {
+        size_t j = j0 + k;
+        if (I.copy1 || pa.type[base + j] == 1) v += pa.w1[base + j] * o1[j];
+    }
+    if (!I.copy1)
+        for (int k = 0; k < nd; k++) {
+            size_t j = j0 + k;
+            if (pa.type[base + j] != 1) v += pa.w1[base + j] * o1[j];
+        }
+    for (int k = 0; k < nd; k++) {
+        size_t j = j0 + k;
+        if (I.copy2 || pa.type[base + j] == 3) v += pa.w2[base + j] * o2[j];
+    }
+    if (!I.copy2)
+        for (int k = 0; k < nd; k++) {
+            size_t j = j0 + k;
+            if (pa.type[base + j] != 3) v += pa.w2[base + j] * o2[j];
+        }
+    return v + I.third;
+}
+
+__global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__ insts, DataDev d, PhaseArrays pa,
+                                                    const double *__restrict__ out, double *calc, double *part) {
     __shared__ double lds[256];
-    const int inst = blockIdx.x;
+    const int inst = blockIdx.y, nblk = gridDim.x;
     const InstDev &I = insts[inst];
     if (!I.active) return;
     const size_t base = (size_t)inst * d.S;
@@ -148,28 +181,9 @@ __global__ __launch_bounds__(256) void k_scatter(const InstDev *__restrict__ ins
     const double *o2 = out + ((size_t)inst * 2 + 1) * d.S;
     double *c = calc + (size_t)inst * d.n;
     double sdy = 0., syy = 0., wn = 0.;
-    for (int np = threadIdx.x; np < d.n; np += 256) {
-        const int j0 = d.sub_off[np], nd = d.ndiv[np];
-        double v = 0.;
-        for (int k = 0; k < nd; k++) {
-            size_t j = j0 + k;
-            if (I.copy1 || pa.type[base + j] == 1) v += pa.w1[base + j] * o1[j];
-        }
-        if (!I.copy1)
-            for (int k = 0; k < nd; k++) {
-                size_t j = j0 + k;
-                if (pa.type[base + j] != 1) v += pa.w1[base + j] * o1[j];
-            }
-        for (int k = 0; k < nd; k++) {
-            size_t j = j0 + k;
-            if (I.copy2 || pa.type[base + j] == 3) v += pa.w2[base + j] * o2[j];
-        }
-        if (!I.copy2)
-            for (int k = 0; k < nd; k++) {
-                size_t j = j0 + k;
-                if (pa.type[base + j] != 3) v += pa.w2[base + j] * o2[j];
-            }
-        v += I.third;
+    const int p1 = min(d.n, (int)(blockIdx.x + 1) * SCAT_PTS);
+    for (int np = blockIdx.x * SCAT_PTS + threadIdx.x; np < p1; np += 256) {
+        const double v = point_value(I, d, pa, base, o1, o2, np);
         c[np] = v;
         if (d.weight[np] > 0.) {
             double wgt = d.weight[np] / sqr(d.ferr[np]);
@@ -178,17 +192,50 @@ __global__ __launch_bounds__(256) void k_scatter(const InstDev *__restrict__ ins
             wn += d.weight[np];
         }
     }
-    double scale = I.sfac0;
-    bool want_chi = I.rdata != 0;
+    if (!I.scale) return;
+    sdy = block_sum<256>(sdy, lds);
+    syy = block_sum<256>(syy, lds);
+    wn = block_sum<256>(wn, lds);
+    if (threadIdx.x == 0) {
+        double *pp = part + ((size_t)inst * nblk + blockIdx.x) * 5;
+        pp[0] = sdy;
+        pp[1] = syy;
+        pp[2] = wn;
+    }
+}
+
+// the instance's scale from the block partials, summed in block order
+__device__ __forceinline__ void inst_scale(const InstDev &I, const double *pi, int nblk, double &scale, double &wn,
+                                           bool &want_chi) {
+    scale = I.sfac0;
+    want_chi = I.rdata != 0;
+    wn = 0.;
     if (I.scale) {
-        sdy = block_sum<256>(sdy, lds);
-        syy = block_sum<256>(syy, lds);
-        wn = block_sum<256>(wn, lds);
+        double sdy = 0., syy = 0.;
+        for (int b = 0; b < nblk; b++) {
+            sdy += pi[b * 5 + 0];
+            syy += pi[b * 5 + 1];
+            wn += pi[b * 5 + 2];
+        }
         scale = wn > 0.0 ? sdy / syy : 1.;
         want_chi = wn > 0.0;
     }
+}
+
+__global__ __launch_bounds__(256) void k_scatter_scale(const InstDev *__restrict__ insts, DataDev d, double *calc,
+                                                      double *part) {
+    __shared__ double lds[256];
+    const int inst = blockIdx.y, nblk = gridDim.x;
+    const InstDev &I = insts[inst];
+    if (!I.active) return;
+    double *pi = part + (size_t)inst * nblk * 5;
+    double scale, wn;
+    bool want_chi;
+    inst_scale(I, pi, nblk, scale, wn, want_chi);
+    double *c = calc + (size_t)inst * d.n;
     double chi = 0., wn2 = 0.;
-    for (int np = threadIdx.x; np < d.n; np += 256) {
+    const int p1 = min(d.n, (int)(blockIdx.x + 1) * SCAT_PTS);
+    for (int np = blockIdx.x * SCAT_PTS + threadIdx.x; np < p1; np += 256) {
         double v = c[np];
         if (!I.scale || wn > 0.0) v *= scale;
         c[np] = v;
@@ -200,17 +247,34 @@ __global__ __launch_bounds__(256) void k_scatter(const InstDev *__restrict__ ins
     chi = block_sum<256>(chi, lds);
     wn2 = block_sum<256>(wn2, lds);
     if (threadIdx.x == 0) {
-        double *s = scal + (size_t)inst * 8;
-        s[0] = scale;
-        if (I.scale) {
-            s[1] = chi;
-            s[2] = wn;
-            s[3] = 1.;
-        } else {
-            s[1] = chi;
-            s[2] = wn2;
-            s[3] = want_chi ? 1. : 0.;
-        }
+        pi[blockIdx.x * 5 + 3] = chi;
+        pi[blockIdx.x * 5 + 4] = wn2;
+    }
+}
+
+__global__ void k_scatter_final(const InstDev *__restrict__ insts, int nblk, const double *part, double *scal) {
+    const int inst = blockIdx.x;
+    const InstDev &I = insts[inst];
+    if (!I.active || threadIdx.x != 0) return;
+    const double *pi = part + (size_t)inst * nblk * 5;
+    double scale, wn;
+    bool want_chi;
+    inst_scale(I, pi, nblk, scale, wn, want_chi);
+    double chi = 0., wn2 = 0.;
+    for (int b = 0; b < nblk; b++) {
+        chi += pi[b * 5 + 3];
+        wn2 += pi[b * 5 + 4];
+    }
+    double *s = scal + (size_t)inst * 8;
+    s[0] = scale;
+    if (I.scale) {
+        s[1] = chi;
+        s[2] = wn;
+        s[3] = 1.;
+    } else {
+        s[1] = chi;
+        s[2] = wn2;
+        s[3] = want_chi ? 1. : 0.;
     }
 }
 
@@ -227,9 +291,12 @@ void launch_route(hipStream_t s, const InstDev *insts, int ninst, int S, PhaseAr
 }
 
 void launch_scatter(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa, const double *out,
-                    double *calc, double *scal) {
+                    double *calc, double *scal, double *part) {
     if (ninst <= 0) return;
-    hipLaunchKernelGGL(k_scatter, dim3(ninst), dim3(256), 0, s, insts, d, pa, out, calc, scal);
+    const int nblk = std::max(1, (d.n + SCAT_PTS - 1) / SCAT_PTS);
+    hipLaunchKernelGGL(k_scatter_sum, dim3(nblk, ninst), dim3(256), 0, s, insts, d, pa, out, calc, part);
+    hipLaunchKernelGGL(k_scatter_scale, dim3(nblk, ninst), dim3(256), 0, s, insts, d, calc, part);
+    hipLaunchKernelGGL(k_scatter_final, dim3(ninst), dim3(64), 0, s, insts, nblk, part, scal);
 }
 
 }  // namespace lck

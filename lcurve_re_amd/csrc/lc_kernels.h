@@ -117,7 +117,16 @@ void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays f
 // grids (not the fluxes, which the continuum kernel recomputes).
 void launch_copy_faces(hipStream_t s, const FaceCopy *items, int nitem, FaceArrays src, FaceArrays dst);
 void launch_continuum(hipStream_t s, const GridDev *grids, int ngrid, int nface_total, FaceArrays fa);
-void launch_flat_build(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, FlatArrays fl, int fp64);
+// One segment of a grid for the FlatGrid build: faces [start, start + n)
+// of grid `grid` (start counted from the grid's first face); the grid's
+// segments are seg0 .. seg0 + nseg - 1.  A grid without faces has one
+// empty segment.
+constexpr int FLAT_SEG = 8192;
+struct FlatSeg {
+    int grid, start, n, seg0, nseg, pad[3];
+};
+void launch_flat_build(hipStream_t s, const GridDev *grids, const FlatSeg *segs, int nseg, int *seg_cnt,
+                       FaceArrays fa, FlatArrays fl, int fp64);
 void launch_point_sums(hipStream_t s, const GridDev *grids, const InstDev *insts, int ninst, FaceArrays fa,
                        double *sums);
 void launch_gint_scales(hipStream_t s, InstDev *insts, int ninst, const double *sums);
@@ -135,8 +144,10 @@ void launch_make_tasks(hipStream_t s, const int *first, const int *order, int nb
                        int ntask, TaskDev *tasks);
 void launch_flux_reduce(hipStream_t s, const BatchDev *batches, int nbatch, int S, int threads_per_tile,
                         const int *lists, const int *counts, const double *partial, double *out);
+// part: workspace of ninst * ceil(n / SCAT_PTS) * 5 doubles
+constexpr int SCAT_PTS = 1024;
 void launch_scatter(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa, const double *out,
-                    double *calc, double *scal);
+                    double *calc, double *scal, double *part);
 void launch_diag(hipStream_t s, const GridDev *grids, const int *gidx, int ng, FaceArrays fa, double *sums);
 
 }  // namespace lck

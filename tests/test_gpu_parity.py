@@ -168,6 +168,29 @@ def test_batch_matches_single(engine):
         assert np.array_equal(c1, calc[i]), i
 
 
+def test_packed_batch_and_out_array(engine):
+    """A ModelBatch and a caller-owned output array give the list path's
+    curves bit for bit; rows of failed instances keep their old contents."""
+    import lcurve_re_amd as L
+    from lcurve_re_amd import workloads as W
+    w = W.batch_prior(m=4, nphase=300, nlat_lo=20, nlat_hi=30)
+    models = [L.Model.from_json(c) for c in w["models"]]
+    models.insert(2, L.Model.from_json(W.model_json(phys={"r2": 0.6})))  # Roche overflow
+    data = L.make_data(w["time"])
+    ref, _, st_ref = engine.light_curve_comp_batch(models, data)
+    out = np.full((len(models), len(data)), -7.0)
+    got, _, st = engine.light_curve_comp_batch(L.ModelBatch(models), data, out=out)
+    assert got is out
+    assert list(st) == list(st_ref) and st[2] == 1
+    for i in range(len(models)):
+        if i == 2:
+            assert (out[i] == -7.0).all()
+        else:
+            assert np.array_equal(out[i], ref[i]), i
+    with pytest.raises(ValueError):
+        engine.light_curve_comp_batch(models, data, out=np.zeros((len(models), len(data) + 1)))
+
+
 def test_roche_overflow_raises(engine):
     import lcurve_re_amd as L
     from lcurve_re_amd import workloads as W
