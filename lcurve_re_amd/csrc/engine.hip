@@ -1064,7 +1064,10 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<int>(std::max<size_t>(task_order.size(), 1));
     sz.add<double>(std::max<size_t>(M, 1) * n);  // calc
     sz.add<double>(std::max<size_t>(M, 1) * 8);  // scalars
-    sz.add<double>(std::max<size_t>(M, 1) * 8);  // gint sums
+    int max_nface = 1;
+    for (int g = 0; g < ngrid; g++) max_nface = std::max(max_nface, gdev[g].nface);
+    const int psum_chunks = any_route ? (max_nface + PSUM_FACES - 1) / PSUM_FACES : 1;
+    sz.add<double>(std::max<size_t>(M, 1) * 8 * psum_chunks);  // Ginterp partial sums
     const size_t scat_part = std::max<size_t>(M, 1) * std::max<size_t>(1, (n + SCAT_PTS - 1) / SCAT_PTS) * 5;
     sz.add<double>(scat_part);
     sz.add<int>(std::max<size_t>(diag_idx.size(), 1));
@@ -1139,7 +1142,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     TaskDev *d_tasks = A.take<TaskDev>(std::max(ntask, 1));
     double *d_calc = A.take<double>(std::max<size_t>(M, 1) * n);
     double *d_scal = A.take<double>(std::max<size_t>(M, 1) * 8);
-    double *d_gsums = A.take<double>(std::max<size_t>(M, 1) * 8);
+    double *d_gsums = A.take<double>(std::max<size_t>(M, 1) * 8 * psum_chunks);
     double *d_scat_part = A.take<double>(scat_part);
     double *d_diag = A.take<double>(std::max<size_t>(diag_idx.size(), 1) * 8);
 
@@ -1242,8 +1245,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     launch_continuum(s, d_grids, ngrid, nface_total, fa);
     HIPCHK(hipEventRecord(ev[3], s));
     if (any_route) {
-        launch_point_sums(s, d_grids, d_insts, (int)M, fa, d_gsums);
-        launch_gint_scales(s, d_insts, (int)M, d_gsums);
+        launch_point_sums(s, d_grids, d_insts, (int)M, fa, psum_chunks, d_gsums);
+        launch_gint_scales(s, d_insts, (int)M, psum_chunks, d_gsums);
     }
     launch_flat_build(s, d_grids, d_fsegs, (int)fsegs.size(), d_seg_cnt, fa, fl, fp64 ? 1 : 0);
     launch_phases(s, d_insts, (int)M, dd, pa);

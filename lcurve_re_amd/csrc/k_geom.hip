@@ -343,26 +343,6 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
     return r;
 }
 
-// Exclusive prefix count of a 0/1 flag over the workgroup in thread order
-// (NT a multiple of 64), from wave ballots; the total through `total`.
-template <int NT>
-__device__ __forceinline__ int block_count_excl(bool flag, int *lds_w, int &total) {
-    const unsigned long long b = __ballot(flag);
-    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-    const int in_wave = __popcll(b & ((1ull << lane) - 1ull));
-    if (lane == 0) lds_w[w] = __popcll(b);
-    __syncthreads();
-    int before = 0, tot = 0;
-    for (int k = 0; k < NT / 64; k++) {
-        const int c = lds_w[k];
-        before += k < w ? c : 0;
-        tot += c;
-    }
-    __syncthreads();
-    total = tot;
-    return before + in_wave;
-}
-
 // Stable partition of every grid into never-eclipsed / eclipsed faces
 // (FlatGrid::build, comp_light_flat.cpp:26-65) and the packed float4
 // records of the mixed-precision flux kernel, in two passes over segments
@@ -505,12 +485,14 @@ __device__ __forceinline__ double gint_phase(int k, const InstDev &I) {
     }
 }
 
-// Raw Point-path sums for the Ginterp scale factors: block (k, inst) sums
-// the grid type(phi) selects.  sums[inst * 8 + k].
-__global__ __launch_bounds__(1024) void k_point_sums(const GridDev *__restrict__ grids,
-                                                    const InstDev *__restrict__ insts, FaceArrays fa, double *sums) {
-    __shared__ double lds[1024];
-    const int k = blockIdx.x, inst = blockIdx.y;
+// Raw Point-path sums for the Ginterp scale factors: block (c, k, inst)
+// sums chunk c of the grid that type(phi_k) selects into
+// part[(inst * 8 + k) * nchunk + c] (zero past the grid's end).
+__global__ __launch_bounds__(256) void k_point_sums(const GridDev *__restrict__ grids,
+                                                   const InstDev *__restrict__ insts, FaceArrays fa, int nchunk,
+                                                   double *part) {
+    __shared__ double lds[256];
+    const int c = blockIdx.x, k = blockIdx.y, inst = blockIdx.z;
     const InstDev &I = insts[inst];
     const bool star1 = k < 4;
     if (!I.active || (star1 && I.copy1) || (!star1 && I.copy2)) return;
@@ -519,10 +501,11 @@ __global__ __launch_bounds__(1024) void k_point_sums(const GridDev *__restrict__
     int gi = star1 ? (ty == 1 ? I.grid1f : I.grid1c) : (ty == 3 ? I.grid2f : I.grid2c);
     const GridDev &G = grids[gi];
     V3 earth = set_earth(I.cosi, I.sini, phi);
+    const int i1 = min(G.nface, (c + 1) * PSUM_FACES);
     double s = 0.0;
-    for (int i = threadIdx.x; i < G.nface; i += 1024) s += point_elem(G, fa, G.face_off + i, earth, phi);
-    s = block_sum<1024>(s, lds);
-    if (threadIdx.x == 0) sums[inst * 8 + k] = s;
+    for (int i = c * PSUM_FACES + (int)threadIdx.x; i < i1; i += 256) s += point_elem(G, fa, G.face_off + i, earth, phi);
+    s = block_sum<256>(s, lds);
+    if (threadIdx.x == 0) part[((size_t)inst * 8 + k) * nchunk + c] = s;
 }
 
 __device__ __forceinline__ double gscale1(const InstDev &I, double phase) {
@@ -540,13 +523,19 @@ __device__ __forceinline__ double gscale2(const InstDev &I, double phase) {
 // Ginterp scale factors in the reference's update order
 // (light_curve_comp.cpp:214-254): each comp_star value is
 // (0 + 1 * scale(phi) * s) / 1 with the scales updated so far.
-__global__ void k_gint_scales(InstDev *insts, int ninst, const double *sums) {
+__global__ void k_gint_scales(InstDev *insts, int ninst, int nchunk, const double *part) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
     InstDev I = insts[i];
     if (!I.active) return;
     I.scale11 = I.scale12 = I.scale21 = I.scale22 = 1.;
-    const double *s = sums + i * 8;
+    double s[8];
+    for (int k = 0; k < 8; k++) {  // the chunk sums in chunk order
+        const bool used = k < 4 ? !I.copy1 : !I.copy2;
+        s[k] = 0.;
+        if (used)
+            for (int c = 0; c < nchunk; c++) s[k] += part[((size_t)i * 8 + k) * nchunk + c];
+    }
     if (!I.copy1) {
         double ff = (0.0 + 1.0 * gscale1(I, gint_phase(0, I)) * s[0]) / 1;
         double fc = (0.0 + 1.0 * gscale1(I, gint_phase(1, I)) * s[1]) / 1;
@@ -669,14 +658,14 @@ void launch_flat_build(hipStream_t s, const GridDev *grids, const FlatSeg *segs,
 }
 
 void launch_point_sums(hipStream_t s, const GridDev *grids, const InstDev *insts, int ninst, FaceArrays fa,
-                       double *sums) {
+                       int nchunk, double *part) {
     if (ninst <= 0) return;
-    hipLaunchKernelGGL(k_point_sums, dim3(8, ninst), dim3(1024), 0, s, grids, insts, fa, sums);
+    hipLaunchKernelGGL(k_point_sums, dim3(nchunk, 8, ninst), dim3(256), 0, s, grids, insts, fa, nchunk, part);
 }
 
-void launch_gint_scales(hipStream_t s, InstDev *insts, int ninst, const double *sums) {
+void launch_gint_scales(hipStream_t s, InstDev *insts, int ninst, int nchunk, const double *part) {
     if (ninst <= 0) return;
-    hipLaunchKernelGGL(k_gint_scales, dim3((ninst + 63) / 64), dim3(64), 0, s, insts, ninst, sums);
+    hipLaunchKernelGGL(k_gint_scales, dim3((ninst + 63) / 64), dim3(64), 0, s, insts, ninst, nchunk, part);
 }
 
 void launch_diag(hipStream_t s, const GridDev *grids, const int *gidx, int ng, FaceArrays fa, double *sums) {

@@ -69,27 +69,12 @@ __global__ __launch_bounds__(256) void k_phases(const InstDev *__restrict__ inst
     }
 }
 
-__device__ __forceinline__ int scan1024(int v, int *lds, int &total) {
-    lds[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = 1; s < 1024; s <<= 1) {
-        int add = (int)threadIdx.x >= s ? lds[threadIdx.x - s] : 0;
-        __syncthreads();
-        lds[threadIdx.x] += add;
-        __syncthreads();
-    }
-    int incl = lds[threadIdx.x];
-    total = lds[1023];
-    __syncthreads();
-    return incl - v;
-}
-
 // Fine/coarse sub-phase lists of one (instance, star) in ascending order
 // (the PhaseBatch routing of light_curve_comp.cpp:448-453).  Only runs for
 // stars whose coarse grid is not an alias of the fine one.
 __global__ __launch_bounds__(1024) void k_route(const InstDev *__restrict__ insts, int S, PhaseArrays pa, int *lists,
                                                int *counts) {
-    __shared__ int lds[1024];
+    __shared__ int lds_w[16];
     const int star = blockIdx.x, inst = blockIdx.y;
     const InstDev &I = insts[inst];
     if (!I.active || (star == 0 ? I.copy1 : I.copy2)) return;
@@ -98,13 +83,13 @@ __global__ __launch_bounds__(1024) void k_route(const InstDev *__restrict__ inst
     int nf = 0, nc = 0;
     for (int c = 0; c < S; c += 1024) {
         const int j = c + (int)threadIdx.x;
-        int isf = 0;
+        bool isf = false;
         if (j < S) {
             int t = pa.type[(size_t)inst * S + j];
             isf = star == 0 ? (t == 1) : (t == 3);
         }
         int tot;
-        int ex = scan1024(isf, lds, tot);
+        int ex = block_count_excl<1024>(isf, lds_w, tot);
         if (j < S) {
             if (isf)
                 fine[nf + ex] = j;

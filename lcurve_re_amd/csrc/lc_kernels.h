@@ -117,6 +117,27 @@ void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays f
 // grids (not the fluxes, which the continuum kernel recomputes).
 void launch_copy_faces(hipStream_t s, const FaceCopy *items, int nitem, FaceArrays src, FaceArrays dst);
 void launch_continuum(hipStream_t s, const GridDev *grids, int ngrid, int nface_total, FaceArrays fa);
+// Exclusive prefix count of a 0/1 flag over the workgroup in thread order
+// (NT a multiple of 64), from wave ballots; the total through `total`.
+// lds_w holds NT / 64 ints.
+template <int NT>
+__device__ __forceinline__ int block_count_excl(bool flag, int *lds_w, int &total) {
+    const unsigned long long b = __ballot(flag);
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int in_wave = __popcll(b & ((1ull << lane) - 1ull));
+    if (lane == 0) lds_w[w] = __popcll(b);
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int k = 0; k < NT / 64; k++) {
+        const int c = lds_w[k];
+        before += k < w ? c : 0;
+        tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return before + in_wave;
+}
+
 // One segment of a grid for the FlatGrid build: faces [start, start + n)
 // of grid `grid` (start counted from the grid's first face); the grid's
 // segments are seg0 .. seg0 + nseg - 1.  A grid without faces has one
@@ -127,9 +148,13 @@ struct FlatSeg {
 };
 void launch_flat_build(hipStream_t s, const GridDev *grids, const FlatSeg *segs, int nseg, int *seg_cnt,
                        FaceArrays fa, FlatArrays fl, int fp64);
+// Ginterp single-phase sums in chunks of PSUM_FACES faces: part holds
+// ninst * 8 * nchunk partial sums, reduced in chunk order by
+// launch_gint_scales.
+constexpr int PSUM_FACES = 16384;
 void launch_point_sums(hipStream_t s, const GridDev *grids, const InstDev *insts, int ninst, FaceArrays fa,
-                       double *sums);
-void launch_gint_scales(hipStream_t s, InstDev *insts, int ninst, const double *sums);
+                       int nchunk, double *part);
+void launch_gint_scales(hipStream_t s, InstDev *insts, int ninst, int nchunk, const double *part);
 void launch_phases(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa);
 void launch_route(hipStream_t s, const InstDev *insts, int ninst, int S, PhaseArrays pa, int *lists, int *counts);
 // variant = ltype | beaming << 1 | lensing << 2 | (mucrit > 0) << 3;
