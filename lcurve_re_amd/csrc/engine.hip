@@ -583,6 +583,11 @@ struct lc_ctx {
     int flux_rphase = 2;
     int flux_chunk = 1024;
     bool dedupe = true;  // build grids equal within one batch once
+    bool ecl_stop = true;  // eclipse searches end once no data phase can tell
+    // below this many faces per call the searches are latency bound: the
+    // longest one sets the time and stopping others early saves nothing
+    long long ecl_stop_min_faces = 1LL << 17;
+    long long ecl_round_min = 1LL << 21;  // faces x 2 from which early-stopping searches run in rounds
     GeoCache geo;
 };
 
@@ -917,6 +922,26 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         }
     }
 
+    // Eclipse searches that may stop early: grids used by one instance only
+    // (no in-batch copies) and offered to no cache, whose ranges are then
+    // tested only at that instance's phases.
+    bool any_ecl_stop = false;
+    {
+        std::vector<char> shared(ngrid, 0);
+        for (int g = 0; g < ngrid; g++)
+            if (dup_of[g] >= 0) shared[dup_of[g]] = 1;
+        for (const auto &ins : inserts) shared[ins.first] = 1;
+        const bool stop = ctx->ecl_stop && nface_total >= ctx->ecl_stop_min_faces;
+        for (int g = 0; g < ngrid; g++) {
+            gdev[g].ecl_stop = (stop && computed(g) && !shared[g]) ? 1 : 0;
+            any_ecl_stop = any_ecl_stop || gdev[g].ecl_stop;
+        }
+    }
+    // phase bins: about 64 bins per phase, 2^10 .. 2^22 per instance
+    int bits_lb = 10;
+    while (bits_lb < 22 && (double)(1u << bits_lb) < 64.0 * (S + 9)) bits_lb++;
+    const size_t bits_words = any_ecl_stop ? (std::max<size_t>(M, 1) << (bits_lb - 5)) : 1;
+
     // FlatGrid build segments
     std::vector<FlatSeg> fsegs;
     for (int g = 0; g < ngrid; g++) {
@@ -1033,6 +1058,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<int>(FN);
     sz.add<int>(FN);
     for (int k = 0; k < 6; k++) sz.add<double>(FN);
+    for (int k = 0; k < 4; k++) sz.add<double>(FN);  // bisection brackets
+    for (int k = 0; k < 4; k++) sz.add<int>(FN);     // bisection round lists
+    sz.add<int>(ECL_ROUNDS);
     sz.add<float4>(FN);
     sz.add<float4>(FN);
     sz.add<float4>(FN);
@@ -1043,6 +1071,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<int>(std::max(ngrid, 1));
     sz.add<FlatSeg>(fsegs.size());
     sz.add<int>(fsegs.size());
+    sz.add<unsigned>(bits_words);
     sz.add<unsigned long long>(16 * 64);
     sz.add<FaceCopy>(std::max<size_t>(copy_hit.size(), 1));
     sz.add<FaceCopy>(std::max<size_t>(copy_dup.size(), 1));
@@ -1114,6 +1143,14 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     ew.lim = A.take<double>(2 * FN);
     ew.acc = A.take<double>(FN);
     ew.res = A.take<double>(2 * FN);
+    ew.b_in = A.take<double>(2 * FN);
+    ew.b_out = A.take<double>(2 * FN);
+    ew.alist[0] = A.take<int>(2 * FN);
+    ew.alist[1] = A.take<int>(2 * FN);
+    ew.acount = A.take<int>(ECL_ROUNDS);
+    unsigned *d_bits = A.take<unsigned>(bits_words);
+    ew.phase_bits = d_bits;
+    ew.bits_lb = bits_lb;
     FlatArrays fl;
     std::memset(&fl, 0, sizeof(fl));
     fl.rec0 = A.take<float4>(FN);
@@ -1211,6 +1248,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     }
     HIPCHK(hipMemsetAsync(d_cand_count, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(ew.hit_count, 0, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(ew.acount, 0, ECL_ROUNDS * sizeof(int), s));
     for (auto &g : gaps) {
         HIPCHK(hipMemsetAsync(fa.px + g.first, 0, g.second * 8, s));
         HIPCHK(hipMemsetAsync(fa.py + g.first, 0, g.second * 8, s));
@@ -1232,10 +1270,15 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
 
     // ---------------- launches
     hipEvent_t *ev = ctx->ev;
+    if (any_ecl_stop) {
+        HIPCHK(hipMemsetAsync(d_bits, 0, bits_words * sizeof(unsigned), s));
+        launch_phase_bits(s, d_insts, (int)M, dd, d_bits, bits_lb);
+    }
     HIPCHK(hipEventRecord(ev[0], s));
     launch_faces(s, d_grids, d_rings, nring, nhalf_total, fa, d_cand, d_cand_count, d_status);
     HIPCHK(hipEventRecord(ev[1], s));
-    launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, ew, d_status);
+    launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, ew, d_status,
+                   any_ecl_stop && 2LL * nface_total >= ctx->ecl_round_min);
     // reused grids: cache hits first (a duplicate may copy from one), then
     // duplicates within the batch, then the new grids into the cache
     if (!copy_hit.empty()) launch_copy_faces(s, d_copy_hit, (int)copy_hit.size(), ctx->geo.pool, fa);
@@ -1524,6 +1567,9 @@ lc_status lc_ctx_create(int device, lc_ctx **out) {
         // (disabled by the same environment variable)
         c->geo.cap = std::getenv("LCURVE_NO_GRID_CACHE") ? 0 : DEFAULT_GEO_CACHE_FACES;
         if (const char *d = std::getenv("LCURVE_AMD_DEDUPE")) c->dedupe = std::atoi(d) != 0;
+        if (std::getenv("LCURVE_AMD_FULL_ECLIPSE")) c->ecl_stop = false;
+        if (const char *r = std::getenv("LCURVE_AMD_ECL_ROUNDS_MIN")) c->ecl_round_min = std::atoll(r);
+        if (const char *r = std::getenv("LCURVE_AMD_ECL_STOP_MIN_FACES")) c->ecl_stop_min_faces = std::atoll(r);
         *out = c.release();
         return LC_OK;
     });
@@ -1796,7 +1842,8 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     G.face_off = 0; G.ring_off = 0; G.nring = (int)rings.size(); G.half_off = 0; G.nhalf = ho; G.inst = 0;
     if (nf == 0) return LC_OK;
     const size_t FN = (size_t)nf;
-    ctx->dev.reserve(FN * (6 * 8 + 3 * 4 + 4 + 16 + 4 + 4 + 4 + 6 * 8) + rings.size() * sizeof(RingDev) + 65536);
+    ctx->dev.reserve(FN * (6 * 8 + 3 * 4 + 4 + 16 + 4 + 4 + 4 + 6 * 8 + 4 * 8 + 4 * 4) +
+                     rings.size() * sizeof(RingDev) + 65536);
     ctx->dev.reset();
     DevArena &A = ctx->dev;
     hipStream_t s = ctx->stream;
@@ -1816,11 +1863,19 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     ew.lim = A.take<double>(2 * FN);
     ew.acc = A.take<double>(FN);
     ew.res = A.take<double>(2 * FN);
+    ew.b_in = A.take<double>(2 * FN);
+    ew.b_out = A.take<double>(2 * FN);
+    ew.alist[0] = A.take<int>(2 * FN);
+    ew.alist[1] = A.take<int>(2 * FN);
+    ew.acount = A.take<int>(ECL_ROUNDS);
+    ew.phase_bits = nullptr;  // no data: full searches (G.ecl_stop = 0)
+    ew.bits_lb = 10;
     GridDev *d_grid = dev_upload(A, &G, 1, s);
     RingDev *d_rings = dev_upload(A, rings.data(), rings.size(), s);
     HIPCHK(hipMemsetAsync(d_count, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(d_status, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(ew.hit_count, 0, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(ew.acount, 0, ECL_ROUNDS * sizeof(int), s));
     if (gp.ring_faces < nf) {
         int gap = nf - gp.ring_faces, g0 = gp.ring_faces;
         for (double *p : {fa.px, fa.py, fa.pz, fa.dx, fa.dy, fa.dz, fa.ein, fa.eout})
@@ -1834,7 +1889,8 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     HIPCHK(hipEventRecord(ctx->ev[0], s));
     launch_faces(s, d_grid, d_rings, (int)rings.size(), ho, fa, d_cand, d_count, d_status);
     HIPCHK(hipEventRecord(ctx->ev[1], s));
-    launch_eclipse(s, d_grid, 1, fa, d_cand, d_count, nf, ew, d_status);
+    // full searches gain nothing from rounds; LCURVE_AMD_ECL_ROUNDS_MIN=0 forces them (tests)
+    launch_eclipse(s, d_grid, 1, fa, d_cand, d_count, nf, ew, d_status, ctx->ecl_round_min <= 0);
     HIPCHK(hipEventRecord(ctx->ev[2], s));
     HIPCHK(hipGetLastError());
     int st = 0;

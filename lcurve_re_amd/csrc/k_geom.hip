@@ -185,15 +185,51 @@ __global__ __launch_bounds__(128) void k_eclipse_potmin(const GridDev *__restric
     }
 }
 
+// True when no phase at which the instance's eclipse ranges are tested can
+// lie in the bisection bracket [lo, hi] (raw phases).  Every test is a
+// comparison of such a phase with the normalised ingress or egress
+// (ecl_range, comp_light_flat.cpp:171-173; Point::visible), and the
+// normalisation x - floor(x) and the egress + 1 shift are monotone, so any
+// two values inside a clear bracket give the same answer to every test: the
+// search may end there instead of halving on to delta_phase.  Conservative:
+// the bracket may not contain an integer, margins of 1e-12 cover the
+// roundings of the normalisation and of the shift, and brackets wider than
+// 64 bins are not looked at.
+__device__ __forceinline__ bool bracket_clear(const unsigned *bits, int lb, double lo, double hi) {
+    const double F = floor(lo);
+    if (floor(hi) != F) return false;
+    const double a = (lo - F) - 1e-12, b = (hi - F) + 1e-12;
+    if (!(a >= 0.0) || !(b < 1.0)) return false;
+    const double B = (double)(1u << lb);
+    if ((b - a) * B > 64.0) return false;
+    const int b0 = (int)(a * B), b1 = (int)(b * B);
+    for (int w = b0 >> 5; w <= (b1 >> 5); w++) {
+        unsigned m = bits[w];
+        if (w == (b0 >> 5)) m &= ~0u << (b0 & 31);
+        if (w == (b1 >> 5)) m &= ~0u >> (31 - (b1 & 31));
+        if (m) return false;
+    }
+    return true;
+}
+
 #ifndef LC_ECL_WAVES
-#define LC_ECL_WAVES 4  // 128 VGPRs: four waves per SIMD without spills
+#define LC_ECL_WAVES 3  // up to 168 VGPRs: three waves per SIMD, no spills
 #endif
+// One round of the ingress/egress bisections: every open search (the
+// items 2*hit + side; all of them in round 0) takes up to nstep more steps;
+// a finished search stores its result, an open one its bracket and its
+// index in the next round's list.  Splitting the searches into rounds keeps
+// lanes whose search has ended from idling until the longest search of
+// their wave ends.
 __global__ __launch_bounds__(128, LC_ECL_WAVES) void k_eclipse_bisect(const GridDev *__restrict__ grids, int ngrid,
                                                                      FaceArrays fa, EclipseWork ew,
-                                                                     int *inst_status) {
-    const int n = *ew.hit_count * 2;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int k = i >> 1, side = i & 1;
+                                                                     int *inst_status, int round, int nstep) {
+    const int *in = round == 0 ? nullptr : ew.alist[(round - 1) & 1];
+    const int n = round == 0 ? *ew.hit_count * 2 : ew.acount[round - 1];
+    int *out = ew.alist[round & 1];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int i = in ? in[t] : t;
+        const int k = i >> 1;
         const int f = ew.face[k];
         const GridDev &G = grids[ew.grid[k]];
         // the grid's constants in registers for the whole search
@@ -202,9 +238,12 @@ __global__ __launch_bounds__(128, LC_ECL_WAVES) void k_eclipse_bisect(const Grid
         const double rref = G.rref_other, pref = G.pref_other;
         const V3 r = v3(fa.px[f], fa.py[f], fa.pz[f]);
         const double ACC = ew.acc[k];
-        double pin = ew.phi[k], pout = ew.lim[2 * k + side];
+        double pin = in ? ew.b_in[i] : ew.phi[k], pout = in ? ew.b_out[i] : ew.lim[i];
+        const unsigned *bits = G.ecl_stop ? ew.phase_bits + ((size_t)G.inst << (ew.bits_lb - 5)) : nullptr;
         int st = ST_OK;
-        while (fabs(pin - pout) > delta) {
+        bool done = false;
+        for (int s = 0; s < nstep; s++) {
+            if (!(fabs(pin - pout) > delta)) break;
             const double pmid = (pin + pout) / 2.;
             bool h;
             st = fblink(q, other, spin, ACC, set_earth(cosi, sini, pmid), r, rref, pref, h);
@@ -213,14 +252,22 @@ __global__ __launch_bounds__(128, LC_ECL_WAVES) void k_eclipse_bisect(const Grid
                 pin = pmid;
             else
                 pout = pmid;
+            if (bits && bracket_clear(bits, ew.bits_lb, fmin(pin, pout), fmax(pin, pout))) {
+                done = true;
+                break;
+            }
         }
         if (st) {
             set_status(inst_status, G.inst, st);
             ew.res[i] = 0.;
-            continue;
+        } else if (done || !(fabs(pin - pout) > delta)) {
+            const double x = (pin + pout) / 2.;
+            ew.res[i] = x - floor(x);
+        } else {
+            ew.b_in[i] = pin;
+            ew.b_out[i] = pout;
+            out[atomicAdd(&ew.acount[round], 1)] = i;
         }
-        const double x = (pin + pout) / 2.;
-        ew.res[i] = x - floor(x);
     }
 }
 
@@ -471,19 +518,6 @@ __device__ __forceinline__ int gint_type(double phase, double phase1, double pha
     return 3;
 }
 
-// Phases at which light_curve_comp.cpp:220-254 samples the Point-path sums.
-__device__ __forceinline__ double gint_phase(int k, const InstDev &I) {
-    switch (k) {
-    case 0: return 0.9999999999 * I.phase1;
-    case 1: return 1.0000000001 * I.phase1;
-    case 2: return 1. - 0.9999999999 * I.phase1;
-    case 3: return 1. - 1.0000000001 * I.phase1;
-    case 4: return 1 - 1.0000000001 * I.phase2;
-    case 5: return 1 - 0.9999999999 * I.phase2;
-    case 6: return 1.0000000001 * I.phase2;
-    default: return 0.9999999999 * I.phase2;
-    }
-}
 
 // Raw Point-path sums for the Ginterp scale factors: block (c, k, inst)
 // sums chunk c of the grid that type(phi_k) selects into
@@ -630,13 +664,18 @@ void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int
 }
 
 void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int *cand,
-                    const int *cand_count, int max_cand, EclipseWork ew, int *inst_status) {
+                    const int *cand_count, int max_cand, EclipseWork ew, int *inst_status, bool rounds) {
     if (max_cand <= 0 || ngrid <= 0) return;
     const int blocks = std::min((max_cand + 127) / 128, 16384);
     hipLaunchKernelGGL(k_eclipse_potmin, dim3(blocks), dim3(128), 0, s, grids, ngrid, fa, cand, cand_count, ew,
                        inst_status);
     const int blocks2 = std::min((2 * max_cand + 127) / 128, 16384);
-    hipLaunchKernelGGL(k_eclipse_bisect, dim3(blocks2), dim3(128), 0, s, grids, ngrid, fa, ew, inst_status);
+    const int nround = rounds ? ECL_ROUNDS : 1;
+    for (int round = 0; round < nround; round++) {
+        const int nstep = round == nround - 1 ? (1 << 30) : (round == 0 ? ECL_FIRST_STEPS : ECL_STEPS);
+        hipLaunchKernelGGL(k_eclipse_bisect, dim3(blocks2), dim3(128), 0, s, grids, ngrid, fa, ew, inst_status, round,
+                           nstep);
+    }
     hipLaunchKernelGGL(k_eclipse_final, dim3(std::min((max_cand + 255) / 256, 4096)), dim3(256), 0, s, fa, ew);
 }
 

@@ -31,6 +31,23 @@ __device__ __forceinline__ double gscale2(const InstDev &I, double phase) {
 // deltat term, the slope/quad/cube factor, then every sub-exposure's earth
 // vector, normalised phase, trapezoid weights and grid type
 // (light_curve_comp.cpp:358-376, 425-455).
+// Orbital phase of a data point: four Newton steps on
+// t0 + phase (P + Pdot phase) = time, then the deltat term
+// (light_curve_comp.cpp:358-376).
+__device__ __forceinline__ double point_phase(const InstDev &I, double time) {
+    double phase = (time - I.t0) / I.period;
+    for (int it = 0; it < 4; it++)
+        phase -= (I.t0 + phase * (I.period + I.pdot * phase) - time) / (I.period + 2. * I.pdot * phase);
+    phase += I.deltat / I.period / 2. * (cos(2. * PI * phase) - 1.);
+    return phase;
+}
+// Phase of sub-exposure nd of ndiv (light_curve_comp.cpp:425-455);
+// expose in units of the period.
+__device__ __forceinline__ double sub_phase(double phase, double expose, int ndiv, int nd) {
+    if (ndiv == 1) return phase;
+    return phase + expose * (nd - (ndiv - 1) / 2.0) / (ndiv - 1);
+}
+
 __global__ __launch_bounds__(256) void k_phases(const InstDev *__restrict__ insts, DataDev d, PhaseArrays pa) {
     const int np = blockIdx.x * blockDim.x + threadIdx.x;
     const int inst = blockIdx.y;
@@ -38,10 +55,7 @@ __global__ __launch_bounds__(256) void k_phases(const InstDev *__restrict__ inst
     const InstDev &I = insts[inst];
     if (!I.active) return;
     const double time = d.time[np];
-    double phase = (time - I.t0) / I.period;
-    for (int it = 0; it < 4; it++)
-        phase -= (I.t0 + phase * (I.period + I.pdot * phase) - time) / (I.period + 2. * I.pdot * phase);
-    phase += I.deltat / I.period / 2. * (cos(2. * PI * phase) - 1.);
+    const double phase = point_phase(I, time);
     const double frac = (time - d.middle) / d.range;
     const double slfac = 1. + frac * (I.slope + frac * (I.quad + frac * I.cube));
     const double expose = d.expose[np] / I.period;
@@ -49,14 +63,8 @@ __global__ __launch_bounds__(256) void k_phases(const InstDev *__restrict__ inst
     const double norm = (double)max(1, ndiv - 1);
     const size_t j0 = (size_t)inst * d.S + d.sub_off[np];
     for (int nd = 0; nd < ndiv; ++nd) {
-        double phi, wgt;
-        if (ndiv == 1) {
-            phi = phase;
-            wgt = 1.0;
-        } else {
-            phi = phase + expose * (nd - (ndiv - 1) / 2.0) / (ndiv - 1);
-            wgt = (nd == 0 || nd == ndiv - 1) ? 0.5 : 1.0;
-        }
+        const double phi = sub_phase(phase, expose, ndiv, nd);
+        const double wgt = (ndiv == 1) ? 1.0 : ((nd == 0 || nd == ndiv - 1) ? 0.5 : 1.0);
         V3 e = set_earth(I.cosi, I.sini, phi);
         const size_t j = j0 + nd;
         pa.ex[j] = e.x;
@@ -67,6 +75,33 @@ __global__ __launch_bounds__(256) void k_phases(const InstDev *__restrict__ inst
         pa.w2[j] = slfac * wgt * gscale2(I, phi) / norm;
         pa.type[j] = (signed char)gint_type(phi, I.phase1, I.phase2);
     }
+}
+
+// Bins of [0, 1) holding a phase at which the instance's eclipse ranges
+// are tested, with the arithmetic of k_phases (phin), of the Point-path
+// sums (the Ginterp phases) and of the diagnostics (0.5).
+__global__ __launch_bounds__(256) void k_phase_bits(const InstDev *__restrict__ insts, DataDev d, unsigned *bits,
+                                                   int lb) {
+    const int np = blockIdx.x * blockDim.x + threadIdx.x;
+    const int inst = blockIdx.y;
+    const InstDev &I = insts[inst];
+    if (!I.active) return;
+    unsigned *b = bits + ((size_t)inst << (lb - 5));
+    const double B = (double)(1u << lb);
+    auto mark = [&](double phi) {
+        const double ph = phi - floor(phi);
+        const int bin = min((int)(ph * B), (int)(1u << lb) - 1);
+        atomicOr(&b[bin >> 5], 1u << (bin & 31));
+    };
+    if (np == 0) {
+        for (int k = 0; k < 8; k++) mark(gint_phase(k, I));
+        mark(0.5);
+    }
+    if (np >= d.n) return;
+    const double phase = point_phase(I, d.time[np]);
+    const double expose = d.expose[np] / I.period;
+    const int ndiv = d.ndiv[np];
+    for (int nd = 0; nd < ndiv; ++nd) mark(sub_phase(phase, expose, ndiv, nd));
 }
 
 // Fine/coarse sub-phase lists of one (instance, star) in ascending order
@@ -268,6 +303,11 @@ __global__ void k_scatter_final(const InstDev *__restrict__ insts, int nblk, con
 void launch_phases(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa) {
     if (ninst <= 0 || d.n <= 0) return;
     hipLaunchKernelGGL(k_phases, dim3((d.n + 255) / 256, ninst), dim3(256), 0, s, insts, d, pa);
+}
+
+void launch_phase_bits(hipStream_t s, const InstDev *insts, int ninst, DataDev d, unsigned *bits, int lb) {
+    if (ninst <= 0) return;
+    hipLaunchKernelGGL(k_phase_bits, dim3((std::max(d.n, 1) + 255) / 256, ninst), dim3(256), 0, s, insts, d, bits, lb);
 }
 
 void launch_route(hipStream_t s, const InstDev *insts, int ninst, int S, PhaseArrays pa, int *lists, int *counts) {

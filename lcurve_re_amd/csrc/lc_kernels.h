@@ -31,6 +31,9 @@ struct GridDev {
     double beam, spin_beam, VFAC, XCOFM, rlens1;
     double ldc[4], mucrit;
     int ltype, glens;
+    // 1: the eclipse search may end once its bracket holds none of the
+    // instance's phases (a grid used by one instance and kept by no cache)
+    int ecl_stop, pad_;
 };
 
 // One latitude ring (set_star_grid.cpp:262-320).
@@ -47,6 +50,20 @@ struct InstDev {
     int copy1, copy2, grid1f, grid2f, grid1c, grid2c;
     int scale, rdata, active, pad;
 };
+
+// Phases at which light_curve_comp.cpp:220-254 samples the Point-path sums.
+__device__ __forceinline__ double gint_phase(int k, const InstDev &I) {
+    switch (k) {
+    case 0: return 0.9999999999 * I.phase1;
+    case 1: return 1.0000000001 * I.phase1;
+    case 2: return 1. - 0.9999999999 * I.phase1;
+    case 3: return 1. - 1.0000000001 * I.phase1;
+    case 4: return 1 - 1.0000000001 * I.phase2;
+    case 5: return 1 - 0.9999999999 * I.phase2;
+    case 6: return 1.0000000001 * I.phase2;
+    default: return 0.9999999999 * I.phase2;
+    }
+}
 
 struct FaceArrays {
     double *px, *py, *pz, *dx, *dy, *dz;
@@ -77,7 +94,21 @@ struct EclipseWork {
     double *lim;     // [2*hit] sphere window phi1, phi2
     double *acc;     // [hit] position accuracy of fblink
     double *res;     // [2*hit + side] ingress (side 0) / egress (side 1) in [0, 1)
+    // bisection in rounds: brackets of unfinished items (indexed like res),
+    // the item lists of consecutive rounds (alternating) and their counts
+    double *b_in, *b_out;
+    int *alist[2];
+    int *acount;     // [ECL_ROUNDS]
+    // per instance, a bitmap of the 2^bits_lb bins of [0, 1) that hold a
+    // phase at which eclipse ranges are tested (bin = floor(phase * 2^lb))
+    const unsigned *phase_bits;
+    int bits_lb;
 };
+
+// Bisection rounds for searches that may stop early: the first runs
+// ECL_FIRST_STEPS steps of every search, the second ECL_STEPS more on the
+// searches still open (compacted), the last one runs them to the end.
+constexpr int ECL_ROUNDS = 3, ECL_FIRST_STEPS = 10, ECL_STEPS = 3;
 
 struct DataDev {
     const double *time, *expose, *flux, *ferr, *weight;
@@ -111,8 +142,9 @@ struct FaceCopy {
 // ------------------------------------------------------------ launches
 void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int nring, int nhalf_total,
                   FaceArrays fa, int *cand, int *cand_count, int *inst_status);
+// rounds: run the bisections in ECL_ROUNDS launches (else one)
 void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int *cand,
-                    const int *cand_count, int max_cand, EclipseWork ew, int *inst_status);
+                    const int *cand_count, int max_cand, EclipseWork ew, int *inst_status, bool rounds);
 // Copies positions, normals, areas, gravities and eclipse ranges of whole
 // grids (not the fluxes, which the continuum kernel recomputes).
 void launch_copy_faces(hipStream_t s, const FaceCopy *items, int nitem, FaceArrays src, FaceArrays dst);
@@ -156,6 +188,10 @@ void launch_point_sums(hipStream_t s, const GridDev *grids, const InstDev *insts
                        int nchunk, double *part);
 void launch_gint_scales(hipStream_t s, InstDev *insts, int ninst, int nchunk, const double *part);
 void launch_phases(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa);
+// Marks in bits (ninst x 2^lb bits, cleared by the caller) the bin of every
+// phase at which an instance's eclipse ranges are tested: its sub-phases,
+// the eight Ginterp phases and phase 0.5 (diagnostics).
+void launch_phase_bits(hipStream_t s, const InstDev *insts, int ninst, DataDev d, unsigned *bits, int lb);
 void launch_route(hipStream_t s, const InstDev *insts, int ninst, int S, PhaseArrays pa, int *lists, int *counts);
 // variant = ltype | beaming << 1 | lensing << 2 | (mucrit > 0) << 3;
 // rphase = sub-phases per lane
