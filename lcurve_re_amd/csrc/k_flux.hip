@@ -151,6 +151,8 @@ __device__ __forceinline__ int compact_slot(bool act, int &total, int *s_cnt) {
 
 struct Coef32 {
     float c1, c2, c3, c4, base, mucrit, beam, rlens;
+    // POLY: mu * I(mu) = mu (q1 + mu (q2 + mu (q3 + mu (q4 + mu q5))))
+    float q1, q2, q3, q4, q5;
 };
 
 // FP32 element without the face flux: returns the factor that multiplies
@@ -171,13 +173,14 @@ __device__ __forceinline__ float elem32(float ex, float ey, float ez, const floa
         const float m = __builtin_amdgcn_fmed3f(mu, 0.f, 1.f);
         float im;
         if (LT == 0) {
-            const float om = 1.f - m;
-            im = fmaf(-om, fmaf(om, fmaf(om, fmaf(om, K.c4, K.c3), K.c2), K.c1), 1.f);
+            // one Horner chain for mu I(mu): the polynomial vanishes at 0, so
+            // the clip to [0, 1] also zeroes faces turned away
+            val = m * fmaf(m, fmaf(m, fmaf(m, fmaf(m, K.q5, K.q4), K.q3), K.q2), K.q1);
         } else {
             const float s = __builtin_amdgcn_sqrtf(m);
             im = fmaf(s, fmaf(s, fmaf(s, fmaf(s, K.c4, K.c3), K.c2), K.c1), K.base);
+            val = m * im;
         }
-        val = m * im;
         if (MUC) val = mu > K.mucrit ? val : 0.f;
     } else {
         // v = (vx, vy, vn): vr_ref = -(ex vx + ey vy)
@@ -301,6 +304,15 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
     K.c3 = (float)G.ldc[2];
     K.c4 = (float)G.ldc[3];
     K.base = (float)(1.0 - (G.ldc[0] + G.ldc[1] + G.ldc[2] + G.ldc[3]));
+    {
+        // mu I(mu) = mu - sum_j c_j mu (1 - mu)^j expanded in powers of mu
+        const double c1 = G.ldc[0], c2 = G.ldc[1], c3 = G.ldc[2], c4 = G.ldc[3];
+        K.q1 = (float)(1.0 - (c1 + c2 + c3 + c4));
+        K.q2 = (float)(c1 + 2.0 * c2 + 3.0 * c3 + 4.0 * c4);
+        K.q3 = (float)(-(c2 + 3.0 * c3 + 6.0 * c4));
+        K.q4 = (float)(c3 + 4.0 * c4);
+        K.q5 = (float)(-c4);
+    }
     K.mucrit = (float)G.mucrit;
     K.beam = (float)G.beam;
     K.rlens = (float)G.rlens1;
