@@ -192,6 +192,19 @@ def roofline_from_profile(prof, steps, precision, workload, R=None):
             "grids_reused_per_step": prof["grids_reused"] / steps}
 
 
+def eager_profile(eng, step, nsteps):
+    """Profile of nsteps calls with graph replay off: a replayed graph is
+    timed as a whole, so per-kernel and per-stage times come from eager
+    calls of the same step (same kernels, same shapes)."""
+    eng.set_graphs(0)
+    eng.reset_profile()
+    for _ in range(nsteps):
+        step()
+    prof = eng.profile()
+    eng.set_graphs(1)
+    return prof
+
+
 def run_batch_workload(L, eng, args):
     if args.precision is None:
         args.precision = L.LC_PREC_MIXED
@@ -229,10 +242,14 @@ def run_batch_workload(L, eng, args):
     for i in range(args.steps):
         _, io, status = eng.light_curve_comp_batch(packed, data, precision=args.precision, out=calc)
     el = time.perf_counter() - t0
-    prof = eng.profile()
+    timed = eng.profile()
     nok = int(np.count_nonzero(status == 0))
-    log(f"bench: {args.steps} steps in {el:.3f} s; profile {prof}")
-    roofline = roofline_from_profile(prof, args.steps, args.precision, cfg["workload"], R)
+    log(f"bench: {args.steps} steps in {el:.3f} s; profile {timed}")
+    cfg["graph_replays_per_step"] = timed["graph_launches"] / args.steps
+    nprof = max(1, min(args.steps, 3))
+    prof = eager_profile(eng, lambda: eng.light_curve_comp_batch(packed, data, precision=args.precision, out=calc),
+                         nprof)
+    roofline = roofline_from_profile(prof, nprof, args.precision, cfg["workload"], R)
     cpu, err = (None, None)
     if not args.no_cpu:
         cpu, err = cpu_baseline(models, data, calc, args.cpu_budget)
@@ -387,9 +404,12 @@ def run_fit_batch(L, eng, args):
         out = fb.jacobian(walkers=walkers)
         ninst += sum(out["instances_per_call"])
     el = time.perf_counter() - t0
-    prof = eng.profile()
-    log(f"bench: {args.steps} steps in {el:.3f} s; profile {prof}")
-    roofline = roofline_from_profile(prof, args.steps, prec, "fit_batch")
+    timed = eng.profile()
+    log(f"bench: {args.steps} steps in {el:.3f} s; profile {timed}")
+    cfg["graph_replays_per_step"] = timed["graph_launches"] / args.steps
+    nprof = max(1, min(args.steps, 3))
+    prof = eager_profile(eng, lambda: fb.jacobian(walkers=walkers), nprof)
+    roofline = roofline_from_profile(prof, nprof, prec, "fit_batch")
     # the same step with grid reuse inside the batch switched off
     os.environ["LCURVE_AMD_DEDUPE"] = "0"
     e2 = L.Engine(0)

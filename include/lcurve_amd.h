@@ -170,6 +170,12 @@ typedef struct lc_profile {
     int64_t grids_computed; /* star grids built by the face + eclipse kernels     */
     int64_t grids_reused;   /* star grids copied (same geometry earlier in the
                                batch, or kept from an earlier call)              */
+    /* batched calls whose kernel sequence was replayed as a captured graph:
+     * their time is in graph_ms_total only (no per-stage split); the
+     * stage and flux-variant fields above count the other calls */
+    int64_t graph_launches;
+    int64_t graph_captures;
+    double graph_ms_total;
 } lc_profile;
 
 typedef struct lc_ctx lc_ctx;
@@ -190,6 +196,14 @@ lc_status lc_ctx_reset_profile(lc_ctx *ctx);
  * off (grids equal within one batch are still built once).  Default 4M
  * faces, or 0 when LCURVE_NO_GRID_CACHE is set. */
 lc_status lc_ctx_set_stage_cache(lc_ctx *ctx, size_t max_faces);
+
+/* No reference counterpart.  on != 0 (the default; LCURVE_AMD_GRAPHS=0 at
+ * creation turns it off): a batched call whose launch shape (sizes, device
+ * buffers, tile shape) equals the previous call's replays that call's
+ * kernels as one captured hipGraph; the second call of a shape captures it.
+ * Uploads, clears and downloads stay ordinary stream operations, so the
+ * graph holds kernel launches only.  Results are the same bits either way. */
+lc_status lc_ctx_set_graphs(lc_ctx *ctx, int on);
 /* Autotuner of the flux kernel's tile shape: runs the given workload with
  * every (sub-phases per lane R in {1, 2, 4}) x (face chunk in {512, 1024,
  * 2048}) and keeps for this ctx the pair with the shortest flux-stage time

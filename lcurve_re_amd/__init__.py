@@ -306,5 +306,13 @@ class Engine:
         if st != LC_OK:
             self._err(st)
 
+    def set_graphs(self, on):
+        """Replay repeated batched calls of one launch shape as a captured
+        graph (on by default); the profile's stage split covers only the
+        calls that were not replayed."""
+        st = _lib().lc_ctx_set_graphs(self._ctx, int(bool(on)))
+        if st != LC_OK:
+            self._err(st)
+
 
 from .data_io import read_data, write_data  # noqa: E402

@@ -119,7 +119,8 @@ class lc_profile(C.Structure):
                 ("eclipse_candidates", C.c_double), ("eclipse_hits", C.c_double),
                 ("flux_var_ms", C.c_double * 16), ("flux_var_flops", C.c_double * 16),
                 ("flux_var_launches", C.c_int64 * 16), ("flux_var_evaluated", C.c_double * 16),
-                ("grids_computed", C.c_int64), ("grids_reused", C.c_int64)]
+                ("grids_computed", C.c_int64), ("grids_reused", C.c_int64),
+                ("graph_launches", C.c_int64), ("graph_captures", C.c_int64), ("graph_ms_total", C.c_double)]
 
 
 # Every entry point include/lcurve_amd.h declares, with its signature.
@@ -131,6 +132,7 @@ SIGNATURES = {
     "lc_ctx_get_profile": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_profile)]),
     "lc_ctx_reset_profile": (_c.c_int, [_c.c_void_p]),
     "lc_ctx_set_stage_cache": (_c.c_int, [_c.c_void_p, _c.c_size_t]),
+    "lc_ctx_set_graphs": (_c.c_int, [_c.c_void_p, _c.c_int]),
     "lc_ctx_autotune": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_model), _c.c_size_t, _c.POINTER(lc_datum), _c.c_size_t,
                                    _c.POINTER(lc_eval_opts), _IP, _IP]),
     "lc_version": (_c.c_char_p, []),

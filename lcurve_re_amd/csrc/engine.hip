@@ -589,6 +589,18 @@ struct lc_ctx {
     long long ecl_stop_min_faces = 1LL << 17;
     long long ecl_round_min = 1LL << 21;  // faces x 2 from which early-stopping searches run in rounds
     GeoCache geo;
+    // kernel sequence of the last repeated launch shape, as a graph
+    bool graphs = true;
+    std::string g_key, g_seen;
+    hipGraph_t g_graph = nullptr;
+    hipGraphExec_t g_exec = nullptr;
+    void drop_graph() {
+        if (g_exec) (void)hipGraphExecDestroy(g_exec);
+        if (g_graph) (void)hipGraphDestroy(g_graph);
+        g_exec = nullptr;
+        g_graph = nullptr;
+        g_key.clear();
+    }
 };
 
 namespace {
@@ -614,8 +626,6 @@ double elem_flops(const GridDev &G) {
     return c[(G.ltype ? 1 : 0) + (G.beam != 0.0 ? 2 : 0)];
 }
 
-// Runs f(0..n-1) on up to 16 host threads (work taken one index at a
-// time); the first exception is rethrown after all threads joined.
 // Host planning workers: one process-wide pool of up to 15 threads that
 // sleep between jobs (starting threads for every call cost up to 0.5 ms).
 // A job runs f(0..n-1) on the pool and the calling thread; one job at a
@@ -1136,6 +1146,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     int *d_cand = A.take<int>(FN);
     int *d_cand_count = A.take<int>(1);
     EclipseWork ew;
+    std::memset(&ew, 0, sizeof(ew));  // padding too: the struct is part of the launch key
     ew.hit_count = A.take<int>(1);
     ew.face = A.take<int>(FN);
     ew.grid = A.take<int>(FN);
@@ -1270,47 +1281,100 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
 
     // ---------------- launches
     hipEvent_t *ev = ctx->ev;
-    if (any_ecl_stop) {
-        HIPCHK(hipMemsetAsync(d_bits, 0, bits_words * sizeof(unsigned), s));
-        launch_phase_bits(s, d_insts, (int)M, dd, d_bits, bits_lb);
-    }
-    HIPCHK(hipEventRecord(ev[0], s));
-    launch_faces(s, d_grids, d_rings, nring, nhalf_total, fa, d_cand, d_cand_count, d_status);
-    HIPCHK(hipEventRecord(ev[1], s));
-    launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, ew, d_status,
-                   any_ecl_stop && 2LL * nface_total >= ctx->ecl_round_min);
-    // reused grids: cache hits first (a duplicate may copy from one), then
-    // duplicates within the batch, then the new grids into the cache
-    if (!copy_hit.empty()) launch_copy_faces(s, d_copy_hit, (int)copy_hit.size(), ctx->geo.pool, fa);
-    if (!copy_dup.empty()) launch_copy_faces(s, d_copy_dup, (int)copy_dup.size(), fa, fa);
-    if (!copy_out.empty()) launch_copy_faces(s, d_copy_out, (int)copy_out.size(), fa, ctx->geo.pool);
-    HIPCHK(hipEventRecord(ev[2], s));
-    launch_continuum(s, d_grids, ngrid, nface_total, fa);
-    HIPCHK(hipEventRecord(ev[3], s));
-    if (any_route) {
-        launch_point_sums(s, d_grids, d_insts, (int)M, fa, psum_chunks, d_gsums);
-        launch_gint_scales(s, d_insts, (int)M, psum_chunks, d_gsums);
-    }
-    launch_flat_build(s, d_grids, d_fsegs, (int)fsegs.size(), d_seg_cnt, fa, fl, fp64 ? 1 : 0);
-    launch_phases(s, d_insts, (int)M, dd, pa);
-    if (any_route) launch_route(s, d_insts, (int)M, S, pa, d_lists, d_counts);
-    launch_make_tasks(s, d_task_first, d_task_order, (int)task_order.size(), d_batches, ntask, d_tasks);
+    // clears for the kernel sequence, on the stream ahead of it
     HIPCHK(hipMemsetAsync(d_out, 0, SM * 2 * sizeof(double), s));
     HIPCHK(hipMemsetAsync(fl.evald, 0, 16 * 64 * sizeof(unsigned long long), s));
-    HIPCHK(hipEventRecord(ev[4], s));
+    if (any_ecl_stop) HIPCHK(hipMemsetAsync(d_bits, 0, bits_words * sizeof(unsigned), s));
+    const bool ecl_rounds = any_ecl_stop && 2LL * nface_total >= ctx->ecl_round_min;
+
+    // ---------------- launches: kernels only, so the sequence can be a graph
     std::vector<int> var_used;
     for (int v = 0; v < 16; v++)
-        if (var_range[v].second) {
-            if (var_used.empty()) HIPCHK(hipEventRecord(ctx->ev_var[0], s));
+        if (var_range[v].second) var_used.push_back(v);
+    auto issue = [&](bool timed) {
+        auto mark = [&](hipEvent_t e) {
+            if (timed) HIPCHK(hipEventRecord(e, s));
+        };
+        if (any_ecl_stop) launch_phase_bits(s, d_insts, (int)M, dd, d_bits, bits_lb);
+        mark(ev[0]);
+        launch_faces(s, d_grids, d_rings, nring, nhalf_total, fa, d_cand, d_cand_count, d_status);
+        mark(ev[1]);
+        launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, ew, d_status, ecl_rounds);
+        // reused grids: cache hits first (a duplicate may copy from one), then
+        // duplicates within the batch, then the new grids into the cache
+        if (!copy_hit.empty()) launch_copy_faces(s, d_copy_hit, (int)copy_hit.size(), ctx->geo.pool, fa);
+        if (!copy_dup.empty()) launch_copy_faces(s, d_copy_dup, (int)copy_dup.size(), fa, fa);
+        if (!copy_out.empty()) launch_copy_faces(s, d_copy_out, (int)copy_out.size(), fa, ctx->geo.pool);
+        mark(ev[2]);
+        launch_continuum(s, d_grids, ngrid, nface_total, fa);
+        mark(ev[3]);
+        if (any_route) {
+            launch_point_sums(s, d_grids, d_insts, (int)M, fa, psum_chunks, d_gsums);
+            launch_gint_scales(s, d_insts, (int)M, psum_chunks, d_gsums);
+        }
+        launch_flat_build(s, d_grids, d_fsegs, (int)fsegs.size(), d_seg_cnt, fa, fl, fp64 ? 1 : 0);
+        launch_phases(s, d_insts, (int)M, dd, pa);
+        if (any_route) launch_route(s, d_insts, (int)M, S, pa, d_lists, d_counts);
+        launch_make_tasks(s, d_task_first, d_task_order, (int)task_order.size(), d_batches, ntask, d_tasks);
+        mark(ev[4]);
+        for (size_t k = 0; k < var_used.size(); k++) {
+            const int v = var_used[k];
+            if (k == 0) mark(ctx->ev_var[0]);
             launch_flux(s, v, fp64 ? 1 : 0, R, d_tasks + var_range[v].first, var_range[v].second, d_batches, d_grids,
                         fl, pa, S, d_lists, d_counts, d_partial);
-            var_used.push_back(v);
-            HIPCHK(hipEventRecord(ctx->ev_var[var_used.size()], s));
+            mark(ctx->ev_var[k + 1]);
         }
-    HIPCHK(hipEventRecord(ev[5], s));
-    launch_flux_reduce(s, d_batches, (int)batches.size(), S, TILE, d_lists, d_counts, d_partial, d_out);
-    launch_scatter(s, d_insts, (int)M, dd, pa, d_out, d_calc, d_scal, d_scat_part);
-    if (!diag_idx.empty()) launch_diag(s, d_grids, d_diag_idx, (int)diag_idx.size(), fa, d_diag);
+        mark(ev[5]);
+        launch_flux_reduce(s, d_batches, (int)batches.size(), S, TILE, d_lists, d_counts, d_partial, d_out);
+        launch_scatter(s, d_insts, (int)M, dd, pa, d_out, d_calc, d_scal, d_scat_part);
+        if (!diag_idx.empty()) launch_diag(s, d_grids, d_diag_idx, (int)diag_idx.size(), fa, d_diag);
+    };
+    // A graph replays the exact launches (grids, blocks, arguments) of its
+    // capture: the key holds every size, flag and device pointer they use.
+    bool graphed = false;
+    if (ctx->graphs) {
+        std::string key;
+        auto put = [&key](const void *p, size_t nb) { key.append(static_cast<const char *>(p), nb); };
+        const long long sizes[] = {(long long)M, (long long)n, S, ngrid, nring, nhalf_total, nface_total,
+                                   (long long)copy_hit.size(), (long long)copy_dup.size(), (long long)copy_out.size(),
+                                   any_route, any_ecl_stop, ecl_rounds, bits_lb, psum_chunks,
+                                   (long long)fsegs.size(), (long long)task_order.size(), ntask, R, fp64,
+                                   (long long)batches.size(), TILE, (long long)diag_idx.size()};
+        put(sizes, sizeof(sizes));
+        for (int v = 0; v < 16; v++) put(&var_range[v], sizeof(var_range[v]));
+        const void *ptrs[] = {d_grids, d_rings, d_insts, d_status, d_cand, d_cand_count, d_copy_hit, d_copy_dup,
+                              d_copy_out, d_gsums, d_fsegs, d_seg_cnt, d_lists, d_counts, d_task_first, d_task_order,
+                              d_batches, d_tasks, d_partial, d_out, d_calc, d_scal, d_scat_part, d_diag_idx, d_diag,
+                              d_bits};
+        put(ptrs, sizeof(ptrs));
+        put(&ctx->geo.pool, sizeof(ctx->geo.pool));
+        put(&dd, sizeof(dd));
+        put(&fa, sizeof(fa));
+        put(&fl, sizeof(fl));
+        put(&pa, sizeof(pa));
+        put(&ew, sizeof(ew));
+        if (ctx->g_exec && key == ctx->g_key) {
+            graphed = true;
+        } else if (key == ctx->g_seen) {  // second call of this shape: capture it
+            ctx->drop_graph();
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            issue(false);
+            HIPCHK(hipStreamEndCapture(s, &ctx->g_graph));
+            HIPCHK(hipGraphInstantiate(&ctx->g_exec, ctx->g_graph, nullptr, nullptr, 0));
+            ctx->g_key = key;
+            ctx->prof.graph_captures++;
+            graphed = true;
+        } else {
+            ctx->g_seen = key;
+        }
+    }
+    if (graphed) {
+        HIPCHK(hipEventRecord(ev[0], s));
+        HIPCHK(hipGraphLaunch(ctx->g_exec, s));
+        HIPCHK(hipEventRecord(ev[5], s));
+    } else {
+        issue(true);
+    }
     HIPCHK(hipGetLastError());
 
     // ---------------- downloads
@@ -1349,22 +1413,27 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             var_flops[flux_variant(G)] += (double)G.nface * c * elem_flops(G);
         }
     }
-    ctx->prof.grid_ms_total += elapsed(ev[0], ev[1]);
-    ctx->prof.eclipse_ms_total += elapsed(ev[1], ev[2]);
-    ctx->prof.cont_ms_total += elapsed(ev[2], ev[3]);
-    ctx->prof.other_ms_total += elapsed(ev[3], ev[4]) + elapsed(ev[5], ev[6]);
-    ctx->prof.flux_ms_total += elapsed(ev[4], ev[5]);
-    for (int v = 0; v < 16; v++) ctx->prof.flux_launches += var_range[v].second ? 1 : 0;
-    for (size_t k = 0; k < var_used.size(); k++) {
-        const int v = var_used[k];
-        ctx->prof.flux_var_ms[v] += elapsed(ctx->ev_var[k], ctx->ev_var[k + 1]);
-        ctx->prof.flux_var_flops[v] += var_flops[v];
-        ctx->prof.flux_var_launches[v] += 1;
+    if (graphed) {
+        ctx->prof.graph_launches++;
+        ctx->prof.graph_ms_total += elapsed(ev[0], ev[5]);
+    } else {
+        ctx->prof.grid_ms_total += elapsed(ev[0], ev[1]);
+        ctx->prof.eclipse_ms_total += elapsed(ev[1], ev[2]);
+        ctx->prof.cont_ms_total += elapsed(ev[2], ev[3]);
+        ctx->prof.other_ms_total += elapsed(ev[3], ev[4]) + elapsed(ev[5], ev[6]);
+        ctx->prof.flux_ms_total += elapsed(ev[4], ev[5]);
+        ctx->prof.flux_launches += (int64_t)var_used.size();
+        for (size_t k = 0; k < var_used.size(); k++) {
+            const int v = var_used[k];
+            ctx->prof.flux_var_ms[v] += elapsed(ctx->ev_var[k], ctx->ev_var[k + 1]);
+            ctx->prof.flux_var_flops[v] += var_flops[v];
+            ctx->prof.flux_var_launches[v] += 1;
+        }
+        for (int k = 0; k < 16 * 64; k++) ctx->prof.flux_var_evaluated[k / 64] += (double)h_evald[k];
+        ctx->prof.flux_elements += alg_elems;
+        ctx->prof.flux_flops += alg_flops;
+        ctx->prof.flux_bytes += alg_bytes;
     }
-    for (int k = 0; k < 16 * 64; k++) ctx->prof.flux_var_evaluated[k / 64] += (double)h_evald[k];
-    ctx->prof.flux_elements += alg_elems;
-    ctx->prof.flux_flops += alg_flops;
-    ctx->prof.flux_bytes += alg_bytes;
 
     // ---------------- stage reuse bookkeeping
     for (int g = 0; g < ngrid; g++) {  // a copy inherits its source's failure
@@ -1568,6 +1637,7 @@ lc_status lc_ctx_create(int device, lc_ctx **out) {
         c->geo.cap = std::getenv("LCURVE_NO_GRID_CACHE") ? 0 : DEFAULT_GEO_CACHE_FACES;
         if (const char *d = std::getenv("LCURVE_AMD_DEDUPE")) c->dedupe = std::atoi(d) != 0;
         if (std::getenv("LCURVE_AMD_FULL_ECLIPSE")) c->ecl_stop = false;
+        if (const char *g = std::getenv("LCURVE_AMD_GRAPHS")) c->graphs = std::atoi(g) != 0;
         if (const char *r = std::getenv("LCURVE_AMD_ECL_ROUNDS_MIN")) c->ecl_round_min = std::atoll(r);
         if (const char *r = std::getenv("LCURVE_AMD_ECL_STOP_MIN_FACES")) c->ecl_stop_min_faces = std::atoll(r);
         *out = c.release();
@@ -1584,6 +1654,7 @@ void lc_ctx_destroy(lc_ctx *ctx) {
     for (auto &e : ctx->ev_var)
         if (e) (void)hipEventDestroy(e);
     ctx->geo.release();
+    ctx->drop_graph();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1611,6 +1682,18 @@ lc_status lc_ctx_set_stage_cache(lc_ctx *ctx, size_t max_faces) {
         ctx->geo.cap = max_faces;
         return LC_OK;
     });
+}
+
+lc_status lc_ctx_set_graphs(lc_ctx *ctx, int on) {
+    if (!ctx) return LC_E_ARG;
+    ctx->graphs = on != 0;
+    ctx->g_seen.clear();
+    if (!ctx->graphs) {
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        ctx->drop_graph();
+    }
+    return LC_OK;
 }
 
 lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const lc_datum *data, size_t n,
