@@ -1062,7 +1062,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<int>(FN);
     sz.add<double>(FN);
     sz.add<double>(FN);
-    sz.add<int>(FN);  // cand
+    sz.add<int2>(FN);  // cand
     sz.add<int>(1);
     sz.add<int>(1);      // eclipse work lists
     sz.add<int>(FN);
@@ -1143,7 +1143,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     fa.ecl = A.take<int>(FN);
     fa.ein = A.take<double>(FN);
     fa.eout = A.take<double>(FN);
-    int *d_cand = A.take<int>(FN);
+    int2 *d_cand = A.take<int2>(FN);
     int *d_cand_count = A.take<int>(1);
     EclipseWork ew;
     std::memset(&ew, 0, sizeof(ew));  // padding too: the struct is part of the launch key
@@ -1925,7 +1925,7 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     G.face_off = 0; G.ring_off = 0; G.nring = (int)rings.size(); G.half_off = 0; G.nhalf = ho; G.inst = 0;
     if (nf == 0) return LC_OK;
     const size_t FN = (size_t)nf;
-    ctx->dev.reserve(FN * (6 * 8 + 3 * 4 + 4 + 16 + 4 + 4 + 4 + 6 * 8 + 4 * 8 + 4 * 4) +
+    ctx->dev.reserve(FN * (6 * 8 + 3 * 4 + 4 + 16 + 8 + 4 + 4 + 6 * 8 + 4 * 8 + 4 * 4) +
                      rings.size() * sizeof(RingDev) + 65536);
     ctx->dev.reset();
     DevArena &A = ctx->dev;
@@ -1935,7 +1935,7 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     fa.dx = A.take<double>(FN); fa.dy = A.take<double>(FN); fa.dz = A.take<double>(FN);
     fa.area = A.take<float>(FN); fa.grav = A.take<float>(FN); fa.flux = A.take<float>(FN);
     fa.ecl = A.take<int>(FN); fa.ein = A.take<double>(FN); fa.eout = A.take<double>(FN);
-    int *d_cand = A.take<int>(FN);
+    int2 *d_cand = A.take<int2>(FN);
     int *d_count = A.take<int>(1);
     int *d_status = A.take<int>(1);
     EclipseWork ew;

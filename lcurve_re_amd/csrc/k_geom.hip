@@ -22,23 +22,31 @@ __device__ __forceinline__ void set_status(int *inst_status, int inst, int st) {
     if (st) atomicCAS(&inst_status[inst], 0, st);
 }
 
-__device__ __forceinline__ int find_grid(const GridDev *grids, int ngrid, int face) {
-    int lo = 0, hi = ngrid - 1;
+// Last index i < n with key(i) <= t, for keys ascending in i and t
+// ascending across the active lanes of a wave: one binary search on the
+// wave's first t (wave-uniform, so scalar loads) and a short forward scan
+// per lane -- a wave's 64 consecutive items span only a few rings or grids.
+template <class Key>
+__device__ __forceinline__ int wave_lookup(int n, int t, Key key) {
+    const int t0 = __builtin_amdgcn_readfirstlane(t);
+    int lo = 0, hi = n - 1;
     while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (grids[mid].face_off <= face)
+        const int mid = (lo + hi + 1) >> 1;
+        if (key(mid) <= t0)
             lo = mid;
         else
             hi = mid - 1;
     }
-    return lo;
+    int r = lo;
+    while (r + 1 < n && key(r + 1) <= t) r++;
+    return r;
 }
 
 // Eclipse of one face by the companion (set_star_grid.cpp:372-386).  For a
 // Roche companion only the reference-sphere test runs here: a hit makes the
 // face a candidate for the full search.
-__device__ __forceinline__ void eclipse_pretest(const GridDev &G, const V3 &pos, int face, FaceArrays fa, int *cand,
-                                                int *cand_count) {
+__device__ __forceinline__ void eclipse_pretest(const GridDev &G, int gidx, const V3 &pos, int face, FaceArrays fa,
+                                                int2 *cand, int *cand_count) {
     fa.ecl[face] = 0;
     fa.ein[face] = 0.;
     fa.eout[face] = 0.;
@@ -48,7 +56,7 @@ __device__ __forceinline__ void eclipse_pretest(const GridDev &G, const V3 &pos,
     if (G.roche_other) {
         if (sphere_eclipse(G.cosi, G.sini, pos, cofm_other, G.rref_other, phi1, phi2, lam1, lam2)) {
             int k = atomicAdd(cand_count, 1);
-            cand[k] = face;
+            cand[k] = make_int2(face, gidx);  // the search needs the face's grid
         }
     } else if (sphere_eclipse(G.cosi, G.sini, pos, cofm_other, G.r_other, phi1, phi2, lam1, lam2)) {
         fa.ecl[face] = 1;
@@ -58,18 +66,11 @@ __device__ __forceinline__ void eclipse_pretest(const GridDev &G, const V3 &pos,
 }
 
 __global__ __launch_bounds__(256) void k_faces(const GridDev *__restrict__ grids, const RingDev *__restrict__ rings,
-                                              int nring, int nhalf_total, FaceArrays fa, int *cand, int *cand_count,
+                                              int nring, int nhalf_total, FaceArrays fa, int2 *cand, int *cand_count,
                                               int *inst_status) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nhalf_total) return;
-    int lo = 0, hi = nring - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (rings[mid].half_off <= t)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
+    const int lo = wave_lookup(nring, t, [&](int k) { return rings[k].half_off; });
     const RingDev R = rings[lo];
     const GridDev &G = grids[R.grid];
     const int np = t - R.half_off;
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(256) void k_faces(const GridDev *__restrict__ grids
         fa.area[f] = farea;
         fa.grav[f] = fgrav;
         fa.flux[f] = 0.f;
-        eclipse_pretest(G, posn, f, fa, cand, cand_count);
+        eclipse_pretest(G, R.grid, posn, f, fa, cand, cand_count);
     }
 
     int np2 = nphi - 1 - np;
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(256) void k_faces(const GridDev *__restrict__ grids
         fa.area[f2] = farea;
         fa.grav[f2] = fgrav;
         fa.flux[f2] = 0.f;
-        eclipse_pretest(G, posn2, f2, fa, cand, cand_count);
+        eclipse_pretest(G, R.grid, posn2, f2, fa, cand, cand_count);
     }
 }
 
@@ -151,13 +152,14 @@ __global__ __launch_bounds__(256) void k_faces(const GridDev *__restrict__ grids
 // Counts live on the device, so the chain needs no host round trip; every
 // grid-stride loop ends once its list is consumed.
 __global__ __launch_bounds__(128) void k_eclipse_potmin(const GridDev *__restrict__ grids, int ngrid, FaceArrays fa,
-                                                       const int *__restrict__ cand,
+                                                       const int2 *__restrict__ cand,
                                                        const int *__restrict__ cand_count, EclipseWork ew,
                                                        int *inst_status) {
     const int n = *cand_count;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int f = cand[i];
-        const GridDev &G = grids[find_grid(grids, ngrid, f)];
+        const int2 c = cand[i];
+        const int f = c.x;
+        const GridDev &G = grids[c.y];
         const int other = G.star == PRIMARY ? SECONDARY : PRIMARY;
         const V3 cofm = other == PRIMARY ? v3(0., 0., 0.) : v3(1., 0., 0.);
         const V3 r = v3(fa.px[f], fa.py[f], fa.pz[f]);
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(256) void k_continuum(const GridDev *__restrict__ g
                                                   FaceArrays fa) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nface_total) return;
-    const GridDev &G = grids[find_grid(grids, ngrid, f)];
+    const GridDev &G = grids[wave_lookup(ngrid, f, [&](int g) { return grids[g].face_off; })];
     const V3 posn = v3(fa.px[f], fa.py[f], fa.pz[f]);
     const V3 dirn = v3(fa.dx[f], fa.dy[f], fa.dz[f]);
     const V3 cofm_other = G.star == PRIMARY ? v3(1., 0., 0.) : v3(0., 0., 0.);
@@ -657,13 +659,13 @@ __global__ __launch_bounds__(1024) void k_diag(const GridDev *__restrict__ grids
 }  // namespace
 
 void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int nring, int nhalf_total,
-                  FaceArrays fa, int *cand, int *cand_count, int *inst_status) {
+                  FaceArrays fa, int2 *cand, int *cand_count, int *inst_status) {
     if (nhalf_total <= 0) return;
     hipLaunchKernelGGL(k_faces, dim3((nhalf_total + 255) / 256), dim3(256), 0, s, grids, rings, nring, nhalf_total,
                        fa, cand, cand_count, inst_status);
 }
 
-void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int *cand,
+void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int2 *cand,
                     const int *cand_count, int max_cand, EclipseWork ew, int *inst_status, bool rounds) {
     if (max_cand <= 0 || ngrid <= 0) return;
     const int blocks = std::min((max_cand + 127) / 128, 16384);

@@ -141,9 +141,9 @@ struct FaceCopy {
 
 // ------------------------------------------------------------ launches
 void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int nring, int nhalf_total,
-                  FaceArrays fa, int *cand, int *cand_count, int *inst_status);
+                  FaceArrays fa, int2 *cand, int *cand_count, int *inst_status);
 // rounds: run the bisections in ECL_ROUNDS launches (else one)
-void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int *cand,
+void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int2 *cand,
                     const int *cand_count, int max_cand, EclipseWork ew, int *inst_status, bool rounds);
 // Copies positions, normals, areas, gravities and eclipse ranges of whole
 // grids (not the fluxes, which the continuum kernel recomputes).
