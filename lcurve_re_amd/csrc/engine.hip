@@ -1075,7 +1075,6 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<float4>(FN);
     sz.add<float4>(FN);
     sz.add<double2>(FN);
-    sz.add<int>(FN);
     if (fp64)
         for (int k = 0; k < 7; k++) sz.add<double>(FN);
     sz.add<int>(std::max(ngrid, 1));
@@ -1168,7 +1167,6 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     fl.rec1 = A.take<float4>(FN);
     fl.rec2 = A.take<float4>(FN);
     fl.ecl = A.take<double2>(FN);
-    fl.perm = A.take<int>(FN);
     if (fp64) {
         fl.fdx = A.take<double>(FN); fl.fdy = A.take<double>(FN); fl.fdz = A.take<double>(FN);
         fl.fpx = A.take<double>(FN); fl.fpy = A.take<double>(FN); fl.fpz = A.take<double>(FN);

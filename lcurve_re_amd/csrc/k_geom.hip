@@ -54,7 +54,7 @@ __device__ __forceinline__ void eclipse_pretest(const GridDev &G, int gidx, cons
     const V3 cofm_other = G.star == PRIMARY ? v3(1., 0., 0.) : v3(0., 0., 0.);
     double phi1, phi2, lam1, lam2;
     if (G.roche_other) {
-        if (sphere_eclipse(G.cosi, G.sini, pos, cofm_other, G.rref_other, phi1, phi2, lam1, lam2)) {
+        if (sphere_eclipse_test(G.cosi, G.sini, pos, cofm_other, G.rref_other)) {
             int k = atomicAdd(cand_count, 1);
             cand[k] = make_int2(face, gidx);  // the search needs the face's grid
         }
@@ -439,17 +439,18 @@ __global__ __launch_bounds__(1024) void k_flat_build(const GridDev *__restrict__
         if (in) {
             const int pos = plain ? before0 + done0 + ex : n0 + before1 + done1 + ((int)threadIdx.x - ex);
             const int f = base + i, p = base + pos;
-            fl.perm[p] = f;
             double dx = fa.dx[f], dy = fa.dy[f], dz = fa.dz[f];
             double px = fa.px[f], py = fa.py[f], pz = fa.pz[f];
             float flux = fa.flux[f];
             fl.rec0[p] = make_float4((float)dx, (float)dy, (float)dz, flux);
-            double vx = -G.VFAC * G.spin_beam * py;
-            double vy = G.star == PRIMARY ? G.VFAC * (G.spin_beam * px - G.XCOFM)
-                                          : G.VFAC * (G.spin_beam * (px - 1.0) + 1.0 - G.XCOFM);
-            double vn = dx * vx + dy * vy;
-            fl.rec1[p] = make_float4((float)vx, (float)vy, (float)vn, 0.f);
-            fl.rec2[p] = make_float4((float)px, (float)py, (float)pz, 0.f);
+            if (G.beam != 0.0) {  // read only by the beaming variants of the flux kernel
+                double vx = -G.VFAC * G.spin_beam * py;
+                double vy = G.star == PRIMARY ? G.VFAC * (G.spin_beam * px - G.XCOFM)
+                                              : G.VFAC * (G.spin_beam * (px - 1.0) + 1.0 - G.XCOFM);
+                double vn = dx * vx + dy * vy;
+                fl.rec1[p] = make_float4((float)vx, (float)vy, (float)vn, 0.f);
+            }
+            if (G.glens) fl.rec2[p] = make_float4((float)px, (float)py, (float)pz, 0.f);  // lensing variants
             fl.ecl[p] = make_double2(fa.ein[f], fa.eout[f]);
             if (fp64) {
                 fl.fdx[p] = dx; fl.fdy[p] = dy; fl.fdz[p] = dz;

@@ -79,7 +79,6 @@ struct FlatArrays {
     float4 *rec1;   // vx, vy, vn (beaming), 0
     float4 *rec2;   // px, py, pz (lensing), 0
     double2 *ecl;   // ingress, egress
-    int *perm;      // flat position -> face index
     double *fdx, *fdy, *fdz, *fpx, *fpy, *fpz, *fflux;  // FP64 copies
     int *n0;        // never-eclipsed count per grid
     unsigned long long *evald;  // [16 x 64] face x sub-phase elements evaluated, per variant

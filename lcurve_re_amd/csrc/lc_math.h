@@ -297,6 +297,16 @@ LC_HD int face_scalar(double q, int star, double spin, const V3 &dirn, double rr
 // ------------------------------------------------------- sphere eclipses
 // Phase/multiplier window of the eclipse of point r by a sphere
 // (sphere_eclipse.cpp:33-75)
+// Whether sphere_eclipse reports an eclipse, without the window: the same
+// operations up to its two early returns.
+LC_HD bool sphere_eclipse_test(double cosi, double sini, const V3 &r, const V3 &c, double rs) {
+    const V3 d = sub(r, c);
+    const double PDIST = sqrt(sqr(d.x) + sqr(d.y));
+    const double BQUAD = d.z * cosi - PDIST * sini;
+    if (BQUAD >= 0.) return false;
+    const double CQUAD = sqrlen(d) - sqr(rs);
+    return !(sqr(BQUAD) - CQUAD <= 0.);  // as written there, NaN included
+}
 LC_HD bool sphere_eclipse(double cosi, double sini, const V3 &r, const V3 &c, double rs, double &phi1, double &phi2,
                           double &lam1, double &lam2) {
     const V3 d = sub(r, c);
