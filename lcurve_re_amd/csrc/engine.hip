@@ -1356,7 +1356,14 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         } else if (key == ctx->g_seen) {  // second call of this shape: capture it
             ctx->drop_graph();
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            issue(false);
+            try {
+                issue(false);
+            } catch (...) {  // leave the stream out of capture mode
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
             HIPCHK(hipStreamEndCapture(s, &ctx->g_graph));
             HIPCHK(hipGraphInstantiate(&ctx->g_exec, ctx->g_graph, nullptr, nullptr, 0));
             ctx->g_key = key;
@@ -1711,6 +1718,24 @@ lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const l
         EvalRequest rq{models, m, data, n, *opts, calc.data(), io.data(), status.data()};
         const lc_profile saved = ctx->prof;
         const int r0 = ctx->flux_rphase, c0 = ctx->flux_chunk;
+        // candidates are timed from the flux-stage events of eager calls
+        // (a replayed graph is timed only as a whole)
+        const bool graphs = ctx->graphs;
+        ctx->graphs = false;
+        struct Restore {  // the context's own settings if a timing run throws
+            lc_ctx *c;
+            bool graphs;
+            int r, ch;
+            lc_profile prof;
+            bool armed = true;
+            ~Restore() {
+                if (!armed) return;
+                c->graphs = graphs;
+                c->flux_rphase = r;
+                c->flux_chunk = ch;
+                c->prof = prof;
+            }
+        } restore{ctx, graphs, r0, c0, saved};
         run_eval(ctx, rq);  // warm the arenas once
         auto time_of = [&](int r, int c) {
             ctx->flux_rphase = r;
@@ -1740,6 +1765,8 @@ lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const l
         ctx->flux_rphase = br;
         ctx->flux_chunk = bc;
         ctx->prof = saved;
+        ctx->graphs = graphs;
+        restore.armed = false;
         if (best_rphase) *best_rphase = br;
         if (best_chunk) *best_chunk = bc;
         return LC_OK;
