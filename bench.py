@@ -162,7 +162,7 @@ def roofline_from_profile(prof, steps, precision, workload, R=None):
     flop_per_launch = prof["flux_var_flops"][v] / launches
     achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     if R is None:
-        R = int(os.environ.get("LCURVE_AMD_FLUX_R", "2"))
+        R = int(prof["last_rphase"])  # the shape the engine used
     kname = (f"k_flux_{'fp64' if precision else 'mixed'}<{v & 1}, {bool(v & 2)}, {bool(v & 4)}"
              + ("" if precision else f", {bool(v & 8)}") + f", {R}>")
     peak = PEAK_FP64_TFLOPS if precision else PEAK_FP32_TFLOPS
@@ -249,7 +249,7 @@ def run_batch_workload(L, eng, args):
     nprof = max(1, min(args.steps, 3))
     prof = eager_profile(eng, lambda: eng.light_curve_comp_batch(packed, data, precision=args.precision, out=calc),
                          nprof)
-    roofline = roofline_from_profile(prof, nprof, args.precision, cfg["workload"], R)
+    roofline = roofline_from_profile(prof, nprof, args.precision, cfg["workload"])
     cpu, err = (None, None)
     if not args.no_cpu:
         cpu, err = cpu_baseline(models, data, calc, args.cpu_budget)

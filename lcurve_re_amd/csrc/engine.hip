@@ -580,8 +580,11 @@ struct lc_ctx {
     lc_profile prof;
     hipEvent_t ev[16];
     hipEvent_t ev_var[17];  // boundaries between flux-variant launches
-    int flux_rphase = 2;
-    int flux_chunk = 1024;
+    // flux tile shape; 0: chosen per call (R = 2 with beaming, else 1; the
+    // chunk from the number of flux tasks); last_* hold the last call's
+    int flux_rphase = 0;
+    int flux_chunk = 0;
+    int last_rphase = 1, last_chunk = 1024;
     bool dedupe = true;  // build grids equal within one batch once
     bool ecl_stop = true;  // eclipse searches end once no data phase can tell
     // below this many faces per call the searches are latency bound: the
@@ -967,9 +970,35 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             gaps.push_back({gdev[g].face_off + grids[g].ring_faces, gdev[g].nface - grids[g].ring_faces});
 
     // ---------------- flux batches and tasks
-    const int R = ctx->flux_rphase;
+    int R = ctx->flux_rphase;
+    if (R <= 0) {  // beaming elements cost twice as much: two sub-phases per lane pay off
+        R = 1;
+        for (int g = 0; g < ngrid; g++)
+            if (gdev[g].beam != 0.0) R = 2;
+    }
     const int TILE = R * 256;
-    const int chunk = fp64 ? 1024 : ctx->flux_chunk;
+    const int ntile = S > 0 ? (S + TILE - 1) / TILE : 0;
+    auto count_tasks = [&](int ch) {
+        long long t = 0;
+        for (size_t i = 0; i < M; i++) {
+            if (ip[i].status != LC_OK) continue;
+            for (int star = 0; star < 2; star++) {
+                const bool copy = star == 0 ? ip[i].copy1 : ip[i].copy2;
+                for (int part = 0; part < (copy ? 1 : 2); part++)
+                    t += (long long)ntile * ((gdev[ip[i].g[star + (part ? 2 : 0)]].nface + ch - 1) / ch);
+            }
+        }
+        return t;
+    };
+    int chunk = fp64 ? 1024 : ctx->flux_chunk;
+    if (chunk <= 0) {  // the largest face chunk that still gives every CU several tasks
+        chunk = 2048;
+        while (chunk > 512 && count_tasks(chunk) < 2048) chunk /= 2;
+    }
+    ctx->last_rphase = R;
+    ctx->last_chunk = chunk;
+    ctx->prof.last_rphase = R;
+    ctx->prof.last_chunk = chunk;
     const bool any_route = std::any_of(ip.begin(), ip.end(), [](const InstPlan &p) {
         return p.status == LC_OK && (!p.copy1 || !p.copy2);
     });
@@ -978,7 +1007,6 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     long long part_total = 0;
     double alg_flops = 0., alg_elems = 0., alg_bytes = 0.;
     double var_flops[16] = {0.};
-    const int ntile = S > 0 ? (S + TILE - 1) / TILE : 0;
     for (size_t i = 0; i < M; i++) {
         if (ip[i].status != LC_OK || S == 0) continue;
         for (int star = 0; star < 2; star++) {
@@ -1635,7 +1663,7 @@ lc_status lc_ctx_create(int device, lc_ctx **out) {
         for (auto &e : c->ev_var) HIPCHK(hipEventCreate(&e));
         std::memset(&c->prof, 0, sizeof(c->prof));
         if (const char *r = std::getenv("LCURVE_AMD_FLUX_R")) c->flux_rphase = std::atoi(r);
-        if (c->flux_rphase != 1 && c->flux_rphase != 2 && c->flux_rphase != 4) c->flux_rphase = 2;
+        if (c->flux_rphase != 1 && c->flux_rphase != 2 && c->flux_rphase != 4) c->flux_rphase = 0;
         if (const char *ch = std::getenv("LCURVE_AMD_FLUX_CHUNK")) c->flux_chunk = std::max(32, std::atoi(ch));
         // grids kept across calls, as the reference's per-thread cache
         // (disabled by the same environment variable)
@@ -1749,12 +1777,13 @@ lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const l
             return ms;
         };
         // a candidate replaces the current shape only when it is more than
-        // 2% faster, so timing noise does not flip between near-equal shapes
+        // 2% faster, so timing noise does not flip between near-equal shapes;
+        // the current shape is reported as what it resolved to here
         double best = time_of(r0, c0) / 1.02;
-        int br = r0, bc = c0;
+        int br = ctx->last_rphase, bc = ctx->last_chunk;
         for (int r : {1, 2, 4})
             for (int c : {512, 1024, 2048}) {
-                if (r == r0 && c == c0) continue;
+                if (r == br && c == bc) continue;
                 const double ms = time_of(r, c);
                 if (ms < best) {
                     best = ms;
@@ -1862,8 +1891,8 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
         r2[i] = make_float4((float)g->px[i], (float)g->py[i], (float)g->pz[i], 0.f);
         ec[i] = i >= g->n0 ? make_double2(g->in1[i - g->n0], g->out1[i - g->n0]) : make_double2(0., 0.);
     }
-    const int R = ctx->flux_rphase, TILE = 256 * R;
-    const int chunk = fp64 ? 1024 : ctx->flux_chunk;
+    const int R = ctx->flux_rphase > 0 ? ctx->flux_rphase : (beam != 0.0 ? 2 : 1), TILE = 256 * R;
+    const int chunk = fp64 ? 1024 : (ctx->flux_chunk > 0 ? ctx->flux_chunk : 1024);
     BatchDev B;
     B.inst = 0; B.star = star; B.grid = 0; B.list_off = -1; B.count_idx = -1;
     B.chunk = chunk;
