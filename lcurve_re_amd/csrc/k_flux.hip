@@ -263,8 +263,55 @@ __device__ __forceinline__ void sweep(const float4 *s_a0, const float4 *s_a1, co
     }
 }
 
+// Polynomial limb darkening without beaming, lensing or mucrit: for faces
+// seen and not eclipsed at every sub-phase of a tile, mu I(mu) = Q(d.e) is a
+// polynomial of degree 5 in e (Q has no constant term), so their sum is
+//   sum_i f_i Q(d_i.e) = sum_alpha q_|alpha| (|alpha|! / alpha!) M_alpha e^alpha,
+//   M_alpha = sum_i f_i d_i^alpha,
+// over the 55 monomials of degree 1..5.  The moments M are FP64 sums over
+// the face chunk (each wave owns the monomials k = wave + 4t); the tile's
+// sub-phases then evaluate the polynomial in FP64.  Terms reach ~2e4 times
+// the result, so FP64 keeps the evaluation error near 1e-11 relative.
+constexpr int NMOM = 55, MOM_PER_WAVE = 14;
+constexpr unsigned char MOM_A[NMOM] = {1,0,0,2,1,1,0,0,0,3,2,2,1,1,1,0,0,0,0,4,3,3,2,2,2,1,1,1,1,0,0,0,0,0,
+                                        5,4,4,3,3,3,2,2,2,2,1,1,1,1,1,0,0,0,0,0,0};
+constexpr unsigned char MOM_B[NMOM] = {0,1,0,0,1,0,2,1,0,0,1,0,2,1,0,3,2,1,0,0,1,0,2,1,0,3,2,1,0,4,3,2,1,0,
+                                        0,1,0,2,1,0,3,2,1,0,4,3,2,1,0,5,4,3,2,1,0};
+constexpr unsigned char MOM_C[NMOM] = {0,0,1,0,0,1,0,1,2,0,0,1,0,1,2,0,1,2,3,0,0,1,0,1,2,0,1,2,3,0,1,2,3,4,
+                                        0,0,1,0,1,2,0,1,2,3,0,1,2,3,4,0,1,2,3,4,5};
+__constant__ unsigned char c_mom_n[NMOM] = {1,1,1,2,2,2,2,2,2,3,3,3,3,3,3,3,3,3,3,4,4,4,4,4,4,4,4,4,4,4,4,4,4,4,
+                                            5,5,5,5,5,5,5,5,5,5,5,5,5,5,5,5,5,5,5,5,5};
+__constant__ double c_mom_mult[NMOM] = {1.,1.,1.,1.,2.,2.,1.,2.,1.,1.,3.,3.,3.,6.,3.,1.,3.,3.,1.,1.,4.,4.,6.,
+                                        12.,6.,4.,12.,12.,4.,1.,4.,6.,4.,1.,1.,5.,5.,10.,20.,10.,10.,30.,30.,
+                                        10.,5.,20.,30.,20.,5.,1.,5.,10.,10.,5.,1.};
+
+// m[t] += f x^a y^b z^c for monomial k = W + 4t of record (x, y, z, f)
+template <int W>
+__device__ __forceinline__ void mom_add(const float4 &rec, double (&m)[MOM_PER_WAVE]) {
+    const double x = rec.x, y = rec.y, z = rec.z;
+    double xp[6], yp[6], zp[6];
+    xp[0] = rec.w;
+    yp[0] = 1.0;
+    zp[0] = 1.0;
+#pragma unroll
+    for (int i = 1; i < 6; i++) {
+        xp[i] = xp[i - 1] * x;
+        yp[i] = yp[i - 1] * y;
+        zp[i] = zp[i - 1] * z;
+    }
+#pragma unroll
+    for (int t = 0; t < MOM_PER_WAVE; t++) {
+        if (W + 4 * t < NMOM)
+            m[t] = fma(xp[MOM_A[W + 4 * t]], yp[MOM_B[W + 4 * t]] * zp[MOM_C[W + 4 * t]], m[t]);
+    }
+}
+template <int W>
+__device__ __forceinline__ void mom_sweep(const float4 *s_m, int n, double (&m)[MOM_PER_WAVE]) {
+    for (int j = (int)(threadIdx.x & 63); j < n; j += 64) mom_add<W>(s_m[j], m);
+}
+
 template <int LT, bool BEAM, bool GLENS, bool MUC, int R>
-__global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ tasks,
+__global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict__ tasks,
                                                    const BatchDev *__restrict__ batches,
                                                    const GridDev *__restrict__ grids, FlatArrays fl, PhaseArrays pa,
                                                    int S, const int *__restrict__ lists,
@@ -275,7 +322,10 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
     __shared__ float4 s_a1[BEAM ? 2 * FB : 1];
     __shared__ float4 s_a2[GLENS ? 2 * FB : 1];
     __shared__ double2 s_io[FB];
-    __shared__ int s_cnt[2][NT / 64];
+    __shared__ int s_cnt[3][NT / 64];
+    constexpr bool MOM = LT == 0 && !BEAM && !GLENS && !MUC;  // moment sums (above)
+    __shared__ float4 s_m[MOM ? FB : 1];
+    __shared__ double s_coef[MOM ? NMOM : 1];
 
     const TaskDev T = tasks[blockIdx.x];
     const BatchDev B = batches[T.batch];
@@ -332,6 +382,10 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.;
     unsigned long long evc = 0;  // elements evaluated by this workgroup
+    double mom[MOM_PER_WAVE];
+#pragma unroll
+    for (int t = 0; t < MOM_PER_WAVE; t++) mom[t] = 0.;
+    const float full_cut = 1.e-5f + ti.rad;  // d.e_c above this: mu > 0 at every sub-phase
 
     // stage prefetch registers (clamped index keeps the loads unconditional)
     int f = f_lo + (int)threadIdx.x;
@@ -354,16 +408,21 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
             if (GLENS) pa2 = rec2[fc];
             pio = eclr[fc];
         }
-        bool act = in && fmaf(a0.z, ecz, fmaf(a0.y, ecy, a0.x * ecx)) > cull;
+        const float dc = fmaf(a0.z, ecz, fmaf(a0.y, ecy, a0.x * ecx));
+        bool act = in && dc > cull;
         int cls = 0;
         if (act && ecl) {
             cls = tile_eclipse_class(ti, io.x, io.y);
             act = cls != 2;
         }
-        int tot0, tot1;
-        const int slot0 = compact_slot(act && cls == 0, tot0, s_cnt[0]);
+        const bool full = MOM && act && cls == 0 && dc > full_cut;  // to the moment sums
+        int tot0, tot1, totm = 0;
+        const int slot0 = compact_slot(act && cls == 0 && !full, tot0, s_cnt[0]);
         const int slot1 = compact_slot(act && cls == 1, tot1, s_cnt[1]);
-        if (act) {
+        const int slotm = MOM ? compact_slot(full, totm, s_cnt[2]) : 0;
+        if (full) {
+            s_m[slotm] = a0;
+        } else if (act) {
             const int slot = cls == 0 ? slot0 : FB + slot1;
             s_a0[slot] = a0;
             if (BEAM) s_a1[slot] = a1;
@@ -371,7 +430,16 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
             if (cls) s_io[slot1] = io;
         }
         __syncthreads();
-        evc += (unsigned long long)(tot0 + tot1) * (unsigned long long)live;
+        evc += (unsigned long long)(tot0 + tot1 + totm) * (unsigned long long)live;
+        if (MOM && totm) {
+            const int nm = __builtin_amdgcn_readfirstlane(totm);
+            switch (threadIdx.x >> 6) {
+            case 0: mom_sweep<0>(s_m, nm, mom); break;
+            case 1: mom_sweep<1>(s_m, nm, mom); break;
+            case 2: mom_sweep<2>(s_m, nm, mom); break;
+            default: mom_sweep<3>(s_m, nm, mom); break;
+            }
+        }
         sweep<LT, BEAM, GLENS, MUC, false, R>(s_a0, s_a1, s_a2, s_io, tot0, ex, ey, ez, ph, K, acc);
         if (tot1)
             sweep<LT, BEAM, GLENS, MUC, true, R>(s_a0 + FB, BEAM ? s_a1 + FB : s_a1, GLENS ? s_a2 + FB : s_a2, s_io,
@@ -380,6 +448,38 @@ __global__ __launch_bounds__(NT) void k_flux_mixed(const TaskDev *__restrict__ t
         f = fn;
     }
 
+    if (MOM) {
+        // each wave's moments summed over its lanes, scaled into the
+        // polynomial's coefficients; then every sub-phase evaluates it
+        const double c1 = G.ldc[0], c2 = G.ldc[1], c3 = G.ldc[2], c4 = G.ldc[3];
+        const double q[6] = {0., 1.0 - (c1 + c2 + c3 + c4), c1 + 2.0 * c2 + 3.0 * c3 + 4.0 * c4,
+                             -(c2 + 3.0 * c3 + 6.0 * c4), c3 + 4.0 * c4, -c4};
+        const int w = (int)(threadIdx.x >> 6);
+#pragma unroll
+        for (int t = 0; t < MOM_PER_WAVE; t++) {
+            double v = mom[t];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            const int k = w + 4 * t;
+            if ((threadIdx.x & 63) == 0 && k < NMOM) s_coef[k] = v * q[c_mom_n[k]] * c_mom_mult[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const double x = dex[r], y = dey[r], z = dez[r];
+            double xp[6], yp[6], zp[6];
+            xp[0] = yp[0] = zp[0] = 1.0;
+#pragma unroll
+            for (int i = 1; i < 6; i++) {
+                xp[i] = xp[i - 1] * x;
+                yp[i] = yp[i - 1] * y;
+                zp[i] = zp[i - 1] * z;
+            }
+            double e = 0.;
+#pragma unroll
+            for (int k = 0; k < NMOM; k++) e = fma(s_coef[k], xp[MOM_A[k]] * yp[MOM_B[k]] * zp[MOM_C[k]], e);
+            acc[r] += e;
+        }
+    }
     double *out = partial + B.part_off + (size_t)T.chunk * (size_t)(B.ntile * TILE);
 #pragma unroll
     for (int r = 0; r < R; r++) {
