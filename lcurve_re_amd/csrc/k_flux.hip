@@ -263,9 +263,10 @@ __device__ __forceinline__ void sweep(const float4 *s_a0, const float4 *s_a1, co
     }
 }
 
-// Polynomial limb darkening without beaming, lensing or mucrit: for faces
-// seen and not eclipsed at every sub-phase of a tile, mu I(mu) = Q(d.e) is a
-// polynomial of degree 5 in e (Q has no constant term), so their sum is
+// Polynomial limb darkening without beaming or lensing: for faces seen (mu
+// above 0 and mucrit) and not eclipsed at every sub-phase of a tile,
+// mu I(mu) = Q(d.e) is a polynomial of degree 5 in e (Q has no constant
+// term), so their sum is
 //   sum_i f_i Q(d_i.e) = sum_alpha q_|alpha| (|alpha|! / alpha!) M_alpha e^alpha,
 //   M_alpha = sum_i f_i d_i^alpha,
 // over the 55 monomials of degree 1..5.  The moments M are FP64 sums over
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict_
     __shared__ float4 s_a2[GLENS ? 2 * FB : 1];
     __shared__ double2 s_io[FB];
     __shared__ int s_cnt[3][NT / 64];
-    constexpr bool MOM = LT == 0 && !BEAM && !GLENS && !MUC;  // moment sums (above)
+    constexpr bool MOM = LT == 0 && !BEAM && !GLENS;  // moment sums (above)
     __shared__ float4 s_m[MOM ? FB : 1];
     __shared__ double s_coef[MOM ? NMOM : 1];
 
@@ -385,7 +386,8 @@ __global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict_
     double mom[MOM_PER_WAVE];
 #pragma unroll
     for (int t = 0; t < MOM_PER_WAVE; t++) mom[t] = 0.;
-    const float full_cut = 1.e-5f + ti.rad;  // d.e_c above this: mu > 0 at every sub-phase
+    // d.e_c above this: mu > max(0, mucrit) at every sub-phase, no clip or mask
+    const float full_cut = fmaxf(0.f, (float)G.mucrit) + 1.e-5f + ti.rad;
 
     // stage prefetch registers (clamped index keeps the loads unconditional)
     int f = f_lo + (int)threadIdx.x;

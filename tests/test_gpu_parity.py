@@ -139,6 +139,30 @@ def test_light_curve_vs_oracle(case, prec, engine):
         assert abs(getattr(io, k) - getattr(rio, k)) <= 1e-6 * max(1.0, abs(getattr(rio, k))), k
 
 
+@pytest.mark.parametrize("mucrit", [(0.0, 0.0), (0.05, 0.12)])
+@pytest.mark.parametrize("limb", ["Poly", "Claret"])
+def test_limb_laws_and_mucrit_vs_oracle(engine, mucrit, limb):
+    """Both limb-darkening laws with and without a mu cut-off, both GPU
+    precisions (the mixed kernels sum faces seen at a whole tile through
+    FP64 moments for the polynomial law) against the FP64 oracle."""
+    import json
+
+    import lcurve_re_amd as L
+    from lcurve_re_amd import workloads as W
+    from oracle import pyoracle as P
+    m = L.Model.from_json(json.dumps(W.model_json(
+        phys={"q": 0.4, "iangle": 84.0, "r1": 0.12, "r2": 0.25},
+        comp={"nlat1f": 32, "nlat2f": 32, "nlat1c": 32, "nlat2c": 32, "limb1": limb, "limb2": limb,
+              "mucrit1": mucrit[0], "mucrit2": mucrit[1]})))
+    data = L.make_data(np.arange(1500) / 1500.0, ferr=1e-3, flux=1.0)
+    st, ref, _ = P.light_curve(m.pod, data, scale=False, rdata=True)
+    assert st == 0
+    for prec, tol in ((L.LC_PREC_FP64, 1e-9), (L.LC_PREC_MIXED, 1e-5)):
+        calc, _ = engine.light_curve_comp(m, data, scale=False, rdata=True, diagnostics=False, precision=prec)
+        print(f"limb {limb} mucrit {mucrit} precision {prec}: rel err {rel_err(calc, ref):.3e}", flush=True)
+        assert rel_err(calc, ref) <= tol
+
+
 def test_detached_small_fp64_vs_oracle(engine):
     """The small detached benchmark (nlat 40, 1000 phases) end to end."""
     import lcurve_re_amd as L
