@@ -272,7 +272,9 @@ __device__ __forceinline__ void sweep(const float4 *s_a0, const float4 *s_a1, co
 // over the 55 monomials of degree 1..5.  The moments M are FP64 sums over
 // the face chunk (each wave owns the monomials k = wave + 4t); the tile's
 // sub-phases then evaluate the polynomial in FP64.  Terms reach ~2e4 times
-// the result, so FP64 keeps the evaluation error near 1e-11 relative.
+// the result, so FP64 keeps the evaluation error near 1e-11 relative.  For
+// the Claret law only the integer powers of mu are a polynomial (degree 3:
+// the first 19 monomials); sweep_half adds the rest face by face.
 constexpr int NMOM = 55, MOM_PER_WAVE = 14;
 constexpr unsigned char MOM_A[NMOM] = {1,0,0,2,1,1,0,0,0,3,2,2,1,1,1,0,0,0,0,4,3,3,2,2,2,1,1,1,1,0,0,0,0,0,
                                         5,4,4,3,3,3,2,2,2,2,1,1,1,1,1,0,0,0,0,0,0};
@@ -287,7 +289,7 @@ __constant__ double c_mom_mult[NMOM] = {1.,1.,1.,1.,2.,2.,1.,2.,1.,1.,3.,3.,3.,6
                                         10.,5.,20.,30.,20.,5.,1.,5.,10.,10.,5.,1.};
 
 // m[t] += f x^a y^b z^c for monomial k = W + 4t of record (x, y, z, f)
-template <int W>
+template <int W, int NM>
 __device__ __forceinline__ void mom_add(const float4 &rec, double (&m)[MOM_PER_WAVE]) {
     const double x = rec.x, y = rec.y, z = rec.z;
     double xp[6], yp[6], zp[6];
@@ -302,17 +304,58 @@ __device__ __forceinline__ void mom_add(const float4 &rec, double (&m)[MOM_PER_W
     }
 #pragma unroll
     for (int t = 0; t < MOM_PER_WAVE; t++) {
-        if (W + 4 * t < NMOM)
+        if (W + 4 * t < NM)
             m[t] = fma(xp[MOM_A[W + 4 * t]], yp[MOM_B[W + 4 * t]] * zp[MOM_C[W + 4 * t]], m[t]);
     }
 }
-template <int W>
+template <int W, int NM>
 __device__ __forceinline__ void mom_sweep(const float4 *s_m, int n, double (&m)[MOM_PER_WAVE]) {
-    for (int j = (int)(threadIdx.x & 63); j < n; j += 64) mom_add<W>(s_m[j], m);
+    for (int j = (int)(threadIdx.x & 63); j < n; j += 64) mom_add<W, NM>(s_m[j], m);
+}
+
+// Claret law: mu I(mu) = base mu + c2 mu^2 + c4 mu^3 + mu^(3/2) (c1 + c3 mu).
+// The integer powers go through the moments of degree 1..3 (the first 19
+// monomials); this sweep adds the half-integer part of the faces seen at
+// every sub-phase (mu in (0, 1], no clip or mask needed), 8 VALU per element.
+template <int R>
+__device__ __forceinline__ void sweep_half(const float4 *s_m, int n_, const float *ex, const float *ey,
+                                           const float *ez, const Coef32 &K, double *acc) {
+#pragma clang fp contract(fast)
+    const int n = __builtin_amdgcn_readfirstlane(n_);
+    for (int k0 = 0; k0 < n; k0 += 32) {
+        float a32[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) a32[r] = 0.f;
+        const int k1 = min(k0 + 32, n);
+        int k = k0;
+        for (; k + 4 <= k1; k += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float4 a = s_m[k + u];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const float mu = fmaf(ez[r], a.z, fmaf(ey[r], a.y, ex[r] * a.x));
+                    const float v = mu * __builtin_amdgcn_sqrtf(mu) * fmaf(K.c3, mu, K.c1);
+                    a32[r] = fmaf(a.w, v, a32[r]);
+                }
+            }
+        }
+        for (; k < k1; k++) {
+            const float4 a = s_m[k];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const float mu = fmaf(ez[r], a.z, fmaf(ey[r], a.y, ex[r] * a.x));
+                const float v = mu * __builtin_amdgcn_sqrtf(mu) * fmaf(K.c3, mu, K.c1);
+                a32[r] = fmaf(a.w, v, a32[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r] += (double)a32[r];
+    }
 }
 
 template <int LT, bool BEAM, bool GLENS, bool MUC, int R>
-__global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict__ tasks,
+__global__ __launch_bounds__(NT, (LT == 1 && !BEAM && !GLENS) ? 5 : 4) void k_flux_mixed(const TaskDev *__restrict__ tasks,
                                                    const BatchDev *__restrict__ batches,
                                                    const GridDev *__restrict__ grids, FlatArrays fl, PhaseArrays pa,
                                                    int S, const int *__restrict__ lists,
@@ -324,7 +367,8 @@ __global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict_
     __shared__ float4 s_a2[GLENS ? 2 * FB : 1];
     __shared__ double2 s_io[FB];
     __shared__ int s_cnt[3][NT / 64];
-    constexpr bool MOM = LT == 0 && !BEAM && !GLENS;  // moment sums (above)
+    constexpr bool MOM = !BEAM && !GLENS;  // moment sums (above)
+    constexpr int NM = LT == 0 ? NMOM : 19;  // monomials: degree 5 (polynomial law) or 3 (Claret)
     __shared__ float4 s_m[MOM ? FB : 1];
     __shared__ double s_coef[MOM ? NMOM : 1];
 
@@ -436,11 +480,12 @@ __global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict_
         if (MOM && totm) {
             const int nm = __builtin_amdgcn_readfirstlane(totm);
             switch (threadIdx.x >> 6) {
-            case 0: mom_sweep<0>(s_m, nm, mom); break;
-            case 1: mom_sweep<1>(s_m, nm, mom); break;
-            case 2: mom_sweep<2>(s_m, nm, mom); break;
-            default: mom_sweep<3>(s_m, nm, mom); break;
+            case 0: mom_sweep<0, NM>(s_m, nm, mom); break;
+            case 1: mom_sweep<1, NM>(s_m, nm, mom); break;
+            case 2: mom_sweep<2, NM>(s_m, nm, mom); break;
+            default: mom_sweep<3, NM>(s_m, nm, mom); break;
             }
+            if (LT == 1) sweep_half<R>(s_m, nm, ex, ey, ez, K, acc);
         }
         sweep<LT, BEAM, GLENS, MUC, false, R>(s_a0, s_a1, s_a2, s_io, tot0, ex, ey, ez, ph, K, acc);
         if (tot1)
@@ -454,20 +499,33 @@ __global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict_
         // each wave's moments summed over its lanes, scaled into the
         // polynomial's coefficients; then every sub-phase evaluates it
         const double c1 = G.ldc[0], c2 = G.ldc[1], c3 = G.ldc[2], c4 = G.ldc[3];
-        const double q[6] = {0., 1.0 - (c1 + c2 + c3 + c4), c1 + 2.0 * c2 + 3.0 * c3 + 4.0 * c4,
-                             -(c2 + 3.0 * c3 + 6.0 * c4), c3 + 4.0 * c4, -c4};
+        const double base = 1.0 - (c1 + c2 + c3 + c4);
+        // coefficient of mu^n of the polynomial part
+        const double q[6] = {0., base, LT == 0 ? c1 + 2.0 * c2 + 3.0 * c3 + 4.0 * c4 : c2,
+                             LT == 0 ? -(c2 + 3.0 * c3 + 6.0 * c4) : c4, LT == 0 ? c3 + 4.0 * c4 : 0.,
+                             LT == 0 ? -c4 : 0.};
         const int w = (int)(threadIdx.x >> 6);
 #pragma unroll
         for (int t = 0; t < MOM_PER_WAVE; t++) {
             double v = mom[t];
             for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
             const int k = w + 4 * t;
-            if ((threadIdx.x & 63) == 0 && k < NMOM) s_coef[k] = v * q[c_mom_n[k]] * c_mom_mult[k];
+            if ((threadIdx.x & 63) == 0 && k < NM) s_coef[k] = v * q[c_mom_n[k]] * c_mom_mult[k];
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const double x = dex[r], y = dey[r], z = dez[r];
+            // Claret: the FP64 direction is loaded again, since keeping it
+            // live through the face loop costs registers its sweep needs
+            double x = dex[r], y = dey[r], z = dez[r];
+            if (LT == 1) {
+                const int el = tile0 + r * NT + (int)threadIdx.x;
+                const int j = ok[r] ? (B.list_off < 0 ? el : lists[B.list_off + el]) : 0;
+                const size_t idx = (size_t)B.inst * S + j;
+                x = ok[r] ? pa.ex[idx] : 0.;
+                y = ok[r] ? pa.ey[idx] : 0.;
+                z = ok[r] ? pa.ez[idx] : 0.;
+            }
             double xp[6], yp[6], zp[6];
             xp[0] = yp[0] = zp[0] = 1.0;
 #pragma unroll
@@ -478,7 +536,7 @@ __global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict_
             }
             double e = 0.;
 #pragma unroll
-            for (int k = 0; k < NMOM; k++) e = fma(s_coef[k], xp[MOM_A[k]] * yp[MOM_B[k]] * zp[MOM_C[k]], e);
+            for (int k = 0; k < NM; k++) e = fma(s_coef[k], xp[MOM_A[k]] * yp[MOM_B[k]] * zp[MOM_C[k]], e);
             acc[r] += e;
         }
     }
