@@ -100,36 +100,54 @@ LC_HD V3 drpot_star(int star, double q, double spin, const V3 &p) {
 }
 
 // A potential evaluation that keeps its distances, so that the gradient at
-// the same point needs two divisions more instead of a second evaluation
-// (dbrent asks for f(u) and then, unless it stops, for df(u)).  The
-// operations are those of rpot1/rpot2 and drpot_star term for term:
-// (1-mu)/r1 and mu/r2 are the common subexpressions, so the values are
-// bit-identical to calling the two functions.
+// the same point needs two products or divisions more instead of a second
+// evaluation (dbrent asks for f(u) and then, unless it stops, for df(u)).
+// Host code: the operations of rpot1/rpot2 and drpot_star term for term
+// ((1-mu)/r1 and mu/r2 are the common subexpressions), so the values are
+// bit-identical to calling the two functions.  Device code (the line
+// searches of the eclipse kernels): 1/r1 and 1/r2 from reciprocal square
+// roots (within about an ulp), a third of the instructions of a square root
+// and a division each; the searches compare these values with one another
+// and with the surface potential, so results move only where a comparison
+// is within an ulp.
 struct PotPoint {
     V3 r;
-    double r1sq, r2sq, c1, m2;  // c1 = (1-mu)/r1, m2 = mu/r2
+    double d1, d2;  // host: r1^2, r2^2; device: 1/r1, 1/r2
+    double c1, m2;  // (1-mu)/r1, mu/r2
 };
 LC_HD double rpot_keep(int star, double q, double spin, const V3 &p, PotPoint &k) {
     double mu = q / (1 + q);
     double comp = 1. - mu;
     double x2y2 = p.x * p.x + p.y * p.y;
     double r1sq = x2y2 + p.z * p.z;
-    double r1 = sqrt(r1sq);
     double r2sq = r1sq + 1. - 2. * p.x;
-    double r2 = sqrt(r2sq);
     k.r = p;
-    k.r1sq = r1sq;
-    k.r2sq = r2sq;
+#ifdef __HIP_DEVICE_COMPILE__
+    k.d1 = rsqrt(r1sq);
+    k.d2 = rsqrt(r2sq);
+    k.c1 = comp * k.d1;
+    k.m2 = mu * k.d2;
+#else
+    double r1 = sqrt(r1sq);
+    double r2 = sqrt(r2sq);
+    k.d1 = r1sq;
+    k.d2 = r2sq;
     k.c1 = comp / r1;
     k.m2 = mu / r2;
+#endif
     if (star == PRIMARY) return (-k.c1 - k.m2 - sqr(spin) * x2y2 / 2. + mu * p.x);
     return -k.c1 - k.m2 - spin * spin * (0.5 + 0.5 * x2y2 - p.x) - comp * p.x;
 }
 LC_HD V3 drpot_kept(int star, double q, double spin, const PotPoint &k) {
     const V3 &p = k.r;
     double mu = q / (1 + q);
-    double mu1 = k.m2 / k.r2sq;
-    double comp = k.c1 / k.r1sq;
+#ifdef __HIP_DEVICE_COMPILE__
+    double mu1 = k.m2 * (k.d2 * k.d2);
+    double comp = k.c1 * (k.d1 * k.d1);
+#else
+    double mu1 = k.m2 / k.d2;
+    double comp = k.c1 / k.d1;
+#endif
     double ssq = sqr(spin);
     double gx = star == PRIMARY ? comp * p.x + mu1 * (p.x - 1.) - ssq * p.x + mu
                                 : comp * p.x + mu1 * (p.x - 1.) - ssq * (p.x - 1.) + mu - 1.;
@@ -457,6 +475,24 @@ struct PhaseLamLine {
     V3 p;
     double phi, dphi, lam, dlam;
     int star;
+    // the two values, sharing the earth vector and the distances
+    struct Keep {
+        PotPoint pt;
+        V3 earth;
+        double lam;
+    };
+#ifdef __HIP_DEVICE_COMPILE__
+    // device: every value of the search from the kept pair (one arithmetic)
+    LC_HD double f(double x) const {
+        Keep k;
+        return f(x, k);
+    }
+    LC_HD double df(double x) const {
+        Keep k;
+        f(x, k);
+        return df(k);
+    }
+#else
     LC_HD double f(double x) const {
         return rpot_val(q, star, spin, set_earth(cosi, sini, phi + dphi * x), p, lam + dlam * x);
     }
@@ -465,12 +501,7 @@ struct PhaseLamLine {
         rpot_grad(q, star, spin, set_earth(cosi, sini, phi + dphi * x), p, lam + dlam * x, dp, dl);
         return (dp * dphi + dl * dlam);
     }
-    // the same two values, sharing the earth vector and the distances
-    struct Keep {
-        PotPoint pt;
-        V3 earth;
-        double lam;
-    };
+#endif
     LC_HD double f(double x, Keep &k) const {
         k.earth = set_earth(cosi, sini, phi + dphi * x);
         k.lam = lam + dlam * x;
@@ -489,13 +520,20 @@ struct SightLine {
     double q, spin;
     V3 earth, p;
     int star;
+    using Keep = PotPoint;
+#ifdef __HIP_DEVICE_COMPILE__
+    LC_HD double f(double x) const {
+        Keep k;
+        return f(x, k);
+    }
+#else
     LC_HD double f(double x) const { return rpot_val(q, star, spin, earth, p, x); }
     LC_HD double df(double x) const {
         double dp, dl;
         rpot_grad(q, star, spin, earth, p, x, dp, dl);
         return dl;
     }
-    using Keep = PotPoint;
+#endif
     LC_HD double f(double x, Keep &k) const { return rpot_keep(star, q, spin, add(p, scl(x, earth)), k); }
     LC_HD double df(const Keep &k) const { return dot(drpot_kept(star, q, spin, k), earth); }
 };
