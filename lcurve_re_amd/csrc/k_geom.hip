@@ -43,103 +43,124 @@ __device__ __forceinline__ int wave_lookup(int n, int t, Key key) {
 }
 
 // Eclipse of one face by the companion (set_star_grid.cpp:372-386).  For a
-// Roche companion only the reference-sphere test runs here: a hit makes the
+// Roche companion only the reference-sphere test runs here: true makes the
 // face a candidate for the full search.
-__device__ __forceinline__ void eclipse_pretest(const GridDev &G, int gidx, const V3 &pos, int face, FaceArrays fa,
-                                                int2 *cand, int *cand_count) {
+__device__ __forceinline__ bool eclipse_pretest(const GridDev &G, const V3 &pos, int face, FaceArrays fa) {
     fa.ecl[face] = 0;
     fa.ein[face] = 0.;
     fa.eout[face] = 0.;
-    if (!G.eclipse) return;
+    if (!G.eclipse) return false;
     const V3 cofm_other = G.star == PRIMARY ? v3(1., 0., 0.) : v3(0., 0., 0.);
     double phi1, phi2, lam1, lam2;
-    if (G.roche_other) {
-        if (sphere_eclipse_test(G.cosi, G.sini, pos, cofm_other, G.rref_other)) {
-            int k = atomicAdd(cand_count, 1);
-            cand[k] = make_int2(face, gidx);  // the search needs the face's grid
-        }
-    } else if (sphere_eclipse(G.cosi, G.sini, pos, cofm_other, G.r_other, phi1, phi2, lam1, lam2)) {
+    if (G.roche_other) return sphere_eclipse_test(G.cosi, G.sini, pos, cofm_other, G.rref_other);
+    if (sphere_eclipse(G.cosi, G.sini, pos, cofm_other, G.r_other, phi1, phi2, lam1, lam2)) {
         fa.ecl[face] = 1;
         fa.ein[face] = phi1;
         fa.eout[face] = phi2;
     }
+    return false;
 }
 
 __global__ __launch_bounds__(256) void k_faces(const GridDev *__restrict__ grids, const RingDev *__restrict__ rings,
                                               int nring, int nhalf_total, FaceArrays fa, int2 *cand, int *cand_count,
                                               int *inst_status) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nhalf_total) return;
-    const int lo = wave_lookup(nring, t, [&](int k) { return rings[k].half_off; });
-    const RingDev R = rings[lo];
-    const GridDev &G = grids[R.grid];
-    const int np = t - R.half_off;
-    const int nphi = R.nphi;
-    double phi = R.phi1 + R.dphi_span * (np + 0.5) / nphi;
-    double sinp, cosp;
-    sincos(phi, &sinp, &cosp);
-    V3 dirn = G.npole ? v3(R.sint * cosp, R.sint * sinp, R.cost) : v3(R.cost, R.sint * cosp, R.sint * sinp);
-    const V3 cofm = G.star == PRIMARY ? v3(0., 0., 0.) : v3(1., 0., 0.);
-    V3 posn, dvec;
-    double rad, gravity;
-    if (G.roche_self) {
-        RayPot rp = make_raypot(G.q, G.star, G.spin_self, dirn, false);
-        int fail = solve_radius(rp, G.rref_self, G.pref_self, G.acc, rad);
-        if (fail) {
-            int st = face_scalar(G.q, G.star, G.spin_self, dirn, G.rref_self, G.pref_self, G.acc, posn, dvec, rad,
-                                 gravity);
-            if (st) {
-                set_status(inst_status, G.inst, ST_FACE);
-                posn = add(cofm, scl(G.rref_self, dirn));
-                dvec = dirn;
-                rad = G.rref_self;
-                gravity = 1.;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // eclipse candidates of this item (the face and its mirror partner),
+    // appended below with one atomic per workgroup: a per-face atomic on the
+    // one counter serialises the whole launch
+    bool c1 = false, c2 = false;
+    int f1 = 0, f2 = 0, gidx = 0;
+    if (t < nhalf_total) {
+        const int lo = wave_lookup(nring, t, [&](int k) { return rings[k].half_off; });
+        const RingDev R = rings[lo];
+        const GridDev &G = grids[R.grid];
+        const int np = t - R.half_off;
+        const int nphi = R.nphi;
+        double phi = R.phi1 + R.dphi_span * (np + 0.5) / nphi;
+        double sinp, cosp;
+        sincos(phi, &sinp, &cosp);
+        V3 dirn = G.npole ? v3(R.sint * cosp, R.sint * sinp, R.cost) : v3(R.cost, R.sint * cosp, R.sint * sinp);
+        const V3 cofm = G.star == PRIMARY ? v3(0., 0., 0.) : v3(1., 0., 0.);
+        V3 posn, dvec;
+        double rad, gravity;
+        if (G.roche_self) {
+            RayPot rp = make_raypot(G.q, G.star, G.spin_self, dirn, false);
+            int fail = solve_radius(rp, G.rref_self, G.pref_self, G.acc, rad);
+            if (fail) {
+                int st = face_scalar(G.q, G.star, G.spin_self, dirn, G.rref_self, G.pref_self, G.acc, posn, dvec, rad,
+                                     gravity);
+                if (st) {
+                    set_status(inst_status, G.inst, ST_FACE);
+                    posn = add(cofm, scl(G.rref_self, dirn));
+                    dvec = dirn;
+                    rad = G.rref_self;
+                    gravity = 1.;
+                }
+            } else {
+                posn = add(cofm, scl(rad, dirn));
+                dvec = drpot_star(G.star, G.q, G.spin_self, posn);
+                gravity = len(dvec);
+                dvec = v3(dvec.x / gravity, dvec.y / gravity, dvec.z / gravity);
             }
         } else {
+            rad = G.r_self;
             posn = add(cofm, scl(rad, dirn));
-            dvec = drpot_star(G.star, G.q, G.spin_self, posn);
-            gravity = len(dvec);
-            dvec = v3(dvec.x / gravity, dvec.y / gravity, dvec.z / gravity);
+            dvec = dirn;
+            gravity = 1.;
         }
-    } else {
-        rad = G.r_self;
-        posn = add(cofm, scl(rad, dirn));
-        dvec = dirn;
-        gravity = 1.;
-    }
-    double area = ((R.dphi_span / nphi * rad * R.sint) * (R.tspan_nl * rad)) / dot(dirn, dvec);
-    const float farea = (float)area, fgrav = (float)(gravity / G.gref);
+        double area = ((R.dphi_span / nphi * rad * R.sint) * (R.tspan_nl * rad)) / dot(dirn, dvec);
+        const float farea = (float)area, fgrav = (float)(gravity / G.gref);
 
-    // faces past numface() are not part of the star (see plan_grid)
-    const int fend = G.face_off + G.nface;
-    int f = R.face_off + np;
-    if (f < fend) {
-        fa.px[f] = posn.x; fa.py[f] = posn.y; fa.pz[f] = posn.z;
-        fa.dx[f] = dvec.x; fa.dy[f] = dvec.y; fa.dz[f] = dvec.z;
-        fa.area[f] = farea;
-        fa.grav[f] = fgrav;
-        fa.flux[f] = 0.f;
-        eclipse_pretest(G, R.grid, posn, f, fa, cand, cand_count);
-    }
-
-    int np2 = nphi - 1 - np;
-    if (np2 != np && R.face_off + np2 < fend) {
-        V3 posn2 = posn, dvec2 = dvec;
-        if (G.npole) {
-            posn2.y = -posn.y;
-            dvec2.y = -dvec.y;
-        } else {
-            posn2.z = -posn.z;
-            dvec2.z = -dvec.z;
+        // faces past numface() are not part of the star (see plan_grid)
+        const int fend = G.face_off + G.nface;
+        int f = R.face_off + np;
+        if (f < fend) {
+            fa.px[f] = posn.x; fa.py[f] = posn.y; fa.pz[f] = posn.z;
+            fa.dx[f] = dvec.x; fa.dy[f] = dvec.y; fa.dz[f] = dvec.z;
+            fa.area[f] = farea;
+            fa.grav[f] = fgrav;
+            fa.flux[f] = 0.f;
+            c1 = eclipse_pretest(G, posn, f, fa);
+            f1 = f;
         }
-        int f2 = R.face_off + np2;
-        fa.px[f2] = posn2.x; fa.py[f2] = posn2.y; fa.pz[f2] = posn2.z;
-        fa.dx[f2] = dvec2.x; fa.dy[f2] = dvec2.y; fa.dz[f2] = dvec2.z;
-        fa.area[f2] = farea;
-        fa.grav[f2] = fgrav;
-        fa.flux[f2] = 0.f;
-        eclipse_pretest(G, R.grid, posn2, f2, fa, cand, cand_count);
+        gidx = R.grid;
+
+        int np2 = nphi - 1 - np;
+        if (np2 != np && R.face_off + np2 < fend) {
+            V3 posn2 = posn, dvec2 = dvec;
+            if (G.npole) {
+                posn2.y = -posn.y;
+                dvec2.y = -dvec.y;
+            } else {
+                posn2.z = -posn.z;
+                dvec2.z = -dvec.z;
+            }
+            f2 = R.face_off + np2;
+            fa.px[f2] = posn2.x; fa.py[f2] = posn2.y; fa.pz[f2] = posn2.z;
+            fa.dx[f2] = dvec2.x; fa.dy[f2] = dvec2.y; fa.dz[f2] = dvec2.z;
+            fa.area[f2] = farea;
+            fa.grav[f2] = fgrav;
+            fa.flux[f2] = 0.f;
+            c2 = eclipse_pretest(G, posn2, f2, fa);
+        }
     }
+    // candidates in lane order within the workgroup; workgroups take their
+    // ranges in arrival order (the search's results do not depend on it)
+    __shared__ int s_w[4], s_base;
+    const unsigned long long b1 = __ballot(c1), b2 = __ballot(c2);
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (lane == 0) s_w[w] = __popcll(b1) + __popcll(b2);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        s_base = tot ? atomicAdd(cand_count, tot) : 0;
+    }
+    __syncthreads();
+    int k = s_base + __popcll(b1 & below) + __popcll(b2 & below);
+    for (int v = 0; v < w; v++) k += s_w[v];
+    if (c1) cand[k++] = make_int2(f1, gidx);  // the search needs the face's grid
+    if (c2) cand[k] = make_int2(f2, gidx);
 }
 
 // FP64 ingress/egress search (ingress_egress.cpp:35-97) over the compacted
@@ -156,34 +177,55 @@ __global__ __launch_bounds__(128) void k_eclipse_potmin(const GridDev *__restric
                                                        const int *__restrict__ cand_count, EclipseWork ew,
                                                        int *inst_status) {
     const int n = *cand_count;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int2 c = cand[i];
-        const int f = c.x;
-        const GridDev &G = grids[c.y];
-        const int other = G.star == PRIMARY ? SECONDARY : PRIMARY;
-        const V3 cofm = other == PRIMARY ? v3(0., 0., 0.) : v3(1., 0., 0.);
-        const V3 r = v3(fa.px[f], fa.py[f], fa.pz[f]);
-        fa.ecl[f] = 0;
-        fa.ein[f] = 0.;
-        fa.eout[f] = 0.;
-        double phi1, phi2, lam1, lam2, phi, lam;
-        if (!sphere_eclipse(G.cosi, G.sini, r, cofm, G.rref_other, phi1, phi2, lam1, lam2)) continue;
-        const double ACC = 2. * sqrt(2. * TWOPI * (lam2 - lam1) * G.delta);
-        bool hit;
-        int st = pot_min(G.q, G.cosi, G.sini, other, G.spin_other, r, phi1, phi2, lam1, lam2, G.rref_other,
-                         G.pref_other, ACC, phi, lam, hit);
-        if (st) {
-            set_status(inst_status, G.inst, st);
-            continue;
+    __shared__ int s_w[2], s_base;
+    // the loop's trip count is the same for the whole workgroup (its
+    // barriers), items past n are inactive
+    for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const int i = i0 + (int)threadIdx.x;
+        bool hit = false;
+        int f = 0, gidx = 0;
+        double phi1, phi2, lam2, lam1, phi, lam, ACC;
+        if (i < n) {
+            const int2 c = cand[i];
+            f = c.x;
+            gidx = c.y;
+            const GridDev &G = grids[gidx];
+            const int other = G.star == PRIMARY ? SECONDARY : PRIMARY;
+            const V3 cofm = other == PRIMARY ? v3(0., 0., 0.) : v3(1., 0., 0.);
+            const V3 r = v3(fa.px[f], fa.py[f], fa.pz[f]);
+            fa.ecl[f] = 0;
+            fa.ein[f] = 0.;
+            fa.eout[f] = 0.;
+            if (sphere_eclipse(G.cosi, G.sini, r, cofm, G.rref_other, phi1, phi2, lam1, lam2)) {
+                ACC = 2. * sqrt(2. * TWOPI * (lam2 - lam1) * G.delta);
+                const int st = pot_min(G.q, G.cosi, G.sini, other, G.spin_other, r, phi1, phi2, lam1, lam2,
+                                       G.rref_other, G.pref_other, ACC, phi, lam, hit);
+                if (st) {
+                    set_status(inst_status, G.inst, st);
+                    hit = false;
+                }
+            }
         }
-        if (!hit) continue;
-        const int k = atomicAdd(ew.hit_count, 1);
-        ew.face[k] = f;
-        ew.grid[k] = (int)(&G - grids);
-        ew.phi[k] = phi;
-        ew.lim[2 * k] = phi1;
-        ew.lim[2 * k + 1] = phi2;
-        ew.acc[k] = ACC;
+        // hits appended with one atomic per workgroup and pass
+        const unsigned long long b = __ballot(hit);
+        const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+        if (lane == 0) s_w[w] = __popcll(b);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int tot = s_w[0] + s_w[1];
+            s_base = tot ? atomicAdd(ew.hit_count, tot) : 0;
+        }
+        __syncthreads();
+        if (hit) {
+            const int k = s_base + (w ? s_w[0] : 0) + __popcll(b & ((1ull << lane) - 1ull));
+            ew.face[k] = f;
+            ew.grid[k] = gidx;
+            ew.phi[k] = phi;
+            ew.lim[2 * k] = phi1;
+            ew.lim[2 * k + 1] = phi2;
+            ew.acc[k] = ACC;
+        }
+        __syncthreads();  // s_w and s_base are rewritten by the next pass
     }
 }
 

@@ -250,9 +250,18 @@ struct RayPot {
         double x2y2 = px * px + py * py;
         double rr = r * r;
         double down = r;
+#ifdef __HIP_DEVICE_COMPILE__
+        // device: the companion term from a reciprocal square root (within
+        // about an ulp; a chop decision moves only where the potential is
+        // within an ulp of pref, and then the radius by at most acc)
+        const double icomp = rsqrt(rr + 1. + (primary ? -2. : 2.) * rdx);
+        if (primary) return -comp / down - mu * icomp - spin_sq * x2y2 / 2. + mu * px;
+        return -comp * icomp - mu / down - spin_sq * (0.5 + 0.5 * x2y2 - px) - comp * px;
+#else
         double dcomp = sqrt(rr + 1. + (primary ? -2. : 2.) * rdx);
         if (primary) return -comp / down - mu / dcomp - spin_sq * x2y2 / 2. + mu * px;
         return -comp / dcomp - mu / down - spin_sq * (0.5 + 0.5 * x2y2 - px) - comp * px;
+#endif
     }
 };
 
