@@ -993,7 +993,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     int chunk = fp64 ? 1024 : ctx->flux_chunk;
     if (chunk <= 0) {  // the largest face chunk that still gives every CU several tasks
         chunk = 2048;
-        while (chunk > 512 && count_tasks(chunk) < 2048) chunk /= 2;
+        while (chunk > 256 && count_tasks(chunk) < 2048) chunk /= 2;
     }
     ctx->last_rphase = R;
     ctx->last_chunk = chunk;
@@ -1782,7 +1782,7 @@ lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const l
         double best = time_of(r0, c0) / 1.02;
         int br = ctx->last_rphase, bc = ctx->last_chunk;
         for (int r : {1, 2, 4})
-            for (int c : {512, 1024, 2048}) {
+            for (int c : {256, 512, 1024, 2048}) {
                 if (r == br && c == bc) continue;
                 const double ms = time_of(r, c);
                 if (ms < best) {
