@@ -19,6 +19,7 @@
 #include <functional>
 #include <exception>
 #include <mutex>
+#include <shared_mutex>
 #include <thread>
 #include <chrono>
 #include <cmath>
@@ -30,6 +31,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include <immintrin.h>
 #include <unistd.h>
 
 #include "../../include/lcurve_amd.h"
@@ -185,6 +187,7 @@ struct GridPlan {
     GridDev dev;
     std::vector<RingHost> rings;
     int ring_faces;  // faces laid out by the rings (<= nface)
+    int ring_halves;  // half-ring positions of the rings (face kernel work items)
 };
 
 // Exact-match key of a grid's geometry: every GridDev field the face and
@@ -196,17 +199,35 @@ struct GridPlan {
 // The key is split in two: a short signature of the GridDev fields and the
 // ring count (hashed and compared first) and the ring table itself, which
 // is compared only when two signatures are equal.
-std::string geom_sig(const GridPlan &gp) {
+struct GeoSig {
+    double dv[14];
+    int iv[9];
+    int pad;  // zero: no implicit padding, so memcmp and the hash see only values
+    bool operator==(const GeoSig &o) const { return std::memcmp(this, &o, sizeof(GeoSig)) == 0; }
+};
+struct GeoSigHash {
+    size_t operator()(const GeoSig &k) const {
+        uint64_t w[sizeof(GeoSig) / 8], h = 0x9e3779b97f4a7c15ull;
+        std::memcpy(w, &k, sizeof(w));
+        for (uint64_t x : w) {
+            h = (h ^ x) * 0xff51afd7ed558ccdull;
+            h ^= h >> 32;
+        }
+        return (size_t)h;
+    }
+};
+static_assert(sizeof(GeoSig) == 14 * 8 + 10 * 4, "GeoSig has no implicit padding");
+GeoSig geom_sig(const GridPlan &gp) {
     const GridDev &G = gp.dev;
-    std::string k;
-    auto put = [&k](const void *p, size_t n) { k.append(static_cast<const char *>(p), n); };
+    GeoSig k;
+    std::memset(&k, 0, sizeof(k));
     const int iv[9] = {G.star, G.fine,    G.nface,        G.roche_self, G.roche_other,
                        G.eclipse, G.npole, gp.ring_faces, (int)gp.rings.size()};
     const double dv[14] = {G.q,          G.spin_self, G.spin_other, G.rref_self, G.pref_self,
                            G.rref_other, G.pref_other, G.r_self,    G.r_other,   G.acc,
                            G.delta,      G.cosi,      G.sini,       G.gref};
-    put(iv, sizeof(iv));
-    put(dv, sizeof(dv));
+    std::memcpy(k.iv, iv, sizeof(iv));
+    std::memcpy(k.dv, dv, sizeof(dv));
     return k;
 }
 
@@ -217,6 +238,87 @@ bool same_rings(const std::vector<RingHost> &a, const std::vector<RingHost> &b) 
             a[i].dphi_span != b[i].dphi_span || a[i].tspan_nl != b[i].tspan_nl || a[i].nphi != b[i].nphi)
             return false;
     return true;
+}
+
+// Rings of one add_faces segment (set_star_grid.cpp:256-320), appended to
+// `rings`.
+void add_segment(std::vector<RingHost> &rings, int nlat, int star, double tlo, double thi, int nlf, int nlg) {
+    const double dtheta = PI / nlat;
+    const bool inf = (nlf > 0) || (nlg > 0);
+    int nl, nl1 = 0, nl2 = 0;
+    if (inf) {
+        nl1 = int(std::ceil((1 + nlf) * (thi - tlo) / dtheta));
+        nl2 = int(std::ceil((thi - tlo) / dtheta));
+        nl = nl1 + nl2;
+    } else {
+        nl = int(std::ceil((thi - tlo) / dtheta));
+    }
+    rings.reserve(rings.size() + std::max(nl, 0));
+    for (int nt = 0; nt < nl; nt++) {
+        RingHost R;
+        double theta, phi1, phi2;
+        int nls;
+        if (inf) {
+            if (nt < nl1) {
+                theta = tlo + (thi - tlo) * (nt + 0.5) / nl1;
+                R.sint = std::sin(theta);
+                R.cost = std::cos(theta);
+                R.nphi = std::max(8, int(PI * R.sint * (1 + nlg) / dtheta));
+                nls = nl1;
+                phi1 = star == 0 ? -PI / 2. : +PI / 2.;
+                phi2 = star == 0 ? +PI / 2. : +3. * PI / 2.;
+            } else {
+                theta = tlo + (thi - tlo) * (nt - nl1 + 0.5) / nl2;
+                R.sint = std::sin(theta);
+                R.cost = std::cos(theta);
+                R.nphi = std::max(8, int(PI * R.sint / dtheta));
+                nls = nl2;
+                phi1 = star == 0 ? +PI / 2. : -PI / 2.;
+                phi2 = star == 0 ? +3. * PI / 2. : +PI / 2.;
+            }
+        } else {
+            theta = tlo + (thi - tlo) * (nt + 0.5) / nl;
+            R.sint = std::sin(theta);
+            R.cost = std::cos(theta);
+            R.nphi = std::max(16, int(2 * PI * R.sint / dtheta));
+            nls = nl;
+            phi1 = 0.;
+            phi2 = TWOPI;
+        }
+        R.phi1 = phi1;
+        R.dphi_span = phi2 - phi1;
+        R.tspan_nl = (thi - tlo) / nls;
+        rings.push_back(R);
+    }
+}
+
+// A grid without infill has the ring table and face count of its nlat
+// alone (the one segment 0..pi, the same for both stars): those are
+// computed once per nlat in the process and copied.
+struct PlainGrid {
+    std::vector<RingHost> rings;
+    int nface, faces, halves;  // numface(); sums of nphi and (nphi + 1) / 2 over the rings
+};
+const PlainGrid &plain_grid(int nlat) {
+    static std::shared_mutex mu;
+    static std::unordered_map<int, std::unique_ptr<PlainGrid>> memo;
+    {
+        std::shared_lock<std::shared_mutex> lk(mu);
+        auto it = memo.find(nlat);
+        if (it != memo.end()) return *it->second;
+    }
+    auto pg = std::make_unique<PlainGrid>();
+    pg->nface = numface(nlat, false, 0., 0., 0, 0);
+    add_segment(pg->rings, nlat, 0, 0., PI, 0, 0);
+    pg->faces = pg->halves = 0;
+    for (const RingHost &R : pg->rings) {
+        pg->faces += R.nphi;
+        pg->halves += (R.nphi + 1) / 2;
+    }
+    std::unique_lock<std::shared_mutex> lk(mu);
+    auto &slot = memo[nlat];
+    if (!slot) slot = std::move(pg);
+    return *slot;
 }
 
 // Everything set_star_grid derives before add_faces, plus the ring table
@@ -302,7 +404,8 @@ GridPlan plan_grid(const lc_model &m, int star, bool fine) {
             thelo = std::min(PI / 2. - lcx::deg2rad(lathi),
                              std::max(0., PI / 2. - std::acos(ratio) + rangle - lcx::deg2rad(m.lfudge)));
     }
-    G.nface = numface(nlat, infill, thelo, thehi, nlatfill, nlngfill);
+    const PlainGrid *plain = !infill && nlat > 0 ? &plain_grid(nlat) : nullptr;
+    G.nface = plain ? plain->nface : numface(nlat, infill, thelo, thehi, nlatfill, nlngfill);
 
     const double delta = m.delta_phase;
     const double ACC = delta / 10.;
@@ -319,69 +422,27 @@ GridPlan plan_grid(const lc_model &m, int star, bool fine) {
     }
 
     // ring tables of the add_faces segments (set_star_grid.cpp:256-320)
-    auto segment = [&](double tlo, double thi, int nlf, int nlg) {
-        const double dtheta = PI / nlat;
-        const bool inf = (nlf > 0) || (nlg > 0);
-        int nl, nl1 = 0, nl2 = 0;
-        if (inf) {
-            nl1 = int(std::ceil((1 + nlf) * (thi - tlo) / dtheta));
-            nl2 = int(std::ceil((thi - tlo) / dtheta));
-            nl = nl1 + nl2;
-        } else {
-            nl = int(std::ceil((thi - tlo) / dtheta));
-        }
-        for (int nt = 0; nt < nl; nt++) {
-            RingHost R;
-            double theta, phi1, phi2;
-            int nls;
-            if (inf) {
-                if (nt < nl1) {
-                    theta = tlo + (thi - tlo) * (nt + 0.5) / nl1;
-                    R.sint = std::sin(theta);
-                    R.cost = std::cos(theta);
-                    R.nphi = std::max(8, int(PI * R.sint * (1 + nlg) / dtheta));
-                    nls = nl1;
-                    phi1 = star == 0 ? -PI / 2. : +PI / 2.;
-                    phi2 = star == 0 ? +PI / 2. : +3. * PI / 2.;
-                } else {
-                    theta = tlo + (thi - tlo) * (nt - nl1 + 0.5) / nl2;
-                    R.sint = std::sin(theta);
-                    R.cost = std::cos(theta);
-                    R.nphi = std::max(8, int(PI * R.sint / dtheta));
-                    nls = nl2;
-                    phi1 = star == 0 ? +PI / 2. : -PI / 2.;
-                    phi2 = star == 0 ? +3. * PI / 2. : +PI / 2.;
-                }
-            } else {
-                theta = tlo + (thi - tlo) * (nt + 0.5) / nl;
-                R.sint = std::sin(theta);
-                R.cost = std::cos(theta);
-                R.nphi = std::max(16, int(2 * PI * R.sint / dtheta));
-                nls = nl;
-                phi1 = 0.;
-                phi2 = TWOPI;
-            }
-            R.phi1 = phi1;
-            R.dphi_span = phi2 - phi1;
-            R.tspan_nl = (thi - tlo) / nls;
-            gp.rings.push_back(R);
-        }
-    };
-    if (nlat > 0) {
-        if (infill) {
-            segment(0., thelo, 0, 0);
-            segment(thelo, thehi, nlatfill, nlngfill);
-            segment(thehi, PI, 0, 0);
-        } else {
-            segment(0., PI, 0, 0);
-        }
+    if (plain) {
+        gp.rings = plain->rings;
+    } else if (nlat > 0) {
+        add_segment(gp.rings, nlat, star, 0., thelo, 0, 0);
+        add_segment(gp.rings, nlat, star, thelo, thehi, nlatfill, nlngfill);
+        add_segment(gp.rings, nlat, star, thehi, PI, 0, 0);
     }
     // The rings may hold one or two faces more than numface() allots (nlat =
     // 153, 297, 315, 333, 339, ...): the reference writes those past the end
     // of its vector, so they are not part of the star; the face kernel
     // drops every face at or beyond nface.
-    gp.ring_faces = 0;
-    for (auto &R : gp.rings) gp.ring_faces += R.nphi;
+    if (plain) {
+        gp.ring_faces = plain->faces;
+        gp.ring_halves = plain->halves;
+    } else {
+        gp.ring_faces = gp.ring_halves = 0;
+        for (auto &R : gp.rings) {
+            gp.ring_faces += R.nphi;
+            gp.ring_halves += (R.nphi + 1) / 2;
+        }
+    }
     gp.ring_faces = std::min(gp.ring_faces, G.nface);
 
     // geometry scalars of the face and eclipse kernels
@@ -523,7 +584,7 @@ struct GeoCache {
     size_t used = 0;
     void *mem = nullptr;
     FaceArrays pool{};
-    std::unordered_map<std::string, std::vector<Rec>> map;  // by geom_sig
+    std::unordered_map<GeoSig, std::vector<Rec>, GeoSigHash> map;  // by geom_sig
     void clear() {
         map.clear();
         used = 0;
@@ -640,26 +701,36 @@ class HostPool {
         static HostPool pool;
         return pool;
     }
+    // Runs f(0..n-1) on the caller and the workers.  Items are claimed
+    // through a counter tagged with the job's generation, so a worker that
+    // wakes after the job has ended (or after the next one has started)
+    // cannot claim an item of it; the caller waits only for workers inside
+    // the claim loop, not for every worker to wake.
     bool try_run(size_t n, const std::function<void(size_t)> &f) {
         // a forked child has the pool object but not its threads
-        if (workers_.empty() || getpid() != pid_) return false;
+        if (workers_.empty() || getpid() != pid_ || n >= (1ull << 32)) return false;
         std::unique_lock<std::mutex> job(submit_, std::try_to_lock);
         if (!job.owns_lock()) return false;
+        uint64_t g;
         {
             std::lock_guard<std::mutex> lk(mu_);
             fn_ = &f;
             n_ = n;
-            next_.store(0);
             err_ = nullptr;
-            active_ = (int)workers_.size();
-            gen_++;
+            g = ++gen_;
+            claim_.store(g << 32);
+            pub_.store(g);
         }
         wake_.notify_all();
-        work();
-        std::unique_lock<std::mutex> lk(mu_);
-        done_.wait(lk, [&] { return active_ == 0; });
-        fn_ = nullptr;
-        if (err_) std::rethrow_exception(err_);
+        run_items(g, &f, n);
+        while (busy_.load() != 0) _mm_pause();
+        std::exception_ptr e;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            e = err_;
+            err_ = nullptr;
+        }
+        if (e) std::rethrow_exception(e);
         return true;
     }
     ~HostPool() {
@@ -677,38 +748,61 @@ class HostPool {
         const unsigned nw = std::min(16u, hc) - 1;
         for (unsigned k = 0; k < nw; k++) workers_.emplace_back([this] { loop(); });
     }
-    void work() {
-        try {
-            for (size_t i; (i = next_.fetch_add(1)) < n_;) (*fn_)(i);
-        } catch (...) {
-            std::lock_guard<std::mutex> lk(mu_);
-            if (!err_) err_ = std::current_exception();
+    // claims runs of about n/64 items of job g until none is left (fn is
+    // called only after a successful claim, so it is never touched once job
+    // g has ended)
+    void run_items(uint64_t g, const std::function<void(size_t)> *fn, size_t n) {
+        const uint64_t step = std::max<uint64_t>(1, n / 64);
+        for (;;) {
+            uint64_t c = claim_.load();
+            do {
+                if ((c >> 32) != g || (c & 0xffffffffull) >= n) return;
+            } while (!claim_.compare_exchange_weak(c, c + step));
+            const size_t i0 = (size_t)(c & 0xffffffffull), i1 = std::min<size_t>(n, i0 + step);
+            try {
+                for (size_t i = i0; i < i1; i++) (*fn)(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (!err_) err_ = std::current_exception();
+            }
         }
     }
     void loop() {
-        unsigned long long seen = 0;
+        uint64_t seen = 0;
         for (;;) {
+            // the planning of one call runs several loops within a
+            // millisecond: spin briefly for the next job before sleeping
+            if (pub_.load() == seen) {
+                const auto t0 = std::chrono::steady_clock::now();
+                while (pub_.load() == seen && std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(200))
+                    _mm_pause();
+            }
+            const std::function<void(size_t)> *fn;
+            size_t n;
             {
                 std::unique_lock<std::mutex> lk(mu_);
                 wake_.wait(lk, [&] { return stop_ || gen_ != seen; });
                 if (stop_) return;
                 seen = gen_;
+                fn = fn_;
+                n = n_;
+                busy_.fetch_add(1);  // under mu_, before any claim of job `seen`
             }
-            work();
-            std::lock_guard<std::mutex> lk(mu_);
-            if (--active_ == 0) done_.notify_all();
+            run_items(seen, fn, n);
+            busy_.fetch_sub(1);
         }
     }
     const pid_t pid_;
     std::mutex submit_, mu_;
-    std::condition_variable wake_, done_;
+    std::condition_variable wake_;
     std::vector<std::thread> workers_;
     const std::function<void(size_t)> *fn_ = nullptr;
     size_t n_ = 0;
-    std::atomic<size_t> next_{0};
+    std::atomic<uint64_t> claim_{0};  // (generation << 32) | next item
+    std::atomic<uint64_t> pub_{0};    // generation of the last job, for spinning workers
+    std::atomic<int> busy_{0};        // workers inside run_items
     std::exception_ptr err_;
-    int active_ = 0;
-    unsigned long long gen_ = 0;
+    uint64_t gen_ = 0;
     bool stop_ = false;
 };
 
@@ -840,10 +934,11 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     // copied; the rest are computed (and offered to the cache).
     const int ngrid = (int)grids.size();
     std::vector<int> dup_of(ngrid, -1), hit_off(ngrid, -1);
-    std::vector<std::string> sigs(ngrid);
+    std::vector<GeoSig> sigs(ngrid);
     parallel_for((size_t)ngrid, [&](size_t g) { sigs[g] = geom_sig(grids[g]); });
     {
-        std::unordered_map<std::string, std::vector<int>> first;  // computed grids by signature
+        std::unordered_map<GeoSig, std::vector<int>, GeoSigHash> first;  // computed grids by signature
+        first.reserve((size_t)ngrid);
         for (int g = 0; g < ngrid; g++) {
             if (ctx->dedupe) {
                 auto it = first.find(sigs[g]);
@@ -877,17 +972,16 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         G.ring_off = ring_cur;
         G.nring = computed(g) ? (int)gp.rings.size() : 0;
         G.half_off = half_cur;
-        int nh = 0;
-        for (int k = 0; k < G.nring; k++) nh += (gp.rings[k].nphi + 1) / 2;
+        const int nh = G.nring ? gp.ring_halves : 0;
         G.nhalf = nh;
         half_cur += nh;
         ring_cur += G.nring;
         face_cur += G.nface;
         gdev[g] = G;
     }
-    // ring table, filled per grid on the planning threads
-    std::unique_ptr<RingDev[]> rdev(new RingDev[std::max(ring_cur, 1)]);  // not zero-filled
-    parallel_for((size_t)ngrid, [&](size_t g) {
+    // ring table of grid g, written by the planning threads straight into
+    // the pinned staging block (below)
+    auto fill_rings = [&](size_t g, RingDev *rdev) {
         const GridDev &G = gdev[g];
         int fo = G.face_off, ho = G.half_off;
         for (int k = 0; k < G.nring; k++) {
@@ -905,7 +999,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             fo += R.nphi;
             ho += (R.nphi + 1) / 2;
         }
-    });
+    };
     const int nface_total = face_cur, nhalf_total = half_cur, nring = ring_cur;
     if ((long long)face_cur > (1LL << 30)) throw EngineError(LC_E_ARG, "batch too large: more than 2^30 faces");
 
@@ -964,10 +1058,10 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     }
 
     // gaps between the ring layout and numface() hold default faces
-    std::vector<std::pair<int, int>> gaps;
+    std::vector<int2> gaps;  // (first face, count)
     for (int g = 0; g < ngrid; g++)
         if (computed(g) && grids[g].ring_faces < gdev[g].nface)
-            gaps.push_back({gdev[g].face_off + grids[g].ring_faces, gdev[g].nface - grids[g].ring_faces});
+            gaps.push_back(make_int2(gdev[g].face_off + grids[g].ring_faces, gdev[g].nface - grids[g].ring_faces));
 
     // ---------------- flux batches and tasks
     int R = ctx->flux_rphase;
@@ -1113,6 +1207,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<FaceCopy>(std::max<size_t>(copy_hit.size(), 1));
     sz.add<FaceCopy>(std::max<size_t>(copy_dup.size(), 1));
     sz.add<FaceCopy>(std::max<size_t>(copy_out.size(), 1));
+    sz.add<int2>(std::max<size_t>(gaps.size(), 1));
     for (int k = 0; k < 5; k++) sz.add<double>(std::max<size_t>(n, 1));
     sz.add<int>(std::max<size_t>(n, 1));
     sz.add<int>(n + 1);
@@ -1150,6 +1245,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     FaceCopy *d_copy_hit = A.take<FaceCopy>(std::max<size_t>(copy_hit.size(), 1));
     FaceCopy *d_copy_dup = A.take<FaceCopy>(std::max<size_t>(copy_dup.size(), 1));
     FaceCopy *d_copy_out = A.take<FaceCopy>(std::max<size_t>(copy_out.size(), 1));
+    int2 *d_gaps = A.take<int2>(std::max<size_t>(gaps.size(), 1));
     double *d_time = A.take<double>(std::max<size_t>(n, 1));
     double *d_expose = A.take<double>(std::max<size_t>(n, 1));
     double *d_flux = A.take<double>(std::max<size_t>(n, 1));
@@ -1241,7 +1337,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     };
     std::vector<Up> ups = {
         {d_grids, gdev.data(), gdev.size() * sizeof(GridDev)},
-        {d_rings, rdev.get(), (size_t)nring * sizeof(RingDev)},
+        {d_rings, nullptr, (size_t)nring * sizeof(RingDev)},  // filled in place (fill_rings)
         {d_insts, insts.data(), insts.size() * sizeof(InstDev)},
         {d_status, st0.data(), st0.size() * sizeof(int)},
         {d_time, col[0].data(), n * 8}, {d_expose, col[1].data(), n * 8}, {d_flux, col[2].data(), n * 8},
@@ -1256,6 +1352,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         {d_copy_hit, copy_hit.data(), copy_hit.size() * sizeof(FaceCopy)},
         {d_copy_dup, copy_dup.data(), copy_dup.size() * sizeof(FaceCopy)},
         {d_copy_out, copy_out.data(), copy_out.size() * sizeof(FaceCopy)},
+        {d_gaps, gaps.data(), gaps.size() * sizeof(int2)},
     };
     size_t up_end = 0;  // the uploaded arrays lead the arena
     for (auto &u : ups)
@@ -1276,30 +1373,27 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         const size_t PIECE = (size_t)1 << 20;
         std::vector<Piece> pieces;
         for (auto &u : ups)
-            for (size_t o = 0; o < u.bytes; o += PIECE)
-                pieces.push_back(Piece{ctx->pin.base + (static_cast<char *>(u.dst) - A.base) + o,
-                                       static_cast<const char *>(u.src) + o, std::min(PIECE, u.bytes - o)});
+            if (u.src)
+                for (size_t o = 0; o < u.bytes; o += PIECE)
+                    pieces.push_back(Piece{ctx->pin.base + (static_cast<char *>(u.dst) - A.base) + o,
+                                           static_cast<const char *>(u.src) + o, std::min(PIECE, u.bytes - o)});
+        RingDev *pin_rings = reinterpret_cast<RingDev *>(ctx->pin.base + (reinterpret_cast<char *>(d_rings) - A.base));
+        const size_t np = pieces.size();
         parallel_for(
-            pieces.size(), [&](size_t k) { std::memcpy(pieces[k].dst, pieces[k].src, pieces[k].bytes); }, 4);
+            np + (size_t)ngrid,
+            [&](size_t k) {
+                if (k < np)
+                    std::memcpy(pieces[k].dst, pieces[k].src, pieces[k].bytes);
+                else
+                    fill_rings(k - np, pin_rings);
+            },
+            4);
         HIPCHK(hipMemcpyAsync(A.base, ctx->pin.base, up_end, hipMemcpyHostToDevice, s));
     }
     HIPCHK(hipMemsetAsync(d_cand_count, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(ew.hit_count, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(ew.acount, 0, ECL_ROUNDS * sizeof(int), s));
-    for (auto &g : gaps) {
-        HIPCHK(hipMemsetAsync(fa.px + g.first, 0, g.second * 8, s));
-        HIPCHK(hipMemsetAsync(fa.py + g.first, 0, g.second * 8, s));
-        HIPCHK(hipMemsetAsync(fa.pz + g.first, 0, g.second * 8, s));
-        HIPCHK(hipMemsetAsync(fa.dx + g.first, 0, g.second * 8, s));
-        HIPCHK(hipMemsetAsync(fa.dy + g.first, 0, g.second * 8, s));
-        HIPCHK(hipMemsetAsync(fa.dz + g.first, 0, g.second * 8, s));
-        HIPCHK(hipMemsetAsync(fa.area + g.first, 0, g.second * 4, s));
-        std::vector<float> ones(g.second, 1.f);
-        HIPCHK(hipMemcpy(fa.grav + g.first, ones.data(), g.second * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemsetAsync(fa.ecl + g.first, 0, g.second * 4, s));
-        HIPCHK(hipMemsetAsync(fa.ein + g.first, 0, g.second * 8, s));
-        HIPCHK(hipMemsetAsync(fa.eout + g.first, 0, g.second * 8, s));
-    }
+    launch_fill_gaps(s, d_gaps, (int)gaps.size(), fa);  // one launch for all gaps
 
     DataDev dd;
     dd.time = d_time; dd.expose = d_expose; dd.flux = d_flux; dd.ferr = d_ferr; dd.weight = d_weight;
@@ -1428,14 +1522,14 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     HIPCHK(hipMemcpyAsync(&h_counts[1], ew.hit_count, sizeof(int), hipMemcpyDeviceToHost, s));
     unsigned long long h_evald[16 * 64];
     HIPCHK(hipMemcpyAsync(h_evald, fl.evald, sizeof(h_evald), hipMemcpyDeviceToHost, s));
+    std::vector<int> cnt(any_route ? M * 4 : 0);  // routed sub-phase counts
+    if (any_route) HIPCHK(hipMemcpyAsync(cnt.data(), d_counts, M * 4 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(ev[6], s));
     HIPCHK(hipStreamSynchronize(s));
     ctx->prof.eclipse_candidates += h_counts[0];
     ctx->prof.eclipse_hits += h_counts[1];
 
     if (any_route) {  // algorithmic work of the fine/coarse batches
-        std::vector<int> cnt(M * 4);
-        HIPCHK(hipMemcpy(cnt.data(), d_counts, M * 4 * sizeof(int), hipMemcpyDeviceToHost));
         for (const BatchDev &B : batches) {
             if (B.count_idx < 0) continue;
             const GridDev &G = gdev[B.grid];
@@ -2019,9 +2113,7 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
             HIPCHK(hipMemsetAsync(p + g0, 0, gap * 8, s));
         HIPCHK(hipMemsetAsync(fa.area + g0, 0, gap * 4, s));
         HIPCHK(hipMemsetAsync(fa.ecl + g0, 0, gap * 4, s));
-        std::vector<float> ones(gap, 1.f);
-        HIPCHK(hipMemcpyAsync(fa.grav + g0, ones.data(), gap * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(fa.grav + g0), 0x3f800000u, gap, s));  // 1.0f
     }
     HIPCHK(hipEventRecord(ctx->ev[0], s));
     launch_faces(s, d_grid, d_rings, (int)rings.size(), ho, fa, d_cand, d_count, d_status);

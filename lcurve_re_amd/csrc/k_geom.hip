@@ -328,6 +328,23 @@ __global__ __launch_bounds__(256) void k_eclipse_final(FaceArrays fa, EclipseWor
     }
 }
 
+// Faces past the ring layout of a grid, up to numface(): no ring writes
+// them, so they keep the default face (zeros, gravity 1).  One workgroup
+// row per gap (blockIdx.y).
+__global__ __launch_bounds__(256) void k_fill_gaps(const int2 *__restrict__ gaps, FaceArrays fa) {
+    const int2 g = gaps[blockIdx.y];
+    for (int i = (int)threadIdx.x; i < g.y; i += (int)blockDim.x) {
+        const int f = g.x + i;
+        fa.px[f] = 0.; fa.py[f] = 0.; fa.pz[f] = 0.;
+        fa.dx[f] = 0.; fa.dy[f] = 0.; fa.dz[f] = 0.;
+        fa.area[f] = 0.f;
+        fa.grav[f] = 1.f;
+        fa.ecl[f] = 0;
+        fa.ein[f] = 0.;
+        fa.eout[f] = 0.;
+    }
+}
+
 // Stage reuse: a grid whose geometry inputs equal those of another grid
 // of the batch, or of a grid kept from an earlier call, takes a copy of its
 // faces instead of recomputing them (the reference's exact-match cache,
@@ -727,6 +744,11 @@ void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays f
 void launch_copy_faces(hipStream_t s, const FaceCopy *items, int nitem, FaceArrays src, FaceArrays dst) {
     for (int k = 0; k < nitem; k += 65535)
         hipLaunchKernelGGL(k_copy_faces, dim3(32, std::min(65535, nitem - k)), dim3(256), 0, s, items + k, src, dst);
+}
+
+void launch_fill_gaps(hipStream_t s, const int2 *gaps, int ngap, FaceArrays fa) {
+    for (int k = 0; k < ngap; k += 65535)
+        hipLaunchKernelGGL(k_fill_gaps, dim3(1, std::min(65535, ngap - k)), dim3(256), 0, s, gaps + k, fa);
 }
 
 void launch_continuum(hipStream_t s, const GridDev *grids, int ngrid, int nface_total, FaceArrays fa) {

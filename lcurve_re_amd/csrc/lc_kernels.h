@@ -177,6 +177,9 @@ constexpr int FLAT_SEG = 8192;
 struct FlatSeg {
     int grid, start, n, seg0, nseg, pad[3];
 };
+// Faces between a grid's ring layout and numface() (first face, count):
+// the default face (zeros, gravity 1).
+void launch_fill_gaps(hipStream_t s, const int2 *gaps, int ngap, FaceArrays fa);
 void launch_flat_build(hipStream_t s, const GridDev *grids, const FlatSeg *segs, int nseg, int *seg_cnt,
                        FaceArrays fa, FlatArrays fl, int fp64);
 // Ginterp single-phase sums in chunks of PSUM_FACES faces: part holds

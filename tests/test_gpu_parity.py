@@ -163,6 +163,29 @@ def test_limb_laws_and_mucrit_vs_oracle(engine, mucrit, limb):
         assert rel_err(calc, ref) <= tol
 
 
+def test_ring_gap_grid_vs_oracle(engine):
+    """nlat 153: the rings lay out fewer faces than numface() allots, and
+    the faces in between keep the default face (filled by one kernel for
+    all such grids of a call); a batch of such instances and others."""
+    import json
+
+    import lcurve_re_amd as L
+    from lcurve_re_amd import workloads as W
+    from oracle import pyoracle as P
+    models = [L.Model.from_json(json.dumps(W.model_json(
+        phys={"q": 0.45, "iangle": 86.0, "r1": 0.1, "r2": 0.3},
+        comp={"nlat1f": nl, "nlat2f": nl, "nlat1c": nl, "nlat2c": nl}))) for nl in (153, 60, 153)]
+    data = L.make_data(np.arange(900) / 900.0, ferr=1e-3, flux=1.0)
+    for prec, tol in ((L.LC_PREC_FP64, 1e-9), (L.LC_PREC_MIXED, 1e-5)):
+        calc, _, status = engine.light_curve_comp_batch(models, data, scale=False, rdata=True, precision=prec)
+        assert (status == 0).all(), status
+        for i, m in enumerate(models):
+            st, ref, _ = P.light_curve(m.pod, data, scale=False, rdata=True)
+            assert st == 0
+            print(f"gap grid model {i} precision {prec}: rel err {rel_err(calc[i], ref):.3e}", flush=True)
+            assert rel_err(calc[i], ref) <= tol
+
+
 def test_detached_small_fp64_vs_oracle(engine):
     """The small detached benchmark (nlat 40, 1000 phases) end to end."""
     import lcurve_re_amd as L
