@@ -1192,7 +1192,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     for (int k = 0; k < 6; k++) sz.add<double>(FN);
     for (int k = 0; k < 4; k++) sz.add<double>(FN);  // bisection brackets
     for (int k = 0; k < 4; k++) sz.add<int>(FN);     // bisection round lists
-    sz.add<int>(ECL_ROUNDS);
+    sz.add<int>(ECL_COUNTS);
     sz.add<float4>(FN);
     sz.add<float4>(FN);
     sz.add<float4>(FN);
@@ -1281,7 +1281,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     ew.b_out = A.take<double>(2 * FN);
     ew.alist[0] = A.take<int>(2 * FN);
     ew.alist[1] = A.take<int>(2 * FN);
-    ew.acount = A.take<int>(ECL_ROUNDS);
+    ew.acount = A.take<int>(ECL_COUNTS);
     unsigned *d_bits = A.take<unsigned>(bits_words);
     ew.phase_bits = d_bits;
     ew.bits_lb = bits_lb;
@@ -1392,7 +1392,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     }
     HIPCHK(hipMemsetAsync(d_cand_count, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(ew.hit_count, 0, sizeof(int), s));
-    HIPCHK(hipMemsetAsync(ew.acount, 0, ECL_ROUNDS * sizeof(int), s));
+    HIPCHK(hipMemsetAsync(ew.acount, 0, ECL_COUNTS * sizeof(int), s));
     launch_fill_gaps(s, d_gaps, (int)gaps.size(), fa);  // one launch for all gaps
 
     DataDev dd;
@@ -2098,7 +2098,7 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     ew.b_out = A.take<double>(2 * FN);
     ew.alist[0] = A.take<int>(2 * FN);
     ew.alist[1] = A.take<int>(2 * FN);
-    ew.acount = A.take<int>(ECL_ROUNDS);
+    ew.acount = A.take<int>(ECL_COUNTS);
     ew.phase_bits = nullptr;  // no data: full searches (G.ecl_stop = 0)
     ew.bits_lb = 10;
     GridDev *d_grid = dev_upload(A, &G, 1, s);
@@ -2106,7 +2106,7 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     HIPCHK(hipMemsetAsync(d_count, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(d_status, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(ew.hit_count, 0, sizeof(int), s));
-    HIPCHK(hipMemsetAsync(ew.acount, 0, ECL_ROUNDS * sizeof(int), s));
+    HIPCHK(hipMemsetAsync(ew.acount, 0, ECL_COUNTS * sizeof(int), s));
     if (gp.ring_faces < nf) {
         int gap = nf - gp.ring_faces, g0 = gp.ring_faces;
         for (double *p : {fa.px, fa.py, fa.pz, fa.dx, fa.dy, fa.dz, fa.ein, fa.eout})

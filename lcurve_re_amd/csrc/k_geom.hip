@@ -61,7 +61,10 @@ __device__ __forceinline__ bool eclipse_pretest(const GridDev &G, const V3 &pos,
     return false;
 }
 
-__global__ __launch_bounds__(256) void k_faces(const GridDev *__restrict__ grids, const RingDev *__restrict__ rings,
+#ifndef LC_FACES_NT
+#define LC_FACES_NT 256  // 1024 (one list atomic per 2048 faces) halves the occupancy: 0.94 vs 0.66 ms
+#endif
+__global__ __launch_bounds__(LC_FACES_NT) void k_faces(const GridDev *__restrict__ grids, const RingDev *__restrict__ rings,
                                               int nring, int nhalf_total, FaceArrays fa, int2 *cand, int *cand_count,
                                               int *inst_status) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -146,14 +149,16 @@ __global__ __launch_bounds__(256) void k_faces(const GridDev *__restrict__ grids
     }
     // candidates in lane order within the workgroup; workgroups take their
     // ranges in arrival order (the search's results do not depend on it)
-    __shared__ int s_w[4], s_base;
+    constexpr int NW = LC_FACES_NT / 64;
+    __shared__ int s_w[NW], s_base;
     const unsigned long long b1 = __ballot(c1), b2 = __ballot(c2);
     const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
     const unsigned long long below = (1ull << lane) - 1ull;
     if (lane == 0) s_w[w] = __popcll(b1) + __popcll(b2);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        int tot = 0;
+        for (int v = 0; v < NW; v++) tot += s_w[v];
         s_base = tot ? atomicAdd(cand_count, tot) : 0;
     }
     __syncthreads();
@@ -165,59 +170,93 @@ __global__ __launch_bounds__(256) void k_faces(const GridDev *__restrict__ grids
 
 // FP64 ingress/egress search (ingress_egress.cpp:35-97) over the compacted
 // candidates, in three kernels so that each keeps little live state:
-//   k_eclipse_potmin  reference-sphere window + pot_min per candidate; hits
-//                     are appended to a second list;
+//   k_eclipse_potmin  reference-sphere window and pot_min's first point for
+//                     every candidate, then the full pot_min for the ones
+//                     that need it (compacted); hits go to a second list;
 //   k_eclipse_bisect  one work item per (hit, side): the ingress or egress
 //                     bisection with fblink at every step;
 //   k_eclipse_final   normalises the phases and marks the face eclipsed.
 // Counts live on the device, so the chain needs no host round trip; every
 // grid-stride loop ends once its list is consumed.
-__global__ __launch_bounds__(128) void k_eclipse_potmin(const GridDev *__restrict__ grids, int ngrid, FaceArrays fa,
+// Pass FIRST (every candidate): the sphere window and pot_min's first point
+// (pot_min_start); a potential there at or below the surface value is a hit
+// with pot_min's own result, the other candidates go to a list.  Second
+// pass (that list, compacted, so the long searches share waves): the full
+// pot_min.  Hits and list entries are appended with one atomic per
+// workgroup and pass.
+template <bool FIRST, int NT>
+__global__ __launch_bounds__(NT) void k_eclipse_potmin(const GridDev *__restrict__ grids, FaceArrays fa,
                                                        const int2 *__restrict__ cand,
                                                        const int *__restrict__ cand_count, EclipseWork ew,
                                                        int *inst_status) {
-    const int n = *cand_count;
-    __shared__ int s_w[2], s_base;
+    const int n = FIRST ? *cand_count : ew.acount[ECL_ROUNDS];
+    int *more_list = ew.alist[1];
+    constexpr int NW = NT / 64;
+    __shared__ int s_w[2][NW], s_base[2];
     // the loop's trip count is the same for the whole workgroup (its
     // barriers), items past n are inactive
     for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
-        const int i = i0 + (int)threadIdx.x;
-        bool hit = false;
-        int f = 0, gidx = 0;
+        const int t = i0 + (int)threadIdx.x;
+        bool hit = false, more = false;
+        int ci = 0, f = 0, gidx = 0;
         double phi1, phi2, lam2, lam1, phi, lam, ACC;
-        if (i < n) {
-            const int2 c = cand[i];
+        if (t < n) {
+            ci = FIRST ? t : more_list[t];
+            const int2 c = cand[ci];
             f = c.x;
             gidx = c.y;
             const GridDev &G = grids[gidx];
             const int other = G.star == PRIMARY ? SECONDARY : PRIMARY;
             const V3 cofm = other == PRIMARY ? v3(0., 0., 0.) : v3(1., 0., 0.);
             const V3 r = v3(fa.px[f], fa.py[f], fa.pz[f]);
-            fa.ecl[f] = 0;
-            fa.ein[f] = 0.;
-            fa.eout[f] = 0.;
+            if (FIRST) {
+                fa.ecl[f] = 0;
+                fa.ein[f] = 0.;
+                fa.eout[f] = 0.;
+            }
             if (sphere_eclipse(G.cosi, G.sini, r, cofm, G.rref_other, phi1, phi2, lam1, lam2)) {
                 ACC = 2. * sqrt(2. * TWOPI * (lam2 - lam1) * G.delta);
-                const int st = pot_min(G.q, G.cosi, G.sini, other, G.spin_other, r, phi1, phi2, lam1, lam2,
-                                       G.rref_other, G.pref_other, ACC, phi, lam, hit);
-                if (st) {
-                    set_status(inst_status, G.inst, st);
-                    hit = false;
+                if (FIRST) {
+                    V3 earth, rr;
+                    const double pot = pot_min_start(G.q, G.cosi, G.sini, other, G.spin_other, r, phi1, phi2, lam1,
+                                                     lam2, phi, lam, earth, rr);
+                    hit = pot <= G.pref_other;
+                    more = !hit;
+                } else {
+                    const int st = pot_min(G.q, G.cosi, G.sini, other, G.spin_other, r, phi1, phi2, lam1, lam2,
+                                           G.rref_other, G.pref_other, ACC, phi, lam, hit);
+                    if (st) {
+                        set_status(inst_status, G.inst, st);
+                        hit = false;
+                    }
                 }
             }
         }
-        // hits appended with one atomic per workgroup and pass
-        const unsigned long long b = __ballot(hit);
+        const unsigned long long bh = __ballot(hit), bm = FIRST ? __ballot(more) : 0ull;
         const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-        if (lane == 0) s_w[w] = __popcll(b);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const int tot = s_w[0] + s_w[1];
-            s_base = tot ? atomicAdd(ew.hit_count, tot) : 0;
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (lane == 0) {
+            s_w[0][w] = __popcll(bh);
+            s_w[1][w] = __popcll(bm);
         }
         __syncthreads();
+        if (threadIdx.x == 0) {
+            int th = 0, tm = 0;
+            for (int v = 0; v < NW; v++) {
+                th += s_w[0][v];
+                tm += s_w[1][v];
+            }
+            s_base[0] = th ? atomicAdd(ew.hit_count, th) : 0;
+            s_base[1] = tm ? atomicAdd(&ew.acount[ECL_ROUNDS], tm) : 0;
+        }
+        __syncthreads();
+        int bh0 = s_base[0], bm0 = s_base[1];
+        for (int v = 0; v < w; v++) {
+            bh0 += s_w[0][v];
+            bm0 += s_w[1][v];
+        }
         if (hit) {
-            const int k = s_base + (w ? s_w[0] : 0) + __popcll(b & ((1ull << lane) - 1ull));
+            const int k = bh0 + __popcll(bh & below);
             ew.face[k] = f;
             ew.grid[k] = gidx;
             ew.phi[k] = phi;
@@ -225,6 +264,7 @@ __global__ __launch_bounds__(128) void k_eclipse_potmin(const GridDev *__restric
             ew.lim[2 * k + 1] = phi2;
             ew.acc[k] = ACC;
         }
+        if (FIRST && more) more_list[bm0 + __popcll(bm & below)] = ci;
         __syncthreads();  // s_w and s_base are rewritten by the next pass
     }
 }
@@ -256,6 +296,9 @@ __device__ __forceinline__ bool bracket_clear(const unsigned *bits, int lb, doub
     return true;
 }
 
+#ifndef LC_FIRST_NT
+#define LC_FIRST_NT 512  // workgroup size of the first pot_min pass (4x fewer list atomics than 128)
+#endif
 #ifndef LC_ECL_WAVES
 #define LC_ECL_WAVES 3  // up to 168 VGPRs: three waves per SIMD, no spills
 #endif
@@ -721,7 +764,8 @@ __global__ __launch_bounds__(1024) void k_diag(const GridDev *__restrict__ grids
 void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int nring, int nhalf_total,
                   FaceArrays fa, int2 *cand, int *cand_count, int *inst_status) {
     if (nhalf_total <= 0) return;
-    hipLaunchKernelGGL(k_faces, dim3((nhalf_total + 255) / 256), dim3(256), 0, s, grids, rings, nring, nhalf_total,
+    hipLaunchKernelGGL(k_faces, dim3((nhalf_total + LC_FACES_NT - 1) / LC_FACES_NT), dim3(LC_FACES_NT), 0, s, grids,
+                       rings, nring, nhalf_total,
                        fa, cand, cand_count, inst_status);
 }
 
@@ -729,7 +773,9 @@ void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays f
                     const int *cand_count, int max_cand, EclipseWork ew, int *inst_status, bool rounds) {
     if (max_cand <= 0 || ngrid <= 0) return;
     const int blocks = std::min((max_cand + 127) / 128, 16384);
-    hipLaunchKernelGGL(k_eclipse_potmin, dim3(blocks), dim3(128), 0, s, grids, ngrid, fa, cand, cand_count, ew,
+    hipLaunchKernelGGL((k_eclipse_potmin<true, LC_FIRST_NT>), dim3(std::min((max_cand + LC_FIRST_NT - 1) / LC_FIRST_NT, 16384)),
+                       dim3(LC_FIRST_NT), 0, s, grids, fa, cand, cand_count, ew, inst_status);
+    hipLaunchKernelGGL((k_eclipse_potmin<false, 128>), dim3(blocks), dim3(128), 0, s, grids, fa, cand, cand_count, ew,
                        inst_status);
     const int blocks2 = std::min((2 * max_cand + 127) / 128, 16384);
     const int nround = rounds ? ECL_ROUNDS : 1;

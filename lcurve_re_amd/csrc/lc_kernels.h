@@ -96,8 +96,8 @@ struct EclipseWork {
     // bisection in rounds: brackets of unfinished items (indexed like res),
     // the item lists of consecutive rounds (alternating) and their counts
     double *b_in, *b_out;
-    int *alist[2];
-    int *acount;     // [ECL_ROUNDS]
+    int *alist[2];   // alist[1] first holds the candidates pot_min has to search (k_eclipse_first)
+    int *acount;     // [ECL_COUNTS]: open searches after each round, then the pot_min list count
     // per instance, a bitmap of the 2^bits_lb bins of [0, 1) that hold a
     // phase at which eclipse ranges are tested (bin = floor(phase * 2^lb))
     const unsigned *phase_bits;
@@ -108,6 +108,7 @@ struct EclipseWork {
 // ECL_FIRST_STEPS steps of every search, the second ECL_STEPS more on the
 // searches still open (compacted), the last one runs them to the end.
 constexpr int ECL_ROUNDS = 3, ECL_FIRST_STEPS = 10, ECL_STEPS = 3;
+constexpr int ECL_COUNTS = ECL_ROUNDS + 1;
 
 struct DataDev {
     const double *time, *expose, *flux, *ferr, *weight;

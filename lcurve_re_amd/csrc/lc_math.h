@@ -708,16 +708,24 @@ LC_HD int linmin(double q, int star, double spin, double cosi, double sini, cons
     return ST_OK;
 }
 
+// pot_min's first point, the middle of the region, and its potential
+// (pot_min.cpp:45-49); the search ends there when it is <= pref.
+LC_HD double pot_min_start(double q, double cosi, double sini, int star, double spin, const V3 &p, double phi1,
+                           double phi2, double lam1, double lam2, double &phi, double &lam, V3 &earth, V3 &r) {
+    phi = (phi1 + phi2) / 2.;
+    lam = (lam1 + lam2) / 2.;
+    earth = set_earth(cosi, sini, phi);
+    r = add(p, scl(lam, earth));
+    return rpot_star(star, q, spin, r);
+}
+
 // Conjugate-gradient search for a potential below pref over the region
 // (phi1..phi2, lam1..lam2) (pot_min.cpp:38-100).  *hit = eclipse found.
 LC_HD int pot_min(double q, double cosi, double sini, int star, double spin, const V3 &p, double phi1, double phi2,
                   double lam1, double lam2, double rref, double pref, double acc, double &phi, double &lam, bool &hit) {
     double dphi, dlam;
-    phi = (phi1 + phi2) / 2.;
-    lam = (lam1 + lam2) / 2.;
-    V3 earth = set_earth(cosi, sini, phi);
-    V3 r = add(p, scl(lam, earth));
-    double pot = rpot_star(star, q, spin, r);
+    V3 earth, r;
+    double pot = pot_min_start(q, cosi, sini, star, spin, p, phi1, phi2, lam1, lam2, phi, lam, earth, r);
     {
         V3 d = drpot_star(star, q, spin, r);
         V3 ed = v3(earth.y, -earth.x, 0.);
