@@ -732,8 +732,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         InstDev &I = insts[i];
         std::memset(&I, 0, sizeof(I));
         try {
-            if (m.add_disc || m.add_spot || m.iscale)
-                throw EngineError(LC_E_UNSUPPORTED, "disc, bright spot and iscale components are not supported");
+            if (m.add_disc || m.add_spot)
+                throw EngineError(LC_E_UNSUPPORTED, "disc and bright spot components are not supported");
             double r1, r2;
             get_r1r2(m, r1, r2);
             double rl2 = 1. - l1(2, m.q.value, m.spin2.value);
@@ -769,6 +769,11 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             I.phase2 = m.phase2;
             I.scale11 = I.scale12 = I.scale21 = I.scale22 = 1.;
             I.sfac0 = rq.io[i].sfac[0];
+            // iscale: each star scaled on its own (light_curve_comp.cpp:378-416, 511-573)
+            I.iscale = m.iscale ? 1 : 0;
+            I.star2 = m.t2.value > 0. ? 1 : 0;
+            I.sfac4 = rq.io[i].sfac[4];
+            I.zcols = rq.io[i].sfac[1] * 0. + rq.io[i].sfac[2] * 0. + rq.io[i].sfac[3] * 0.;
             I.copy1 = P.copy1;
             I.copy2 = P.copy2;
             I.scale = o.scale;
@@ -1104,7 +1109,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     for (int g = 0; g < ngrid; g++) max_nface = std::max(max_nface, gdev[g].nface);
     const int psum_chunks = any_route ? (max_nface + PSUM_FACES - 1) / PSUM_FACES : 1;
     sz.add<double>(std::max<size_t>(M, 1) * 8 * psum_chunks);  // Ginterp partial sums
-    const size_t scat_part = std::max<size_t>(M, 1) * std::max<size_t>(1, (n + SCAT_PTS - 1) / SCAT_PTS) * 5;
+    const size_t scat_part = std::max<size_t>(M, 1) * std::max<size_t>(1, (n + SCAT_PTS - 1) / SCAT_PTS) * SCAT_PARTS;
     sz.add<double>(scat_part);
     sz.add<int>(std::max<size_t>(diag_idx.size(), 1));
     sz.add<double>(std::max<size_t>(diag_idx.size(), 1) * 8);
@@ -1468,7 +1473,13 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         lc_scalars &io = rq.io[i];
         const double *sc = h_scal + i * 8;
         if (o.scale) {
-            for (int k = 0; k < 5; k++) io.sfac[k] = sc[0];
+            if (rq.models[i].iscale) {  // star 1, disc, edge, spot (zero columns), star 2
+                io.sfac[0] = sc[0];
+                io.sfac[1] = io.sfac[2] = io.sfac[3] = 0.;
+                io.sfac[4] = sc[4];
+            } else {
+                for (int k = 0; k < 5; k++) io.sfac[k] = sc[0];
+            }
             io.chisq = sc[1];
             io.wnok = sc[2];
         } else if (o.rdata) {

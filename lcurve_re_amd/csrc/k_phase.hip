@@ -165,8 +165,14 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
 //   k_scatter_scale  the scale (every block sums the instance's partials in
 //                    block order, so all agree), scaled calc, chi**2 sums;
 //   k_scatter_final  the instance's chi**2 in block order and its scalars.
-__device__ __forceinline__ double point_value(const InstDev &I, const DataDev &d, const PhaseArrays &pa, size_t base,
-                                              const double *o1, const double *o2, int np) {
+// The point's value: the weighted star sums (Ginterp choosing the fine or
+// coarse grid per sub-phase) in the order star 1 fine, star 1 coarse,
+// star 2 fine, star 2 coarse, plus third light (light_curve_comp.cpp:
+// 476-498); with SPLIT the two stars apart and no third light, the iscale
+// components (light_curve_comp.cpp:378-416).
+template <bool SPLIT>
+__device__ __forceinline__ void point_sums(const InstDev &I, const DataDev &d, const PhaseArrays &pa, size_t base,
+                                           const double *o1, const double *o2, int np, double &v1, double &v2) {
     const int j0 = d.sub_off[np], nd = d.ndiv[np];
     double v = 0.;
     for (int k = 0; k < nd; k++) {
@@ -178,6 +184,10 @@ __device__ __forceinline__ double point_value(const InstDev &I, const DataDev &d
             size_t j = j0 + k;
             if (pa.type[base + j] != 1) v += pa.w1[base + j] * o1[j];
         }
+    if (SPLIT) {
+        v1 = v;
+        v = 0.;
+    }
     for (int k = 0; k < nd; k++) {
         size_t j = j0 + k;
         if (I.copy2 || pa.type[base + j] == 3) v += pa.w2[base + j] * o2[j];
@@ -187,7 +197,10 @@ __device__ __forceinline__ double point_value(const InstDev &I, const DataDev &d
             size_t j = j0 + k;
             if (pa.type[base + j] != 3) v += pa.w2[base + j] * o2[j];
         }
-    return v + I.third;
+    if (SPLIT)
+        v2 = I.star2 ? v : 0.;  // star 2 is a component only when t2 > 0
+    else
+        v1 = v + I.third;
 }
 
 __global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__ insts, DataDev d, PhaseArrays pa,
@@ -200,10 +213,31 @@ __global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__
     const double *o1 = out + ((size_t)inst * 2 + 0) * d.S;
     const double *o2 = out + ((size_t)inst * 2 + 1) * d.S;
     double *c = calc + (size_t)inst * d.n;
-    double sdy = 0., syy = 0., wn = 0.;
+    double sdy = 0., syy = 0., wn = 0., s12 = 0., s22 = 0., b2 = 0.;
     const int p1 = min(d.n, (int)(blockIdx.x + 1) * SCAT_PTS);
     for (int np = blockIdx.x * SCAT_PTS + threadIdx.x; np < p1; np += 256) {
-        const double v = point_value(I, d, pa, base, o1, o2, np);
+        if (I.iscale) {
+            // rows of the weighted design matrix: weight > 0 and
+            // sigma = ferr / sqrt(weight) > 0 (svdfit.cpp:34-53)
+            double v1, v2;
+            point_sums<true>(I, d, pa, base, o1, o2, np, v1, v2);
+            c[np] = v1;
+            if (d.weight[np] > 0.) {
+                wn += d.weight[np];
+                const double z = d.ferr[np] / sqrt(d.weight[np]);
+                if (z > 0.) {
+                    const double is = 1. / z, u1 = is * v1, u2 = is * v2, yb = is * d.flux[np];
+                    sdy += u1 * yb;
+                    syy += u1 * u1;
+                    s12 += u1 * u2;
+                    s22 += u2 * u2;
+                    b2 += u2 * yb;
+                }
+            }
+            continue;
+        }
+        double v, unused;
+        point_sums<false>(I, d, pa, base, o1, o2, np, v, unused);
         c[np] = v;
         if (d.weight[np] > 0.) {
             double wgt = d.weight[np] / sqr(d.ferr[np]);
@@ -216,48 +250,114 @@ __global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__
     sdy = block_sum<256>(sdy, lds);
     syy = block_sum<256>(syy, lds);
     wn = block_sum<256>(wn, lds);
+    double *pp = part + ((size_t)inst * nblk + blockIdx.x) * SCAT_PARTS;
+    if (I.iscale) {
+        s12 = block_sum<256>(s12, lds);
+        s22 = block_sum<256>(s22, lds);
+        b2 = block_sum<256>(b2, lds);
+        if (threadIdx.x == 0) {
+            pp[5] = s12;
+            pp[6] = s22;
+            pp[7] = b2;
+        }
+    }
     if (threadIdx.x == 0) {
-        double *pp = part + ((size_t)inst * nblk + blockIdx.x) * 5;
         pp[0] = sdy;
         pp[1] = syy;
         pp[2] = wn;
     }
 }
 
-// the instance's scale from the block partials, summed in block order
-__device__ __forceinline__ void inst_scale(const InstDev &I, const double *pi, int nblk, double &scale, double &wn,
-                                           bool &want_chi) {
-    scale = I.sfac0;
-    want_chi = I.rdata != 0;
-    wn = 0.;
-    if (I.scale) {
-        double sdy = 0., syy = 0.;
-        for (int b = 0; b < nblk; b++) {
-            sdy += pi[b * 5 + 0];
-            syy += pi[b * 5 + 1];
-            wn += pi[b * 5 + 2];
-        }
-        scale = wn > 0.0 ? sdy / syy : 1.;
-        want_chi = wn > 0.0;
+// Least squares for the two star factors through the 2x2 Gram matrix of
+// the weighted design columns: its eigenvalues are the squared singular
+// values, so an eigenvalue below 1e-10 of the largest is a singular value
+// below 1e-5 of the largest, which Subs::svdfit sets to zero
+// (svdfit.cpp:58-64); the factors are then the sum over the kept
+// eigenvectors v of v (v . g) / lambda, g = (b1, b2).  The disc, edge and
+// spot columns are zero: their singular values are zero and their factors 0.
+__device__ __forceinline__ void iscale_factors(double s11, double s12, double s22, double b1, double b2, double &a0,
+                                               double &a4) {
+    a0 = a4 = 0.;
+    const double h = 0.5 * (s11 + s22), dd = 0.5 * (s11 - s22);
+    const double l1 = h + sqrt(dd * dd + s12 * s12);
+    if (!(l1 > 0.)) return;  // no rows, or both columns zero
+    const double det = s11 * s22 - s12 * s12;
+    const double l2 = fmax(0., det / l1);
+    if (l2 > 0. && l2 >= 1e-10 * l1) {
+        a0 = (s22 * b1 - s12 * b2) / det;
+        a4 = (s11 * b2 - s12 * b1) / det;
+        return;
     }
+    // one singular value kept: the eigenvector of l1
+    double v0 = s11 >= s22 ? l1 - s22 : s12, v1 = s11 >= s22 ? s12 : l1 - s11;
+    const double nv = sqrt(v0 * v0 + v1 * v1);
+    v0 /= nv;
+    v1 /= nv;
+    const double f = (v0 * b1 + v1 * b2) / l1;
+    a0 = v0 * f;
+    a4 = v1 * f;
 }
 
-__global__ __launch_bounds__(256) void k_scatter_scale(const InstDev *__restrict__ insts, DataDev d, double *calc,
-                                                      double *part) {
+// The instance's factors (star 1 / all, star 2) from the block partials,
+// summed in block order.
+__device__ __forceinline__ void inst_scale(const InstDev &I, const double *pi, int nblk, double &a0, double &a4,
+                                           double &wn, bool &want_chi) {
+    a0 = I.sfac0;
+    a4 = I.sfac4;
+    want_chi = I.rdata != 0;
+    wn = 0.;
+    if (!I.scale) return;
+    double sdy = 0., syy = 0., s12 = 0., s22 = 0., b2 = 0.;
+    for (int b = 0; b < nblk; b++) {
+        const double *q = pi + b * SCAT_PARTS;
+        sdy += q[0];
+        syy += q[1];
+        wn += q[2];
+        if (I.iscale) {
+            s12 += q[5];
+            s22 += q[6];
+            b2 += q[7];
+        }
+    }
+    if (I.iscale) {
+        iscale_factors(syy, s12, s22, sdy, b2, a0, a4);
+        want_chi = true;  // light_curve_comp.cpp:539-547
+        return;
+    }
+    a0 = a4 = wn > 0.0 ? sdy / syy : 1.;
+    want_chi = wn > 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_scatter_scale(const InstDev *__restrict__ insts, DataDev d, PhaseArrays pa,
+                                                      const double *__restrict__ out, double *calc, double *part) {
     __shared__ double lds[256];
     const int inst = blockIdx.y, nblk = gridDim.x;
     const InstDev &I = insts[inst];
     if (!I.active) return;
-    double *pi = part + (size_t)inst * nblk * 5;
-    double scale, wn;
+    double *pi = part + (size_t)inst * nblk * SCAT_PARTS;
+    double a0, a4, wn;
     bool want_chi;
-    inst_scale(I, pi, nblk, scale, wn, want_chi);
+    inst_scale(I, pi, nblk, a0, a4, wn, want_chi);
+    const size_t base = (size_t)inst * d.S;
+    const double *o1 = out + ((size_t)inst * 2 + 0) * d.S;
+    const double *o2 = out + ((size_t)inst * 2 + 1) * d.S;
+    // iscale: sfac[1..3] multiply the zero disc, edge and spot columns
+    // (0 after a fit; the caller's values otherwise, as written there)
+    const double zc = I.scale ? 0. : I.zcols;
     double *c = calc + (size_t)inst * d.n;
     double chi = 0., wn2 = 0.;
     const int p1 = min(d.n, (int)(blockIdx.x + 1) * SCAT_PTS);
     for (int np = blockIdx.x * SCAT_PTS + threadIdx.x; np < p1; np += 256) {
-        double v = c[np];
-        if (!I.scale || wn > 0.0) v *= scale;
+        double v;
+        if (I.iscale) {
+            double v1, v2;
+            point_sums<true>(I, d, pa, base, o1, o2, np, v1, v2);
+            v = a0 * v1 + zc;
+            if (I.star2) v += a4 * v2;
+        } else {
+            v = c[np];
+            if (!I.scale || wn > 0.0) v *= a0;
+        }
         c[np] = v;
         if (want_chi && d.weight[np] > 0.) {
             chi += d.weight[np] * sqr((d.flux[np] - v) / d.ferr[np]);
@@ -267,8 +367,8 @@ __global__ __launch_bounds__(256) void k_scatter_scale(const InstDev *__restrict
     chi = block_sum<256>(chi, lds);
     wn2 = block_sum<256>(wn2, lds);
     if (threadIdx.x == 0) {
-        pi[blockIdx.x * 5 + 3] = chi;
-        pi[blockIdx.x * 5 + 4] = wn2;
+        pi[blockIdx.x * SCAT_PARTS + 3] = chi;
+        pi[blockIdx.x * SCAT_PARTS + 4] = wn2;
     }
 }
 
@@ -276,18 +376,19 @@ __global__ void k_scatter_final(const InstDev *__restrict__ insts, int nblk, con
     const int inst = blockIdx.x;
     const InstDev &I = insts[inst];
     if (!I.active || threadIdx.x != 0) return;
-    const double *pi = part + (size_t)inst * nblk * 5;
-    double scale, wn;
+    const double *pi = part + (size_t)inst * nblk * SCAT_PARTS;
+    double a0, a4, wn;
     bool want_chi;
-    inst_scale(I, pi, nblk, scale, wn, want_chi);
+    inst_scale(I, pi, nblk, a0, a4, wn, want_chi);
     double chi = 0., wn2 = 0.;
     for (int b = 0; b < nblk; b++) {
-        chi += pi[b * 5 + 3];
-        wn2 += pi[b * 5 + 4];
+        chi += pi[b * SCAT_PARTS + 3];
+        wn2 += pi[b * SCAT_PARTS + 4];
     }
     double *s = scal + (size_t)inst * 8;
-    s[0] = scale;
-    if (I.scale) {
+    s[0] = a0;
+    s[4] = a4;
+    if (I.scale && !I.iscale) {
         s[1] = chi;
         s[2] = wn;
         s[3] = 1.;
@@ -320,7 +421,7 @@ void launch_scatter(hipStream_t s, const InstDev *insts, int ninst, DataDev d, P
     if (ninst <= 0) return;
     const int nblk = std::max(1, (d.n + SCAT_PTS - 1) / SCAT_PTS);
     hipLaunchKernelGGL(k_scatter_sum, dim3(nblk, ninst), dim3(256), 0, s, insts, d, pa, out, calc, part);
-    hipLaunchKernelGGL(k_scatter_scale, dim3(nblk, ninst), dim3(256), 0, s, insts, d, calc, part);
+    hipLaunchKernelGGL(k_scatter_scale, dim3(nblk, ninst), dim3(256), 0, s, insts, d, pa, out, calc, part);
     hipLaunchKernelGGL(k_scatter_final, dim3(ninst), dim3(64), 0, s, insts, nblk, part, scal);
 }
 

@@ -46,9 +46,11 @@ struct RingDev {
 struct InstDev {
     double t0, period, pdot, deltat, slope, quad, cube, third, cosi, sini;
     double phase1, phase2, scale11, scale12, scale21, scale22;
-    double sfac0;
+    double sfac0, sfac4;  // the caller's factors (scale off): star 1 / all, star 2 (iscale)
+    double zcols;         // iscale, scale off: sfac[1..3] times the disc, edge and spot columns (0)
     int copy1, copy2, grid1f, grid2f, grid1c, grid2c;
-    int scale, rdata, active, pad;
+    int scale, rdata, active, iscale;
+    int star2, pad[3];  // iscale: star 2 is a component (t2 > 0)
 };
 
 // Phases at which light_curve_comp.cpp:220-254 samples the Point-path sums.
@@ -208,8 +210,8 @@ void launch_make_tasks(hipStream_t s, const int *first, const int *order, int nb
                        int ntask, TaskDev *tasks);
 void launch_flux_reduce(hipStream_t s, const BatchDev *batches, int nbatch, int S, int threads_per_tile,
                         const int *lists, const int *counts, const double *partial, double *out);
-// part: workspace of ninst * ceil(n / SCAT_PTS) * 5 doubles
-constexpr int SCAT_PTS = 1024;
+// part: workspace of ninst * ceil(n / SCAT_PTS) * SCAT_PARTS doubles
+constexpr int SCAT_PTS = 1024, SCAT_PARTS = 8;
 void launch_scatter(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa, const double *out,
                     double *calc, double *scal, double *part);
 void launch_diag(hipStream_t s, const GridDev *grids, const int *gidx, int ng, FaceArrays fa, double *sums);

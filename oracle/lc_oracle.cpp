@@ -1814,12 +1814,77 @@ lc_ldc make_ldc(const lc_model &m, int star) {
     return l;
 }
 
+// Weighted linear least squares through a singular value decomposition
+// whose singular values below TOL = 1e-5 of the largest are set to zero
+// (Subs::svdfit, svdfit.cpp:22-76; svbksb then skips them).  The
+// decomposition here is the one-sided Jacobi method (Hestenes 1958): pairs
+// of columns of the weighted design matrix are rotated until orthogonal;
+// the column norms are then the singular values and the accumulated
+// rotations the right singular vectors, so a = sum over kept j of
+// v_j (col_j . b) / sigma_j^2.  cols[j][k] = design value of row k scaled by
+// 1/sigma_k, b[k] = y_k / sigma_k.
+void svd_lsq(std::vector<std::vector<double>> cols, const std::vector<double> &b, std::vector<double> &a) {
+    const size_t nc = cols.size(), m = b.size();
+    std::vector<std::vector<double>> V(nc, std::vector<double>(nc, 0.));
+    for (size_t j = 0; j < nc; j++) V[j][j] = 1.;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.;
+        for (size_t p = 0; p < nc; p++)
+            for (size_t q = p + 1; q < nc; q++) {
+                double al = 0., be = 0., ga = 0.;
+                for (size_t k = 0; k < m; k++) {
+                    al += cols[p][k] * cols[p][k];
+                    be += cols[q][k] * cols[q][k];
+                    ga += cols[p][k] * cols[q][k];
+                }
+                if (al == 0. || be == 0. || ga == 0.) continue;
+                off = std::max(off, std::fabs(ga) / std::sqrt(al * be));
+                const double zeta = (be - al) / (2. * ga);
+                const double t = (zeta >= 0. ? 1. : -1.) / (std::fabs(zeta) + std::sqrt(1. + zeta * zeta));
+                const double c = 1. / std::sqrt(1. + t * t), sn = c * t;
+                for (size_t k = 0; k < m; k++) {
+                    const double x = cols[p][k], y = cols[q][k];
+                    cols[p][k] = c * x - sn * y;
+                    cols[q][k] = sn * x + c * y;
+                }
+                for (size_t k = 0; k < nc; k++) {
+                    const double x = V[k][p], y = V[k][q];
+                    V[k][p] = c * x - sn * y;
+                    V[k][q] = sn * x + c * y;
+                }
+            }
+        if (off < 1e-15) break;
+    }
+    std::vector<double> sig(nc);
+    double wmax = 0.;
+    for (size_t j = 0; j < nc; j++) {
+        double ss = 0.;
+        for (size_t k = 0; k < m; k++) ss += cols[j][k] * cols[j][k];
+        sig[j] = std::sqrt(ss);
+        wmax = std::max(wmax, sig[j]);
+    }
+    const double thresh = 1e-5 * wmax;
+    a.assign(nc, 0.);
+    for (size_t j = 0; j < nc; j++) {
+        if (!(sig[j] >= thresh) || sig[j] == 0.) continue;
+        double ub = 0.;
+        for (size_t k = 0; k < m; k++) ub += cols[j][k] * b[k];
+        const double f = ub / (sig[j] * sig[j]);
+        for (size_t i = 0; i < nc; i++) a[i] += V[i][j] * f;
+    }
+}
+
 // Lcurve::light_curve_comp (light_curve_comp.cpp:119-599), no disc, no
-// bright spot, no iscale.
+// bright spot.  With iscale the two stars are separate components scaled
+// by svd_lsq (light_curve_comp.cpp:378-416, 511-573): the disc, edge and
+// spot columns are zero, so their factors come out 0, and star 2 is a
+// component only when t2 > 0.  Each star's curve is summed here in the
+// order of the batched path (fine sub-phases, then coarse), not per point
+// as the reference's single-phase kernels do: the same sums up to rounding.
 void light_curve_comp(const lc_model &mdl, const lc_datum *data, size_t n, const lc_eval_opts &o, double *calc,
-                      lc_scalars &io) {
-    if (mdl.add_disc || mdl.add_spot || mdl.iscale)
-        throw OracleError(LC_E_UNSUPPORTED, "disc, bright spot and iscale are not part of this path");
+                      lc_scalars &io, double *comp1 = nullptr, double *comp2 = nullptr) {
+    if (mdl.add_disc || mdl.add_spot)
+        throw OracleError(LC_E_UNSUPPORTED, "disc and bright spot are not part of this path");
     if (n == 0) throw OracleError(LC_E_ARG, "light_curve_comp: empty data");
     double r1, r2;
     get_r1r2(mdl, r1, r2);
@@ -1955,12 +2020,24 @@ void light_curve_comp(const lc_model &mdl, const lc_datum *data, size_t n, const
     if (!o2f.empty()) flat_sum_multi(flat_view(fg2f), pb2f.view(), a2, o2f.data());
     if (!o2c.empty()) flat_sum_multi(flat_view(fg2c), pb2c.view(), a2, o2c.data());
 
-    for (size_t np = 0; np < n; np++) calc[np] = 0.;
-    for (size_t k = 0; k < pb1f.idx.size(); k++) calc[pb1f.idx[k]] += pb1f.w[k] * o1f[k];
-    for (size_t k = 0; k < pb1c.idx.size(); k++) calc[pb1c.idx[k]] += pb1c.w[k] * o1c[k];
-    for (size_t k = 0; k < pb2f.idx.size(); k++) calc[pb2f.idx[k]] += pb2f.w[k] * o2f[k];
-    for (size_t k = 0; k < pb2c.idx.size(); k++) calc[pb2c.idx[k]] += pb2c.w[k] * o2c[k];
-    for (size_t np = 0; np < n; np++) calc[np] += mdl.third.value;
+    std::vector<double> f1, f2;  // iscale: each star's curve
+    if (mdl.iscale) {
+        f1.assign(n, 0.);
+        f2.assign(n, 0.);
+        for (size_t k = 0; k < pb1f.idx.size(); k++) f1[pb1f.idx[k]] += pb1f.w[k] * o1f[k];
+        for (size_t k = 0; k < pb1c.idx.size(); k++) f1[pb1c.idx[k]] += pb1c.w[k] * o1c[k];
+        for (size_t k = 0; k < pb2f.idx.size(); k++) f2[pb2f.idx[k]] += pb2f.w[k] * o2f[k];
+        for (size_t k = 0; k < pb2c.idx.size(); k++) f2[pb2c.idx[k]] += pb2c.w[k] * o2c[k];
+        if (comp1) std::copy(f1.begin(), f1.end(), comp1);
+        if (comp2) std::copy(f2.begin(), f2.end(), comp2);
+    } else {
+        for (size_t np = 0; np < n; np++) calc[np] = 0.;
+        for (size_t k = 0; k < pb1f.idx.size(); k++) calc[pb1f.idx[k]] += pb1f.w[k] * o1f[k];
+        for (size_t k = 0; k < pb1c.idx.size(); k++) calc[pb1c.idx[k]] += pb1c.w[k] * o1c[k];
+        for (size_t k = 0; k < pb2f.idx.size(); k++) calc[pb2f.idx[k]] += pb2f.w[k] * o2f[k];
+        for (size_t k = 0; k < pb2c.idx.size(); k++) calc[pb2c.idx[k]] += pb2c.w[k] * o2c[k];
+        for (size_t np = 0; np < n; np++) calc[np] += mdl.third.value;
+    }
 
     // wdwarf: comp_star1_flat at phase 0.5 (light_curve_comp.cpp:504-508;
     // comp_light_flat.cpp:543-571), beaming with spin = 1
@@ -1976,7 +2053,46 @@ void light_curve_comp(const lc_model &mdl, const lc_datum *data, size_t n, const
         wdwarf = s / std::max(1, 0);
     }
 
-    if (o.scale) {
+    const bool has2 = mdl.t2.value > 0.;
+    auto chi_all = [&] {
+        io.wnok = 0.;
+        io.chisq = 0.;
+        for (size_t i = 0; i < n; i++) {
+            if (data[i].weight > 0.) {
+                io.wnok += data[i].weight;
+                io.chisq += data[i].weight * sqr((data[i].flux - calc[i]) / data[i].ferr);
+            }
+        }
+    };
+    if (mdl.iscale) {
+        if (o.scale) {
+            // rows of points with weight > 0 and sigma = ferr / sqrt(weight) > 0
+            std::vector<std::vector<double>> cols(has2 ? 5 : 4);
+            std::vector<double> b;
+            for (size_t np = 0; np < n; np++) {
+                if (!(data[np].weight > 0.)) continue;
+                const double z = data[np].ferr / std::sqrt(data[np].weight);
+                if (!(z > 0.)) continue;
+                const double is = 1. / z;
+                cols[0].push_back(is * f1[np]);
+                for (int c = 1; c < 4; c++) cols[c].push_back(0.);
+                if (has2) cols[4].push_back(is * f2[np]);
+                b.push_back(is * data[np].flux);
+            }
+            std::vector<double> a;
+            svd_lsq(cols, b, a);
+            a.resize(5, 0.);
+            for (int k = 0; k < 5; k++) io.sfac[k] = a[k];
+            wdwarf *= io.sfac[0];
+        } else {
+            wdwarf *= io.sfac[0];
+        }
+        for (size_t np = 0; np < n; np++) {
+            calc[np] = io.sfac[0] * f1[np] + io.sfac[1] * 0. + io.sfac[2] * 0. + io.sfac[3] * 0.;
+            if (has2) calc[np] += io.sfac[4] * f2[np];
+        }
+        if (o.scale || o.rdata) chi_all();
+    } else if (o.scale) {
         double ssfac = re_scale(data, n, calc, io.chisq, io.wnok);
         wdwarf *= ssfac;
         for (int k = 0; k < 5; k++) io.sfac[k] = ssfac;
@@ -2249,6 +2365,18 @@ int orc_light_curve_comp(const lc_model *m, const lc_datum *data, size_t n, cons
             light_curve_comp_fast(*m, data, n, *opts, calc, *io);
         else
             light_curve_comp(*m, data, n, *opts, calc, *io);
+    });
+}
+
+int orc_star_curves(const lc_model *m, const lc_datum *data, size_t n, double *f1, double *f2) {
+    return guarded([&] {
+        lc_model mi = *m;
+        mi.iscale = 1;
+        lc_eval_opts o{};
+        lc_scalars io{};
+        for (double &f : io.sfac) f = 1.;
+        std::vector<double> calc(n);
+        light_curve_comp(mi, data, n, o, calc.data(), io, f1, f2);
     });
 }
 

@@ -106,6 +106,11 @@ int orc_light_curve_comp(const lc_model *m, const lc_datum *data, size_t n,
                          const lc_eval_opts *opts, double *calc,
                          lc_scalars *io);
 
+/* The two stars' curves that the iscale path scales separately (each with
+ * the slope factor and exposure weights, no third light). */
+int orc_star_curves(const lc_model *m, const lc_datum *data, size_t n,
+                    double *f1, double *f2);
+
 /* Number of OpenMP threads the oracle uses (0 = runtime default). */
 void orc_set_threads(int n);
 

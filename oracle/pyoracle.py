@@ -81,6 +81,7 @@ def oracle():
         "orc_re_scale": (C.c_double, [C.POINTER(abi.lc_datum), C.c_size_t, _DP, _DP, _DP]),
         "orc_light_curve_comp": (C.c_int, [C.POINTER(abi.lc_model), C.POINTER(abi.lc_datum), C.c_size_t,
                                            C.POINTER(abi.lc_eval_opts), _DP, C.POINTER(abi.lc_scalars)]),
+        "orc_star_curves": (C.c_int, [C.POINTER(abi.lc_model), C.POINTER(abi.lc_datum), C.c_size_t, _DP, _DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -273,6 +274,17 @@ def light_curve(model, data, scale=False, rdata=True, diagnostics=False, max_mod
     o = abi.lc_eval_opts(int(scale), int(rdata), 0, int(diagnostics), int(max_model_points), 0)
     st = lib.orc_light_curve_comp(C.byref(model), data, n, C.byref(o), dptr(calc), C.byref(io))
     return st, calc, io
+
+
+def star_curves(model, data):
+    """The two stars' curves of the iscale path (f1, f2), or raises."""
+    lib = oracle()
+    n = len(data)
+    f1, f2 = np.zeros(n, np.float64), np.zeros(n, np.float64)
+    st = lib.orc_star_curves(C.byref(model), data, n, dptr(f1), dptr(f2))
+    if st:
+        raise RuntimeError(f"oracle star_curves failed: {st}")
+    return f1, f2
 
 
 def evaluator():
