@@ -427,6 +427,19 @@ def run_fit_batch(L, eng, args):
     el2 = time.perf_counter() - t1
     e2.close()
     del os.environ["LCURVE_AMD_DEDUPE"]
+    # a Levenberg-Marquardt solve of the same fit (up to 100 iterations from
+    # a start 0.1% off in every parameter): batched damping candidates and
+    # one trial per engine call take the same steps; calls and time differ
+    start = F.walkers_around(model, 1, w["walker_seed"] + 1, rel=1e-3)[0]
+    lm = {"start_chisq": float(fb.sum_squares([start])[0])}
+    for k in (4, 1):
+        tl = time.perf_counter()
+        r = F.levmarq(F.FitBatch(model, data, F.engine_evaluator(eng), scale=True, precision=prec), pars=start,
+                      max_iter=100, candidates=k)
+        lm[f"candidates_{k}"] = {"s": time.perf_counter() - tl, "engine_calls": r["calls"],
+                                 "iterations": r["iterations"], "chisq": r["chisq"], "stop": r["stop"],
+                                 "rejections": sum(1 for h in r["history"] if h[3] == "reject")}
+    log(f"bench: levmarq {lm}")
     cpu, jerr = None, None
     if not args.no_cpu:
         from oracle import pyoracle as P
@@ -452,7 +465,7 @@ def run_fit_batch(L, eng, args):
             "fp32 per face, fp64 geometry/accumulation",
             "data": "synthetic", "config": cfg, "roofline": roofline,
             "no_grid_reuse": {"value": ninst / el2 if el2 > 0 else None, "ms_per_step": el2 / args.steps * 1e3},
-            "cpu_baseline": cpu, "parity": jerr}
+            "levmarq": lm, "cpu_baseline": cpu, "parity": jerr}
 
 
 if __name__ == "__main__":

@@ -122,15 +122,16 @@ class FitBatch:
                 index.append((j, len(vecs) - 1, None))
         return vecs, index
 
-    def jacobian(self, pars=None, walkers=()):
+    def jacobian(self, pars=None, walkers=(), shrink=1.0):
         """One batched call: base point, Jacobian columns, walkers.
 
-        Returns dict with J (n x npar), resid0, chisq0, calc0, steps,
-        walker_chisq (nan where a walker failed), walker_status, and the
-        number of instances evaluated per call.
+        shrink scales the difference steps (the solver shrinks them when it
+        stalls).  Returns dict with J (n x npar), resid0, chisq0, calc0,
+        steps, walker_chisq (nan where a walker failed), walker_status, and
+        the number of instances evaluated per call.
         """
         pars = np.array(self.model.get_param() if pars is None else pars, float)
-        h = fd_steps(pars, self.limits)
+        h = fd_steps(pars, self.limits, shrink)
         cols = self.plan(pars, h)
         vecs, index = self._vectors(pars, cols, range(self.npar))
         walkers = [np.asarray(w, float) for w in walkers]
@@ -185,6 +186,14 @@ class FitBatch:
                 "kinds": [k for k, _ in cols], "walker_chisq": wchi, "walker_status": wst,
                 "instances_per_call": calls}
 
+    def sum_squares(self, vectors):
+        """sum r_i^2 at each vector (nan where the evaluation failed)."""
+        if not len(vectors):
+            return np.zeros(0)
+        calc, status = self._run([np.asarray(v, float) for v in vectors])
+        return np.array([float(np.sum(residuals(self.data, calc[k]) ** 2)) if status[k] == 0 else np.nan
+                         for k in range(len(vectors))])
+
     def _column(self, kind, hj, ca, cb, r0):
         ra = residuals(self.data, ca)
         if kind == "c":
@@ -209,6 +218,192 @@ def walkers_around(model, nwalk, seed, rel=1e-3):
     return out
 
 
+SQRT_EPS = math.sqrt(np.finfo(np.float64).eps)
+
+
+def levmarq(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, gtol=0.0, tau0=1e-3, lambda_conv=1.0,
+            lambda_max=1e8, max_recoveries=3, fd_shrink_factor=0.1, candidates=4, log=None):
+    """Levenberg-Marquardt fit of fb.model's free parameters to fb.data
+    (the inner loop of levmarq_solver.cpp:1225-1500, one stage, no priors).
+
+    Damping as there: (J^T J + lambda D^2) delta = -J^T r with D_j the
+    running maximum of the Jacobian column norms (update_scaling,
+    levmarq_solver.cpp:935-950), the trial clamped to the parameter limits,
+    gain ratio rho = (|r|^2 - |r_trial|^2) / (|J delta|^2 + 2 lambda
+    |D delta|^2); rho > 0 accepts and scales lambda by
+    max(1/3, 1 - (2 rho - 1)^3), otherwise lambda *= nu, nu *= 2; lambda is
+    kept in [1e-20, lambda_max]; gtol, xtol (tested only while lambda <=
+    lambda_conv) and ftol stop tests; a rejection at lambda_max shrinks the
+    difference steps by fd_shrink_factor and rebuilds J (up to
+    max_recoveries times), after which it ends the fit at the noise floor.
+
+    Batching: while trials are rejected the damping sequence is fixed in
+    advance, so one engine call evaluates `candidates` trial steps of that
+    sequence together with the Jacobian around the first one; they are then
+    taken in order exactly as the one-at-a-time loop would take them (the
+    first accepted one ends the batch; if it is not the first, its Jacobian
+    is one more call).  The iterates are those of the sequential loop
+    (candidates=1).  Returns dict with pars, chisq (= |r|^2), iterations
+    (trials, as the reference counts them), calls, stop reason, lambda and
+    the iteration log.
+    """
+    npar = fb.npar
+    lims = fb.limits
+    p = np.array(fb.model.get_param() if pars is None else pars, float)
+    shrink = 1.0
+    jac = fb.jacobian(p, shrink=shrink)
+    calls = len(jac["instances_per_call"])
+    D = None
+
+    def scaling(J, D):
+        cn = np.sqrt(np.sum(J * J, axis=0))
+        if D is None:
+            return np.where(cn > 0.0, cn, 1.0)
+        return np.maximum(D, cn)
+
+    J = jac["J"]
+    D = scaling(J, D)
+    r = jac["resid0"]
+    sum_sq = float(np.sum(r * r))
+    JtJ, Jtr = J.T @ J, J.T @ r
+    lam, nu = tau0, 2.0
+    it = 0
+    fails = recoveries = 0
+    stop = "max_iter"
+    hist = []
+
+    def solve(lam):
+        A = JtJ + np.diag(lam * D * D)
+        try:
+            L = np.linalg.cholesky(A)
+        except np.linalg.LinAlgError:
+            return None
+        return np.linalg.solve(L.T, np.linalg.solve(L, -Jtr))
+
+    while it < max_iter:
+        if gtol > 0.0:
+            gmax = float(np.max(np.abs(Jtr) / np.maximum(1.0, D)))
+            if gmax / max(1.0, math.sqrt(sum_sq)) < gtol:
+                stop = "gtol"
+                break
+        # the trials of the damping sequence, in order, until `candidates`
+        # of them are planned (Cholesky failures and the xtol test are steps
+        # of the sequence too)
+        plan, l, n_, k = [], lam, nu, it
+        done = None
+        while len(plan) < candidates and k < max_iter:
+            d = solve(l)
+            if d is None:
+                plan.append(("chol", l, n_, None, None))
+                l, n_ = l * n_, n_ * 2.0
+                k += 1
+                continue
+            trial = np.array([min(max(p[j] + d[j], lims[j][0]), lims[j][1]) for j in range(npar)])
+            d = trial - p
+            step_norm = float(np.linalg.norm(d * D))
+            if l <= lambda_conv:
+                xnorm = float(np.linalg.norm(p * D))
+                if xnorm > 0.0 and step_norm / xnorm < xtol:
+                    plan.append(("xtol", l, n_, None, None))
+                    break
+            plan.append(("trial", l, n_, trial, d))
+            k += 1
+            if l * n_ >= lambda_max:  # a rejection here hits the ceiling: stall handling follows
+                break
+            l, n_ = l * n_, n_ * 2.0
+        trials = [e[3] for e in plan if e[0] == "trial"]
+        spec = None
+        ssq = np.zeros(0)
+        if trials:
+            try:
+                spec = fb.jacobian(trials[0], walkers=trials[1:], shrink=shrink)
+                calls += len(spec["instances_per_call"])
+                ssq = np.concatenate([[spec["chisq0"]], spec["walker_chisq"]])
+            except RuntimeError:  # the first trial failed to evaluate
+                ssq = np.concatenate([[np.nan], fb.sum_squares(trials[1:])])
+                calls += 1 if len(trials) > 1 else 0
+        ti = 0
+        accepted = False
+        for kind, l, n_, trial, d in plan:
+            it += 1  # every pass of the loop counts, the one ending at xtol too
+            if kind == "xtol":
+                done = "xtol"
+                break
+            if kind == "chol":
+                lam, nu = l * n_, n_ * 2.0
+                fails += 1
+                hist.append((it, sum_sq, lam, "cholesky"))
+                if fails > 50:
+                    done = "singular"
+                    break
+                continue
+            s_t = ssq[ti]
+            first = ti == 0
+            ti += 1
+            if not np.isfinite(s_t):
+                lam, nu = l * n_, n_ * 2.0
+                fails += 1
+                hist.append((it, sum_sq, lam, "eval_fail"))
+                if fails > 50:
+                    done = "max consecutive failures"
+                    break
+                continue  # as after a failed Cholesky: no clamp, no stall test
+            Jd = J @ d
+            pred = float(Jd @ Jd) + 2.0 * l * float(np.sum((D * d) ** 2))
+            actual = sum_sq - s_t
+            rho = actual / pred if pred > 0.0 else 0.0
+            if rho > 0.0:
+                p, sum_sq = trial, s_t
+                lam = l * max(1.0 / 3.0, 1.0 - (2.0 * rho - 1.0) ** 3)
+                nu = 2.0
+                if first:
+                    jac = spec
+                else:
+                    jac = fb.jacobian(p, shrink=shrink)
+                    calls += len(jac["instances_per_call"])
+                J = jac["J"]
+                r = jac["resid0"]
+                D = scaling(J, D)
+                JtJ, Jtr = J.T @ J, J.T @ r
+                fails = 0
+                status = "accept"
+                if sum_sq > 0.0 and lam <= lambda_conv:
+                    if abs(actual) / sum_sq < ftol and pred / sum_sq < ftol:
+                        done = "ftol"
+                        status = "ftol"
+                lam = min(max(lam, 1e-20), lambda_max)
+                hist.append((it, sum_sq, lam, status))
+                accepted = True
+                break
+            lam, nu = l * n_, n_ * 2.0
+            fails += 1
+            lam = min(max(lam, 1e-20), lambda_max)
+            hist.append((it, sum_sq, lam, "reject"))
+            if lam >= lambda_max:
+                if recoveries < max_recoveries:
+                    recoveries += 1
+                    shrink *= fd_shrink_factor
+                    jac = fb.jacobian(p, shrink=shrink)
+                    calls += len(jac["instances_per_call"])
+                    J = jac["J"]
+                    r = jac["resid0"]
+                    D = scaling(J, None)
+                    JtJ, Jtr = J.T @ J, J.T @ r
+                    lam, nu = tau0, 2.0
+                    fails = 0
+                    hist.append((it, sum_sq, lam, "recover"))
+                else:
+                    done = "noise floor"
+                break
+        if log:
+            log(f"levmarq: iteration {it} sum_sq {sum_sq:.8g} lambda {lam:.3g} calls {calls}")
+        if done:
+            stop = done
+            break
+    return {"pars": p, "chisq": sum_sq, "iterations": it, "calls": calls, "stop": stop, "lambda": lam,
+            "history": hist}
+
+
 def engine_evaluator(engine):
     """evaluate(models, data, opts) over Engine.light_curve_comp_batch."""
     def evaluate(models, data, opts):
@@ -219,4 +414,4 @@ def engine_evaluator(engine):
     return evaluate
 
 
-__all__ = ["FitBatch", "fd_steps", "residuals", "walkers_around", "engine_evaluator", "Model"]
+__all__ = ["FitBatch", "fd_steps", "residuals", "walkers_around", "engine_evaluator", "levmarq", "Model"]

@@ -1,0 +1,108 @@
+"""The batched Levenberg-Marquardt driver (lcurve_re_amd/fit.py levmarq,
+the inner loop of levmarq_solver.cpp:1225-1500).
+
+Pinned by: on a toy nonlinear model (a exp(-b t) + c sin(2 pi t) + d) it
+reaches the least-squares optimum that scipy's MINPACK lmder finds
+(scipy.optimize.least_squares, method 'lm'); batched damping candidates
+give exactly the iterates of the one-trial-at-a-time loop (candidates=1),
+from a start that forces rejections; on a light curve evaluated by the
+FP64 oracle it recovers the parameters the data were made with.  CPU only
+(the GPU engine's version is in test_gpu_levmarq.py)."""
+import json
+
+import numpy as np
+import pytest
+
+import lcurve_re_amd as L
+from lcurve_re_amd import fit as F
+
+
+class ToyModel:
+    """Just the Model methods FitBatch uses."""
+
+    def __init__(self, p, lims):
+        self.p = np.array(p, float)
+        self.lims = lims
+
+    def copy(self):
+        return ToyModel(self.p.copy(), self.lims)
+
+    def get_param(self):
+        return self.p.copy()
+
+    def set_param(self, v):
+        self.p = np.array(v, float)
+
+    def get_limit(self):
+        return self.lims
+
+    def nvary(self):
+        return len(self.p)
+
+
+T = np.linspace(0.0, 3.0, 240)
+
+
+def toy_curve(p):
+    a, b, c, d = p
+    return a * np.exp(-b * T) + c * np.sin(2 * np.pi * T) + d
+
+
+def toy_eval(models, data, opts):
+    return np.array([toy_curve(m.p) for m in models]), np.zeros(len(models), int)
+
+
+def toy_problem():
+    truth = np.array([2.0, 1.3, 0.4, 0.5])
+    rng = np.random.default_rng(7)
+    ferr = 0.01 + 0.01 * rng.random(len(T))
+    flux = toy_curve(truth) + ferr * rng.standard_normal(len(T))
+    weight = np.where(rng.random(len(T)) < 0.1, 0.0, 1.0 + rng.random(len(T)))
+    data = L.make_data(T, ferr=ferr, flux=flux, weight=weight)
+    lims = [(-10.0, 10.0), (0.01, 10.0), (-5.0, 5.0), (-5.0, 5.0)]
+    return data, lims, ferr, flux, weight
+
+
+def test_levmarq_reaches_the_minpack_optimum():
+    from scipy.optimize import least_squares
+    data, lims, ferr, flux, weight = toy_problem()
+    start = np.array([1.0, 0.5, 0.0, 0.0])
+    fb = F.FitBatch(ToyModel(start, lims), data, toy_eval, scale=False)
+    res = F.levmarq(fb, max_iter=200)
+    w = np.where(weight > 0, weight, 1.0)
+    ref = least_squares(lambda p: np.sqrt(w) * (flux - toy_curve(p)) / ferr, start, method="lm",
+                        xtol=1e-15, ftol=1e-15, gtol=1e-15)
+    np.testing.assert_allclose(res["pars"], ref.x, rtol=1e-6, atol=1e-8)
+    assert abs(res["chisq"] / float(np.sum(ref.fun ** 2)) - 1) <= 1e-10
+    assert res["stop"] in ("ftol", "xtol", "noise floor"), res["stop"]
+
+
+@pytest.mark.parametrize("k", [2, 4, 7])
+def test_batched_candidates_give_the_sequential_iterates(k):
+    data, lims, *_ = toy_problem()
+    start = np.array([8.0, 5.0, -3.0, 3.0])  # far off: rejections happen
+    seq = F.levmarq(F.FitBatch(ToyModel(start, lims), data, toy_eval, scale=False), tau0=1e-6, candidates=1)
+    bat = F.levmarq(F.FitBatch(ToyModel(start, lims), data, toy_eval, scale=False), tau0=1e-6, candidates=k)
+    assert any(h[3] == "reject" for h in seq["history"])
+    assert seq["history"] == bat["history"]
+    assert np.array_equal(seq["pars"], bat["pars"]) and seq["iterations"] == bat["iterations"]
+    assert bat["calls"] < seq["calls"]
+
+
+def test_levmarq_fits_an_oracle_light_curve():
+    from lcurve_re_amd import workloads as W
+    from oracle import pyoracle as P
+    phys = {"q": 0.4, "iangle": 84.0, "r1": 0.12, "r2": 0.25}
+    comp = {"nlat1f": 16, "nlat2f": 16, "nlat1c": 16, "nlat2c": 16}
+    truth = L.Model.from_json(json.dumps(W.model_json(phys=phys, comp=comp)))
+    t = np.arange(300) / 300.0
+    st, clean, _ = P.light_curve(truth.pod, L.make_data(t, ferr=1.0, flux=1.0), scale=False, rdata=False)
+    assert st == 0
+    ferr = 1e-3 * np.abs(clean).max() * np.ones_like(t)
+    data = L.make_data(t, ferr=ferr, flux=clean)
+    start = dict(phys, iangle=82.5, r1=0.125)
+    m = L.Model.from_json(json.dumps(W.model_json(phys=start, comp=comp, vary=("iangle", "r1"))))
+    fb = F.FitBatch(m, data, P.evaluator(), scale=False)
+    res = F.levmarq(fb, max_iter=40)
+    assert res["chisq"] < 1e-6 * float(np.sum(((clean - clean.mean()) / ferr) ** 2))
+    np.testing.assert_allclose(res["pars"], [84.0, 0.12], rtol=2e-4)
