@@ -553,7 +553,9 @@ __global__ __launch_bounds__(1024) void k_flat_build(const GridDev *__restrict__
                 fl.rec1[p] = make_float4((float)vx, (float)vy, (float)vn, 0.f);
             }
             if (G.glens) fl.rec2[p] = make_float4((float)px, (float)py, (float)pz, 0.f);  // lensing variants
-            fl.ecl[p] = make_double2(fa.ein[f], fa.eout[f]);
+            // the flux kernels read a face's range only past n0 (the
+            // faces that can be eclipsed), so plain faces skip it
+            if (!plain) fl.ecl[p] = make_double2(fa.ein[f], fa.eout[f]);
             if (fp64) {
                 fl.fdx[p] = dx; fl.fdy[p] = dy; fl.fdz[p] = dz;
                 fl.fpx[p] = px; fl.fpy[p] = py; fl.fpz[p] = pz;
