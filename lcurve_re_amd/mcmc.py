@@ -1,0 +1,215 @@
+"""Adaptive-Metropolis MCMC over the batched engine: many chains advanced
+in lockstep, the proposals of all chains evaluated in one batched call per
+step.
+
+Each chain follows the reference's sampler (mcmc_solver.cpp:196-1175,
+without the physical prior): a Gaussian proposal with per-parameter steps
+dstep * exp(s), or, once the covariance is ready, (2.38^2/d) C + eps I
+through its Cholesky factor times exp(s); proposals reflected off the
+limits (at most 20 bounces, then clamped); an evaluation failure is a
+rejection; log alpha = -0.5 (chi2' - chi2) / (T s2) with the annealing
+temperature T cooling linearly from anneal_T0 to 1 over the first
+anneal_steps and s2 = chi2_best / (N - k) fixed at the end of burn-in
+(data-error rescale); Robbins-Monro scale adaptation every adapt_interval
+steps (gamma = rate (1 + batch)^-decay, log scale clamped); Welford
+covariance of the chain's states, reset once at cov_reset_step, its
+Cholesky factor refreshed every adapt_interval steps once cov_n passes the
+warm-up (the scale adaptation restarts when the covariance proposal is
+switched on).  chi^2 is the reference's: sum over weight > 0 of
+weight ((flux - fit) / ferr)^2 (light_curve_comp.cpp:574-584).
+
+Each chain draws from its own numpy PCG64 generator (seed + chain index),
+in the reference's order (the proposal's normals, then a uniform only when
+log alpha < 0), so a chain does not depend on the evaluator or on the
+other chains.
+"""
+import math
+
+import numpy as np
+
+from . import LC_PREC_FP64, data_view
+
+
+def chisq(data, calc):
+    v = data_view(data)
+    w = v["weight"]
+    ok = w > 0.0
+    return float(np.sum(w[ok] * ((v["flux"][ok] - calc[ok]) / v["ferr"][ok]) ** 2))
+
+
+class _Chain:
+    def __init__(self, p, dsteps, lims, seed, cfg):
+        self.rng = np.random.Generator(np.random.PCG64(seed))
+        self.p = np.array(p, float)
+        self.npar = len(p)
+        self.lims = lims
+        self.d0 = np.array(dsteps, float)
+        self.dsteps = self.d0.copy()
+        self.cfg = cfg
+        self.log_scale = 0.0
+        self.batch = self.win_acc = self.win_n = 0
+        self.cov_n = 0
+        self.mean = np.zeros(self.npar)
+        self.M2 = np.zeros((self.npar, self.npar))
+        self.L = None
+        self.cov_ready = self.cov_reset = False
+        self.chi = self.best = math.inf
+        self.best_p = self.p.copy()
+        self.chi2_scale = 1.0
+        self.accepted = 0
+
+    def propose(self):
+        sf = math.exp(self.log_scale)
+        z = self.rng.standard_normal(self.npar)
+        if self.cov_ready:
+            prop = self.p + sf * (self.L @ z)
+        else:
+            prop = self.p + self.dsteps * z
+        for i, (lo, hi) in enumerate(self.lims):
+            b = 0
+            while (prop[i] < lo or prop[i] > hi) and b < 20:
+                prop[i] = 2 * lo - prop[i] if prop[i] < lo else 2 * hi - prop[i]
+                b += 1
+            prop[i] = min(max(prop[i], lo), hi)
+        return prop
+
+    def record(self, accepted):
+        c = self.cfg
+        if c["adapt"]:
+            self.win_n += 1
+            self.win_acc += 1 if accepted else 0
+            if self.win_n >= c["adapt_interval"]:
+                rate = self.win_acc / self.win_n
+                gamma = c["adapt_rate"] * (1.0 + self.batch) ** (-c["adapt_decay"])
+                self.log_scale = min(max(self.log_scale + gamma * (rate - c["target"]), c["log_min"]),
+                                     c["log_max"])
+                self.dsteps = self.d0 * math.exp(self.log_scale)
+                self.win_acc = self.win_n = 0
+                self.batch += 1
+        if c["adapt_covariance"]:
+            self.cov_n += 1
+            delta = self.p - self.mean
+            self.mean = self.mean + delta / self.cov_n
+            self.M2 = self.M2 + np.outer(delta, self.p - self.mean)
+
+    def update_cholesky(self, step):
+        c = self.cfg
+        if not c["adapt_covariance"]:
+            return
+        threshold = c["cov_min_after_reset"] if self.cov_reset else c["cov_warmup"]
+        if self.cov_n >= threshold and (not self.cov_ready or step % c["adapt_interval"] == 0):
+            L = None
+            if self.cov_n >= self.npar + 1:
+                C = c["cov_sd"] ** 2 * self.M2 / (self.cov_n - 1) + c["cov_epsilon"] * np.eye(self.npar)
+                try:
+                    L = np.linalg.cholesky(C)
+                except np.linalg.LinAlgError:
+                    L = None
+            if L is not None:
+                self.L = L
+                if not self.cov_ready:
+                    self.cov_ready = True
+                    self.log_scale = 0.0
+                    self.batch = self.win_acc = self.win_n = 0
+                    self.dsteps = self.d0.copy()
+            elif self.cov_ready:
+                self.cov_ready = False
+
+
+def adaptive_metropolis(model, data, evaluate, nchains=8, nsteps=2000, burn_in=None, seed=1, start=None,
+                        scale=True, max_model_points=0, precision=LC_PREC_FP64, anneal=True, anneal_T0=10.0,
+                        anneal_steps=None, adapt=True, target=None, adapt_interval=100, adapt_rate=1.0,
+                        adapt_decay=0.6, adapt_min_stepscale=1e-4, adapt_max_stepscale=1e4, adapt_covariance=True,
+                        cov_warmup=None, cov_epsilon=1e-6, cov_reset_step=None, scale_data_errors=True, log=None):
+    """Runs nchains chains of nsteps (the reference's defaults where it
+    has them: burn_in = nsteps / 4, anneal_steps = burn_in / 2, target 0.234
+    (0.44 for one parameter), cov_warmup = max(50 d, 1000), covariance
+    reset at 2/5 of the burn-in).  evaluate(models, data, opts) ->
+    (calc[m, n], status[m]) as fit.engine_evaluator gives.  start: one
+    vector or one per chain (default: the model's parameters).
+
+    Returns dict: chain [nchains, nsteps - burn_in, d] and chain_chisq of
+    the recorded states, acceptance per chain, best chi^2 and parameters,
+    final scale factors, chi2_scale, and the number of batched calls."""
+    npar = model.nvary()
+    burn_in = nsteps // 4 if burn_in is None else burn_in
+    anneal_steps = max(0, burn_in // 2 if anneal_steps is None else anneal_steps)
+    cfg = dict(adapt=adapt, adapt_interval=adapt_interval, adapt_rate=adapt_rate, adapt_decay=adapt_decay,
+               target=(0.44 if npar == 1 else 0.234) if target is None else target,
+               log_min=math.log(adapt_min_stepscale), log_max=math.log(adapt_max_stepscale),
+               adapt_covariance=adapt_covariance,
+               cov_warmup=max(50 * npar, 1000) if cov_warmup is None else cov_warmup,
+               cov_min_after_reset=max(10 * npar, 200), cov_epsilon=cov_epsilon,
+               cov_sd=2.38 / math.sqrt(npar))
+    cov_reset_step = burn_in * 2 // 5 if cov_reset_step is None else cov_reset_step
+    lims = [tuple(x) for x in model.get_limit()]
+    p0 = model.get_param() if start is None else start
+    starts = [np.asarray(p0, float)] * nchains if np.ndim(p0) == 1 else [np.asarray(s, float) for s in p0]
+    chains = [_Chain(starts[c], model.get_dstep(), lims, seed + c, cfg) for c in range(nchains)]
+    opts = dict(scale=scale, rdata=True, max_model_points=max_model_points, precision=precision)
+    ndata = len(data)
+
+    def run(vectors):
+        ms = []
+        for v in vectors:
+            m = model.copy()
+            m.set_param(v)
+            ms.append(m)
+        calc, status = evaluate(ms, data, opts)
+        return [chisq(data, calc[k]) if status[k] == 0 else None for k in range(len(vectors))]
+
+    calls = 1
+    for ch, x in zip(chains, run([c.p for c in chains])):
+        if x is None:
+            raise RuntimeError("a chain's starting point does not evaluate")
+        ch.chi = ch.best = x
+    keep = nsteps - burn_in
+    rec = np.zeros((nchains, max(keep, 0), npar))
+    rec_chi = np.zeros((nchains, max(keep, 0)))
+    for step in range(nsteps):
+        for ch in chains:
+            if adapt_covariance and cov_reset_step > 0 and step == cov_reset_step and not ch.cov_reset:
+                ch.cov_n = 0
+                ch.mean[:] = 0.0
+                ch.M2[:] = 0.0
+                ch.cov_ready = False
+                ch.cov_reset = True
+            if step == burn_in and scale_data_errors:
+                dof = ndata - npar
+                if dof > 0 and ch.best > 0.0:
+                    ch.chi2_scale = ch.best / dof
+        props = [ch.propose() for ch in chains]
+        res = run(props)
+        calls += 1
+        T = 1.0
+        if anneal and step < anneal_steps and anneal_steps > 0:
+            frac = step / anneal_steps
+            T = anneal_T0 * (1.0 - frac) + 1.0 * frac
+        for k, ch in enumerate(chains):
+            acc = False
+            x = res[k]
+            if x is not None:
+                la = -0.5 * (x - ch.chi) / (T * ch.chi2_scale)
+                if la >= 0.0 or math.log(ch.rng.random()) < la:
+                    ch.p, ch.chi = props[k], x
+                    ch.accepted += 1
+                    acc = True
+                    if x < ch.best:
+                        ch.best, ch.best_p = x, props[k].copy()
+            ch.record(acc)
+            if x is not None:
+                ch.update_cholesky(step)
+            if step >= burn_in:
+                rec[k, step - burn_in] = ch.p
+                rec_chi[k, step - burn_in] = ch.chi
+        if log and (step % 100 == 0 or step == nsteps - 1):
+            log(f"mcmc: step {step} accepted {np.mean([c.accepted for c in chains]) / (step + 1):.3f} "
+                f"best {min(c.best for c in chains):.6g}")
+    return {"chain": rec, "chain_chisq": rec_chi, "acceptance": np.array([c.accepted / nsteps for c in chains]),
+            "best_chisq": np.array([c.best for c in chains]), "best_pars": np.array([c.best_p for c in chains]),
+            "scale_factor": np.array([math.exp(c.log_scale) for c in chains]),
+            "cov_ready": np.array([c.cov_ready for c in chains]),
+            "chi2_scale": np.array([c.chi2_scale for c in chains]), "calls": calls}
+
+
+__all__ = ["adaptive_metropolis", "chisq"]
