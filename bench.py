@@ -16,6 +16,7 @@ workload) and ``max_rel_err`` of the GPU curves against that oracle sample.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -432,14 +433,38 @@ def run_fit_batch(L, eng, args):
     # one trial per engine call take the same steps; calls and time differ
     start = F.walkers_around(model, 1, w["walker_seed"] + 1, rel=1e-3)[0]
     lm = {"start_chisq": float(fb.sum_squares([start])[0])}
+    lm_pars = None
     for k in (4, 1):
         tl = time.perf_counter()
         r = F.levmarq(F.FitBatch(model, data, F.engine_evaluator(eng), scale=True, precision=prec), pars=start,
                       max_iter=100, candidates=k)
+        lm_pars = r["pars"] if lm_pars is None else lm_pars
         lm[f"candidates_{k}"] = {"s": time.perf_counter() - tl, "engine_calls": r["calls"],
                                  "iterations": r["iterations"], "chisq": r["chisq"], "stop": r["stop"],
                                  "rejections": sum(1 for h in r["history"] if h[3] == "reject")}
     log(f"bench: levmarq {lm}")
+    # the adaptive-Metropolis sampler: 64 chains, 60 steps, all proposals of
+    # a step in one batched call, started at the LM solution (inside the
+    # limits: the workload's limb-darkening start is not) with initial steps
+    # 2.38 / sqrt(d) times each parameter's conditional Gauss-Newton
+    # deviation there, 1 / sqrt((J^T J)_ii) (the model file's 1e-4 are far
+    # wider than this posterior, and the limb-darkening coefficients are too
+    # degenerate for marginal deviations)
+    from lcurve_re_amd import mcmc as MC
+    mmc = model.copy()
+    mmc.set_param(lm_pars)
+    Jl = fb.jacobian(lm_pars)["J"]
+    sd = 1.0 / np.sqrt(np.maximum(np.diag(Jl.T @ Jl), 1e-300))
+    for i, name in enumerate(cfg["free_parameters"]):
+        getattr(mmc.pod, name).dstep = 2.38 / math.sqrt(model.nvary()) * float(sd[i])
+    tm = time.perf_counter()
+    mc = MC.adaptive_metropolis(mmc, data, F.engine_evaluator(eng), nchains=64, nsteps=60, burn_in=20,
+                                seed=w["walker_seed"], precision=prec, cov_warmup=30, adapt_interval=10)
+    elm = time.perf_counter() - tm
+    mcmc = {"chains": 64, "steps": 60, "s": elm, "chain_steps_per_s": 64 * 60 / elm, "engine_calls": mc["calls"],
+            "acceptance_mean": float(mc["acceptance"].mean()), "start": "LM solution",
+            "best_chisq_min": float(mc["best_chisq"].min())}
+    log(f"bench: mcmc {mcmc}")
     cpu, jerr = None, None
     if not args.no_cpu:
         from oracle import pyoracle as P
@@ -465,7 +490,7 @@ def run_fit_batch(L, eng, args):
             "fp32 per face, fp64 geometry/accumulation",
             "data": "synthetic", "config": cfg, "roofline": roofline,
             "no_grid_reuse": {"value": ninst / el2 if el2 > 0 else None, "ms_per_step": el2 / args.steps * 1e3},
-            "levmarq": lm, "cpu_baseline": cpu, "parity": jerr}
+            "levmarq": lm, "mcmc": mcmc, "cpu_baseline": cpu, "parity": jerr}
 
 
 if __name__ == "__main__":
