@@ -315,4 +315,4 @@ class Engine:
             self._err(st)
 
 
-from .data_io import read_data, write_data  # noqa: E402
+from .data_io import read_data, write_chain, write_config_and_model, write_data  # noqa: E402

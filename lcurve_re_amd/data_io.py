@@ -43,3 +43,57 @@ def write_data(path, data):
         for r in v:
             f.write(f"{r['time']:17.8f} {r['expose']:.8f} {r['flux']:10.5f} {r['ferr']:.5f} "
                     f"{r['weight']:.5f} {int(r['ndiv'])}\n")
+
+
+# Parameters written back into a configuration, in the reference's order
+# (new_helpers.h:480-533, write_config_and_model_to_json).
+CONFIG_PARAMS = (
+    "q iangle r1 r2 cphi3 cphi4 spin1 spin2 t1 t2 "
+    "ldc1_1 ldc1_2 ldc1_3 ldc1_4 ldc2_1 ldc2_2 ldc2_3 ldc2_4 "
+    "velocity_scale beam_factor1 beam_factor2 t0 period pdot deltat "
+    "gravity_dark1 gravity_dark2 absorb slope quad cube third "
+    "rdisc1 rdisc2 height_disc beta_disc temp_disc texp_disc lin_limb_disc quad_limb_disc temp_edge absorb_edge "
+    "radius_spot length_spot height_spot expon_spot epow_spot angle_spot yaw_spot temp_spot tilt_spot cfrac_spot "
+    "stsp11_long stsp11_lat stsp11_fwhm stsp11_tcen stsp12_long stsp12_lat stsp12_fwhm stsp12_tcen "
+    "stsp13_long stsp13_lat stsp13_fwhm stsp13_tcen stsp21_long stsp21_lat stsp21_fwhm stsp21_tcen "
+    "stsp22_long stsp22_lat stsp22_fwhm stsp22_tcen uesp_long1 uesp_long2 uesp_lathw uesp_taper uesp_temp").split()
+
+
+def pparam_string(p):
+    """'value range dstep vary defined' as new_helpers.h:23-30 writes it
+    (value with 10 decimals, range and dstep with 6)."""
+    return "%.10f %.6f %.6f %d %d" % (p.value, p.range, p.dstep, int(p.vary), int(p.defined))
+
+
+def write_config_and_model(model, config, path):
+    """A copy of `config` (dict or JSON text) with its model_parameters set
+    from `model` (Helpers::write_config_and_model_to_json,
+    new_helpers.h:470-541): pretty-printed with two-space indents and keys
+    in sorted order, as nlohmann::json prints an object."""
+    import copy
+    import json
+    cfg = json.loads(config) if isinstance(config, str) else copy.deepcopy(config)
+    mp = cfg.get("model_parameters")
+    if not isinstance(mp, dict):
+        raise ValueError("write_config_and_model: config misses 'model_parameters' object")
+    for name in CONFIG_PARAMS:
+        mp[name] = pparam_string(getattr(model.pod, name))
+    with open(path, "w") as f:
+        f.write(json.dumps(cfg, indent=2, sort_keys=True, ensure_ascii=False) + "\n")
+
+
+def write_chain(path, names, pars, chisq, log_prior=None, thin=1):
+    """One MCMC chain as the reference's sampler writes it
+    (mcmc_solver.cpp:1488-1506): header step,<names>,chisq,log_prior, then
+    every thin-th recorded state with the stream's default formatting (six
+    significant digits).  pars: [steps, d]; log_prior defaults to 0 (no
+    physical prior)."""
+    pars = np.asarray(pars, float)
+    chisq = np.asarray(chisq, float)
+    lp = np.zeros(len(pars)) if log_prior is None else np.asarray(log_prior, float)
+    thin = max(1, int(thin))
+    with open(path, "w") as f:
+        f.write("step" + "".join("," + n for n in names) + ",chisq,log_prior\n")
+        for i in range(0, len(pars), thin):
+            f.write("%d%s,%s,%s\n" % (i, "".join("," + format(v, "g") for v in pars[i]), format(chisq[i], "g"),
+                                      format(lp[i], "g")))
