@@ -122,6 +122,17 @@ class FitBatch:
                 index.append((j, len(vecs) - 1, None))
         return vecs, index
 
+    def drive(self, gen):
+        """Runs a request generator (yields lists of parameter vectors,
+        receives (calc, status) for them) against this batch's evaluator;
+        returns the generator's result."""
+        try:
+            req = next(gen)
+            while True:
+                req = gen.send(self._run(req))
+        except StopIteration as e:
+            return e.value
+
     def jacobian(self, pars=None, walkers=(), shrink=1.0):
         """One batched call: base point, Jacobian columns, walkers.
 
@@ -130,13 +141,18 @@ class FitBatch:
         steps, walker_chisq (nan where a walker failed), walker_status, and
         the number of instances evaluated per call.
         """
+        return self.drive(self.jacobian_requests(pars, walkers, shrink))
+
+    def jacobian_requests(self, pars=None, walkers=(), shrink=1.0):
+        """jacobian() as a request generator (see drive)."""
         pars = np.array(self.model.get_param() if pars is None else pars, float)
         h = fd_steps(pars, self.limits, shrink)
         cols = self.plan(pars, h)
         vecs, index = self._vectors(pars, cols, range(self.npar))
         walkers = [np.asarray(w, float) for w in walkers]
         batch = [pars] + vecs + walkers
-        calc, status = self._run(batch)
+        calc, status = yield batch
+        status = np.asarray(status)
         n = calc.shape[1]
         if status[0] != 0:
             raise RuntimeError(f"base evaluation failed with status {int(status[0])}")
@@ -161,7 +177,8 @@ class FitBatch:
                 cols[j] = (kind, 0.5 * hj)
                 tries[j] += 1
             vecs, index = self._vectors(pars, cols, pending)
-            c2, s2 = self._run(vecs)
+            c2, s2 = yield vecs
+            s2 = np.asarray(s2)
             calls.append(len(vecs))
             nxt = []
             for (j, a, b) in index:
@@ -188,9 +205,12 @@ class FitBatch:
 
     def sum_squares(self, vectors):
         """sum r_i^2 at each vector (nan where the evaluation failed)."""
+        return self.drive(self.sum_squares_requests(vectors))
+
+    def sum_squares_requests(self, vectors):
         if not len(vectors):
             return np.zeros(0)
-        calc, status = self._run([np.asarray(v, float) for v in vectors])
+        calc, status = yield [np.asarray(v, float) for v in vectors]
         return np.array([float(np.sum(residuals(self.data, calc[k]) ** 2)) if status[k] == 0 else np.nan
                          for k in range(len(vectors))])
 
@@ -221,10 +241,12 @@ def walkers_around(model, nwalk, seed, rel=1e-3):
 SQRT_EPS = math.sqrt(np.finfo(np.float64).eps)
 
 
-def levmarq(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, gtol=0.0, tau0=1e-3, lambda_conv=1.0,
-            lambda_max=1e8, max_recoveries=3, fd_shrink_factor=0.1, candidates=4, log=None):
+def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, gtol=0.0, tau0=1e-3,
+                     lambda_conv=1.0, lambda_max=1e8, max_recoveries=3, fd_shrink_factor=0.1, candidates=4, log=None):
     """Levenberg-Marquardt fit of fb.model's free parameters to fb.data
-    (the inner loop of levmarq_solver.cpp:1225-1500, one stage, no priors).
+    (the inner loop of levmarq_solver.cpp:1225-1500, one stage, no priors),
+    as a request generator: it yields lists of parameter vectors to
+    evaluate and receives (calc, status) for them (FitBatch.drive).
 
     Damping as there: (J^T J + lambda D^2) delta = -J^T r with D_j the
     running maximum of the Jacobian column norms (update_scaling,
@@ -251,7 +273,7 @@ def levmarq(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, gtol=0.0,
     lims = fb.limits
     p = np.array(fb.model.get_param() if pars is None else pars, float)
     shrink = 1.0
-    jac = fb.jacobian(p, shrink=shrink)
+    jac = yield from fb.jacobian_requests(p, shrink=shrink)
     calls = len(jac["instances_per_call"])
     D = None
 
@@ -316,11 +338,11 @@ def levmarq(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, gtol=0.0,
         ssq = np.zeros(0)
         if trials:
             try:
-                spec = fb.jacobian(trials[0], walkers=trials[1:], shrink=shrink)
+                spec = yield from fb.jacobian_requests(trials[0], walkers=trials[1:], shrink=shrink)
                 calls += len(spec["instances_per_call"])
                 ssq = np.concatenate([[spec["chisq0"]], spec["walker_chisq"]])
             except RuntimeError:  # the first trial failed to evaluate
-                ssq = np.concatenate([[np.nan], fb.sum_squares(trials[1:])])
+                ssq = np.concatenate([[np.nan], (yield from fb.sum_squares_requests(trials[1:]))])
                 calls += 1 if len(trials) > 1 else 0
         ti = 0
         accepted = False
@@ -359,7 +381,7 @@ def levmarq(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, gtol=0.0,
                 if first:
                     jac = spec
                 else:
-                    jac = fb.jacobian(p, shrink=shrink)
+                    jac = yield from fb.jacobian_requests(p, shrink=shrink)
                     calls += len(jac["instances_per_call"])
                 J = jac["J"]
                 r = jac["resid0"]
@@ -383,7 +405,7 @@ def levmarq(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, gtol=0.0,
                 if recoveries < max_recoveries:
                     recoveries += 1
                     shrink *= fd_shrink_factor
-                    jac = fb.jacobian(p, shrink=shrink)
+                    jac = yield from fb.jacobian_requests(p, shrink=shrink)
                     calls += len(jac["instances_per_call"])
                     J = jac["J"]
                     r = jac["resid0"]
@@ -404,6 +426,48 @@ def levmarq(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, gtol=0.0,
             "history": hist}
 
 
+def levmarq(fb, pars=None, **kw):
+    """Levenberg-Marquardt fit of fb.model's free parameters to fb.data; see
+    levmarq_requests for the algorithm and the keyword arguments."""
+    return fb.drive(levmarq_requests(fb, pars, **kw))
+
+
+def levmarq_multistart(fb, starts, **kw):
+    """Independent LM fits from several starting vectors (the reference's
+    multistart, levmarq_solver.cpp:1000-1640, without its clustering of
+    optima), advanced in lockstep: each round, the pending evaluations of
+    every unfinished fit go to the engine in one batched call.  Each fit is
+    exactly the one levmarq() runs from its start.  Returns the results
+    (levmarq's dicts, plus "start") sorted by chi^2, and the number of
+    batched calls."""
+    gens, reqs, out = [], [], []
+    for k, s0 in enumerate(starts):
+        g = levmarq_requests(fb, np.asarray(s0, float), **kw)
+        try:
+            reqs.append(next(g))
+            gens.append((k, g))
+        except StopIteration as e:
+            out.append(dict(e.value, start=k))
+    calls = 0
+    while gens:
+        batch = [v for r in reqs for v in r]
+        calc, status = fb._run(batch)
+        calls += 1
+        status = np.asarray(status)
+        nxt_g, nxt_r, o = [], [], 0
+        for (k, g), r in zip(gens, reqs):
+            part = (calc[o:o + len(r)], status[o:o + len(r)])
+            o += len(r)
+            try:
+                nxt_r.append(g.send(part))
+                nxt_g.append((k, g))
+            except StopIteration as e:
+                out.append(dict(e.value, start=k))
+        gens, reqs = nxt_g, nxt_r
+    out.sort(key=lambda r: r["chisq"])
+    return out, calls
+
+
 def engine_evaluator(engine):
     """evaluate(models, data, opts) over Engine.light_curve_comp_batch."""
     def evaluate(models, data, opts):
@@ -414,4 +478,5 @@ def engine_evaluator(engine):
     return evaluate
 
 
-__all__ = ["FitBatch", "fd_steps", "residuals", "walkers_around", "engine_evaluator", "levmarq", "Model"]
+__all__ = ["FitBatch", "fd_steps", "residuals", "walkers_around", "engine_evaluator", "levmarq",
+           "levmarq_multistart", "levmarq_requests", "Model"]

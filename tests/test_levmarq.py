@@ -106,3 +106,33 @@ def test_levmarq_fits_an_oracle_light_curve():
     res = F.levmarq(fb, max_iter=40)
     assert res["chisq"] < 1e-6 * float(np.sum(((clean - clean.mean()) / ferr) ** 2))
     np.testing.assert_allclose(res["pars"], [84.0, 0.12], rtol=2e-4)
+
+
+def wave_curve(p):
+    a, b = p
+    return a * np.sin(b * T)
+
+
+def wave_eval(models, data, opts):
+    return np.array([wave_curve(m.p) for m in models]), np.zeros(len(models), int)
+
+
+def test_multistart_runs_are_the_single_runs_and_find_the_global_minimum():
+    """a sin(b t): chi^2 has many local minima in b; every start of the
+    lockstep multistart ends where levmarq alone ends from it, in fewer
+    engine calls than the runs one after another, and the best start finds
+    the true b."""
+    rng = np.random.default_rng(2)
+    ferr = 0.05 * np.ones(len(T))
+    data = L.make_data(T, ferr=ferr, flux=wave_curve([1.5, 7.0]) + ferr * rng.standard_normal(len(T)))
+    lims = [(-10.0, 10.0), (0.1, 20.0)]
+    starts = [np.array([1.0, b]) for b in (2.0, 4.5, 6.5, 9.0, 12.0)]
+    fb = F.FitBatch(ToyModel(starts[0], lims), data, wave_eval, scale=False)
+    res, calls = F.levmarq_multistart(fb, starts, max_iter=60)
+    alone = [F.levmarq(fb, s, max_iter=60) for s in starts]
+    for r in res:
+        a = alone[r["start"]]
+        assert np.array_equal(r["pars"], a["pars"]) and r["history"] == a["history"]
+    assert calls < sum(a["calls"] for a in alone)
+    assert len({round(r["pars"][1], 3) for r in res}) > 1  # distinct local minima
+    assert abs(res[0]["pars"][1] - 7.0) < 0.01 and abs(res[0]["pars"][0] - 1.5) < 0.05
