@@ -442,6 +442,18 @@ def run_fit_batch(L, eng, args):
         lm[f"candidates_{k}"] = {"s": time.perf_counter() - tl, "engine_calls": r["calls"],
                                  "iterations": r["iterations"], "chisq": r["chisq"], "stop": r["stop"],
                                  "rejections": sum(1 for h in r["history"] if h[3] == "reject")}
+    # multistart: 8 starts 0.1% off, up to 30 iterations each, lockstep
+    # (one engine call per round for all starts) and one after another
+    starts = F.walkers_around(model, 8, w["walker_seed"] + 2, rel=1e-3)
+    fbm = F.FitBatch(model, data, F.engine_evaluator(eng), scale=True, precision=prec)
+    tl = time.perf_counter()
+    ms_res, ms_calls = F.levmarq_multistart(fbm, starts, max_iter=30)
+    t_lock = time.perf_counter() - tl
+    tl = time.perf_counter()
+    seq_calls = sum(F.levmarq(fbm, s0, max_iter=30)["calls"] for s0 in starts)
+    t_seq = time.perf_counter() - tl
+    lm["multistart_8"] = {"lockstep_s": t_lock, "lockstep_engine_calls": ms_calls, "sequential_s": t_seq,
+                          "sequential_engine_calls": seq_calls, "best_chisq": float(ms_res[0]["chisq"])}
     log(f"bench: levmarq {lm}")
     # the adaptive-Metropolis sampler: 64 chains, 60 steps, all proposals of
     # a step in one batched call, started at the LM solution (inside the
