@@ -2,12 +2,14 @@
 in lockstep, the proposals of all chains evaluated in one batched call per
 step.
 
-Each chain follows the reference's sampler (mcmc_solver.cpp:196-1175,
-without the physical prior): a Gaussian proposal with per-parameter steps
+Each chain follows the reference's sampler (mcmc_solver.cpp:196-1175):
+a Gaussian proposal with per-parameter steps
 dstep * exp(s), or, once the covariance is ready, (2.38^2/d) C + eps I
 through its Cholesky factor times exp(s); proposals reflected off the
 limits (at most 20 bounces, then clamped); an evaluation failure is a
-rejection; log alpha = -0.5 (chi2' - chi2) / (T s2) with the annealing
+rejection; log alpha = -0.5 (chi2' - chi2) / (T s2) + (lp' - lp), lp the
+optional physical log prior (prior.py; never tempered, a proposal it puts
+at <= -1e29 rejected before evaluation), with the annealing
 temperature T cooling linearly from anneal_T0 to 1 over the first
 anneal_steps and s2 = chi2_best / (N - k) fixed at the end of burn-in
 (data-error rescale); Robbins-Monro scale adaptation every adapt_interval
@@ -57,6 +59,7 @@ class _Chain:
         self.best_p = self.p.copy()
         self.chi2_scale = 1.0
         self.accepted = 0
+        self.lp = 0.0  # log prior of the current state
 
     def propose(self):
         sf = math.exp(self.log_scale)
@@ -120,13 +123,17 @@ def adaptive_metropolis(model, data, evaluate, nchains=8, nsteps=2000, burn_in=N
                         scale=True, max_model_points=0, precision=LC_PREC_FP64, anneal=True, anneal_T0=10.0,
                         anneal_steps=None, adapt=True, target=None, adapt_interval=100, adapt_rate=1.0,
                         adapt_decay=0.6, adapt_min_stepscale=1e-4, adapt_max_stepscale=1e4, adapt_covariance=True,
-                        cov_warmup=None, cov_epsilon=1e-6, cov_reset_step=None, scale_data_errors=True, log=None):
+                        cov_warmup=None, cov_epsilon=1e-6, cov_reset_step=None, scale_data_errors=True, prior=None,
+                        log=None):
     """Runs nchains chains of nsteps (the reference's defaults where it
     has them: burn_in = nsteps / 4, anneal_steps = burn_in / 2, target 0.234
     (0.44 for one parameter), cov_warmup = max(50 d, 1000), covariance
     reset at 2/5 of the burn-in).  evaluate(models, data, opts) ->
     (calc[m, n], status[m]) as fit.engine_evaluator gives.  start: one
-    vector or one per chain (default: the model's parameters).
+    vector or one per chain (default: the model's parameters).  prior:
+    None or p -> log prior (prior.model_prior): it enters log alpha
+    untempered, and a proposal with log prior <= -1e29 is rejected without
+    an evaluation (mcmc_solver.cpp:1033-1056, 1097-1099).
 
     Returns dict: chain [nchains, nsteps - burn_in, d] and chain_chisq of
     the recorded states, acceptance per chain, best chi^2 and parameters,
@@ -163,9 +170,11 @@ def adaptive_metropolis(model, data, evaluate, nchains=8, nsteps=2000, burn_in=N
         if x is None:
             raise RuntimeError("a chain's starting point does not evaluate")
         ch.chi = ch.best = x
+        ch.lp = prior(ch.p) if prior else 0.0
     keep = nsteps - burn_in
     rec = np.zeros((nchains, max(keep, 0), npar))
     rec_chi = np.zeros((nchains, max(keep, 0)))
+    rec_lp = np.zeros((nchains, max(keep, 0)))
     for step in range(nsteps):
         for ch in chains:
             if adapt_covariance and cov_reset_step > 0 and step == cov_reset_step and not ch.cov_reset:
@@ -179,8 +188,13 @@ def adaptive_metropolis(model, data, evaluate, nchains=8, nsteps=2000, burn_in=N
                 if dof > 0 and ch.best > 0.0:
                     ch.chi2_scale = ch.best / dof
         props = [ch.propose() for ch in chains]
-        res = run(props)
-        calls += 1
+        lps = [prior(v) if prior else 0.0 for v in props]
+        live = [k for k in range(nchains) if lps[k] > -1e29]
+        res = [None] * nchains
+        if live:
+            for k, x in zip(live, run([props[k] for k in live])):
+                res[k] = x
+            calls += 1
         T = 1.0
         if anneal and step < anneal_steps and anneal_steps > 0:
             frac = step / anneal_steps
@@ -189,9 +203,9 @@ def adaptive_metropolis(model, data, evaluate, nchains=8, nsteps=2000, burn_in=N
             acc = False
             x = res[k]
             if x is not None:
-                la = -0.5 * (x - ch.chi) / (T * ch.chi2_scale)
+                la = -0.5 * (x - ch.chi) / (T * ch.chi2_scale) + (lps[k] - ch.lp)
                 if la >= 0.0 or math.log(ch.rng.random()) < la:
-                    ch.p, ch.chi = props[k], x
+                    ch.p, ch.chi, ch.lp = props[k], x, lps[k]
                     ch.accepted += 1
                     acc = True
                     if x < ch.best:
@@ -202,10 +216,11 @@ def adaptive_metropolis(model, data, evaluate, nchains=8, nsteps=2000, burn_in=N
             if step >= burn_in:
                 rec[k, step - burn_in] = ch.p
                 rec_chi[k, step - burn_in] = ch.chi
+                rec_lp[k, step - burn_in] = ch.lp
         if log and (step % 100 == 0 or step == nsteps - 1):
             log(f"mcmc: step {step} accepted {np.mean([c.accepted for c in chains]) / (step + 1):.3f} "
                 f"best {min(c.best for c in chains):.6g}")
-    return {"chain": rec, "chain_chisq": rec_chi, "acceptance": np.array([c.accepted / nsteps for c in chains]),
+    return {"chain": rec, "chain_chisq": rec_chi, "chain_log_prior": rec_lp, "acceptance": np.array([c.accepted / nsteps for c in chains]),
             "best_chisq": np.array([c.best for c in chains]), "best_pars": np.array([c.best_p for c in chains]),
             "scale_factor": np.array([math.exp(c.log_scale) for c in chains]),
             "cov_ready": np.array([c.cov_ready for c in chains]),

@@ -1,5 +1,6 @@
 """The batched adaptive-Metropolis sampler (lcurve_re_amd/mcmc.py, the
-reference's mcmc_solver.cpp:196-1175 without the physical prior).
+reference's mcmc_solver.cpp:196-1175; the physical prior's own checks
+are in test_prior.py).
 
 Pinned by: on a linear model y = A p the chi^2 is exactly quadratic, so
 after burn-in the chains must sample N(p_hat, s2 (A^T W A)^-1), with p_hat
@@ -82,3 +83,45 @@ def test_a_chain_does_not_depend_on_the_others():
                                     seed=10 + k, cov_warmup=300)
         assert np.array_equal(many["chain"][k], one["chain"][0])
         assert np.array_equal(many["chain_chisq"][k], one["chain_chisq"][0])
+
+
+def test_gaussian_prior_shifts_the_posterior_to_the_product():
+    """A Gaussian prior on p0 (untempered, unscaled by s2) multiplies the
+    tempered-free likelihood: the posterior is N(m, C) with C^-1 =
+    A^T W A / s2 + e0 e0^T / s0^2 and m = C (A^T W y / s2 + e0 m0 / s0^2)."""
+    data, y, sig = lin_data()
+    W = 1.0 / sig ** 2
+    F = A.T @ (A * W[:, None])
+    m0, s0 = 1.1, 0.01
+    prior = lambda p: -0.5 * ((p[0] - m0) / s0) ** 2
+    r = M.adaptive_metropolis(LinModel([1.0, -0.5, 0.3]), data, lin_eval, nchains=8, nsteps=6000, seed=5,
+                              prior=prior)
+    s2 = r["chi2_scale"].mean()
+    P = F / s2
+    P[0, 0] += 1.0 / s0 ** 2
+    C = np.linalg.inv(P)
+    b = A.T @ (W * y) / s2
+    b[0] += m0 / s0 ** 2
+    m = C @ b
+    samples = r["chain"].reshape(-1, 3)
+    sd = np.sqrt(np.diag(C))
+    assert np.all(np.abs(samples.mean(axis=0) - m) < 0.15 * sd), (samples.mean(axis=0), m, sd)
+    np.testing.assert_allclose(samples.std(axis=0), sd, rtol=0.2)
+    lp = r["chain_log_prior"].reshape(-1)
+    np.testing.assert_allclose(lp, [prior(v) for v in samples], rtol=1e-12, atol=1e-300)
+
+
+def test_proposals_the_prior_forbids_are_never_evaluated():
+    data, *_ = lin_data()
+    bound = 1.0
+    seen = []
+
+    def ev(models, d, opts):
+        seen.extend(m.p[0] for m in models)
+        return lin_eval(models, d, opts)
+
+    prior = lambda p: -1e30 if p[0] > bound else 0.0
+    r = M.adaptive_metropolis(LinModel([0.95, -0.5, 0.3]), data, ev, nchains=4, nsteps=800, seed=6, prior=prior)
+    assert max(seen[4:]) <= bound  # every evaluation after the starting points
+    assert r["chain"][..., 0].max() <= bound
+    assert np.all(r["chain_log_prior"] == 0.0)
