@@ -133,6 +133,13 @@ class FitBatch:
         except StopIteration as e:
             return e.value
 
+    def _resid(self, calc_row, vec, prior):
+        r = residuals(self.data, calc_row)
+        if prior is None:
+            return r
+        fn, eff = prior
+        return np.concatenate([r, eff * fn(vec)])
+
     def jacobian(self, pars=None, walkers=(), shrink=1.0):
         """One batched call: base point, Jacobian columns, walkers.
 
@@ -143,8 +150,12 @@ class FitBatch:
         """
         return self.drive(self.jacobian_requests(pars, walkers, shrink))
 
-    def jacobian_requests(self, pars=None, walkers=(), shrink=1.0):
-        """jacobian() as a request generator (see drive)."""
+    def jacobian_requests(self, pars=None, walkers=(), shrink=1.0, prior=None):
+        """jacobian() as a request generator (see drive).  prior: None or
+        (fn, eff): the residual vector gains eff * fn(p) (prior residuals
+        computed on the host, differenced with the same steps); eff may be
+        a function of the base point's light-curve chi^2, evaluated once
+        (its value is returned as prior_eff)."""
         pars = np.array(self.model.get_param() if pars is None else pars, float)
         h = fd_steps(pars, self.limits, shrink)
         cols = self.plan(pars, h)
@@ -156,15 +167,19 @@ class FitBatch:
         n = calc.shape[1]
         if status[0] != 0:
             raise RuntimeError(f"base evaluation failed with status {int(status[0])}")
-        r0 = residuals(self.data, calc[0])
-        J = np.zeros((n, self.npar))
+        chisq_lc0 = float(np.sum(residuals(self.data, calc[0]) ** 2))
+        if prior is not None and callable(prior[1]):
+            prior = (prior[0], prior[1](chisq_lc0))
+        r0 = self._resid(calc[0], pars, prior)
+        J = np.zeros((len(r0), self.npar))
         calls = [len(batch)]
         pending = []
         for (j, a, b) in index:
             kind, hj = cols[j]
             ok = status[1 + a] == 0 and (b is None or status[1 + b] == 0)
             if ok:
-                J[:, j] = self._column(kind, hj, calc[1 + a], None if b is None else calc[1 + b], r0)
+                J[:, j] = self._column(kind, hj, calc[1 + a], None if b is None else calc[1 + b], r0,
+                                       vecs[a], None if b is None else vecs[b], prior)
             else:
                 pending.append(j)
         # failed columns: halve the step and retry, batched; a central
@@ -184,7 +199,8 @@ class FitBatch:
             for (j, a, b) in index:
                 kind, hj = cols[j]
                 if s2[a] == 0 and (b is None or s2[b] == 0):
-                    J[:, j] = self._column(kind, hj, c2[a], None if b is None else c2[b], r0)
+                    J[:, j] = self._column(kind, hj, c2[a], None if b is None else c2[b], r0,
+                                           vecs[a], None if b is None else vecs[b], prior)
                 elif tries[j] < 4:
                     nxt.append(j)
                 elif kind == "c":
@@ -197,27 +213,29 @@ class FitBatch:
         wchi = np.full(nw, np.nan)
         for k in range(nw):
             if wst[k] == 0:
-                wchi[k] = float(np.sum(residuals(self.data, calc[len(batch) - nw + k]) ** 2))
+                wchi[k] = float(np.sum(self._resid(calc[len(batch) - nw + k], walkers[k], prior) ** 2))
         steps = np.array([hj for _, hj in cols])
-        return {"J": J, "resid0": r0, "chisq0": float(np.sum(r0 ** 2)), "calc0": calc[0], "steps": steps,
+        return {"J": J, "resid0": r0, "chisq0": float(np.sum(r0 ** 2)), "chisq_lc0": chisq_lc0,
+                "prior_eff": None if prior is None else prior[1], "calc0": calc[0], "steps": steps,
                 "kinds": [k for k, _ in cols], "walker_chisq": wchi, "walker_status": wst,
                 "instances_per_call": calls}
 
-    def sum_squares(self, vectors):
+    def sum_squares(self, vectors, prior=None):
         """sum r_i^2 at each vector (nan where the evaluation failed)."""
-        return self.drive(self.sum_squares_requests(vectors))
+        return self.drive(self.sum_squares_requests(vectors, prior))
 
-    def sum_squares_requests(self, vectors):
+    def sum_squares_requests(self, vectors, prior=None):
         if not len(vectors):
             return np.zeros(0)
-        calc, status = yield [np.asarray(v, float) for v in vectors]
-        return np.array([float(np.sum(residuals(self.data, calc[k]) ** 2)) if status[k] == 0 else np.nan
+        vectors = [np.asarray(v, float) for v in vectors]
+        calc, status = yield vectors
+        return np.array([float(np.sum(self._resid(calc[k], vectors[k], prior) ** 2)) if status[k] == 0 else np.nan
                          for k in range(len(vectors))])
 
-    def _column(self, kind, hj, ca, cb, r0):
-        ra = residuals(self.data, ca)
+    def _column(self, kind, hj, ca, cb, r0, va, vb, prior):
+        ra = self._resid(ca, va, prior)
         if kind == "c":
-            d = (ra - residuals(self.data, cb)) / (2.0 * hj)
+            d = (ra - self._resid(cb, vb, prior)) / (2.0 * hj)
         elif kind == "f":
             d = (ra - r0) / hj
         else:
@@ -241,12 +259,64 @@ def walkers_around(model, nwalk, seed, rel=1e-3):
 SQRT_EPS = math.sqrt(np.finfo(np.float64).eps)
 
 
-def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, gtol=0.0, tau0=1e-3,
-                     lambda_conv=1.0, lambda_max=1e8, max_recoveries=3, fd_shrink_factor=0.1, candidates=4, log=None):
+def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, prior=None, continuation=True,
+                     stages=6, auto_balance=True, balance_target=1.0, **kw):
     """Levenberg-Marquardt fit of fb.model's free parameters to fb.data
-    (the inner loop of levmarq_solver.cpp:1225-1500, one stage, no priors),
-    as a request generator: it yields lists of parameter vectors to
-    evaluate and receives (calc, status) for them (FitBatch.drive).
+    (levmarq_solver.cpp:985-1530), as a request generator: it yields lists
+    of parameter vectors to evaluate and receives (calc, status) for them
+    (FitBatch.drive).  Without a prior it is one pass of the inner loop
+    (lm_stage_requests).
+
+    prior: None or prior.LMPrior, whose residuals extend the residual
+    vector scaled by sqrt(s b).  With continuation the prior scale s steps
+    through 0, 1/stages, ..., 1 (one inner-loop pass each, starting where
+    the last ended, lambda, D and the difference steps fresh); a stage
+    before the last runs at most max(20, max_iter // (stages + 1))
+    iterations with ftol and xtol at least 1e-4.  b, the balance factor,
+    is min(balance_target chi2_LC / (n_prior prior_weight), 1) at the
+    stage's starting point with auto_balance, else 1 (0 while s is 0).
+    MI355X-side change: chi2_LC comes from the base point of the stage's
+    first Jacobian call instead of an evaluation of its own (the same
+    engine result).  Iterations add up over the stages; calls counts
+    batched engine calls.
+    """
+    p = np.array(fb.model.get_param() if pars is None else pars, float)
+    if prior is None or len(prior) == 0:
+        return (yield from lm_stage_requests(fb, p, max_iter=max_iter, ftol=ftol, xtol=xtol, **kw))
+    sched = [s / stages for s in range(stages + 1)] if continuation else [1.0]
+    total = {"iterations": 0, "calls": 0, "history": [], "stages": []}
+    res = None
+    for k, scale in enumerate(sched):
+        final = k == len(sched) - 1
+
+        def eff(chisq_lc, scale=scale):
+            if auto_balance and scale > 0.0:
+                raw = prior.weight_sum
+                b = min(balance_target * chisq_lc / raw, 1.0) if raw > 0.0 and chisq_lc > 1e-30 else 1.0
+            else:
+                b = 1.0 if scale > 0.0 else 0.0
+            return math.sqrt(scale * b)
+        start = p
+        res = yield from lm_stage_requests(
+            fb, p, prior=(prior, eff), max_iter=max_iter if final else max(20, max_iter // len(sched)),
+            ftol=ftol if final else max(ftol, 1e-4), xtol=xtol if final else max(xtol, 1e-4), **kw)
+        p = res["pars"]
+        total["iterations"] += res["iterations"]
+        total["calls"] += res["calls"]
+        total["history"] += [(total["iterations"] - res["iterations"] + h[0],) + tuple(h[1:]) + (scale,)
+                             for h in res["history"]]
+        total["stages"].append({"scale": scale, "start": start, "eff": res["prior_eff"], "iterations": res["iterations"],
+                                "stop": res["stop"], "sum_sq": res["chisq"]})
+    return dict(res, iterations=total["iterations"], calls=total["calls"], history=total["history"],
+                stages=total["stages"])
+
+
+def lm_stage_requests(fb, p, prior=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, gtol=0.0, tau0=1e-3,
+                      lambda_conv=1.0, lambda_max=1e8, max_recoveries=3, fd_shrink_factor=0.1, candidates=4,
+                      log=None):
+    """One pass of the LM inner loop (levmarq_solver.cpp:1225-1500) from p,
+    with the prior residuals (fn, eff) when given (eff may be a function of
+    the starting point's chi2_LC; see FitBatch.jacobian_requests).
 
     Damping as there: (J^T J + lambda D^2) delta = -J^T r with D_j the
     running maximum of the Jacobian column norms (update_scaling,
@@ -271,10 +341,12 @@ def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, 
     """
     npar = fb.npar
     lims = fb.limits
-    p = np.array(fb.model.get_param() if pars is None else pars, float)
+    p = np.array(p, float)
     shrink = 1.0
-    jac = yield from fb.jacobian_requests(p, shrink=shrink)
+    jac = yield from fb.jacobian_requests(p, shrink=shrink, prior=prior)
     calls = len(jac["instances_per_call"])
+    if prior is not None:
+        prior = (prior[0], jac["prior_eff"])
     D = None
 
     def scaling(J, D):
@@ -338,11 +410,11 @@ def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, 
         ssq = np.zeros(0)
         if trials:
             try:
-                spec = yield from fb.jacobian_requests(trials[0], walkers=trials[1:], shrink=shrink)
+                spec = yield from fb.jacobian_requests(trials[0], walkers=trials[1:], shrink=shrink, prior=prior)
                 calls += len(spec["instances_per_call"])
                 ssq = np.concatenate([[spec["chisq0"]], spec["walker_chisq"]])
             except RuntimeError:  # the first trial failed to evaluate
-                ssq = np.concatenate([[np.nan], (yield from fb.sum_squares_requests(trials[1:]))])
+                ssq = np.concatenate([[np.nan], (yield from fb.sum_squares_requests(trials[1:], prior))])
                 calls += 1 if len(trials) > 1 else 0
         ti = 0
         accepted = False
@@ -381,7 +453,7 @@ def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, 
                 if first:
                     jac = spec
                 else:
-                    jac = yield from fb.jacobian_requests(p, shrink=shrink)
+                    jac = yield from fb.jacobian_requests(p, shrink=shrink, prior=prior)
                     calls += len(jac["instances_per_call"])
                 J = jac["J"]
                 r = jac["resid0"]
@@ -405,7 +477,7 @@ def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, 
                 if recoveries < max_recoveries:
                     recoveries += 1
                     shrink *= fd_shrink_factor
-                    jac = yield from fb.jacobian_requests(p, shrink=shrink)
+                    jac = yield from fb.jacobian_requests(p, shrink=shrink, prior=prior)
                     calls += len(jac["instances_per_call"])
                     J = jac["J"]
                     r = jac["resid0"]
@@ -422,8 +494,9 @@ def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, 
         if done:
             stop = done
             break
-    return {"pars": p, "chisq": sum_sq, "iterations": it, "calls": calls, "stop": stop, "lambda": lam,
-            "history": hist}
+    return {"pars": p, "chisq": sum_sq, "chisq_lc": jac["chisq_lc0"],
+            "iterations": it, "calls": calls, "stop": stop, "lambda": lam, "history": hist,
+            "prior_eff": None if prior is None else prior[1]}
 
 
 def levmarq(fb, pars=None, **kw):

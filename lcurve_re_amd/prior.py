@@ -7,6 +7,8 @@ sin i < 1e-10, q <= 0, velocity_scale <= 0 or r1 outside (0, 1)."""
 import math
 from dataclasses import dataclass
 
+import numpy as np
+
 DEG2RAD = math.pi / 180.0
 DAY2SEC = 86400.0
 RSUN_KM = 695700.0
@@ -163,17 +165,116 @@ def model_prior(model, constraints):
     """p -> log prior for the model's free-parameter vectors p: varied
     parameters from p, the others from the model (get_par,
     mcmc_solver.cpp:323-328 / levmarq_solver.cpp:323-328)."""
-    names = [model.get_name(i) for i in range(model.nvary())]
-    fixed = {k: getattr(model.pod, k).value for k in ("iangle", "q", "velocity_scale", "r1", "r2", "t1", "t2")}
+    get = _param_getter(model)
 
     def f(p):
-        v = dict(fixed)
-        for j, n in enumerate(names):
-            if n in v:
-                v[n] = float(p[j])
+        v = get(p)
         return log_prior(v["iangle"], v["q"], v["velocity_scale"], v["r1"], v["r2"], v["t1"], v["t2"], constraints)
     return f
 
 
 __all__ = ["Constraints", "constraints_from_config", "implied", "log_prior", "model_prior", "log_gauss",
-           "log_split_gauss", "parse_three"]
+           "log_split_gauss", "parse_three", "LMPrior", "lm_derived", "lm_residual"]
+
+
+# ------------------------------------------------------------------
+# The LM solver's form of the prior: one residual per constraint
+# (levmarq_solver.cpp:195-320), with its own constants and definitions.
+LM_G, LM_MSUN, LM_RSUN = 6.67430e-11, 1.98892e30, 6.9634e8
+
+
+def lm_derived(key, i_deg, q, vs, r1, r2, t1, t2, P_days):
+    """evaluate_derived (levmarq_solver.cpp:225-290): SI constants, M2_min
+    as M2 sin^3 i, log g = log10(100 g) of g in m/s^2 (0 when g <= 0), R2
+    from r2 whatever its value."""
+    sin_i = math.sin(i_deg * (math.pi / 180.0))
+    P_s = P_days * 86400.0
+    a_m = vs * 1e3 * P_s / (2.0 * math.pi)
+    a_rsun = a_m / LM_RSUN
+    M_total = 4.0 * math.pi * math.pi * a_m * a_m * a_m / (LM_G * P_s * P_s * LM_MSUN)
+    M1 = M_total / (1.0 + q)
+    M2 = M_total * q / (1.0 + q)
+    R1 = r1 * a_rsun
+    R2 = r2 * a_rsun
+    if key == "K":
+        return vs * sin_i * q / (1.0 + q)
+    if key == "K2":
+        return vs * sin_i / (1.0 + q)
+    if key == "M1":
+        return M1
+    if key == "M2":
+        return M2
+    if key == "M2min":
+        return M2 * sin_i * sin_i * sin_i
+    if key == "Mtotal":
+        return M_total
+    if key == "q":
+        return q
+    if key == "R1":
+        return R1
+    if key == "R2":
+        return R2
+    if key in ("logg1", "logg2"):
+        M, R = (M1, R1) if key == "logg1" else (M2, R2)
+        g = LM_G * M * LM_MSUN / ((R * LM_RSUN) * (R * LM_RSUN))
+        return math.log10(g * 100.0) if g > 0 else 0.0
+    if key == "T1":
+        return t1
+    if key == "T2":
+        return t2
+    raise KeyError(key)
+
+
+def lm_residual(key, obs, weight, i_deg, q, vs, r1, r2, t1, t2, P_days):
+    """prior_residual (levmarq_solver.cpp:292-318): sqrt(weight) (X - X_obs)
+    / sigma, sigma err_hi when X >= X_obs else err_lo (the other when that
+    one is 0; 1% of |X_obs| when both are)."""
+    val, lo, hi = obs
+    diff = lm_derived(key, i_deg, q, vs, r1, r2, t1, t2, P_days) - val
+    sigma = hi if diff >= 0.0 else lo
+    if sigma <= 0.0:
+        sigma = max(lo, hi)
+    if sigma <= 0.0:
+        sigma = 0.01 * abs(val) + 1e-30
+    return math.sqrt(weight) * diff / sigma
+
+
+class LMPrior:
+    """The LM's prior residuals for a model's free-parameter vectors, in
+    the order of the configuration's "priors" object; weight is the plain
+    prior_weight (levmarq_solver.cpp:457-550).  Calling it with p gives
+    the unscaled residual vector; weight_sum is what the balance factor
+    divides by (levmarq_solver.cpp:1100-1104)."""
+
+    def __init__(self, model, config):
+        c = constraints_from_config(config, sampler=False)
+        self.specs = []
+        if c is not None:
+            for key in config.get("priors", {}):
+                name = PRIOR_KEYS[key]
+                self.specs.append((name, getattr(c, name)))
+        self.P_days = c.P_days if c is not None else 1.0
+        self.weight = c.prior_weight if c is not None else 1.0
+        self.weight_sum = self.weight * len(self.specs)
+        self._get = _param_getter(model)
+
+    def __len__(self):
+        return len(self.specs)
+
+    def __call__(self, p):
+        v = self._get(p)
+        return np.array([lm_residual(name, obs, self.weight, v["iangle"], v["q"], v["velocity_scale"], v["r1"],
+                                     v["r2"], v["t1"], v["t2"], self.P_days) for name, obs in self.specs])
+
+
+def _param_getter(model):
+    names = [model.get_name(i) for i in range(model.nvary())]
+    fixed = {k: getattr(model.pod, k).value for k in ("iangle", "q", "velocity_scale", "r1", "r2", "t1", "t2")}
+    idx = {n: j for j, n in enumerate(names) if n in fixed}
+
+    def get(p):
+        v = dict(fixed)
+        for n, j in idx.items():
+            v[n] = float(p[j])
+        return v
+    return get

@@ -54,3 +54,32 @@ def test_levmarq_engine_vs_oracle(engine, prec):
         np.testing.assert_allclose(got["pars"], ref["pars"], rtol=1e-2)
         assert abs(got["chisq"] / ref["chisq"] - 1) <= 1e-2
     assert ref["chisq"] < 1.1 * 600  # close to the noise level (600 points)
+
+
+def test_levmarq_with_prior_engine_vs_oracle(engine):
+    """The continuation over the prior scale (levmarq_solver.cpp:985-1530)
+    with the physical prior's residuals (a K1 and an R1 constraint, R1 in
+    tension with the data) on the engine and on the oracle: the same
+    stages, balance factors and accept/reject sequence in FP64."""
+    from lcurve_re_amd import fit as F
+    from lcurve_re_amd import prior as PR
+    from oracle import pyoracle as P
+    make, data = _problem()
+    m = make()
+    pod = m.pod
+    args = (84.0, pod.q.value, pod.velocity_scale.value, 0.12, pod.r2.value, pod.t1.value, pod.t2.value, 0.25)
+    K1 = PR.lm_derived("K", *args)
+    R1 = PR.lm_derived("R1", *args)
+    cfg = {"use_priors": True, "true_period": 0.25, "prior_weight": 50.0,
+           "priors": {"K1": f"{K1!r} {0.01 * K1!r} {0.01 * K1!r}", "R1": f"{1.02 * R1!r} {0.01 * R1!r} {0.01 * R1!r}"}}
+    kw = dict(prior=PR.LMPrior(m, cfg), stages=2, max_iter=6)
+    ref = F.levmarq(F.FitBatch(make(), data, P.evaluator(), scale=True), **kw)
+    got = F.levmarq(F.FitBatch(make(), data, F.engine_evaluator(engine), scale=True), **kw)
+    print(f"levmarq+prior: engine {got['pars']} sum_sq {got['chisq']:.6f} it {got['iterations']} calls "
+          f"{got['calls']}; oracle {ref['pars']} sum_sq {ref['chisq']:.6f} it {ref['iterations']}; "
+          f"eff {[s['eff'] for s in ref['stages']]}", flush=True)
+    assert len(ref["stages"]) == 3 and ref["stages"][0]["eff"] == 0.0 and ref["stages"][-1]["eff"] > 0.0
+    assert [h[3] for h in got["history"]] == [h[3] for h in ref["history"]]
+    np.testing.assert_allclose([s["eff"] for s in got["stages"]], [s["eff"] for s in ref["stages"]], rtol=1e-6)
+    np.testing.assert_allclose(got["pars"], ref["pars"], rtol=1e-6)
+    assert abs(got["chisq"] / ref["chisq"] - 1) <= 1e-6

@@ -136,3 +136,59 @@ def test_multistart_runs_are_the_single_runs_and_find_the_global_minimum():
     assert calls < sum(a["calls"] for a in alone)
     assert len({round(r["pars"][1], 3) for r in res}) > 1  # distinct local minima
     assert abs(res[0]["pars"][1] - 7.0) < 0.01 and abs(res[0]["pars"][0] - 1.5) < 0.05
+
+
+class ToyPrior:
+    """One prior residual on b, the form LMPrior has: sqrt(weight) (b - mu) / sigma."""
+
+    def __init__(self, mu, sigma, weight):
+        self.mu, self.sigma, self.weight = mu, sigma, weight
+        self.weight_sum = weight
+
+    def __len__(self):
+        return 1
+
+    def __call__(self, p):
+        return np.array([np.sqrt(self.weight) * (p[1] - self.mu) / self.sigma])
+
+
+@pytest.mark.parametrize("weight", [2.0, 1e4])
+def test_prior_continuation_reaches_the_augmented_optimum(weight):
+    """levmarq_solver.cpp:985-1530: stages at prior scale 0, 1/6, .., 1, the
+    balance factor min(chi2_LC / (n w), 1) at each stage's start; the last
+    stage ends at the least-squares optimum of the data residuals plus the
+    scaled prior residual (scipy MINPACK on the same augmented vector)."""
+    from scipy.optimize import least_squares
+    data, lims, ferr, flux, wt = toy_problem()
+    w = np.where(wt > 0, wt, 1.0)
+    pr = ToyPrior(1.5, 0.05, weight)
+    start = np.array([1.0, 0.5, 0.0, 0.0])
+    res = F.levmarq(F.FitBatch(ToyModel(start, lims), data, toy_eval, scale=False), prior=pr, max_iter=200)
+    st = res["stages"]
+    assert [s["scale"] for s in st] == [k / 6 for k in range(7)]
+    assert st[0]["eff"] == 0.0 and np.array_equal(st[0]["start"], start)
+    for s in st[1:]:
+        chi_lc = float(np.sum(w * ((flux - toy_curve(s["start"])) / ferr) ** 2))
+        assert s["eff"] ** 2 == pytest.approx(s["scale"] * min(chi_lc / weight, 1.0), rel=1e-12)
+    eff = st[-1]["eff"]
+    ref = least_squares(lambda p: np.concatenate([np.sqrt(w) * (flux - toy_curve(p)) / ferr, eff * pr(p)]),
+                        st[-1]["start"], method="lm", xtol=1e-15, ftol=1e-15, gtol=1e-15)
+    np.testing.assert_allclose(res["pars"], ref.x, rtol=1e-6, atol=1e-8)
+    assert abs(res["chisq"] / float(np.sum(ref.fun ** 2)) - 1) <= 1e-10
+    free = least_squares(lambda p: np.sqrt(w) * (flux - toy_curve(p)) / ferr, start, method="lm")
+    assert abs(res["pars"][1] - 1.5) < abs(free.x[1] - 1.5)  # the prior pulls b toward 1.5
+    assert res["iterations"] == sum(s["iterations"] for s in st)
+
+
+def test_prior_candidates_give_the_sequential_iterates():
+    data, lims, *_ = toy_problem()
+    start = np.array([8.0, 5.0, -3.0, 3.0])
+    pr = ToyPrior(1.5, 0.05, 10.0)
+    seq = F.levmarq(F.FitBatch(ToyModel(start, lims), data, toy_eval, scale=False), prior=pr, tau0=1e-6,
+                    candidates=1)
+    bat = F.levmarq(F.FitBatch(ToyModel(start, lims), data, toy_eval, scale=False), prior=pr, tau0=1e-6,
+                    candidates=4)
+    assert seq["history"] == bat["history"] and np.array_equal(seq["pars"], bat["pars"])
+    one = F.levmarq(F.FitBatch(ToyModel(start, lims), data, toy_eval, scale=False), prior=pr, tau0=1e-6,
+                    continuation=False)
+    assert [s["scale"] for s in one["stages"]] == [1.0]

@@ -151,3 +151,57 @@ def test_model_prior_takes_varied_parameters_from_the_vector():
     v2[names.index("q")] *= 1.2
     base["q"] *= 1.2
     assert f(v2) == pytest.approx(PR.log_prior(*base.values(), c), rel=1e-15)
+
+
+# ---------------------------------------------------------------- LM form
+def test_lm_kepler_and_derived_quantities():
+    """The LM's constants are SI (G = 6.67430e-11, M_sun = 1.98892e30 kg,
+    R_sun = 6.9634e8 m): a circular orbit of one such solar mass at 1 au
+    gives M_total 1; K1 + K2 = vs sin i; M2_min = M2 sin^3 i; log g of one
+    solar mass and radius is the solar 4.438 to the precision of these
+    constants."""
+    GM = PR.LM_G * PR.LM_MSUN / 1e9  # km^3 s^-2
+    P_s = 2.0 * math.pi * math.sqrt(AU_KM ** 3 / GM)
+    vs, P = 2.0 * math.pi * AU_KM / P_s, P_s / 86400.0
+    args = (60.0, 0.4, vs, 0.1, 0.2, 5000.0, 3000.0, P)
+    assert PR.lm_derived("Mtotal", *args) == pytest.approx(1.0, rel=1e-12)
+    s = math.sin(math.radians(60.0))
+    assert PR.lm_derived("K", *args) + PR.lm_derived("K2", *args) == pytest.approx(vs * s, rel=1e-14)
+    assert PR.lm_derived("M2min", *args) == pytest.approx(PR.lm_derived("M2", *args) * s ** 3, rel=1e-14)
+    assert PR.lm_derived("R1", *args) == pytest.approx(0.1 * AU_KM * 1e3 / PR.LM_RSUN, rel=1e-12)
+    # M1 = 1, R1 = 1 solar unit
+    q, r1 = 0.5, 0.25
+    a = PR.LM_RSUN / 1e3 / r1
+    P_s = 2.0 * math.pi * math.sqrt(a ** 3 / (GM * (1.0 + q)))
+    args = (90.0, q, 2.0 * math.pi * a / P_s, r1, 0.2, 0.0, 0.0, P_s / 86400.0)
+    assert PR.lm_derived("M1", *args) == pytest.approx(1.0, rel=1e-12)
+    assert abs(PR.lm_derived("logg1", *args) - 4.438) < 2e-3
+
+
+def test_lm_residual_sides_and_fallbacks():
+    args = (80.0, 0.3, 300.0, 0.2, 0.3, 12000.0, 4000.0, 0.5)
+    assert PR.lm_residual("T1", (11000.0, 100.0, 400.0), 4.0, *args) == pytest.approx(2.0 * 1000.0 / 400.0)
+    assert PR.lm_residual("T1", (13000.0, 100.0, 400.0), 4.0, *args) == pytest.approx(2.0 * -1000.0 / 100.0)
+    assert PR.lm_residual("T2", (3000.0, 0.0, 0.0), 1.0, *args) == pytest.approx(1000.0 / 30.0)
+
+
+def test_lm_prior_from_a_configuration():
+    import json
+
+    import lcurve_re_amd as L
+    from lcurve_re_amd import workloads as W
+    m = L.Model.from_json(json.dumps(W.model_json(vary=("q", "iangle", "r1"))))
+    cfg = {"use_priors": True, "true_period": 0.3, "prior_weight": 9.0, "mcmc_prior_weight": 1.0,
+           "priors": {"T1": "15000 300 300", "K1": "150 10 20"}}
+    lp = PR.LMPrior(m, cfg)
+    assert len(lp) == 2 and lp.weight == 9.0 and lp.weight_sum == 18.0
+    names = [m.get_name(i) for i in range(m.nvary())]
+    v = np.array(m.get_param(), float)
+    v[names.index("q")] = 0.31
+    r = lp(v)
+    pod = m.pod
+    base = (pod.iangle.value, 0.31, pod.velocity_scale.value, pod.r1.value, pod.r2.value, pod.t1.value,
+            pod.t2.value, 0.3)
+    assert r[0] == PR.lm_residual("T1", (15000.0, 300.0, 300.0), 9.0, *base)
+    assert r[1] == PR.lm_residual("K", (150.0, 10.0, 20.0), 9.0, *base)
+    assert len(PR.LMPrior(m, {"use_priors": False})) == 0
