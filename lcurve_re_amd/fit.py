@@ -505,40 +505,122 @@ def levmarq(fb, pars=None, **kw):
     return fb.drive(levmarq_requests(fb, pars, **kw))
 
 
-def levmarq_multistart(fb, starts, **kw):
-    """Independent LM fits from several starting vectors (the reference's
-    multistart, levmarq_solver.cpp:1000-1640, without its clustering of
-    optima), advanced in lockstep: each round, the pending evaluations of
-    every unfinished fit go to the engine in one batched call.  Each fit is
-    exactly the one levmarq() runs from its start.  Returns the results
-    (levmarq's dicts, plus "start") sorted by chi^2, and the number of
-    batched calls."""
-    gens, reqs, out = [], [], []
-    for k, s0 in enumerate(starts):
-        g = levmarq_requests(fb, np.asarray(s0, float), **kw)
+def run_lockstep(fb, gens):
+    """Runs request generators (see FitBatch.drive) in lockstep: each
+    round, the pending evaluations of every unfinished generator go to the
+    evaluator in one batched call.  Returns their results in order and the
+    number of batched calls."""
+    out = [None] * len(gens)
+    live, reqs = [], []
+    for k, g in enumerate(gens):
         try:
             reqs.append(next(g))
-            gens.append((k, g))
+            live.append((k, g))
         except StopIteration as e:
-            out.append(dict(e.value, start=k))
+            out[k] = e.value
     calls = 0
-    while gens:
+    while live:
         batch = [v for r in reqs for v in r]
         calc, status = fb._run(batch)
         calls += 1
         status = np.asarray(status)
         nxt_g, nxt_r, o = [], [], 0
-        for (k, g), r in zip(gens, reqs):
+        for (k, g), r in zip(live, reqs):
             part = (calc[o:o + len(r)], status[o:o + len(r)])
             o += len(r)
             try:
                 nxt_r.append(g.send(part))
                 nxt_g.append((k, g))
             except StopIteration as e:
-                out.append(dict(e.value, start=k))
-        gens, reqs = nxt_g, nxt_r
+                out[k] = e.value
+        live, reqs = nxt_g, nxt_r
+    return out, calls
+
+
+def levmarq_multistart(fb, starts, **kw):
+    """Independent LM fits from several starting vectors (the reference's
+    multistart, levmarq_solver.cpp:1550-1640; the starts are the caller's),
+    advanced in lockstep: each round, the pending evaluations of every
+    unfinished fit go to the engine in one batched call.  Each fit is
+    exactly the one levmarq() runs from its start.  Returns the results
+    (levmarq's dicts, plus "start") sorted by chi^2, and the number of
+    batched calls.  laplace_modes() groups them into modes."""
+    res, calls = run_lockstep(fb, [levmarq_requests(fb, np.asarray(s0, float), **kw) for s0 in starts])
+    out = [dict(r, start=k) for k, r in enumerate(res)]
     out.sort(key=lambda r: r["chisq"])
     return out, calls
+
+
+def laplace_modes(fb, runs, dsteps, prior=None, log_prior=None, tol_dsteps=5.0, tol_frac=0.02):
+    """Groups LM optima into modes and weights each by its Laplace
+    posterior mass (levmarq_solver.cpp:1640-1790).
+
+    runs: levmarq results (levmarq_multistart's).  Sorted by |r|^2, a run
+    joins the first mode whose representative is within max(tol_dsteps
+    |dstep_j|, tol_frac (|a_j| + |b_j|) / 2, 1e-12) in every parameter,
+    else it opens a mode.  With s2 = max(chi2_LC(best) / max(1, N - k),
+    1e-30), each mode gets lpost = (-0.5 chi2_LC + log_prior) / s2 and,
+    from the central-difference Jacobian at its optimum with the data rows
+    scaled by 1/sqrt(s2) (prior rows, when `prior` is given, at full scale
+    times the best run's balance factor), Sigma = (J^T J)^-1 and log det
+    Sigma; the weight is exp(lpost + 0.5 log det Sigma) normalised over
+    the modes (the volume term only when every mode has its Sigma).
+    log_prior: None or p -> log prior (the reference uses the physical
+    prior with lm_error_mcmc_prior_weight, by default the balanced
+    prior_weight).  MI355X-side change: the Jacobians of all modes are
+    one batched engine call.  Returns the modes, heaviest first: dicts
+    with pars, run (index into runs), n_runs, weight, lpost, logdet, cov
+    (None when J^T J is not positive definite), chisq_lc; and calls.
+    """
+    order = sorted(range(len(runs)), key=lambda k: runs[k]["chisq"])
+    if not order:
+        raise ValueError("no LM run to group")
+    dsteps = np.abs(np.asarray(dsteps, float))
+
+    def same(a, b):
+        thr = np.maximum(tol_dsteps * dsteps, tol_frac * 0.5 * (np.abs(a) + np.abs(b)))
+        return bool(np.all(np.abs(a - b) <= np.maximum(thr, 1e-12)))
+
+    modes = []
+    for k in order:
+        p = np.asarray(runs[k]["pars"], float)
+        for m in modes:
+            if same(p, m["pars"]):
+                m["n_runs"] += 1
+                break
+        else:
+            modes.append({"pars": p, "run": k, "n_runs": 1})
+    best = runs[order[0]]
+    ndata = len(data_view(fb.data)["flux"])
+    s2 = max(best["chisq_lc"] / max(1, ndata - fb.npar), 1e-30)
+    pr = None
+    if prior is not None and len(prior):
+        pr = (prior, best["stages"][-1]["eff"] if "stages" in best else 1.0)
+    jacs, calls = run_lockstep(fb, [fb.jacobian_requests(m["pars"], prior=pr) for m in modes])
+    dsc = 1.0 / math.sqrt(s2)
+    for m, jac in zip(modes, jacs):
+        m["chisq_lc"] = jac["chisq_lc0"]
+        m["lpost"] = (-0.5 * jac["chisq_lc0"] + (log_prior(m["pars"]) if log_prior else 0.0)) / s2
+        J = jac["J"].copy()
+        J[:ndata] *= dsc
+        A = J.T @ J
+        m["cov"], m["logdet"] = None, 0.0
+        try:
+            L = np.linalg.cholesky(A)
+            d = np.diag(L)
+            if np.all(d * d > 1e-30):
+                m["cov"] = np.linalg.inv(A)
+                m["logdet"] = -2.0 * float(np.sum(np.log(d)))
+        except np.linalg.LinAlgError:
+            pass
+    vol = all(m["cov"] is not None for m in modes)
+    score = np.array([m["lpost"] + (0.5 * m["logdet"] if vol else 0.0) for m in modes])
+    w = np.exp(np.maximum(score - score.max(), -700.0))
+    w /= max(float(w.sum()), 1e-300)
+    for m, wk in zip(modes, w):
+        m["weight"] = float(wk)
+    modes.sort(key=lambda m: -m["weight"])
+    return modes, calls
 
 
 def engine_evaluator(engine):

@@ -83,3 +83,24 @@ def test_levmarq_with_prior_engine_vs_oracle(engine):
     np.testing.assert_allclose([s["eff"] for s in got["stages"]], [s["eff"] for s in ref["stages"]], rtol=1e-6)
     np.testing.assert_allclose(got["pars"], ref["pars"], rtol=1e-6)
     assert abs(got["chisq"] / ref["chisq"] - 1) <= 1e-6
+
+
+def test_laplace_modes_engine_vs_oracle(engine):
+    """Mode grouping and Laplace weights (levmarq_solver.cpp:1640-1790) of
+    the engine's multistart optima, computed with the engine and with the
+    oracle: the same modes, lpost and log det Sigma (FP64)."""
+    from lcurve_re_amd import fit as F
+    from oracle import pyoracle as P
+    make, data = _problem()
+    starts = [[83.0, 0.124, 0.245], [85.0, 0.118, 0.252], [84.2, 0.121, 0.249]]
+    fe = F.FitBatch(make(), data, F.engine_evaluator(engine), scale=True)
+    runs, calls = F.levmarq_multistart(fe, starts, max_iter=8)
+    got, gcalls = F.laplace_modes(fe, runs, dsteps=[0.05, 5e-4, 5e-4])
+    ref, _ = F.laplace_modes(F.FitBatch(make(), data, P.evaluator(), scale=True), runs, dsteps=[0.05, 5e-4, 5e-4])
+    print(f"modes: {[(m['pars'], m['n_runs'], m['weight']) for m in got]} calls {calls} + {gcalls}", flush=True)
+    assert gcalls == 1 and sum(m["n_runs"] for m in got) == 3
+    assert [m["run"] for m in got] == [m["run"] for m in ref]
+    for a, b in zip(got, ref):
+        assert abs(a["lpost"] - b["lpost"]) <= 1e-6 * abs(b["lpost"])
+        assert abs(a["logdet"] - b["logdet"]) <= 1e-4 * max(1.0, abs(b["logdet"]))
+        assert abs(a["weight"] - b["weight"]) <= 1e-4

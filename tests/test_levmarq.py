@@ -9,6 +9,7 @@ from a start that forces rejections; on a light curve evaluated by the
 FP64 oracle it recovers the parameters the data were made with.  CPU only
 (the GPU engine's version is in test_gpu_levmarq.py)."""
 import json
+import math
 
 import numpy as np
 import pytest
@@ -192,3 +193,48 @@ def test_prior_candidates_give_the_sequential_iterates():
     one = F.levmarq(F.FitBatch(ToyModel(start, lims), data, toy_eval, scale=False), prior=pr, tau0=1e-6,
                     continuation=False)
     assert [s["scale"] for s in one["stages"]] == [1.0]
+
+
+# ------------------------------------------------------- modes of a multistart
+XM = np.linspace(0.0, 1.0, 200)
+
+
+def bimodal_curve(a):
+    return a * a * (1.0 + 0.3 * a) * XM + 0.0075 * a * XM ** 2
+
+
+def bimodal_eval(models, data, opts):
+    return np.array([bimodal_curve(m.p[0]) for m in models]), np.zeros(len(models), int)
+
+
+def test_laplace_modes_match_the_integrated_posterior_mass():
+    """levmarq_solver.cpp:1640-1790 on y = a^2 (1 + 0.3 a) x + 0.0075 a x^2,
+    whose chi^2 has three minima of different depth and width: the runs
+    from nine starts fall into three modes, and each mode's weight is its
+    share of the integral of exp(-0.5 chi^2 / s2) over its basin (scipy
+    quadrature; the Laplace approximation is accurate for basins this
+    narrow), s2 = chi^2_best / (N - 1)."""
+    from scipy.integrate import quad
+    rng = np.random.default_rng(11)
+    sig = 0.02
+    flux = bimodal_curve(0.9) + sig * rng.standard_normal(len(XM))
+    data = L.make_data(XM, ferr=sig, flux=flux, weight=1.0)
+    lims = [(-5.0, 5.0)]
+    fb = F.FitBatch(ToyModel([0.5], lims), data, bimodal_eval, scale=False)
+    starts = [[s] for s in (0.6, 0.8, 1.2, -0.8, -1.3, -1.0, -2.6, -3.3, -2.9)]
+    runs, calls = F.levmarq_multistart(fb, starts, max_iter=200)
+    modes, mcalls = F.laplace_modes(fb, runs, dsteps=[0.01])
+    assert len(modes) == 3 and sorted(m["n_runs"] for m in modes) == [3, 3, 3] and mcalls == 1
+    chi = lambda a: float(np.sum(((flux - bimodal_curve(a)) / sig) ** 2))
+    s2 = min(chi(m["pars"][0]) for m in modes) / (len(XM) - 1)
+    c0 = min(chi(m["pars"][0]) for m in modes)
+    mass = []
+    for m in modes:
+        a0, w = m["pars"][0], math.sqrt(m["cov"][0, 0])
+        mass.append(quad(lambda a: math.exp(-0.5 * (chi(a) - c0) / s2), a0 - 12 * w, a0 + 12 * w,
+                         points=[a0], limit=200)[0])
+    mass = np.array(mass) / sum(mass)
+    got = np.array([m["weight"] for m in modes])
+    assert got.max() / got.min() > 1.5  # the modes differ in mass
+    np.testing.assert_allclose(got, mass, rtol=2e-3)
+    assert abs(sum(got) - 1.0) < 1e-12 and list(got) == sorted(got, reverse=True)
