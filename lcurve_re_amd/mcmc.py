@@ -228,3 +228,131 @@ def adaptive_metropolis(model, data, evaluate, nchains=8, nsteps=2000, burn_in=N
 
 
 __all__ = ["adaptive_metropolis", "chisq"]
+
+
+def _reflect(prop, lims):
+    for i, (lo, hi) in enumerate(lims):
+        b = 0
+        while (prop[i] < lo or prop[i] > hi) and b < 20:
+            prop[i] = 2 * lo - prop[i] if prop[i] < lo else 2 * hi - prop[i]
+            b += 1
+        prop[i] = min(max(prop[i], lo), hi)
+    return prop
+
+
+def error_mcmc(fb, modes, err_s2=None, log_prior=None, dsteps=None, steps=8000, min_steps=500,
+               mode_min_weight=0.005, seed=43):
+    """The LM program's error sampling over the posterior modes
+    (levmarq_solver.cpp:2090-2500, one round: no re-anchoring), on a
+    fit.FitBatch and the modes of fit.laplace_modes.
+
+    Modes of weight >= mode_min_weight are sampled (the heaviest if none
+    is), each by one chain of max(min_steps, steps w / w_sampled) kept
+    steps after max(300, that / 5) burn-in steps, from the mode's optimum:
+    proposal p + exp(s) (2.38 / sqrt(d)) L z with L the Cholesky factor of
+    the mode's covariance (dsteps times z when it has none), reflected off
+    the limits (at most 20 bounces, then clamped); a proposal with log
+    prior <= -1e29 is rejected unevaluated, an evaluation failure is a
+    rejection; log alpha = (-0.5 (chi2' - chi2) + (lp' - lp)) / err_s2
+    with chi2 the LM's light-curve chi^2 (a zero weight counting as 1) and
+    err_s2 by default chi2_LC / max(1, N - k) of the heaviest mode;
+    Robbins-Monro during burn-in: every 50 steps s += (1 + batch)^-0.6
+    (rate - 0.234), s in [log 1e-3, log 1e3].  The best state met
+    (largest -0.5 chi2 + lp) is the MAP.
+
+    MI355X-side change: the chains of all modes advance in lockstep, one
+    batched engine call per step for every chain still running, and each
+    chain draws from its own PCG64 generator (seed + mode index) instead of
+    one generator shared by chains run one after another.  Returns dict
+    with samples (kept states of all chains, mode order), per_mode (mode
+    index, kept, accepted), acceptance, map_pars, map_chisq, map_lp, ok
+    (more than 100 samples and acceptance above 1%), calls.
+    """
+    npar = fb.npar
+    lims = fb.limits
+    data = fb.data
+    ndata = len(data_view(data)["flux"])
+    if err_s2 is None:
+        err_s2 = max(modes[0]["chisq_lc"] / max(1, ndata - npar), 1e-30)
+    if dsteps is None:
+        dsteps = np.abs(np.asarray(fb.model.get_dstep(), float))
+    diag = np.where(np.asarray(dsteps, float) > 0.0, dsteps, 1e-6)
+    pick = [k for k, m in enumerate(modes) if m["weight"] >= mode_min_weight] or [0]
+    wincl = sum(modes[k]["weight"] for k in pick)
+    prop_sd = 2.38 / math.sqrt(npar)
+    lp = (lambda v: log_prior(v)) if log_prior else (lambda v: 0.0)
+
+    chains = []
+    for k in pick:
+        m = modes[k]
+        nst = max(min_steps, int(steps * m["weight"] / max(wincl, 1e-300)))
+        L = None
+        if m.get("cov") is not None:
+            try:
+                L = np.linalg.cholesky(m["cov"])
+            except np.linalg.LinAlgError:
+                L = None
+        chains.append({"mode": k, "p": np.array(m["pars"], float), "L": L, "nst": nst,
+                       "nbr": max(300, nst // 5), "step": 0, "rng": np.random.default_rng(seed + k),
+                       "s": 0.0, "batch": 0, "bacc": 0, "bn": 0, "kept": [], "acc": 0})
+    calc, status = fb._run([c["p"] for c in chains])
+    calls = 1
+    for k, c in enumerate(chains):
+        if status[k] != 0:
+            raise RuntimeError("a mode's optimum does not evaluate")
+        c["chi"] = float(np.sum(residuals_lc(data, calc[k]) ** 2))
+        c["lp"] = lp(c["p"])
+    # the MAP starts at the heaviest mode's optimum (the adopted LM fit)
+    map_lp, map_pars, map_chisq = -0.5 * chains[0]["chi"] + chains[0]["lp"], chains[0]["p"].copy(), chains[0]["chi"]
+    while True:
+        live = [c for c in chains if c["step"] < c["nst"] + c["nbr"]]
+        if not live:
+            break
+        props, lps = [], []
+        for c in live:
+            z = c["rng"].standard_normal(npar)
+            sf = math.exp(c["s"])
+            prop = c["p"] + sf * prop_sd * (c["L"] @ z) if c["L"] is not None else c["p"] + sf * prop_sd * diag * z
+            props.append(_reflect(prop, lims))
+            lps.append(lp(props[-1]))
+        ev = [i for i, v in enumerate(lps) if v > -1e29]
+        res = [None] * len(live)
+        if ev:
+            calc, status = fb._run([props[i] for i in ev])
+            calls += 1
+            for j, i in enumerate(ev):
+                if status[j] == 0:
+                    res[i] = float(np.sum(residuals_lc(data, calc[j]) ** 2))
+        for i, c in enumerate(live):
+            acc = False
+            if res[i] is not None:
+                la = (-0.5 * (res[i] - c["chi"]) + (lps[i] - c["lp"])) / err_s2
+                if la >= 0.0 or math.log(c["rng"].random()) < la:
+                    c["p"], c["chi"], c["lp"] = props[i], res[i], lps[i]
+                    acc = True
+            if acc and -0.5 * c["chi"] + c["lp"] > map_lp:
+                map_lp, map_pars, map_chisq = -0.5 * c["chi"] + c["lp"], c["p"].copy(), c["chi"]
+            if c["step"] < c["nbr"]:
+                c["bn"] += 1
+                c["bacc"] += acc
+                if c["bn"] >= 50:
+                    g = (1.0 + c["batch"]) ** -0.6
+                    c["s"] = min(max(c["s"] + g * (c["bacc"] / c["bn"] - 0.234), math.log(1e-3)), math.log(1e3))
+                    c["bacc"] = c["bn"] = 0
+                    c["batch"] += 1
+            else:
+                c["acc"] += acc
+                c["kept"].append(c["p"].copy())
+            c["step"] += 1
+    samples = np.array([v for c in chains for v in c["kept"]]).reshape(-1, npar)
+    drawn = sum(len(c["kept"]) for c in chains)
+    acc = sum(c["acc"] for c in chains) / drawn if drawn else 0.0
+    return {"samples": samples, "per_mode": [(c["mode"], len(c["kept"]), c["acc"]) for c in chains],
+            "acceptance": acc, "map_pars": map_pars, "map_chisq": map_chisq, "map_lp": map_lp,
+            "ok": len(samples) > 100 and acc > 0.01, "calls": calls, "err_s2": err_s2}
+
+
+def residuals_lc(data, calc):
+    """The LM's light-curve residuals (levmarq_solver.cpp:690-700)."""
+    from .fit import residuals
+    return residuals(data, calc)

@@ -104,3 +104,24 @@ def test_laplace_modes_engine_vs_oracle(engine):
         assert abs(a["lpost"] - b["lpost"]) <= 1e-6 * abs(b["lpost"])
         assert abs(a["logdet"] - b["logdet"]) <= 1e-4 * max(1.0, abs(b["logdet"]))
         assert abs(a["weight"] - b["weight"]) <= 1e-4
+
+
+def test_error_mcmc_engine_vs_oracle(engine):
+    """The error sampling over the modes (levmarq_solver.cpp:2090-2500) on
+    the engine and on the oracle from the same modes: the same chains in
+    FP64 (each chain owns its generator, so only the evaluations differ)."""
+    from lcurve_re_amd import fit as F
+    from lcurve_re_amd import mcmc as M
+    from oracle import pyoracle as P
+    make, data = _problem()
+    fe = F.FitBatch(make(), data, F.engine_evaluator(engine), scale=True)
+    runs, _ = F.levmarq_multistart(fe, [[83.0, 0.124, 0.245], [84.2, 0.121, 0.249]], max_iter=8)
+    modes, _ = F.laplace_modes(fe, runs, dsteps=[0.05, 5e-4, 5e-4])
+    kw = dict(dsteps=[0.05, 5e-4, 5e-4], steps=150, min_steps=150, seed=5)
+    got = M.error_mcmc(fe, modes, **kw)
+    ref = M.error_mcmc(F.FitBatch(make(), data, P.evaluator(), scale=True), modes, **kw)
+    print(f"error mcmc: per mode {got['per_mode']} acceptance {got['acceptance']:.3f} calls {got['calls']} "
+          f"map {got['map_pars']}", flush=True)
+    assert got["per_mode"] == ref["per_mode"] and got["acceptance"] > 0.01
+    np.testing.assert_allclose(got["samples"], ref["samples"], rtol=1e-9)
+    np.testing.assert_allclose(got["map_pars"], ref["map_pars"], rtol=1e-9)

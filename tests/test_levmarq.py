@@ -238,3 +238,61 @@ def test_laplace_modes_match_the_integrated_posterior_mass():
     assert got.max() / got.min() > 1.5  # the modes differ in mass
     np.testing.assert_allclose(got, mass, rtol=2e-3)
     assert abs(sum(got) - 1.0) < 1e-12 and list(got) == sorted(got, reverse=True)
+
+
+# ------------------------------------------------------- error MCMC over modes
+def test_error_mcmc_samples_the_gaussian_posterior_of_a_linear_fit():
+    """levmarq_solver.cpp:2090-2500 on a linear model y = A p: the target
+    exp(-0.5 chi^2 / s2) is N(p_hat, s2 (A^T W A)^-1), s2 = chi2_min /
+    (N - k)."""
+    from lcurve_re_amd import mcmc as M
+    X = np.linspace(0.0, 1.0, 150)
+    A = np.stack([np.ones_like(X), X, np.cos(5.0 * X)], axis=1)
+    rng = np.random.default_rng(2)
+    sig = 0.03 + 0.02 * rng.random(len(X))
+    y = A @ np.array([0.5, 1.0, -0.2]) + sig * rng.standard_normal(len(X))
+    data = L.make_data(X, ferr=sig, flux=y, weight=1.0)
+    ev = lambda models, d, o: (np.array([A @ m.p for m in models]), np.zeros(len(models), int))
+    fb = F.FitBatch(ToyModel([0.4, 0.9, 0.0], [(-10.0, 10.0)] * 3), data, ev, scale=False)
+    runs, _ = F.levmarq_multistart(fb, [[0.4, 0.9, 0.0], [0.6, 1.1, -0.3]])
+    modes, _ = F.laplace_modes(fb, runs, dsteps=[0.01] * 3)
+    assert len(modes) == 1 and modes[0]["n_runs"] == 2
+    r = M.error_mcmc(fb, modes, dsteps=[0.01] * 3, steps=12000, seed=4)
+    W = 1.0 / sig ** 2
+    F_ = A.T @ (A * W[:, None])
+    phat = np.linalg.solve(F_, A.T @ (W * y))
+    s2 = float(np.sum(W * (y - A @ phat) ** 2)) / (len(X) - 3)
+    assert r["err_s2"] == pytest.approx(s2, rel=1e-8)
+    C = s2 * np.linalg.inv(F_)
+    sd = np.sqrt(np.diag(C))
+    assert r["per_mode"][0][1] == 12000 and r["ok"]
+    assert np.all(np.abs(r["samples"].mean(axis=0) - phat) < 0.15 * sd)
+    np.testing.assert_allclose(r["samples"].std(axis=0), sd, rtol=0.15)
+    assert 0.1 < r["acceptance"] < 0.5
+    assert -0.5 * r["map_chisq"] >= -0.5 * float(np.sum(W * (y - A @ modes[0]["pars"]) ** 2)) - 1e-9
+
+
+def test_error_mcmc_splits_steps_over_modes_by_weight():
+    from lcurve_re_amd import mcmc as M
+    rng = np.random.default_rng(11)
+    sig = 0.02
+    flux = bimodal_curve(0.9) + sig * rng.standard_normal(len(XM))
+    data = L.make_data(XM, ferr=sig, flux=flux, weight=1.0)
+    fb = F.FitBatch(ToyModel([0.5], [(-5.0, 5.0)]), data, bimodal_eval, scale=False)
+    runs, _ = F.levmarq_multistart(fb, [[s] for s in (0.8, -1.3, -2.9)])
+    modes, _ = F.laplace_modes(fb, runs, dsteps=[0.01])
+    r = M.error_mcmc(fb, modes, dsteps=[0.01], steps=3000, seed=8)
+    w = np.array([m["weight"] for m in modes])
+    kept = [n for _, n, _ in r["per_mode"]]
+    assert kept == [max(500, int(3000 * wk / w.sum())) for wk in w]
+    o = 0
+    for (k, n, _), m in zip(r["per_mode"], modes):
+        s = r["samples"][o:o + n, 0]
+        o += n
+        assert np.all(np.abs(s - m["pars"][0]) < 10 * math.sqrt(m["cov"][0, 0]))  # each chain stays in its basin
+    one = M.error_mcmc(fb, modes[:1], dsteps=[0.01], steps=kept[0], min_steps=kept[0], seed=8)
+    assert np.array_equal(one["samples"], r["samples"][:kept[0]])  # a chain does not depend on the others
+    bound = modes[0]["pars"][0] + 0.5 * math.sqrt(modes[0]["cov"][0, 0])
+    capped = M.error_mcmc(fb, modes[:1], dsteps=[0.01], steps=1000, seed=3,
+                          log_prior=lambda p: -1e30 if p[0] > bound else 0.0)
+    assert capped["samples"][:, 0].max() <= bound
