@@ -442,9 +442,14 @@ def run_fit_batch(L, eng, args):
         lm[f"candidates_{k}"] = {"s": time.perf_counter() - tl, "engine_calls": r["calls"],
                                  "iterations": r["iterations"], "chisq": r["chisq"], "stop": r["stop"],
                                  "rejections": sum(1 for h in r["history"] if h[3] == "reject")}
-    # multistart: 8 starts 0.1% off, up to 30 iterations each, lockstep
-    # (one engine call per round for all starts) and one after another
-    starts = F.walkers_around(model, 8, w["walker_seed"] + 2, rel=1e-3)
+    # multistart: 8 starts 0.1% off the LM solution (the workload's own
+    # start has two limb-darkening coefficients below their lower limit 0,
+    # where every clamped trial is rejected), up to 30 iterations each,
+    # lockstep (one engine call per round for all starts) and one after
+    # another
+    m_lm = model.copy()
+    m_lm.set_param(lm_pars)
+    starts = F.walkers_around(m_lm, 8, w["walker_seed"] + 2, rel=1e-3)
     fbm = F.FitBatch(model, data, F.engine_evaluator(eng), scale=True, precision=prec)
     tl = time.perf_counter()
     ms_res, ms_calls = F.levmarq_multistart(fbm, starts, max_iter=30)
@@ -454,6 +459,39 @@ def run_fit_batch(L, eng, args):
     t_seq = time.perf_counter() - tl
     lm["multistart_8"] = {"lockstep_s": t_lock, "lockstep_engine_calls": ms_calls, "sequential_s": t_seq,
                           "sequential_engine_calls": seq_calls, "best_chisq": float(ms_res[0]["chisq"])}
+    # the multistart's optima grouped into modes (Laplace masses; every
+    # mode's Jacobian in one call), then the error sampling over them (one
+    # chain per mode, 400 kept steps shared by mass, at least 100 each,
+    # burn-in 300), all chains' proposals of a step in one call
+    from lcurve_re_amd import mcmc as MC
+    tl = time.perf_counter()
+    modes, mode_calls = F.laplace_modes(fbm, ms_res, dsteps=model.get_dstep())
+    t_modes = time.perf_counter() - tl
+    tl = time.perf_counter()
+    emc = MC.error_mcmc(fbm, modes, steps=400, min_steps=100, seed=w["walker_seed"] + 3)
+    t_emc = time.perf_counter() - tl
+    lm["modes"] = {"s": t_modes, "engine_calls": mode_calls, "modes": len(modes),
+                   "weights": [round(m["weight"], 6) for m in modes]}
+    lm["error_mcmc"] = {"s": t_emc, "engine_calls": emc["calls"], "kept": [n for _, n, _ in emc["per_mode"]],
+                        "acceptance": emc["acceptance"], "ok": emc["ok"], "chain_steps_per_s": sum(
+                            n + max(300, n // 5) for _, n, _ in emc["per_mode"]) / t_emc}
+    # the LM with physical priors (K1 and R1 at 1% around the solution's
+    # implied values, R1 moved by 2%): continuation over 7 prior scales
+    from lcurve_re_amd import prior as PRI
+    mp = model.copy()
+    mp.set_param(lm_pars)
+    pod = mp.pod
+    pargs = (pod.iangle.value, pod.q.value, pod.velocity_scale.value, pod.r1.value, pod.r2.value, pod.t1.value,
+             pod.t2.value, 1.0)
+    K1, R1 = PRI.lm_derived("K", *pargs), PRI.lm_derived("R1", *pargs)
+    pcfg = {"use_priors": True, "true_period": 1.0, "prior_weight": 10.0,
+            "priors": {"K1": f"{K1!r} {0.01 * K1!r} {0.01 * K1!r}", "R1": f"{1.02 * R1!r} {0.01 * R1!r} {0.01 * R1!r}"}}
+    tl = time.perf_counter()
+    rp = F.levmarq(F.FitBatch(model, data, F.engine_evaluator(eng), scale=True, precision=prec), pars=lm_pars,
+                   max_iter=60, prior=PRI.LMPrior(model, pcfg))
+    lm["with_prior"] = {"s": time.perf_counter() - tl, "engine_calls": rp["calls"], "iterations": rp["iterations"],
+                        "stages": len(rp["stages"]), "sum_sq": rp["chisq"], "chisq_lc": rp["chisq_lc"],
+                        "final_eff": rp["stages"][-1]["eff"]}
     log(f"bench: levmarq {lm}")
     # the adaptive-Metropolis sampler: 64 chains, 60 steps, all proposals of
     # a step in one batched call, started at the LM solution (inside the
@@ -462,7 +500,6 @@ def run_fit_batch(L, eng, args):
     # deviation there, 1 / sqrt((J^T J)_ii) (the model file's 1e-4 are far
     # wider than this posterior, and the limb-darkening coefficients are too
     # degenerate for marginal deviations)
-    from lcurve_re_amd import mcmc as MC
     mmc = model.copy()
     mmc.set_param(lm_pars)
     Jl = fb.jacobian(lm_pars)["J"]
