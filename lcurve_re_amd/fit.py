@@ -505,6 +505,59 @@ def levmarq(fb, pars=None, **kw):
     return fb.drive(levmarq_requests(fb, pars, **kw))
 
 
+class CxxMt19937Uniform:
+    """The draws of std::mt19937(seed) through libstdc++'s
+    std::uniform_real_distribution<double>(0, 1): generate_canonical takes
+    two 32-bit outputs w1, w2 and returns (w1 + w2 2^32) / 2^64 (the sum
+    rounded to double; a result of 1 becomes the largest double below 1).
+    numpy's legacy RandomState seeds MT19937 as std::mt19937 does and its
+    full-range uint32 draws are the raw outputs."""
+
+    def __init__(self, seed):
+        self._rs = np.random.RandomState(int(seed) & 0xFFFFFFFF)
+
+    def __call__(self):
+        w1, w2 = (int(x) for x in self._rs.randint(0, 2 ** 32, size=2, dtype=np.uint64))
+        r = (float(w1) + float(w2) * 4294967296.0) / 18446744073709551616.0
+        return r if r < 1.0 else math.nextafter(1.0, 0.0)
+
+
+def multistart_starts(model, pars=None, n=8, span=1.0, seed=42):
+    """The multistart's starting vectors (levmarq_solver.cpp:1560-1606):
+    the current parameters, then n draws; in each, every parameter but
+    iangle uniform in [p - span range, p + span range] cut to its limits
+    (kept when that is empty or the range is not positive), and iangle
+    stratified over [max(lo, 5), min(hi, 90)]: start s at fraction
+    (s + 0.2 + 0.6 u) / n.  Uniforms from std::mt19937(seed + 2) in the
+    reference's order (CxxMt19937Uniform), so the starts are the
+    reference's.  Not restated: the consistent-parameter solve for iangle
+    when K1, M1 and R1 priors are all given."""
+    p0 = np.array(model.get_param() if pars is None else pars, float)
+    npar = len(p0)
+    lims = [tuple(x) for x in model.get_limit()]
+    ranges = np.asarray(model.get_range(), float)
+    names = [model.get_name(i) for i in range(npar)]
+    ii = names.index("iangle") if "iangle" in names else -1
+    u = CxxMt19937Uniform(seed + 2)
+    starts = [p0.copy()]
+    for s in range(n):
+        st = p0.copy()
+        for j in range(npar):
+            if j == ii:
+                continue
+            rj = span * ranges[j]
+            if not rj > 0.0:
+                continue
+            lo, hi = max(lims[j][0], p0[j] - rj), min(lims[j][1], p0[j] + rj)
+            if hi > lo:
+                st[j] = lo + (hi - lo) * u()
+        if ii >= 0:
+            ilo, ihi = max(lims[ii][0], 5.0), min(lims[ii][1], 90.0)
+            st[ii] = ilo + (s + 0.2 + 0.6 * u()) / n * (ihi - ilo)
+        starts.append(st)
+    return starts
+
+
 def run_lockstep(fb, gens):
     """Runs request generators (see FitBatch.drive) in lockstep: each
     round, the pending evaluations of every unfinished generator go to the
@@ -539,7 +592,8 @@ def run_lockstep(fb, gens):
 
 def levmarq_multistart(fb, starts, **kw):
     """Independent LM fits from several starting vectors (the reference's
-    multistart, levmarq_solver.cpp:1550-1640; the starts are the caller's),
+    multistart, levmarq_solver.cpp:1550-1640; multistart_starts draws its
+    starts),
     advanced in lockstep: each round, the pending evaluations of every
     unfinished fit go to the engine in one batched call.  Each fit is
     exactly the one levmarq() runs from its start.  Returns the results

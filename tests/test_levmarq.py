@@ -296,3 +296,47 @@ def test_error_mcmc_splits_steps_over_modes_by_weight():
     capped = M.error_mcmc(fb, modes[:1], dsteps=[0.01], steps=1000, seed=3,
                           log_prior=lambda p: -1e30 if p[0] > bound else 0.0)
     assert capped["samples"][:, 0].max() <= bound
+
+
+# ------------------------------------------------------- multistart starts
+def test_cxx_uniform_draws_match_libstdcxx():
+    """CxxMt19937Uniform against std::mt19937 + std::uniform_real_distribution
+    compiled here with g++ (the generator the reference's multistart uses)."""
+    import shutil
+    import subprocess
+    import tempfile
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = ("#include <cstdio>\n#include <random>\nint main(){for(int seed: {44, 7, 123456}){std::mt19937 g(seed);"
+           "std::uniform_real_distribution<> u(0.0,1.0);for(int i=0;i<40;++i)std::printf(\"%.17g\\n\",u(g));}}\n")
+    with tempfile.TemporaryDirectory() as d:
+        open(f"{d}/u.cpp", "w").write(src)
+        subprocess.run(["g++", "-O1", "-o", f"{d}/u", f"{d}/u.cpp"], check=True)
+        want = [float(x) for x in subprocess.run([f"{d}/u"], check=True, capture_output=True,
+                                                  text=True).stdout.split()]
+    got = []
+    for seed in (44, 7, 123456):
+        u = F.CxxMt19937Uniform(seed)
+        got += [u() for _ in range(40)]
+    assert got == want
+
+
+def test_multistart_starts_are_inside_the_limits_and_stratified():
+    m = L.Model.from_json(json.dumps(__import__("lcurve_re_amd.workloads", fromlist=["x"]).model_json(
+        vary=("q", "iangle", "r1", "r2", "t2"))))
+    names = [m.get_name(i) for i in range(m.nvary())]
+    p0 = np.array(m.get_param(), float)
+    st = F.multistart_starts(m, n=8, span=0.5, seed=42)
+    assert len(st) == 9 and np.array_equal(st[0], p0)
+    lims, rng = m.get_limit(), np.asarray(m.get_range(), float)
+    ii = names.index("iangle")
+    ilo, ihi = max(lims[ii][0], 5.0), min(lims[ii][1], 90.0)
+    for s, v in enumerate(st[1:]):
+        for j in range(len(v)):
+            assert lims[j][0] <= v[j] <= lims[j][1]
+            if j != ii:
+                assert abs(v[j] - p0[j]) <= 0.5 * rng[j] + 1e-15
+        frac = (v[ii] - ilo) / (ihi - ilo) * 8 - s
+        assert 0.2 <= frac <= 0.8
+    again = F.multistart_starts(m, n=8, span=0.5, seed=42)
+    assert all(np.array_equal(a, b) for a, b in zip(st, again))
