@@ -241,10 +241,10 @@ def _reflect(prop, lims):
 
 
 def error_mcmc(fb, modes, err_s2=None, log_prior=None, dsteps=None, steps=8000, min_steps=500,
-               mode_min_weight=0.005, seed=43):
+               mode_min_weight=0.005, seed=43, reanchor_rounds=2, reanchor_tol=0.5):
     """The LM program's error sampling over the posterior modes
-    (levmarq_solver.cpp:2090-2500, one round: no re-anchoring), on a
-    fit.FitBatch and the modes of fit.laplace_modes.
+    (levmarq_solver.cpp:2090-2500), on a fit.FitBatch and the modes of
+    fit.laplace_modes.
 
     Modes of weight >= mode_min_weight are sampled (the heaviest if none
     is), each by one chain of max(min_steps, steps w / w_sampled) kept
@@ -258,7 +258,11 @@ def error_mcmc(fb, modes, err_s2=None, log_prior=None, dsteps=None, steps=8000, 
     err_s2 by default chi2_LC / max(1, N - k) of the heaviest mode;
     Robbins-Monro during burn-in: every 50 steps s += (1 + batch)^-0.6
     (rate - 0.234), s in [log 1e-3, log 1e3].  The best state met
-    (largest -0.5 chi2 + lp) is the MAP.
+    (largest -0.5 chi2 + lp) is the MAP, starting from the heaviest mode's
+    optimum.  Re-anchoring: when a round raises the MAP's -chi2 + 2 lp by
+    more than reanchor_tol, the sampling restarts (samples cleared, the
+    chain that found the MAP starting from it), at most reanchor_rounds
+    times.
 
     MI355X-side change: the chains of all modes advance in lockstep, one
     batched engine call per step for every chain still running, and each
@@ -266,7 +270,8 @@ def error_mcmc(fb, modes, err_s2=None, log_prior=None, dsteps=None, steps=8000, 
     one generator shared by chains run one after another.  Returns dict
     with samples (kept states of all chains, mode order), per_mode (mode
     index, kept, accepted), acceptance, map_pars, map_chisq, map_lp, ok
-    (more than 100 samples and acceptance above 1%), calls.
+    (more than 100 samples and acceptance above 1%), calls, rounds,
+    reanchored and the gain of each round.
     """
     npar = fb.npar
     lims = fb.limits
@@ -295,61 +300,79 @@ def error_mcmc(fb, modes, err_s2=None, log_prior=None, dsteps=None, steps=8000, 
         chains.append({"mode": k, "p": np.array(m["pars"], float), "L": L, "nst": nst,
                        "nbr": max(300, nst // 5), "step": 0, "rng": np.random.default_rng(seed + k),
                        "s": 0.0, "batch": 0, "bacc": 0, "bn": 0, "kept": [], "acc": 0})
-    calc, status = fb._run([c["p"] for c in chains])
-    calls = 1
-    for k, c in enumerate(chains):
-        if status[k] != 0:
-            raise RuntimeError("a mode's optimum does not evaluate")
-        c["chi"] = float(np.sum(residuals_lc(data, calc[k]) ** 2))
-        c["lp"] = lp(c["p"])
     # the MAP starts at the heaviest mode's optimum (the adopted LM fit)
-    map_lp, map_pars, map_chisq = -0.5 * chains[0]["chi"] + chains[0]["lp"], chains[0]["p"].copy(), chains[0]["chi"]
+    calc, status = fb._run([chains[0]["p"]])
+    calls = 1
+    if status[0] != 0:
+        raise RuntimeError("the heaviest mode's optimum does not evaluate")
+    map_chisq = float(np.sum(residuals_lc(data, calc[0]) ** 2))
+    map_pars, map_lp, map_k = chains[0]["p"].copy(), -0.5 * map_chisq + lp(chains[0]["p"]), -1
+    gains, reanchored, rnd = [], False, 0
     while True:
-        live = [c for c in chains if c["step"] < c["nst"] + c["nbr"]]
-        if not live:
+        round_start_lp = map_lp
+        for k, c in enumerate(chains):
+            c.update(p=(map_pars if reanchored and k == map_k else modes[c["mode"]]["pars"]).copy(), step=0, s=0.0,
+                     batch=0, bacc=0, bn=0, kept=[], acc=0)
+        calc, status = fb._run([c["p"] for c in chains])
+        calls += 1
+        for k, c in enumerate(chains):
+            if status[k] != 0:
+                raise RuntimeError("a chain's starting point does not evaluate")
+            c["chi"] = float(np.sum(residuals_lc(data, calc[k]) ** 2))
+            c["lp"] = lp(c["p"])
+        while True:
+            live = [c for c in chains if c["step"] < c["nst"] + c["nbr"]]
+            if not live:
+                break
+            props, lps = [], []
+            for c in live:
+                z = c["rng"].standard_normal(npar)
+                sf = math.exp(c["s"])
+                prop = c["p"] + sf * prop_sd * (c["L"] @ z) if c["L"] is not None else c["p"] + sf * prop_sd * diag * z
+                props.append(_reflect(prop, lims))
+                lps.append(lp(props[-1]))
+            ev = [i for i, v in enumerate(lps) if v > -1e29]
+            res = [None] * len(live)
+            if ev:
+                calc, status = fb._run([props[i] for i in ev])
+                calls += 1
+                for j, i in enumerate(ev):
+                    if status[j] == 0:
+                        res[i] = float(np.sum(residuals_lc(data, calc[j]) ** 2))
+            for i, c in enumerate(live):
+                acc = False
+                if res[i] is not None:
+                    la = (-0.5 * (res[i] - c["chi"]) + (lps[i] - c["lp"])) / err_s2
+                    if la >= 0.0 or math.log(c["rng"].random()) < la:
+                        c["p"], c["chi"], c["lp"] = props[i], res[i], lps[i]
+                        acc = True
+                if acc and -0.5 * c["chi"] + c["lp"] > map_lp:
+                    map_lp, map_pars, map_chisq = -0.5 * c["chi"] + c["lp"], c["p"].copy(), c["chi"]
+                    map_k = chains.index(c)
+                if c["step"] < c["nbr"]:
+                    c["bn"] += 1
+                    c["bacc"] += acc
+                    if c["bn"] >= 50:
+                        g = (1.0 + c["batch"]) ** -0.6
+                        c["s"] = min(max(c["s"] + g * (c["bacc"] / c["bn"] - 0.234), math.log(1e-3)), math.log(1e3))
+                        c["bacc"] = c["bn"] = 0
+                        c["batch"] += 1
+                else:
+                    c["acc"] += acc
+                    c["kept"].append(c["p"].copy())
+                c["step"] += 1
+        gains.append(2.0 * (map_lp - round_start_lp))
+        if gains[-1] <= reanchor_tol or rnd >= reanchor_rounds:
             break
-        props, lps = [], []
-        for c in live:
-            z = c["rng"].standard_normal(npar)
-            sf = math.exp(c["s"])
-            prop = c["p"] + sf * prop_sd * (c["L"] @ z) if c["L"] is not None else c["p"] + sf * prop_sd * diag * z
-            props.append(_reflect(prop, lims))
-            lps.append(lp(props[-1]))
-        ev = [i for i, v in enumerate(lps) if v > -1e29]
-        res = [None] * len(live)
-        if ev:
-            calc, status = fb._run([props[i] for i in ev])
-            calls += 1
-            for j, i in enumerate(ev):
-                if status[j] == 0:
-                    res[i] = float(np.sum(residuals_lc(data, calc[j]) ** 2))
-        for i, c in enumerate(live):
-            acc = False
-            if res[i] is not None:
-                la = (-0.5 * (res[i] - c["chi"]) + (lps[i] - c["lp"])) / err_s2
-                if la >= 0.0 or math.log(c["rng"].random()) < la:
-                    c["p"], c["chi"], c["lp"] = props[i], res[i], lps[i]
-                    acc = True
-            if acc and -0.5 * c["chi"] + c["lp"] > map_lp:
-                map_lp, map_pars, map_chisq = -0.5 * c["chi"] + c["lp"], c["p"].copy(), c["chi"]
-            if c["step"] < c["nbr"]:
-                c["bn"] += 1
-                c["bacc"] += acc
-                if c["bn"] >= 50:
-                    g = (1.0 + c["batch"]) ** -0.6
-                    c["s"] = min(max(c["s"] + g * (c["bacc"] / c["bn"] - 0.234), math.log(1e-3)), math.log(1e3))
-                    c["bacc"] = c["bn"] = 0
-                    c["batch"] += 1
-            else:
-                c["acc"] += acc
-                c["kept"].append(c["p"].copy())
-            c["step"] += 1
+        reanchored = True  # the MAP's chain restarts there (adopt_map)
+        rnd += 1
     samples = np.array([v for c in chains for v in c["kept"]]).reshape(-1, npar)
     drawn = sum(len(c["kept"]) for c in chains)
     acc = sum(c["acc"] for c in chains) / drawn if drawn else 0.0
     return {"samples": samples, "per_mode": [(c["mode"], len(c["kept"]), c["acc"]) for c in chains],
             "acceptance": acc, "map_pars": map_pars, "map_chisq": map_chisq, "map_lp": map_lp,
-            "ok": len(samples) > 100 and acc > 0.01, "calls": calls, "err_s2": err_s2}
+            "ok": len(samples) > 100 and acc > 0.01, "calls": calls, "err_s2": err_s2, "rounds": rnd + 1,
+            "reanchored": reanchored, "gains": gains}
 
 
 def residuals_lc(data, calc):

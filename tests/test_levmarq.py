@@ -340,3 +340,36 @@ def test_multistart_starts_are_inside_the_limits_and_stratified():
         assert 0.2 <= frac <= 0.8
     again = F.multistart_starts(m, n=8, span=0.5, seed=42)
     assert all(np.array_equal(a, b) for a, b in zip(st, again))
+
+
+def test_error_mcmc_reanchors_on_a_better_state():
+    """levmarq_solver.cpp:2390-2470: a mode reported 5 sigma off the
+    optimum; the first round's chains find states far better than it
+    (gain > 0.5), so the sampling restarts from the MAP, and the second
+    round, started there, gains little and ends it.  The kept samples are
+    the second round's, of the right posterior."""
+    from lcurve_re_amd import mcmc as M
+    X = np.linspace(0.0, 1.0, 150)
+    A = np.stack([np.ones_like(X), X], axis=1)
+    rng = np.random.default_rng(5)
+    sig = 0.05 * np.ones_like(X)
+    y = A @ np.array([0.3, -0.7]) + sig * rng.standard_normal(len(X))
+    data = L.make_data(X, ferr=sig, flux=y, weight=1.0)
+    ev = lambda models, d, o: (np.array([A @ m.p for m in models]), np.zeros(len(models), int))
+    fb = F.FitBatch(ToyModel([0.3, -0.7], [(-10.0, 10.0)] * 2), data, ev, scale=False)
+    W = 1.0 / sig ** 2
+    Fm = A.T @ (A * W[:, None])
+    phat = np.linalg.solve(Fm, A.T @ (W * y))
+    chi_min = float(np.sum(W * (y - A @ phat) ** 2))
+    s2 = chi_min / (len(X) - 2)
+    C = s2 * np.linalg.inv(Fm)
+    off = phat + 5.0 * np.sqrt(np.diag(C))
+    chi_off = float(np.sum(W * (y - A @ off) ** 2))
+    mode = {"pars": off, "weight": 1.0, "cov": C, "chisq_lc": chi_min}
+    r = M.error_mcmc(fb, [mode], dsteps=[0.01, 0.01], steps=6000, seed=2)
+    assert r["reanchored"] and r["rounds"] == 2 and r["gains"][0] > 0.5 >= r["gains"][1]
+    assert r["map_chisq"] < chi_off and r["map_chisq"] - chi_min < 5.0
+    sd = np.sqrt(np.diag(C))
+    assert np.all(np.abs(r["samples"].mean(axis=0) - phat) < 0.2 * sd)
+    one = M.error_mcmc(fb, [mode], dsteps=[0.01, 0.01], steps=6000, seed=2, reanchor_rounds=0)
+    assert one["rounds"] == 1 and not one["reanchored"]
