@@ -522,7 +522,7 @@ class CxxMt19937Uniform:
         return r if r < 1.0 else math.nextafter(1.0, 0.0)
 
 
-def multistart_starts(model, pars=None, n=8, span=1.0, seed=42):
+def multistart_starts(model, pars=None, n=8, span=1.0, seed=42, constraints=None):
     """The multistart's starting vectors (levmarq_solver.cpp:1560-1606):
     the current parameters, then n draws; in each, every parameter but
     iangle uniform in [p - span range, p + span range] cut to its limits
@@ -530,8 +530,10 @@ def multistart_starts(model, pars=None, n=8, span=1.0, seed=42):
     stratified over [max(lo, 5), min(hi, 90)]: start s at fraction
     (s + 0.2 + 0.6 u) / n.  Uniforms from std::mt19937(seed + 2) in the
     reference's order (CxxMt19937Uniform), so the starts are the
-    reference's.  Not restated: the consistent-parameter solve for iangle
-    when K1, M1 and R1 priors are all given."""
+    reference's.  constraints (prior.Constraints) with K1, M1 and R1: q,
+    velocity_scale and r1 of each drawn start come from
+    prior.solve_consistent_params at its inclination, clamped to the
+    limits, where it has a solution."""
     p0 = np.array(model.get_param() if pars is None else pars, float)
     npar = len(p0)
     lims = [tuple(x) for x in model.get_limit()]
@@ -554,6 +556,15 @@ def multistart_starts(model, pars=None, n=8, span=1.0, seed=42):
         if ii >= 0:
             ilo, ihi = max(lims[ii][0], 5.0), min(lims[ii][1], 90.0)
             st[ii] = ilo + (s + 0.2 + 0.6 * u()) / n * (ihi - ilo)
+            if constraints is not None and constraints.K and constraints.M1 and constraints.R1:
+                from .prior import solve_consistent_params
+                sol = solve_consistent_params(st[ii], constraints.K[0], constraints.M1[0], constraints.R1[0],
+                                              constraints.P_days)
+                if sol is not None:
+                    for name, val in zip(("q", "velocity_scale", "r1"), sol):
+                        if name in names:
+                            j = names.index(name)
+                            st[j] = min(max(val, lims[j][0]), lims[j][1])
         starts.append(st)
     return starts
 

@@ -174,7 +174,8 @@ def model_prior(model, constraints):
 
 
 __all__ = ["Constraints", "constraints_from_config", "implied", "log_prior", "model_prior", "log_gauss",
-           "log_split_gauss", "parse_three", "LMPrior", "lm_derived", "lm_residual"]
+           "log_split_gauss", "parse_three", "LMPrior", "lm_derived", "lm_residual", "solve_consistent_params",
+           "consistent_start"]
 
 
 # ------------------------------------------------------------------
@@ -278,3 +279,70 @@ def _param_getter(model):
             v[n] = float(p[j])
         return v
     return get
+
+
+def solve_consistent_params(i_deg, K, M1, R1, P_days):
+    """PhysicalPrior::solve_consistent_params (physical_prior.h:283-318):
+    the q, velocity_scale and r1 for which an orbit at inclination i_deg
+    has K1 = K, M1 and R1 (solar units) -- q the root of (1 + q)^2 / q^3 =
+    2 pi G M1 sin^3 i / (K^3 P) by geometric bisection on [1e-4, 1e4] (200
+    steps or a ratio below 1 + 1e-12), vs = K (1 + q) / (q sin i), r1 = R1
+    / a.  None when sin i < 0.01 or r1 falls outside (0, 1)."""
+    sin_i = math.sin(i_deg * DEG2RAD)
+    if sin_i < 0.01:
+        return None
+    P_s = P_days * DAY2SEC
+    rhs = 2.0 * math.pi * G_MSUN * M1 * sin_i ** 3 / (K * K * K * P_s)
+    lo, hi = 1e-4, 1e4
+    for _ in range(200):
+        mid = math.sqrt(lo * hi)
+        if (1.0 + mid) * (1.0 + mid) / (mid * mid * mid) > rhs:
+            lo = mid
+        else:
+            hi = mid
+        if hi / lo < 1.0 + 1e-12:
+            break
+    q = math.sqrt(lo * hi)
+    vs = K * (1.0 + q) / (q * sin_i)
+    r1 = R1 * RSUN_KM / (vs * P_s / (2.0 * math.pi))
+    if r1 <= 0.0 or r1 >= 1.0:
+        return None
+    return q, vs, r1
+
+
+def consistent_start(model, pars, c):
+    """The LM's auto_consistent_init (levmarq_solver.cpp:580-640): with K1,
+    M1 and R1 constraints, when the starting K1 is more than 3 sigma from
+    the observed one, q, velocity_scale and r1 are replaced by
+    solve_consistent_params at the starting inclination (else at the
+    first of 85, 80, .., 30 degrees that has a solution, which then also
+    becomes the inclination), clamped to the limits.  Returns the new
+    vector (the old one when nothing applies)."""
+    p = np.array(pars, float)
+    if c is None or not (c.K and c.M1 and c.R1):
+        return p
+    names = [model.get_name(i) for i in range(model.nvary())]
+    lims = model.get_limit()
+    v = _param_getter(model)(p)
+    K_impl = v["velocity_scale"] * math.sin(v["iangle"] * DEG2RAD) * v["q"] / (1.0 + v["q"])
+    sig = c.K[1] if K_impl < c.K[0] else c.K[2]
+    if sig <= 0.0:
+        sig = max(c.K[1], c.K[2])
+    if sig <= 0.0:
+        sig = 0.1 * c.K[0]
+    if abs((K_impl - c.K[0]) / sig) <= 3.0:
+        return p
+    i_deg = v["iangle"]
+    sol = solve_consistent_params(i_deg, c.K[0], c.M1[0], c.R1[0], c.P_days)
+    if sol is None:
+        for i_try in np.arange(85.0, 29.0, -5.0):
+            sol = solve_consistent_params(float(i_try), c.K[0], c.M1[0], c.R1[0], c.P_days)
+            if sol is not None:
+                i_deg = float(i_try)
+                break
+    if sol is None:
+        return p
+    for name, val in zip(("iangle", "q", "velocity_scale", "r1"), (i_deg,) + sol):
+        if name in names:
+            p[names.index(name)] = val
+    return np.array([min(max(x, lims[j][0]), lims[j][1]) for j, x in enumerate(p)])

@@ -205,3 +205,43 @@ def test_lm_prior_from_a_configuration():
     assert r[0] == PR.lm_residual("T1", (15000.0, 300.0, 300.0), 9.0, *base)
     assert r[1] == PR.lm_residual("K", (150.0, 10.0, 20.0), 9.0, *base)
     assert len(PR.LMPrior(m, {"use_priors": False})) == 0
+
+
+# ------------------------------------------------- consistent starting point
+@pytest.mark.parametrize("i_deg,K,M1,R1,P", [(85.0, 120.0, 0.5, 0.2, 0.25), (60.0, 45.0, 1.2, 1.1, 2.0),
+                                             (30.0, 200.0, 0.6, 0.015, 0.1)])
+def test_consistent_params_reproduce_the_targets(i_deg, K, M1, R1, P):
+    """physical_prior.h:283-318: the forward quantities of the solved q,
+    velocity_scale and r1 (implied(), Kepler's law) are the targets."""
+    sol = PR.solve_consistent_params(i_deg, K, M1, R1, P)
+    assert sol is not None
+    q, vs, r1 = sol
+    d = PR.implied(i_deg, q, vs, r1, 0.1, P)
+    assert d["K1"] == pytest.approx(K, rel=1e-12)
+    assert d["M1"] == pytest.approx(M1, rel=1e-10)  # q bisected to a ratio of 1e-12
+    assert d["R1"] == pytest.approx(R1, rel=1e-12)
+
+
+def test_consistent_params_fail_where_the_reference_does():
+    assert PR.solve_consistent_params(0.5, 100.0, 1.0, 1.0, 1.0) is None  # sin i < 0.01
+    assert PR.solve_consistent_params(85.0, 100.0, 1.0, 1e4, 1.0) is None  # r1 >= 1
+
+
+def test_consistent_start_and_multistart():
+    import json
+
+    import lcurve_re_amd as L
+    from lcurve_re_amd import fit as F
+    from lcurve_re_amd import workloads as W
+    m = L.Model.from_json(json.dumps(W.model_json(vary=("q", "iangle", "r1", "velocity_scale"))))
+    names = [m.get_name(i) for i in range(m.nvary())]
+    c = PR.Constraints(P_days=0.25, K=(90.0, 2.0, 2.0), M1=(0.47, 0.02, 0.02), R1=(0.2, 0.01, 0.01))
+    p = PR.consistent_start(m, m.get_param(), c)
+    v = dict(zip(names, p))
+    d = PR.implied(v["iangle"], v["q"], v["velocity_scale"], v["r1"], 0.1, 0.25)
+    assert d["K1"] == pytest.approx(90.0, rel=1e-12) and d["M1"] == pytest.approx(0.47, rel=1e-10)
+    assert np.array_equal(PR.consistent_start(m, p, c), p)  # within 3 sigma: unchanged
+    for st in F.multistart_starts(m, pars=p, n=6, span=0.1, seed=1, constraints=c)[1:]:
+        v = dict(zip(names, st))
+        d = PR.implied(v["iangle"], v["q"], v["velocity_scale"], v["r1"], 0.1, 0.25)
+        assert d["K1"] == pytest.approx(90.0, rel=1e-12) and d["R1"] == pytest.approx(0.2, rel=1e-12)
