@@ -609,8 +609,18 @@ def levmarq_multistart(fb, starts, **kw):
     unfinished fit go to the engine in one batched call.  Each fit is
     exactly the one levmarq() runs from its start.  Returns the results
     (levmarq's dicts, plus "start") sorted by chi^2, and the number of
-    batched calls.  laplace_modes() groups them into modes."""
-    res, calls = run_lockstep(fb, [levmarq_requests(fb, np.asarray(s0, float), **kw) for s0 in starts])
+    batched calls; a run whose evaluation failed has ok False (and chisq
+    inf).  laplace_modes() groups them into modes."""
+    def guarded(s0):
+        # a start whose evaluation fails is a failed run, not the end of the
+        # others (LMRun.ok, levmarq_solver.cpp:1052-1062)
+        try:
+            r = yield from levmarq_requests(fb, np.asarray(s0, float), **kw)
+            return dict(r, ok=True)
+        except RuntimeError as e:
+            return {"ok": False, "error": str(e), "pars": np.asarray(s0, float), "chisq": math.inf}
+
+    res, calls = run_lockstep(fb, [guarded(s0) for s0 in starts])
     out = [dict(r, start=k) for k, r in enumerate(res)]
     out.sort(key=lambda r: r["chisq"])
     return out, calls
@@ -620,7 +630,8 @@ def laplace_modes(fb, runs, dsteps, prior=None, log_prior=None, tol_dsteps=5.0, 
     """Groups LM optima into modes and weights each by its Laplace
     posterior mass (levmarq_solver.cpp:1640-1790).
 
-    runs: levmarq results (levmarq_multistart's).  Sorted by |r|^2, a run
+    runs: levmarq results (levmarq_multistart's; failed runs are left
+    out).  Sorted by |r|^2, a run
     joins the first mode whose representative is within max(tol_dsteps
     |dstep_j|, tol_frac (|a_j| + |b_j|) / 2, 1e-12) in every parameter,
     else it opens a mode.  With s2 = max(chi2_LC(best) / max(1, N - k),
@@ -637,7 +648,7 @@ def laplace_modes(fb, runs, dsteps, prior=None, log_prior=None, tol_dsteps=5.0, 
     with pars, run (index into runs), n_runs, weight, lpost, logdet, cov
     (None when J^T J is not positive definite), chisq_lc; and calls.
     """
-    order = sorted(range(len(runs)), key=lambda k: runs[k]["chisq"])
+    order = sorted((k for k in range(len(runs)) if runs[k].get("ok", True)), key=lambda k: runs[k]["chisq"])
     if not order:
         raise ValueError("no LM run to group")
     dsteps = np.abs(np.asarray(dsteps, float))

@@ -373,3 +373,22 @@ def test_error_mcmc_reanchors_on_a_better_state():
     assert np.all(np.abs(r["samples"].mean(axis=0) - phat) < 0.2 * sd)
     one = M.error_mcmc(fb, [mode], dsteps=[0.01, 0.01], steps=6000, seed=2, reanchor_rounds=0)
     assert one["rounds"] == 1 and not one["reanchored"]
+
+
+def test_multistart_survives_starts_that_do_not_evaluate():
+    """A start whose evaluation fails is a failed run (ok False) and the
+    others go on; laplace_modes leaves it out."""
+    data, lims, *_ = toy_problem()
+
+    def ev(models, d, opts):
+        calc, st = toy_eval(models, d, opts)
+        return calc, np.array([1 if m.p[3] > 4.0 else 0 for m in models])
+
+    fb = F.FitBatch(ToyModel([1.0, 0.5, 0.0, 0.0], lims), data, ev, scale=False)
+    starts = [[1.0, 0.5, 0.0, 0.0], [1.0, 0.5, 0.0, 4.5], [2.2, 1.2, 0.3, 0.4]]
+    runs, _ = F.levmarq_multistart(fb, starts, max_iter=50)
+    bad = [r for r in runs if not r["ok"]]
+    assert len(bad) == 1 and bad[0]["start"] == 1 and runs[-1] is bad[0]
+    assert all(r["ok"] for r in runs[:2])
+    modes, _ = F.laplace_modes(fb, runs, dsteps=[1e-3] * 4)
+    assert sum(m["n_runs"] for m in modes) == 2
