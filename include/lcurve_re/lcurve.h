@@ -27,10 +27,18 @@
 //   * Data::rasc does not print the number of lines read;
 //   * every evaluation runs on the GPU through a thread_local engine context
 //     (device from LCURVE_AMD_DEVICE, default 0).  There is no CPU path: a
-//     missing GPU or library is an Lcurve_Error.
+//     missing GPU or library is an Lcurve_Error;
+//   * light_curve_comp / _fast compute each face in FP64 (the reference's
+//     CPU arithmetic, comp_light_flat.cpp:81-168) unless the caller asks for
+//     FP32 per face with FP64 accumulation, the arithmetic of the
+//     reference's CUDA flux path: Lcurve::set_precision(LC_PREC_MIXED) for
+//     the calling thread, or the environment variable LCURVE_AMD_PRECISION
+//     (or the reference's LCURVE_CUDA_PRECISION, comp_light_cuda.cu:523-533)
+//     set to float / fp32 / 32 / mixed.  double / fp64 / 64 select FP64.
 #ifndef LCURVE_RE_LCURVE_H
 #define LCURVE_RE_LCURVE_H
 
+#include <cctype>
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
@@ -81,6 +89,31 @@ inline lc_ctx *thread_ctx() {
     }
     return holder.ctx;
 }
+
+// Precision of the per-face arithmetic for light_curve_comp(_fast):
+// LC_PREC_FP64 unless LCURVE_AMD_PRECISION, else LCURVE_CUDA_PRECISION,
+// names FP32 (case-insensitive).  Read once per process.
+inline int env_precision() {
+    static const int value = [] {
+        const char *env = std::getenv("LCURVE_AMD_PRECISION");
+        if (!env) env = std::getenv("LCURVE_CUDA_PRECISION");
+        if (!env) return int(LC_PREC_FP64);
+        std::string mode(env);
+        for (char &c : mode) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+        if (mode == "float" || mode == "fp32" || mode == "32" || mode == "single" || mode == "mixed")
+            return int(LC_PREC_MIXED);
+        return int(LC_PREC_FP64);
+    }();
+    return value;
+}
+
+// Per-thread choice made with Lcurve::set_precision (-1: none made).
+inline int &thread_precision() {
+    thread_local int p = -1;
+    return p;
+}
+
+inline int precision() { return thread_precision() >= 0 ? thread_precision() : env_precision(); }
 
 template <class Vec> std::vector<double> to_vector(const Vec &v) {
     std::vector<double> out(static_cast<size_t>(v.size()));
@@ -165,6 +198,10 @@ struct Model : lc_model {
         return n;
     }
 
+    // Model::wrasc (model.cpp:1812-1818): the parameter listing of
+    // operator<< written to a file.
+    inline void wrasc(const std::string &file) const;
+
 private:
     std::vector<double> get(lc_status (*fn)(const lc_model *, double *, size_t)) const {
         const size_t n = static_cast<size_t>(nvary());
@@ -174,6 +211,138 @@ private:
         return out;
     }
 };
+
+// Pparam output (model.cpp:1664-1670): value, range and dstep in fixed
+// notation, then vary and defined.  The stream keeps the last precision
+// and the fixed flag, as the reference's does.
+inline std::ostream &print_pparam(std::ostream &s, const lc_pparam &p) {
+    s << std::setw(17) << std::setprecision(10) << std::fixed << p.value << " " << std::setw(8)
+      << std::setprecision(4) << std::fixed << p.range << " " << std::setw(8) << std::setprecision(4) << std::fixed
+      << p.dstep << " " << (p.vary ? 1 : 0) << " " << (p.defined ? 1 : 0);
+    return s;
+}
+
+// Model output (model.cpp:1673-1806): one "name = value" line per field in
+// the reference's order and blank-line grouping.
+inline std::ostream &operator<<(std::ostream &s, const Model &m) {
+    const auto P = [&s](const char *label, const lc_pparam &p) -> std::ostream & {
+        s << label;
+        return print_pparam(s, p);
+    };
+    P("q              = ", m.q) << "\n";
+    P("iangle         = ", m.iangle) << "\n";
+    P("r1             = ", m.r1) << "\n";
+    P("r2             = ", m.r2) << "\n";
+    P("cphi3          = ", m.cphi3) << "\n";
+    P("cphi4          = ", m.cphi4) << "\n";
+    P("spin1          = ", m.spin1) << "\n";
+    P("spin2          = ", m.spin2) << "\n";
+    P("t1             = ", m.t1) << "\n";
+    P("t2             = ", m.t2) << "\n";
+    P("ldc1_1         = ", m.ldc1_1) << "\n";
+    P("ldc1_2         = ", m.ldc1_2) << "\n";
+    P("ldc1_3         = ", m.ldc1_3) << "\n";
+    P("ldc1_4         = ", m.ldc1_4) << "\n";
+    P("ldc2_1         = ", m.ldc2_1) << "\n";
+    P("ldc2_2         = ", m.ldc2_2) << "\n";
+    P("ldc2_3         = ", m.ldc2_3) << "\n";
+    P("ldc2_4         = ", m.ldc2_4) << "\n";
+    P("velocity_scale = ", m.velocity_scale) << "\n";
+    P("beam_factor1   = ", m.beam_factor1) << "\n";
+    P("beam_factor2   = ", m.beam_factor2) << "\n\n";
+    P("t0             = ", m.t0) << "\n";
+    P("period         = ", m.period) << "\n";
+    P("pdot           = ", m.pdot) << "\n";
+    P("deltat         = ", m.deltat) << "\n";
+    P("gravity_dark1  = ", m.gravity_dark1) << "\n";
+    P("gravity_dark2  = ", m.gravity_dark2) << "\n";
+    P("absorb         = ", m.absorb) << "\n";
+    P("slope          = ", m.slope) << "\n\n";
+    P("quad           = ", m.quad) << "\n\n";
+    P("cube           = ", m.cube) << "\n\n";
+    P("third          = ", m.third) << "\n\n";
+    P("rdisc1         = ", m.rdisc1) << "\n";
+    P("rdisc2         = ", m.rdisc2) << "\n";
+    P("height_disc    = ", m.height_disc) << "\n";
+    P("beta_disc      = ", m.beta_disc) << "\n";
+    P("temp_disc      = ", m.temp_disc) << "\n";
+    P("texp_disc      = ", m.texp_disc) << "\n";
+    P("lin_limb_disc  = ", m.lin_limb_disc) << "\n";
+    P("quad_limb_disc = ", m.quad_limb_disc) << "\n";
+    P("temp_edge      = ", m.temp_edge) << "\n";
+    P("absorb_edge    = ", m.absorb_edge) << "\n\n";
+    P("radius_spot    = ", m.radius_spot) << "\n";
+    P("length_spot    = ", m.length_spot) << "\n";
+    P("height_spot    = ", m.height_spot) << "\n";
+    P("expon_spot     = ", m.expon_spot) << "\n";
+    P("epow_spot      = ", m.epow_spot) << "\n";
+    P("angle_spot     = ", m.angle_spot) << "\n";
+    P("yaw_spot       = ", m.yaw_spot) << "\n";
+    P("temp_spot      = ", m.temp_spot) << "\n";
+    P("tilt_spot      = ", m.tilt_spot) << "\n";
+    P("cfrac_spot     = ", m.cfrac_spot) << "\n\n";
+    const lc_pparam *sp[5][4] = {{&m.stsp11_long, &m.stsp11_lat, &m.stsp11_fwhm, &m.stsp11_tcen},
+                                 {&m.stsp12_long, &m.stsp12_lat, &m.stsp12_fwhm, &m.stsp12_tcen},
+                                 {&m.stsp13_long, &m.stsp13_lat, &m.stsp13_fwhm, &m.stsp13_tcen},
+                                 {&m.stsp21_long, &m.stsp21_lat, &m.stsp21_fwhm, &m.stsp21_tcen},
+                                 {&m.stsp22_long, &m.stsp22_lat, &m.stsp22_fwhm, &m.stsp22_tcen}};
+    const char *sn[5] = {"11", "12", "13", "21", "22"};
+    for (int k = 0; k < 5; k++) {
+        s << "stsp" << sn[k] << "_long    = ";
+        print_pparam(s, *sp[k][0]) << "\n";
+        s << "stsp" << sn[k] << "_lat     = ";
+        print_pparam(s, *sp[k][1]) << "\n";
+        s << "stsp" << sn[k] << "_fwhm    = ";
+        print_pparam(s, *sp[k][2]) << "\n";
+        s << "stsp" << sn[k] << "_tcen    = ";
+        print_pparam(s, *sp[k][3]) << "\n\n";
+    }
+    P("uesp_long1     = ", m.uesp_long1) << "\n";
+    P("uesp_long2     = ", m.uesp_long2) << "\n";
+    P("uesp_lathw     = ", m.uesp_lathw) << "\n";
+    P("uesp_taper     = ", m.uesp_taper) << "\n";
+    P("uesp_temp      = ", m.uesp_temp) << "\n\n";
+    s << "delta_phase    = " << m.delta_phase << "\n";
+    s << "nlat1f         = " << m.nlat1f << "\n";
+    s << "nlat2f         = " << m.nlat2f << "\n";
+    s << "nlat1c         = " << m.nlat1c << "\n";
+    s << "nlat2c         = " << m.nlat2c << "\n";
+    s << "npole          = " << (m.npole ? 1 : 0) << "\n";
+    s << "nlatfill       = " << m.nlatfill << "\n";
+    s << "nlngfill       = " << m.nlngfill << "\n";
+    s << "lfudge         = " << m.lfudge << "\n";
+    s << "llo            = " << m.llo << "\n";
+    s << "lhi            = " << m.lhi << "\n";
+    s << "phase1         = " << m.phase1 << "\n";
+    s << "phase2         = " << m.phase2 << "\n";
+    s << "wavelength     = " << m.wavelength << "\n";
+    s << "roche1         = " << (m.roche1 ? 1 : 0) << "\n";
+    s << "roche2         = " << (m.roche2 ? 1 : 0) << "\n";
+    s << "eclipse1       = " << (m.eclipse1 ? 1 : 0) << "\n";
+    s << "eclipse2       = " << (m.eclipse2 ? 1 : 0) << "\n";
+    s << "glens1         = " << (m.glens1 ? 1 : 0) << "\n";
+    s << "use_radii      = " << (m.use_radii ? 1 : 0) << "\n";
+    s << "tperiod        = " << m.tperiod << "\n";
+    s << "gdark_bolom1   = " << (m.gdark_bolom1 ? 1 : 0) << "\n";
+    s << "gdark_bolom2   = " << (m.gdark_bolom2 ? 1 : 0) << "\n";
+    s << "mucrit1        = " << m.mucrit1 << "\n";
+    s << "mucrit2        = " << m.mucrit2 << "\n";
+    s << (m.limb1 == LC_LDC_CLARET ? "limb1          = Claret\n" : "limb1          = Poly\n");
+    s << (m.limb2 == LC_LDC_CLARET ? "limb2          = Claret\n" : "limb2          = Poly\n");
+    s << "mirror         = " << (m.mirror ? 1 : 0) << "\n";
+    s << "add_disc       = " << (m.add_disc ? 1 : 0) << "\n";
+    s << "nrad           = " << m.nrad << "\n";
+    s << "opaque         = " << (m.opaque ? 1 : 0) << "\n";
+    s << "add_spot       = " << (m.add_spot ? 1 : 0) << "\n";
+    s << "nspot          = " << m.nspot << "\n";
+    s << "iscale         = " << (m.iscale ? 1 : 0) << "\n";
+    return s;
+}
+
+inline void Model::wrasc(const std::string &file) const {
+    std::ofstream fout(file.c_str());
+    fout << *this;
+}
 
 // ------------------------------------------------------------- Datum, Data
 struct Datum : lc_datum {
@@ -290,6 +459,18 @@ inline void eval(const Model &mdl, const Data &data, bool scale, bool rdata, boo
 
 }  // namespace detail
 
+// Per-face precision of light_curve_comp(_fast) on the calling thread:
+// LC_PREC_FP64 (the reference's CPU arithmetic) or LC_PREC_MIXED (FP32 per
+// face, FP64 accumulation).  A negative value returns to the environment's
+// choice (FP64 when LCURVE_AMD_PRECISION / LCURVE_CUDA_PRECISION are unset).
+inline void set_precision(int precision) {
+    if (precision != LC_PREC_FP64 && precision != LC_PREC_MIXED && precision >= 0)
+        detail::fail("Lcurve::set_precision", "precision must be LC_PREC_FP64 or LC_PREC_MIXED");
+    detail::thread_precision() = precision < 0 ? -1 : precision;
+}
+
+inline int get_precision() { return detail::precision(); }
+
 // Lcurve::light_curve_comp (lcurve.h:669-674): the model light curve at the
 // times of `data`, optionally rescaled (scale) and compared (rdata).
 inline void light_curve_comp(const Model &model, const Data &data, bool scale, bool rdata, bool info,
@@ -297,7 +478,7 @@ inline void light_curve_comp(const Model &model, const Data &data, bool scale, b
                              double &wnok, double &logg1, double &logg2, double &rv1, double &rv2,
                              bool diagnostics = true) {
     detail::eval(model, data, scale, rdata, info, sfac, calc, wdwarf, chisq, wnok, logg1, logg2, rv1, rv2, 0,
-                 diagnostics, LC_PREC_MIXED, "Lcurve::light_curve_comp");
+                 diagnostics, detail::precision(), "Lcurve::light_curve_comp");
 }
 
 // Lcurve::light_curve_comp_fast (lcurve.h:474-483): evaluates at most
@@ -307,7 +488,7 @@ inline void light_curve_comp_fast(const Model &mdl, const Data &data, bool scale
                                   double &chisq, double &wnok, double &logg1, double &logg2, double &rv1,
                                   double &rv2, int max_model_points = 500, bool diagnostics = true) {
     detail::eval(mdl, data, scale, rdata, info, sfac, calc, wdwarf, chisq, wnok, logg1, logg2, rv1, rv2,
-                 max_model_points, diagnostics, LC_PREC_MIXED, "Lcurve::light_curve_comp_fast");
+                 max_model_points, diagnostics, detail::precision(), "Lcurve::light_curve_comp_fast");
 }
 
 // Results of light_curve_comp_batch: calc is models.size() x data.size(),
@@ -324,9 +505,11 @@ struct BatchResult {
 // Many parameter vectors (MCMC proposals, Jacobian columns) against one
 // data set in one launch sequence.  Instances fail individually (status);
 // an Lcurve_Error is thrown only when the batch itself could not run.
+// precision < 0 takes the thread's choice (get_precision()).
 inline BatchResult light_curve_comp_batch(const std::vector<Model> &models, const Data &data, bool scale, bool rdata,
                                           double sfac = 1.0, int max_model_points = 0, bool diagnostics = false,
-                                          int precision = LC_PREC_MIXED) {
+                                          int precision = -1) {
+    if (precision < 0) precision = detail::precision();
     if (data.empty()) detail::fail("Lcurve::light_curve_comp_batch", "no data points");
     BatchResult r;
     r.n = data.size();

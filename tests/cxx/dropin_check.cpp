@@ -7,7 +7,17 @@
 //   dropin_check eval  <config.json> <in.dat> <out.txt>
 //                                               light_curve_comp, _fast and _batch
 //   dropin_check error <config.json> <in.dat>   expects an Lcurve_Error
+//   dropin_check print <config.json> <out.txt>  Model::wrasc
+//   dropin_check jac   <config.json> <in.dat> <out.txt> <j> [fp64|mixed]
+//                                               central-difference column j of
+//                                               light_curve_comp, as the LM
+//                                               solver forms it, in the
+//                                               precision of the environment
+//                                               or the one named
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <sstream>
 #include <string>
@@ -121,6 +131,42 @@ int run_error(const char *cfg, const char *dat) {
     return 1;
 }
 
+int run_print(const char *cfg, const char *out) {
+    Lcurve::Model m(slurp(cfg));
+    m.wrasc(out);
+    return 0;
+}
+
+// h = eps^(1/4) |p| + 1e-10, at most half the parameter's range
+// (levmarq_solver.cpp:766-778); column = (c(p + h) - c(p - h)) / 2h.
+int run_jac(const char *cfg, const char *dat, const char *out, int j, const char *prec) {
+    if (prec) Lcurve::set_precision(std::string(prec) == "mixed" ? LC_PREC_MIXED : LC_PREC_FP64);
+    Lcurve::Model m(slurp(cfg));
+    Lcurve::Data data(dat);
+    const std::vector<double> p = m.get_param();
+    const auto lim = m.get_limit();
+    double h = std::sqrt(std::sqrt(2.220446049250313e-16)) * std::fabs(p[j]) + 1e-10;
+    const double span = lim[j].second - lim[j].first;
+    if (std::isfinite(span) && span > 0.0) h = std::min(h, 0.5 * span);
+    std::vector<double> sfac(5, 1.0), cp, cm;
+    double wdwarf, chisq, wnok, logg1, logg2, rv1, rv2;
+    std::vector<double> v = p;
+    v[j] = p[j] + h;
+    m.set_param(v);
+    Lcurve::light_curve_comp(m, data, false, false, false, sfac, cp, wdwarf, chisq, wnok, logg1, logg2, rv1, rv2,
+                             false);
+    v[j] = p[j] - h;
+    m.set_param(v);
+    Lcurve::light_curve_comp(m, data, false, false, false, sfac, cm, wdwarf, chisq, wnok, logg1, logg2, rv1, rv2,
+                             false);
+    FILE *f = std::fopen(out, "w");
+    std::fprintf(f, "precision %d\nh %.17g\ncol", Lcurve::get_precision(), h);
+    for (size_t k = 0; k < cp.size(); k++) std::fprintf(f, " %.17g", (cp[k] - cm[k]) / (2.0 * h));
+    std::fprintf(f, "\n");
+    std::fclose(f);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -130,10 +176,13 @@ int main(int argc, char **argv) {
         if (mode == "data" && argc == 4) return run_data(argv[2], argv[3]);
         if (mode == "eval" && argc == 5) return run_eval(argv[2], argv[3], argv[4]);
         if (mode == "error" && argc == 4) return run_error(argv[2], argv[3]);
+        if (mode == "print" && argc == 4) return run_print(argv[2], argv[3]);
+        if (mode == "jac" && (argc == 6 || argc == 7))
+            return run_jac(argv[2], argv[3], argv[4], std::atoi(argv[5]), argc == 7 ? argv[6] : nullptr);
     } catch (const std::exception &e) {
         std::printf("exception: %s\n", e.what());
         return 3;
     }
-    std::fprintf(stderr, "usage: dropin_check model|data|eval|error ...\n");
+    std::fprintf(stderr, "usage: dropin_check model|data|eval|error|print|jac ...\n");
     return 2;
 }
