@@ -536,7 +536,10 @@ void fill_physics(const lc_model &m, GridDev &G) {
     G.VFAC = vs / (lcx::C_SI / 1.e3);
     G.XCOFM = q / (1.0 + q);
     G.beam = star == 0 ? m.beam_factor1.value : m.beam_factor2.value;
-    G.spin_beam = star == 0 ? m.spin1.value : m.spin2.value;
+    // With iscale each star's curve is a separate component summed by
+    // comp_star1_flat / comp_star2_flat, which use spin = 1 in the beaming
+    // term (comp_light_flat.cpp:565-568, 596-599, light_curve_comp.cpp:389-413)
+    G.spin_beam = m.iscale ? 1.0 : (star == 0 ? m.spin1.value : m.spin2.value);
     const lc_pparam *l = star == 0 ? &m.ldc1_1 : &m.ldc2_1;
     const lc_pparam *ls[4] = {star == 0 ? &m.ldc1_1 : &m.ldc2_1, star == 0 ? &m.ldc1_2 : &m.ldc2_2,
                               star == 0 ? &m.ldc1_3 : &m.ldc2_3, star == 0 ? &m.ldc1_4 : &m.ldc2_4};

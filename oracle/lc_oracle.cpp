@@ -2010,10 +2010,14 @@ void light_curve_comp(const lc_model &mdl, const lc_datum *data, size_t n, const
         }
     }
 
-    ElemArgs a1 = {PRIMARY, mdl.beam_factor1.value != 0.0, false, mdl.beam_factor1.value, mdl.spin1.value,
-                   VFAC, XCOFM, 0., ldc_unpack(ldc1)};
+    // iscale components come from comp_star1_flat / comp_star2_flat, which
+    // use spin = 1 in the beaming term (comp_light_flat.cpp:565-568, 596-599;
+    // light_curve_comp.cpp:389-413); the plain path uses the stars' spins
+    // (light_curve_comp.cpp:462-475)
+    ElemArgs a1 = {PRIMARY, mdl.beam_factor1.value != 0.0, false, mdl.beam_factor1.value,
+                   mdl.iscale ? 1.0 : mdl.spin1.value, VFAC, XCOFM, 0., ldc_unpack(ldc1)};
     ElemArgs a2 = {SECONDARY, mdl.beam_factor2.value != 0.0, mdl.glens1 != 0, mdl.beam_factor2.value,
-                   mdl.spin2.value, VFAC, XCOFM, rlens1, ldc_unpack(ldc2)};
+                   mdl.iscale ? 1.0 : mdl.spin2.value, VFAC, XCOFM, rlens1, ldc_unpack(ldc2)};
     std::vector<double> o1f(pb1f.idx.size()), o1c(pb1c.idx.size()), o2f(pb2f.idx.size()), o2c(pb2c.idx.size());
     if (!o1f.empty()) flat_sum_multi(flat_view(fg1f), pb1f.view(), a1, o1f.data());
     if (!o1c.empty()) flat_sum_multi(flat_view(fg1c), pb1c.view(), a1, o1c.data());

@@ -103,6 +103,7 @@ def ref():
     if not have_ref():
         raise RuntimeError("oracle/_ref/libref.so not built")
     lib = C.CDLL(REF_LIB)
+    abi = _abi()
     D, I, Z = C.c_double, C.c_int, C.c_size_t
     sig = {
         "ref_last_error": (C.c_char_p, []),
@@ -125,6 +126,12 @@ def ref():
                               D, I, D, D, C.c_float, I, D, D, D]),
         "ref_re_scale": (D, [Z, _DP, _DP, _DP, _DP, _DP, _DP, _DP]),
         "ref_comp_radius": (D, [I, Z, _DP, _DP, _DP, _DP, _DP, _DP, _FP]),
+        "ref_set_star_grid": (I, [C.POINTER(abi.lc_model), I, I, C.POINTER(abi.lc_star_faces)]),
+        "ref_set_star_continuum": (I, [C.POINTER(abi.lc_model), C.POINTER(abi.lc_star_faces),
+                                       C.POINTER(abi.lc_star_faces)]),
+        "ref_comp_gravity": (D, [C.POINTER(abi.lc_model), I, C.POINTER(abi.lc_star_faces)]),
+        "ref_light_curve_comp": (I, [C.POINTER(abi.lc_model), C.c_void_p, Z, C.POINTER(abi.lc_eval_opts), _DP,
+                                     C.POINTER(abi.lc_scalars)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -170,9 +177,13 @@ class Faces:
         return v
 
     def copy(self):
-        f = Faces(self.n)
+        return self.truncated(self.n)
+
+    def truncated(self, n):
+        """A copy holding the first n faces."""
+        f = Faces(n)
         for k in ("px", "py", "pz", "dx", "dy", "dz", "ingress", "egress", "area", "gravity", "flux", "necl"):
-            setattr(f, k, getattr(self, k).copy())
+            setattr(f, k, getattr(self, k)[:n].copy())
         return f
 
 
@@ -274,6 +285,42 @@ def light_curve(model, data, scale=False, rdata=True, diagnostics=False, max_mod
     o = abi.lc_eval_opts(int(scale), int(rdata), 0, int(diagnostics), int(max_model_points), 0)
     st = lib.orc_light_curve_comp(C.byref(model), data, n, C.byref(o), dptr(calc), C.byref(io))
     return st, calc, io
+
+
+def ref_light_curve(model, data, scale=False, rdata=True, diagnostics=False, max_model_points=0, sfac=1.0):
+    """The compiled reference's light_curve_comp(_fast) (oracle/_ref);
+    returns (status, calc, scalars) like light_curve()."""
+    abi = _abi()
+    lib = ref()
+    n = len(data)
+    calc = np.zeros(n, np.float64)
+    io = abi.lc_scalars()
+    for k in range(5):
+        io.sfac[k] = sfac
+    o = abi.lc_eval_opts(int(scale), int(rdata), 0, int(diagnostics), int(max_model_points), 1)
+    st = lib.ref_light_curve_comp(C.byref(model), data, n, C.byref(o), dptr(calc), C.byref(io))
+    return st, calc, io
+
+
+def ref_star_grid(model, star, fine):
+    """The compiled reference's set_star_grid as Faces."""
+    n = C.c_size_t()
+    st = oracle().orc_star_grid_size(C.byref(model), star, int(fine), C.byref(n))
+    if st:
+        raise RuntimeError(f"orc_star_grid_size: {st}")
+    f = Faces(n.value + 4)   # room for the faces past numface() some nlat lay out
+    v = f.view()
+    st = ref().ref_set_star_grid(C.byref(model), star, int(fine), C.byref(v))
+    if st:
+        raise RuntimeError(f"ref_set_star_grid: {st} {ref().ref_last_error().decode()}")
+    return f.truncated(v.n)
+
+
+def ref_star_continuum(model, f1, f2):
+    v1, v2 = f1.view(), f2.view()
+    st = ref().ref_set_star_continuum(C.byref(model), C.byref(v1), C.byref(v2))
+    if st:
+        raise RuntimeError(f"ref_set_star_continuum: {st} {ref().ref_last_error().decode()}")
 
 
 def star_curves(model, data):

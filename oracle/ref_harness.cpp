@@ -5,14 +5,27 @@
  * tests/golden/make_golden.py to write golden vectors and by the tests that
  * check the oracle against the reference where the library is built.
  *
- * Lcurve::Model cannot be constructed here (model.cpp is not built, see the
- * Makefile), so only functions that take explicit arguments are wrapped:
- * Roche geometry, add_faces, the flux sums, the Point-path sums, re_scale.
+ * The reference's JSON constructor of Lcurve::Model lives in model.cpp,
+ * which is not built (see the Makefile).  Model itself is a plain struct of
+ * Pparam and scalar members: trivially copyable with a trivial destructor,
+ * so it is an implicit-lifetime type and storage obtained from malloc holds
+ * a Model object (C++20 [intro.object]/10-13; the static_asserts below check
+ * the two properties).  ref_model_new fills such an object member by member
+ * from an lc_model, the field-for-field layout that this repository's own
+ * parser produces, so the functions taking a Model -- set_star_grid,
+ * set_star_continuum, light_curve_comp(_fast), comp_gravity -- run here as
+ * compiled from the reference sources.  The other wrappers take explicit
+ * arguments: Roche geometry, add_faces, the flux sums, the Point-path sums,
+ * re_scale, comp_radius.
  */
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <string>
+#include <type_traits>
 #include <vector>
+
+#include "../include/lcurve_amd.h"
 
 #include "lcurve.h"
 #include "../lroche_base/roche.h"
@@ -232,6 +245,159 @@ double ref_comp_radius(int star, size_t n, const double *px, const double *py, c
         v[i] = Lcurve::Point(Subs::Vec3(px[i], py[i], pz[i]), Subs::Vec3(dx[i], dy[i], dz[i]), area[i], 1.,
                              Lcurve::Point::etype());
     return star == 0 ? Lcurve::comp_radius1(v) : Lcurve::comp_radius2(v);
+}
+
+/* ---------------------------------------------------------------- Model */
+static_assert(std::is_trivially_copyable<Lcurve::Model>::value, "Model must be trivially copyable");
+static_assert(std::is_trivially_destructible<Lcurve::Model>::value, "Model must be trivially destructible");
+
+#define REF_PPARAMS(X)                                                                                            \
+    X(q) X(iangle) X(r1) X(r2) X(cphi3) X(cphi4) X(spin1) X(spin2) X(t1) X(t2) X(ldc1_1) X(ldc1_2) X(ldc1_3)      \
+    X(ldc1_4) X(ldc2_1) X(ldc2_2) X(ldc2_3) X(ldc2_4) X(velocity_scale) X(beam_factor1) X(beam_factor2) X(t0)    \
+    X(period) X(pdot) X(deltat) X(gravity_dark1) X(gravity_dark2) X(absorb) X(slope) X(quad) X(cube) X(third)    \
+    X(rdisc1) X(rdisc2) X(height_disc) X(beta_disc) X(temp_disc) X(texp_disc) X(lin_limb_disc)                   \
+    X(quad_limb_disc) X(temp_edge) X(absorb_edge) X(radius_spot) X(length_spot) X(height_spot) X(expon_spot)     \
+    X(epow_spot) X(angle_spot) X(yaw_spot) X(temp_spot) X(tilt_spot) X(cfrac_spot) X(stsp11_long)                \
+    X(stsp11_lat) X(stsp11_fwhm) X(stsp11_tcen) X(stsp12_long) X(stsp12_lat) X(stsp12_fwhm) X(stsp12_tcen)       \
+    X(stsp13_long) X(stsp13_lat) X(stsp13_fwhm) X(stsp13_tcen) X(stsp21_long) X(stsp21_lat) X(stsp21_fwhm)       \
+    X(stsp21_tcen) X(stsp22_long) X(stsp22_lat) X(stsp22_fwhm) X(stsp22_tcen) X(uesp_long1) X(uesp_long2)        \
+    X(uesp_lathw) X(uesp_taper) X(uesp_temp)
+
+#define REF_NUMBERS(X)                                                                                            \
+    X(delta_phase) X(nlat1f) X(nlat2f) X(nlat1c) X(nlat2c) X(nlatfill) X(nlngfill) X(lfudge) X(llo) X(lhi)       \
+    X(phase1) X(phase2) X(wavelength) X(tperiod) X(mucrit1) X(mucrit2) X(nrad) X(nspot)
+
+#define REF_FLAGS(X)                                                                                              \
+    X(npole) X(roche1) X(roche2) X(eclipse1) X(eclipse2) X(glens1) X(use_radii) X(gdark_bolom1) X(gdark_bolom2)  \
+    X(mirror) X(add_disc) X(opaque) X(add_spot) X(iscale)
+
+static Lcurve::Model *model_of(const lc_model *pod) {
+    void *mem = std::malloc(sizeof(Lcurve::Model));
+    if (!mem) throw std::bad_alloc();
+    std::memset(mem, 0, sizeof(Lcurve::Model));
+    Lcurve::Model *m = static_cast<Lcurve::Model *>(mem);
+#define SET_PP(n)                                                                                                 \
+    m->n.value = pod->n.value;                                                                                    \
+    m->n.range = pod->n.range;                                                                                    \
+    m->n.dstep = pod->n.dstep;                                                                                    \
+    m->n.vary = pod->n.vary != 0;                                                                                 \
+    m->n.defined = pod->n.defined != 0;
+    REF_PPARAMS(SET_PP)
+#undef SET_PP
+#define SET_NUM(n) m->n = pod->n;
+    REF_NUMBERS(SET_NUM)
+#undef SET_NUM
+#define SET_FLAG(n) m->n = pod->n != 0;
+    REF_FLAGS(SET_FLAG)
+#undef SET_FLAG
+    m->limb1 = pod->limb1 == LC_LDC_CLARET ? Lcurve::LDC::CLARET : Lcurve::LDC::POLY;
+    m->limb2 = pod->limb2 == LC_LDC_CLARET ? Lcurve::LDC::CLARET : Lcurve::LDC::POLY;
+    return m;
+}
+
+struct ModelHolder {
+    Lcurve::Model *m;
+    explicit ModelHolder(const lc_model *pod) : m(model_of(pod)) {}
+    ~ModelHolder() { std::free(m); }
+    const Lcurve::Model &operator*() const { return *m; }
+};
+
+static void faces_to_points(const lc_star_faces *f, std::vector<Lcurve::Point> &v) {
+    v.resize(f->n);
+    for (size_t i = 0; i < f->n; i++) {
+        Lcurve::Point::etype ecl;
+        if (f->necl && f->necl[i]) ecl.push_back(std::make_pair(f->ingress[i], f->egress[i]));
+        v[i] = Lcurve::Point(Subs::Vec3(f->px[i], f->py[i], f->pz[i]), Subs::Vec3(f->dx[i], f->dy[i], f->dz[i]),
+                             f->area[i], f->gravity[i], ecl);
+        v[i].flux = f->flux ? f->flux[i] : 0.f;
+    }
+}
+
+/* Lcurve::set_star_grid (set_star_grid.cpp:49-523) for one star and grid;
+ * *n in: capacity of the out arrays, out: faces written. */
+int ref_set_star_grid(const lc_model *pod, int star, int fine, lc_star_faces *out) {
+    return guard([&] {
+        ModelHolder m(pod);
+        std::vector<Lcurve::Point> v;
+        Lcurve::set_star_grid(*m, star_of(star), fine != 0, v);
+        if (v.size() > out->n) throw std::runtime_error("ref_set_star_grid: output arrays too small");
+        out->n = v.size();
+        for (size_t i = 0; i < v.size(); i++) {
+            const Lcurve::Point &p = v[i];
+            out->px[i] = p.posn.x(); out->py[i] = p.posn.y(); out->pz[i] = p.posn.z();
+            out->dx[i] = p.dirn.x(); out->dy[i] = p.dirn.y(); out->dz[i] = p.dirn.z();
+            out->area[i] = p.area;
+            out->gravity[i] = p.gravity;
+            out->flux[i] = p.flux;
+            out->necl[i] = p.eclipse.empty() ? 0 : 1;
+            out->ingress[i] = p.eclipse.empty() ? 0. : p.eclipse[0].first;
+            out->egress[i] = p.eclipse.empty() ? 0. : p.eclipse[0].second;
+        }
+    });
+}
+
+/* Lcurve::set_star_continuum (set_star_continuum.cpp:26-297): fluxes of
+ * the faces given (posn, dirn, area, gravity) written to star?->flux. */
+int ref_set_star_continuum(const lc_model *pod, lc_star_faces *star1, lc_star_faces *star2) {
+    return guard([&] {
+        ModelHolder m(pod);
+        std::vector<Lcurve::Point> v1, v2;
+        faces_to_points(star1, v1);
+        faces_to_points(star2, v2);
+        Lcurve::set_star_continuum(*m, v1, v2);
+        for (size_t i = 0; i < v1.size(); i++) star1->flux[i] = v1[i].flux;
+        for (size_t i = 0; i < v2.size(); i++) star2->flux[i] = v2[i].flux;
+    });
+}
+
+/* Lcurve::comp_gravity1 / comp_gravity2 (comp_gravity.cpp) on the faces
+ * given (area, gravity, flux). */
+double ref_comp_gravity(const lc_model *pod, int star, const lc_star_faces *f) {
+    double out = 0.;
+    if (guard([&] {
+            ModelHolder m(pod);
+            std::vector<Lcurve::Point> v;
+            faces_to_points(f, v);
+            out = star == 0 ? Lcurve::comp_gravity1(*m, v) : Lcurve::comp_gravity2(*m, v);
+        }) != 0)
+        return -1.0e300;
+    return out;
+}
+
+/* Lcurve::light_curve_comp (light_curve_comp.cpp:119-599), or
+ * light_curve_comp_fast (light_curve_comp_fast.cpp:6-84) when
+ * o->max_model_points > 0.  io->sfac is read when o->scale is 0. */
+int ref_light_curve_comp(const lc_model *pod, const lc_datum *data, size_t n, const lc_eval_opts *o, double *calc,
+                         lc_scalars *io) {
+    return guard([&] {
+        ModelHolder m(pod);
+        Lcurve::Data d(n);
+        for (size_t i = 0; i < n; i++) {
+            d[i].time = data[i].time;
+            d[i].expose = data[i].expose;
+            d[i].flux = data[i].flux;
+            d[i].ferr = data[i].ferr;
+            d[i].weight = data[i].weight;
+            d[i].ndiv = data[i].ndiv;
+        }
+        std::vector<double> sfac(io->sfac, io->sfac + 5), c;
+        double wdwarf = 0., chisq = 0., wnok = 0., logg1 = 0., logg2 = 0., rv1 = 0., rv2 = 0.;
+        if (o->max_model_points > 0)
+            Lcurve::light_curve_comp_fast(*m, d, o->scale != 0, o->rdata != 0, false, sfac, c, wdwarf, chisq, wnok,
+                                          logg1, logg2, rv1, rv2, o->max_model_points, o->diagnostics != 0);
+        else
+            Lcurve::light_curve_comp(*m, d, o->scale != 0, o->rdata != 0, false, sfac, c, wdwarf, chisq, wnok,
+                                     logg1, logg2, rv1, rv2, o->diagnostics != 0);
+        for (size_t i = 0; i < n && i < c.size(); i++) calc[i] = c[i];
+        for (size_t k = 0; k < 5 && k < sfac.size(); k++) io->sfac[k] = sfac[k];
+        io->wdwarf = wdwarf;
+        io->chisq = chisq;
+        io->wnok = wnok;
+        io->logg1 = logg1;
+        io->logg2 = logg2;
+        io->rv1 = rv1;
+        io->rv2 = rv2;
+    });
 }
 
 } // extern "C"

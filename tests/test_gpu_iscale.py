@@ -43,12 +43,15 @@ def _data(m, t, a0, a4, seed):
     return L.make_data(t, ferr=ferr, flux=flux, weight=weight)
 
 
-@pytest.mark.parametrize("case", ["eclipsing", "one_phase_rank_one"])
+@pytest.mark.parametrize("case", ["eclipsing", "one_phase_rank_one", "async_beamed"])
 def test_iscale_fit_vs_oracle(engine, case):
+    """async_beamed: the star components use spin 1 in the beaming term, as
+    the reference's comp_star1_flat / comp_star2_flat do."""
     import lcurve_re_amd as L
     from oracle import pyoracle as P
-    m = _model()
-    t = np.arange(1200) / 1200.0 if case == "eclipsing" else 0.3 + (np.arange(600) % 2)
+    m = _model(**({"spin1": 1.2, "spin2": 0.9, "beam_factor1": 1.3, "beam_factor2": 1.6}
+                  if case == "async_beamed" else {}))
+    t = 0.3 + (np.arange(600) % 2) if case == "one_phase_rank_one" else np.arange(1200) / 1200.0
     data = _data(m, t, 1.7, 0.6, seed=11)
     st, ref, rio = P.light_curve(m.pod, data, scale=True, rdata=True, diagnostics=True)
     assert st == 0
