@@ -246,7 +246,7 @@ def _jac_case(tmp_path, nphase=800):
     t = np.arange(nphase) / nphase
     with open(tmp_path / "in.dat", "w") as f:
         for x in t:
-            f.write(f"{x!r} 0 1 0.0015 1 1\n")
+            f.write(f"{float(x)!r} 0 1 0.0015 1 1\n")
     return str(p), str(tmp_path / "in.dat"), cfg
 
 
