@@ -26,9 +26,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (packed FP32 instructions)
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector peak
 PEAK_HBM_GBS = 8000.0
+# VALU issue ceiling without packed FP32 (this build emits none): one wave64
+# VALU instruction per SIMD every 4 cycles, 256 CUs x 4 SIMDs x 16 lanes x
+# 2 FLOP (an FMA) x 2.4 GHz
+PEAK_VALU_ISSUE_TFLOPS = 78.6
 
 
 def log(*a):
@@ -57,25 +61,78 @@ def build_workload(name):
     return models, data, cfg
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def curve_errors(calc, ref):
+    """(max |d| / max |ref|, max |d| / |ref| pointwise over |ref| > 1e-12 max)."""
+    calc, ref = np.asarray(calc, np.float64), np.asarray(ref, np.float64)
+    d = np.abs(calc - ref)
+    scale = np.max(np.abs(ref))
+    nz = np.abs(ref) > 1e-12 * scale
+    return float(d.max() / scale), float(np.max(d[nz] / np.abs(ref[nz])))
+
+
 def cpu_baseline(models, data, gpu_calc, budget_s=15.0):
-    """FP64 CPU oracle over the first instances until `budget_s` elapses."""
+    """The CPU path timed on this host's cores over the first instances of
+    the workload until `budget_s` elapses: the reference's own
+    light_curve_comp compiled from its sources (oracle/_ref/libref.so, with
+    its grid cache off, LCURVE_NO_GRID_CACHE, as in an MCMC where every
+    proposal moves the geometry), then the FP64 oracle over the same
+    instances as a cross-check of the port's speed.  Returns the baseline,
+    the GPU curves' errors against the reference and the oracle timing."""
     from oracle import pyoracle as P
-    lib = P.oracle()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    lib.orc_set_threads(threads)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    use_ref = P.have_ref()
+    if use_ref:
+        os.environ["LCURVE_NO_GRID_CACHE"] = "1"
+        P.ref()
+    P.oracle().orc_set_threads(threads)
+    from lcurve_re_amd.workloads import ring_layout_exceeds_numface
+    run = P.ref_light_curve if use_ref else P.light_curve
     t0 = time.perf_counter()
-    done, errs = 0, []
+    done, e_max, e_pt, skipped = 0, 0.0, 0.0, 0
+    sample = []
     for i, m in enumerate(models):
-        st, ref, _ = P.light_curve(m.pod, data, scale=False, rdata=True, diagnostics=False)
+        pod = m.pod
+        if use_ref and any(ring_layout_exceeds_numface(n) for n in (pod.nlat1f, pod.nlat1c, pod.nlat2f, pod.nlat2c)):
+            # the reference writes such grids past the end of their vectors
+            skipped += 1
+            continue
+        sample.append(m)
+        st, ref, _ = run(pod, data, scale=False, rdata=True, diagnostics=False)
         if st == 0 and gpu_calc is not None:
-            errs.append(float(np.max(np.abs(gpu_calc[i] - ref)) / np.max(np.abs(ref))))
+            a, b = curve_errors(gpu_calc[i], ref)
+            e_max, e_pt = max(e_max, a), max(e_pt, b)
         done += 1
         if time.perf_counter() - t0 > budget_s and done >= 2:
             break
     el = time.perf_counter() - t0
-    return {"value": done / el, "unit": "light curves/s", "cores": threads, "kind": "port",
-            "sample": f"first {done} of {len(models)} instances of the same workload, FP64 oracle "
-                      f"(OpenMP, -O3 -ffp-contract=off), {el:.1f} s"}, (max(errs) if errs else None)
+    base = {"value": done / el, "unit": "light curves/s", "cores": threads,
+            "kind": "reference" if use_ref else "port", "cpu_model": cpu_model(), "host_cores": os.cpu_count(),
+            "sample": f"first {done} of {len(models)} instances of the same workload"
+                      + (f" ({skipped} with nlat 153 passed over)" if skipped else "") + ", "
+                      + ("the reference's light_curve_comp compiled from its sources (oracle/_ref, -O3, OpenMP, "
+                         "grid cache off)" if use_ref else "FP64 oracle (OpenMP, -O3 -ffp-contract=off)")
+                      + f", {el:.1f} s"}
+    port = None
+    if use_ref:
+        t1 = time.perf_counter()
+        for m in sample:
+            P.light_curve(m.pod, data, scale=False, rdata=True, diagnostics=False)
+        elp = time.perf_counter() - t1
+        port = {"value": done / elp, "unit": "light curves/s", "cores": threads,
+                "sample": f"the same {done} instances through the FP64 oracle, {elp:.1f} s"}
+    return base, {"max_normalised": e_max, "pointwise": e_pt,
+                  "against": "reference" if use_ref else "oracle", "instances": done}, port
 
 
 def measured_traffic(workload, kernel):
@@ -92,6 +149,51 @@ def measured_traffic(workload, kernel):
             if name.strip() == want:
                 return float(val)
     return None
+
+
+def valu_counters(workload, kernel):
+    """VALU instructions per launch of `kernel` (all, and transcendental
+    FP32) and its lane utilisation, from the committed rocprofv3 counter
+    summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_counters_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    want = kernel.replace("True", "true").replace("False", "false")
+    out = {}
+    for line in open(path):
+        for key in ("valu_insts_per_launch", "valu_trans_per_launch", "valu_lane_utilisation",
+                    "fp32_ops_per_launch", "fp64_ops_per_launch"):
+            if line.startswith(key + " "):
+                name, val = line[len(key) + 1:].rsplit("=", 1)
+                if name.strip() == want:
+                    out[key] = float(val)
+    return out if "valu_insts_per_launch" in out else None
+
+
+def valu_calibration(blocks=8192, iters=4096, reps=5):
+    """Issue rate of kernels with a known VALU instruction count
+    (lcurve_re_amd/csrc/k_calib.hip): FP32 FMAs, FMAs with transcendental
+    reciprocals, FP64 FMAs.  Instructions per second as FMA-equivalent
+    TFLOP/s (wave64 instructions x 64 lanes x 2)."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "lcurve_re_amd", "_lib", "liblcurve_calib.so"))
+    lib.lcc_valu_issue.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+    lib.lcc_valu_issue.restype = C.c_int
+    out = {}
+    for kind, name in ((0, "fma_f32"), (1, "fma_f32_with_rcp_f32_8to1"), (3, "fma_f32_with_rcp_f32_8to2"),
+                       (2, "fma_f64")):
+        ms, per_lane = C.c_double(), C.c_double()
+        st = lib.lcc_valu_issue(kind, blocks, iters, reps, C.byref(ms), C.byref(per_lane))
+        if st:
+            raise RuntimeError(f"lcc_valu_issue({kind}): HIP error {st}")
+        waves = blocks * 256 // 64
+        inst = waves * per_lane.value
+        fma = waves * iters * 8
+        out[name] = {"ms": ms.value, "valu_wave_insts": inst, "fma_wave_insts": fma,
+                     "valu_issue_tflops": inst * 128 / (ms.value * 1e-3) / 1e12,
+                     "fma_tflops": fma * 128 / (ms.value * 1e-3) / 1e12}
+    return out
 
 
 def copy_calibration(mib=256, reps=4):
@@ -130,6 +232,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default flux tile shape")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="run only the VALU issue-rate kernels of known instruction count")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -138,6 +242,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.valu_calibration:
+        print(json.dumps({"valu_calibration": valu_calibration()}), flush=True)
         return
 
     import lcurve_re_amd as L
@@ -155,34 +262,57 @@ def main():
 
 def roofline_from_profile(prof, steps, precision, workload, R=None):
     """Roofline of the dominant flux kernel: the variant (limb-darkening law
-    x beaming x lensing x mucrit) with the most launch time."""
+    x beaming x lensing x mucrit) with the most launch time.
+
+    achieved = the FP32 + FP64 arithmetic the kernel issued per launch, from
+    rocprofv3 counters (profiles/valu_counters_<workload>.md: (2 FMA + ADD +
+    MUL + TRANS + INT32) x 64 for FP32, (2 FMA + ADD + MUL) x 64 for FP64,
+    AMD's TOTAL_32_OPS / TOTAL_64_OPS), divided by the launch time measured
+    here with HIP events; peak = 78.6 TFLOP/s, the VALU ceiling without
+    packed FP32 (which this build does not emit; FP64 has the same peak).
+    valu_issue_* = non-transcendental VALU instructions (x 64 lanes x 2,
+    FMA-equivalent) against one wave64 instruction per SIMD per 4 cycles at
+    2.4 GHz.  alg_* = the algorithmic count of the reference's element
+    arithmetic over every face x sub-phase element, including the elements
+    the kernel proves zero and skips (culled_fraction)."""
     var_ms = prof["flux_var_ms"]
     v = int(np.argmax(var_ms))
     launches = max(int(prof["flux_var_launches"][v]), 1)
     avg_ms = var_ms[v] / launches
     flop_per_launch = prof["flux_var_flops"][v] / launches
-    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    alg = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     if R is None:
         R = int(prof["last_rphase"])  # the shape the engine used
     kname = (f"k_flux_{'fp64' if precision else 'mixed'}<{v & 1}, {bool(v & 2)}, {bool(v & 4)}"
              + ("" if precision else f", {bool(v & 8)}") + f", {R}>")
-    peak = PEAK_FP64_TFLOPS if precision else PEAK_FP32_TFLOPS
     c_elem = {0: 24, 1: 26, 2: 39, 3: 41}[v & 3]
     evald = prof["flux_var_evaluated"][v] / launches
     traffic = measured_traffic(workload, kname)
-    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": traffic,
+    vc = valu_counters(workload, kname)
+    t = avg_ms * 1e-3
+    achieved = issue = None
+    if vc and t > 0:
+        achieved = (vc["fp32_ops_per_launch"] + vc["fp64_ops_per_launch"]) / t / 1e12
+        issue = (vc["valu_insts_per_launch"] - vc["valu_trans_per_launch"]) * 128 / t / 1e12
+    alg_elems = flop_per_launch / c_elem
+    return {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_ISSUE_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_VALU_ISSUE_TFLOPS if achieved is not None else None, "traffic": traffic,
+            "achieved_source": f"profiles/valu_counters_{workload}.md" if vc else None,
             "traffic_source": f"profiles/hbm_traffic_{workload}.md" if traffic else None,
-            "hbm_frac": traffic / (avg_ms * 1e-3) / (PEAK_HBM_GBS * 1e9) if traffic and avg_ms > 0 else None,
+            "hbm_frac": traffic / t / (PEAK_HBM_GBS * 1e9) if traffic and t > 0 else None,
+            "valu_issue_tflops_fma_equivalent": issue,
+            "valu_issue_frac": issue / PEAK_VALU_ISSUE_TFLOPS if issue is not None else None,
+            "valu_trans_insts_per_launch": vc["valu_trans_per_launch"] if vc else None,
+            "valu_lane_utilisation": vc["valu_lane_utilisation"] if vc else None,
+            "packed_fp32_peak": PEAK_FP32_TFLOPS,
             "kernel": kname, "launches_per_step": launches / steps, "avg_launch_ms": avg_ms,
+            "alg_tflops": alg, "alg_frac_of_packed_fp32_peak": alg / PEAK_FP32_TFLOPS if not precision else None,
             "alg_flop_per_launch": flop_per_launch, "flop_per_element": c_elem,
-            "alg_elements_per_launch": flop_per_launch / c_elem,
-            # elements left after tile culling (provably zero ones skipped):
-            # the rate the VALUs actually ran at (mixed kernels count them)
+            "alg_elements_per_launch": alg_elems,
+            # elements left after tile culling (provably zero ones skipped)
             "evaluated_elements_per_launch": evald if not precision else None,
-            "evaluated_tflops": (evald * c_elem / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0)
-            if not precision else None,
-            "flux_stage_tflops": prof["flux_flops"] / (prof["flux_ms_total"] * 1e-3) / 1e12
+            "culled_fraction": 1.0 - evald / alg_elems if not precision and alg_elems > 0 else None,
+            "flux_stage_alg_tflops": prof["flux_flops"] / (prof["flux_ms_total"] * 1e-3) / 1e12
             if prof["flux_ms_total"] > 0 else 0.0,
             "stage_ms_per_step": {k: prof[k] / steps for k in
                                   ("grid_ms_total", "eclipse_ms_total", "cont_ms_total", "flux_ms_total",
@@ -256,9 +386,9 @@ def run_batch_workload(L, eng, args):
     prof = eager_profile(eng, lambda: eng.light_curve_comp_batch(packed, data, precision=args.precision, out=calc),
                          nprof)
     roofline = roofline_from_profile(prof, nprof, args.precision, cfg["workload"])
-    cpu, err = (None, None)
+    cpu, err, port = (None, None, None)
     if not args.no_cpu:
-        cpu, err = cpu_baseline(models, data, calc, args.cpu_budget)
+        cpu, err, port = cpu_baseline(models, data, calc, args.cpu_budget)
     return {"metric": "light curves/sec on 1 MI355X (batched engine, <=1e-5 rel err vs FP64 oracle)",
             "value": M * args.steps / el, "unit": "light curves/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
@@ -266,7 +396,8 @@ def run_batch_workload(L, eng, args):
             "dtype": "fp64 per face and accumulation" if args.precision else
             "fp32 per face, fp64 geometry/accumulation",
             "data": "synthetic", "config": cfg, "instances_ok": nok, "roofline": roofline,
-            "cpu_baseline": cpu, "max_rel_err": err}
+            "cpu_baseline": cpu, "cpu_oracle_port": port,
+            "max_rel_err": err["max_normalised"] if err else None, "rel_err": err}
 
 
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA

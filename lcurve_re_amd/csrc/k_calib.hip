@@ -6,8 +6,9 @@
 // SIMD), so instructions / time is the issue ceiling the hardware reaches,
 // measured on the same box and clock as the flux kernels:
 //   kind 0: v_fma_f32 only                       (8 per iteration)
-//   kind 1: v_fma_f32 + v_rcp_f32, 8 : 2          (transcendental co-issue)
+//   kind 1: v_fma_f32 + v_rcp_f32, 8 : 1          (transcendental co-issue)
 //   kind 2: v_fma_f64 only                       (8 per iteration)
+//   kind 3: v_fma_f32 + v_rcp_f32, 8 : 2
 // Built into its own library (lcurve_re_amd/_lib/liblcurve_calib.so), used
 // by bench.py --valu-calibration; the engine never loads it.
 #include <hip/hip_runtime.h>
@@ -40,10 +41,8 @@ __global__ __launch_bounds__(kThreads) void k_valu_issue(int iters, float a, flo
         for (int i = 0; i < iters; i++) {
 #pragma unroll
             for (int k = 0; k < 8; k++) x[k] = __builtin_fmaf(x[k], a, b);
-            if constexpr (KIND == 1) {
-                y[0] = __builtin_amdgcn_rcpf(y[0]);
-                y[1] = __builtin_amdgcn_rcpf(y[1]);
-            }
+            if constexpr (KIND == 1 || KIND == 3) y[0] = __builtin_amdgcn_rcpf(y[0]);
+            if constexpr (KIND == 3) y[1] = __builtin_amdgcn_rcpf(y[1]);
         }
         float s = y[0] + y[1];
         for (int k = 0; k < 8; k++) s += x[k];
@@ -55,13 +54,13 @@ __global__ __launch_bounds__(kThreads) void k_valu_issue(int iters, float a, flo
 
 extern "C" {
 
-// Runs kind `kind` (0, 1, 2) on `blocks` workgroups of 256 lanes for `iters`
+// Runs kind `kind` (0, 1, 2, 3) on `blocks` workgroups of 256 lanes for `iters`
 // iterations, `reps` times; returns the shortest launch in *ms and the VALU
 // instructions per lane of the loop in *valu_per_lane (the loop's FMA/RCP
 // count; set-up and the final sum add ~20).  0 on success, else the HIP
 // error code.
 int lcc_valu_issue(int kind, int blocks, int iters, int reps, double *ms, double *valu_per_lane) {
-    if (kind < 0 || kind > 2 || blocks <= 0 || iters <= 0 || reps <= 0 || !ms || !valu_per_lane) return -1;
+    if (kind < 0 || kind > 3 || blocks <= 0 || iters <= 0 || reps <= 0 || !ms || !valu_per_lane) return -1;
     const size_t n = (size_t)blocks * kThreads;
     float *out = nullptr;
     double *outd = nullptr;
@@ -82,6 +81,8 @@ int lcc_valu_issue(int kind, int blocks, int iters, int reps, double *ms, double
             k_valu_issue<0><<<blocks, kThreads>>>(iters, 0.999f, 1e-4f, 0.999, 1e-4, out, outd);
         else if (kind == 1)
             k_valu_issue<1><<<blocks, kThreads>>>(iters, 0.999f, 1e-4f, 0.999, 1e-4, out, outd);
+        else if (kind == 3)
+            k_valu_issue<3><<<blocks, kThreads>>>(iters, 0.999f, 1e-4f, 0.999, 1e-4, out, outd);
         else
             k_valu_issue<2><<<blocks, kThreads>>>(iters, 0.999f, 1e-4f, 0.999, 1e-4, out, outd);
         (void)hipEventRecord(t1, 0);
@@ -96,7 +97,7 @@ int lcc_valu_issue(int kind, int blocks, int iters, int reps, double *ms, double
     (void)hipFree(out);
     (void)hipFree(outd);
     *ms = best;
-    *valu_per_lane = (double)iters * (kind == 1 ? 10.0 : 8.0);
+    *valu_per_lane = (double)iters * (kind == 1 ? 9.0 : kind == 3 ? 10.0 : 8.0);
     return e == hipSuccess ? 0 : (int)e;
 }
 

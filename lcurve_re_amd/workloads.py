@@ -65,6 +65,25 @@ def model_json(phys=None, comp=None, vary=()):
     return {"model_parameters": mp}
 
 
+def ring_layout_exceeds_numface(nlat):
+    """True when the rings of a grid without infill hold more faces than
+    numface() allots: add_faces counts max(16, int(2 pi sin(theta) / dtheta))
+    per ring (set_star_grid.cpp:261-273), numface max(16, int(2 pi / dphi))
+    with dphi = dtheta / sin(theta) (numface.cpp:41-50), and the two roundings
+    differ for some nlat (153, 297, 315, ...).  The reference writes the extra
+    faces past the end of its vector."""
+    pi = 3.14159265358979323846
+    dtheta = pi / nlat
+    n_num = n_ring = 0
+    for i in range(int(math.ceil(pi / dtheta))):
+        theta = pi * (i + 0.5) / nlat
+        n_num += max(16, int(2 * pi / (dtheta / math.sin(theta))))
+    for nt in range(nlat):
+        sint = math.sin(0.0 + (pi - 0.0) * (nt + 0.5) / nlat)
+        n_ring += max(16, int(2 * pi * sint / dtheta))
+    return n_ring > n_num
+
+
 def xl12(q, spin=1.0):
     """L1 distance from star 1 for an asynchronous star 2 (roche.cpp:45-80)."""
     ssq = spin * spin

@@ -20,9 +20,6 @@ from oracle import pyoracle as P
 
 pytestmark = pytest.mark.skipif(not P.have_ref(), reason="oracle/_ref/libref.so not built")
 
-OVERFULL_NLAT = {153, 297, 315, 333, 339}
-
-
 def _same(m, data, **kw):
     st, c, io = P.light_curve(m.pod, data, **kw)
     rs, rc, rio = P.ref_light_curve(m.pod, data, **kw)
@@ -39,7 +36,7 @@ def test_batch_prior_draws_at_their_shape():
     data = L.make_data(w["time"], ferr=1.0, weight=1.0)
     done = 0
     for cfg, par in zip(w["models"], w["params"]):
-        if par[-1] in OVERFULL_NLAT:
+        if W.ring_layout_exceeds_numface(par[-1]):
             continue
         m = L.Model.from_json(cfg)
         print(f"batch_prior draw nlat {par[-1]} q {par[0]:.3f}", flush=True)
