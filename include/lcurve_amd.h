@@ -266,6 +266,23 @@ lc_status lc_light_curve_comp_batch(lc_ctx *ctx, const lc_model *models,
                                     const lc_eval_opts *opts, double *calc,
                                     lc_scalars *io, int32_t *status);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): flat_sum_star1_multi (star 0) / flat_sum_star2_multi
+ * (star 1) as lc_flat_sum_star{1,2}_multi compute them, bit for bit, but on
+ * DEVICE arrays: every pointer in *g and *pb, and out (pb->n doubles), are
+ * device addresses on ctx's GPU, written and synchronised by the caller
+ * before the call (g and pb themselves are host structs).  The work memory
+ * (flux records, task table, partial sums) is one static device array of
+ * 64 MiB per GPU (needs about 64 B per face + 8 B per 1024 faces per
+ * sub-phase); larger inputs return LC_E_ARG with the sizes.  The call
+ * allocates nothing, runs on ctx's stream and returns when out is written.
+ * Calls on one GPU share that array: calls that overlap on one GPU (other
+ * threads, other contexts) race. */
+lc_status lc_flat_sum_multi_noalloc(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_batch *pb,
+                                    const lc_ldc *ldc, double beam, double spin, double VFAC, double XCOFM,
+                                    int glens1, double rlens1, int precision, double *out);
+
 /* ------------------------------------------------ stage-level entry points */
 /* Lcurve::flat_sum_star1_multi (lcurve.h:299-301; comp_light_flat.cpp:413) */
 lc_status lc_flat_sum_star1_multi(lc_ctx *ctx, const lc_flat_grid *g,

@@ -160,6 +160,10 @@ SIGNATURES = {
     "lc_flat_sum_star2_multi": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_flat_grid), _c.POINTER(lc_phase_batch),
                                            _c.POINTER(lc_ldc), _c.c_double, _c.c_double, _c.c_double,
                                            _c.c_double, _c.c_int, _c.c_double, _c.c_int, _DP]),
+    "lc_flat_sum_multi_noalloc": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(lc_flat_grid),
+                                             _c.POINTER(lc_phase_batch), _c.POINTER(lc_ldc), _c.c_double,
+                                             _c.c_double, _c.c_double, _c.c_double, _c.c_int, _c.c_double,
+                                             _c.c_int, _c.c_void_p]),
     "lc_set_star_grid": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_model), _c.c_int, _c.c_int,
                                     _c.POINTER(lc_star_faces)]),
     "lc_set_star_continuum": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_model), _c.POINTER(lc_star_faces),

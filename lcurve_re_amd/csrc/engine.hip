@@ -1609,6 +1609,10 @@ void run_eval(lc_ctx *ctx, const EvalRequest &rq) {
     }
 }
 
+// Per-device address of the no-allocation entry point's scratch array
+// (defined with that entry point below).
+void noalloc_scratch_init(int device);
+
 template <class F> lc_status guarded(lc_ctx *ctx, F f) {
     try {
         return (lc_status)f();
@@ -1639,6 +1643,7 @@ lc_status lc_ctx_create(int device, lc_ctx **out) {
         if (ndev <= 0) throw EngineError(LC_E_DEVICE, "no HIP device available");
         if (device < 0 || device >= ndev) device = 0;
         HIPCHK(hipSetDevice(device));
+        noalloc_scratch_init(device);
         std::unique_ptr<lc_ctx> c(new lc_ctx());
         c->device = device;
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1941,6 +1946,182 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
     return LC_OK;
 }
 
+// ------------------------------------------------ flux sums without allocation
+// Work memory of lc_flat_sum_multi_noalloc: one static device array, which
+// every GPU the library runs on holds until the process ends.  Host code
+// reaches it only through the per-device address that noalloc_scratch_init
+// reads with hipGetSymbolAddress (the array's name is a host-side stand-in,
+// not a device address).  Calls on one GPU share the array: calls that
+// overlap on one GPU race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+std::string g_scratch_err[NOALLOC_MAX_DEVICES];
+
+// Called once per device by lc_ctx_create, with that device current.  A
+// failure leaves the address unset: the other entry points work, and
+// lc_flat_sum_multi_noalloc reports the error.
+void noalloc_scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return;
+    }
+    g_scratch_ptr[device] = p;
+}
+
+void *noalloc_scratch(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        throw EngineError(LC_E_DEVICE, "flat_sum_noalloc: device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device])
+        throw EngineError(LC_E_DEVICE, "flat_sum_noalloc: no scratch address for this device (" +
+                                           g_scratch_err[device] + ")");
+    return g_scratch_ptr[device];
+}
+
+// Flux records of the caller's device-resident FlatGrid, the same values
+// flat_sum_stage builds on the host (float casts of the FP64 arrays; the
+// beaming velocities in FP64, comp_light_flat.cpp:110-168), the task table
+// (tile-major), the evaluated-element counters cleared, and the descriptors.
+__global__ void k_pack_flat_dev(lc_flat_grid g, int star, double spin, double VFAC, double XCOFM, int ntask,
+                                int nchunk, GridDev G, BatchDev B, float4 *rec0, float4 *rec1, float4 *rec2,
+                                double2 *ecl, TaskDev *tasks, unsigned long long *evald, GridDev *d_grid,
+                                BatchDev *d_batch, int *d_n0) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < g.n) {
+        const double dx = g.dx[i], dy = g.dy[i], dz = g.dz[i], px = g.px[i], py = g.py[i], pz = g.pz[i];
+        rec0[i] = make_float4((float)dx, (float)dy, (float)dz, (float)g.flux[i]);
+        const double vx = -VFAC * spin * py;
+        const double vy = star == 0 ? VFAC * (spin * px - XCOFM) : VFAC * (spin * (px - 1.0) + 1.0 - XCOFM);
+        const double vn = dx * vx + dy * vy;
+        rec1[i] = make_float4((float)vx, (float)vy, (float)vn, 0.f);
+        rec2[i] = make_float4((float)px, (float)py, (float)pz, 0.f);
+        ecl[i] = i >= g.n0 ? make_double2(g.in1[i - g.n0], g.out1[i - g.n0]) : make_double2(0., 0.);
+    }
+    if (i < (size_t)ntask) tasks[i] = TaskDev{0, (int)(i / nchunk), (int)(i % nchunk), 0};
+    if (i < 16 * 64) evald[i] = 0ull;
+    if (i == 0) {
+        *d_grid = G;
+        *d_batch = B;
+        *d_n0 = (int)g.n0;
+    }
+}
+
+__global__ void k_zero_doubles(double *out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = 0.0;
+}
+
+// flat_sum_stage on device-resident arrays with the work memory carved from
+// the device's scratch array: the same records, tile shape and kernels, so
+// the same sums bit for bit.
+int flat_sum_noalloc_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_batch *pb,
+                           const lc_ldc *ldc, double beam, double spin, double VFAC, double XCOFM, int glens1,
+                           double rlens1, int precision, double *out) {
+    if (!g || !pb || !ldc || !out) throw EngineError(LC_E_ARG, "flat_sum_noalloc: null argument");
+    if (g->n1 != g->n) throw EngineError(LC_E_UNSUPPORTED, "flat_sum_noalloc: faces with several eclipse ranges");
+    if (g->n0 > g->n1) throw EngineError(LC_E_ARG, "flat_sum_noalloc: n0 > n1");
+    const size_t nf = g->n, S = pb->n;
+    if (S == 0) return LC_OK;
+    if (!pb->ex || !pb->ey || !pb->ez || !pb->phin)
+        throw EngineError(LC_E_ARG, "flat_sum_noalloc: null sub-phase array");
+    if (nf > 0 && (!g->dx || !g->dy || !g->dz || !g->px || !g->py || !g->pz || !g->flux ||
+                   (g->n1 > g->n0 && (!g->in1 || !g->out1))))
+        throw EngineError(LC_E_ARG, "flat_sum_noalloc: null face array");
+    const bool fp64 = precision == LC_PREC_FP64;
+    const int R = ctx->flux_rphase > 0 ? ctx->flux_rphase : (beam != 0.0 ? 2 : 1), TILE = 256 * R;
+    const int chunk = fp64 ? 1024 : (ctx->flux_chunk > 0 ? ctx->flux_chunk : 1024);
+    const size_t nchunk = nf == 0 ? 1 : (nf + chunk - 1) / chunk;
+    const size_t ntile = (S + TILE - 1) / TILE;
+    const size_t ntask = nchunk * ntile;
+    const size_t part = nchunk * ntile * (size_t)TILE;
+    // layout of the scratch array, each piece 256-byte aligned
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t o_grid = 0, o_batch = o_grid + up(sizeof(GridDev)), o_n0 = o_batch + up(sizeof(BatchDev));
+    const size_t o_evald = o_n0 + 256, o_tasks = o_evald + up(16 * 64 * sizeof(unsigned long long));
+    const size_t o_rec0 = o_tasks + up(ntask * sizeof(TaskDev)), o_rec1 = o_rec0 + up(nf * sizeof(float4));
+    const size_t o_rec2 = o_rec1 + up(nf * sizeof(float4)), o_ecl = o_rec2 + up(nf * sizeof(float4));
+    const size_t o_part = o_ecl + up(nf * sizeof(double2)), need = o_part + part * sizeof(double);
+    if (nf > (size_t)(1 << 30) || S > (size_t)(1 << 28) || need > NOALLOC_SCRATCH_BYTES)
+        throw EngineError(LC_E_ARG, "flat_sum_noalloc: " + std::to_string(nf) + " faces x " + std::to_string(S) +
+                                        " sub-phases need " + std::to_string(need) +
+                                        " bytes of work memory; the library's scratch holds " +
+                                        std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                        " (at most about 64 MiB = 64 x faces + 8 x faces / 1024 x sub-phases)");
+    unsigned char *base = static_cast<unsigned char *>(noalloc_scratch(ctx->device));
+    hipStream_t s = ctx->stream;
+    if (nf == 0) {
+        // no faces: every sum is zero (a fill kernel, no memset node)
+        hipLaunchKernelGGL(k_zero_doubles, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, out, S);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        return LC_OK;
+    }
+    GridDev G;
+    std::memset(&G, 0, sizeof(G));
+    G.star = star;
+    G.nface = (int)nf;
+    G.beam = beam;
+    G.spin_beam = spin;
+    G.VFAC = VFAC;
+    G.XCOFM = XCOFM;
+    G.rlens1 = rlens1;
+    G.glens = (star == 1 && glens1) ? 1 : 0;
+    G.ldc[0] = ldc->c1; G.ldc[1] = ldc->c2; G.ldc[2] = ldc->c3; G.ldc[3] = ldc->c4;
+    G.mucrit = ldc->mucrit;
+    G.ltype = ldc->type;
+    BatchDev B;
+    std::memset(&B, 0, sizeof(B));
+    B.inst = 0; B.star = 0; B.grid = 0; B.list_off = -1; B.count_idx = -1;   // star 0: out[j] directly
+    B.chunk = chunk;
+    B.nchunk = (int)nchunk;
+    B.ntile = (int)ntile;
+    B.part_off = 0;
+    GridDev *d_grid = reinterpret_cast<GridDev *>(base + o_grid);
+    BatchDev *d_batch = reinterpret_cast<BatchDev *>(base + o_batch);
+    int *d_n0 = reinterpret_cast<int *>(base + o_n0);
+    FlatArrays fl;
+    std::memset(&fl, 0, sizeof(fl));
+    fl.rec0 = reinterpret_cast<float4 *>(base + o_rec0);
+    fl.rec1 = reinterpret_cast<float4 *>(base + o_rec1);
+    fl.rec2 = reinterpret_cast<float4 *>(base + o_rec2);
+    fl.ecl = reinterpret_cast<double2 *>(base + o_ecl);
+    fl.n0 = d_n0;
+    fl.evald = reinterpret_cast<unsigned long long *>(base + o_evald);
+    if (fp64) {
+        fl.fdx = const_cast<double *>(g->dx); fl.fdy = const_cast<double *>(g->dy);
+        fl.fdz = const_cast<double *>(g->dz); fl.fpx = const_cast<double *>(g->px);
+        fl.fpy = const_cast<double *>(g->py); fl.fpz = const_cast<double *>(g->pz);
+        fl.fflux = const_cast<double *>(g->flux);
+    }
+    TaskDev *d_tasks = reinterpret_cast<TaskDev *>(base + o_tasks);
+    double *d_partial = reinterpret_cast<double *>(base + o_part);
+    PhaseArrays pa;
+    std::memset(&pa, 0, sizeof(pa));
+    pa.ex = const_cast<double *>(pb->ex);
+    pa.ey = const_cast<double *>(pb->ey);
+    pa.ez = const_cast<double *>(pb->ez);
+    pa.phin = const_cast<double *>(pb->phin);
+    const size_t nthreads = std::max<size_t>({nf, ntask, (size_t)(16 * 64)});
+    hipLaunchKernelGGL(k_pack_flat_dev, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, *g, star, spin,
+                       VFAC, XCOFM, (int)ntask, (int)nchunk, G, B, fl.rec0, fl.rec1, fl.rec2, fl.ecl, d_tasks,
+                       fl.evald, d_grid, d_batch, d_n0);
+    launch_flux(s, flux_variant(G), fp64 ? 1 : 0, R, d_tasks, (int)ntask, d_batch, d_grid, fl, pa, (int)S, nullptr,
+                nullptr, d_partial);
+    launch_flux_reduce(s, d_batch, 1, (int)S, TILE, nullptr, nullptr, d_partial, out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return LC_OK;
+}
+
 // Uploads a set of grids' face tables and runs faces + eclipse search.
 int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_faces *out) {
     if (!m || !out || (star != 0 && star != 1)) throw EngineError(LC_E_ARG, "set_star_grid: bad argument");
@@ -2100,6 +2281,17 @@ lc_status lc_flat_sum_star2_multi(lc_ctx *ctx, const lc_flat_grid *g, const lc_p
     return guarded(ctx, [&]() -> int {
         HIPCHK(hipSetDevice(ctx->device));
         return flat_sum_stage(ctx, 1, g, pb, ldc, beam, spin, VFAC, XCOFM, glens1, rlens1, precision, out);
+    });
+}
+
+lc_status lc_flat_sum_multi_noalloc(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_batch *pb,
+                                     const lc_ldc *ldc, double beam, double spin, double VFAC, double XCOFM,
+                                     int glens1, double rlens1, int precision, double *out) {
+    if (!ctx || (star != 0 && star != 1)) return LC_E_ARG;
+    return guarded(ctx, [&]() -> int {
+        HIPCHK(hipSetDevice(ctx->device));
+        return flat_sum_noalloc_stage(ctx, star, g, pb, ldc, beam, spin, VFAC, XCOFM, glens1, rlens1, precision,
+                                      out);
     });
 }
 
