@@ -320,7 +320,8 @@ def roofline_from_profile(prof, steps, precision, workload, R=None):
             "eclipse_candidates_per_step": prof["eclipse_candidates"] / steps,
             "eclipse_hits_per_step": prof["eclipse_hits"] / steps,
             "grids_computed_per_step": prof["grids_computed"] / steps,
-            "grids_reused_per_step": prof["grids_reused"] / steps}
+            "grids_reused_per_step": prof["grids_reused"] / steps,
+            "continua_reused_per_step": prof["continua_reused"] / steps}
 
 
 def eager_profile(eng, step, nsteps):

@@ -179,6 +179,10 @@ typedef struct lc_profile {
     /* flux tile shape of the last batched call (sub-phases per lane, face chunk) */
     int64_t last_rphase;
     int64_t last_chunk;
+    /* star grids whose fluxes were copied from a grid with the same faces
+     * and continuum inputs instead of recomputed (ContKey reuse,
+     * light_curve_comp.cpp:43-52, 205-268) */
+    int64_t continua_reused;
 } lc_profile;
 
 typedef struct lc_ctx lc_ctx;

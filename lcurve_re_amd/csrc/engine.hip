@@ -218,6 +218,35 @@ struct GeoSigHash {
     }
 };
 static_assert(sizeof(GeoSig) == 14 * 8 + 10 * 4, "GeoSig has no implicit padding");
+// Exact-match key of a grid's continuum: every GridDev field k_continuum
+// reads besides the faces themselves.  Equal faces (equal GeoSig and ring
+// table) with equal keys give bit-identical fluxes, so those are copied
+// instead of recomputed (the reference's ContKey, light_curve_comp.cpp:43-52,
+// 205-268, keyed here on the derived temperatures, exponents and spot
+// directions).  Compared byte for byte, so -0.0 and 0.0 count as different
+// inputs and the grid is simply recomputed.
+struct ContSig {
+    double v[36];
+    bool operator==(const ContSig &o) const { return std::memcmp(this, &o, sizeof(ContSig)) == 0; }
+};
+ContSig cont_sig(const GridDev &G) {
+    ContSig c;
+    std::memset(&c, 0, sizeof(c));
+    int k = 0;
+    auto put = [&](double x) { c.v[k++] = x; };
+    put(G.star); put(G.t_base); put(G.t_irr); put(G.t_mirror); put(G.gdcbol); put(G.absorb); put(G.wave);
+    put(G.r_cont_other); put(G.mirror); put(G.nspot); put(G.uespot); put(G.spot_running);
+    for (int j = 0; j < G.nspot && j < 3; j++) {
+        put(G.spot_dir[j][0]); put(G.spot_dir[j][1]); put(G.spot_dir[j][2]);
+        put(G.spot_fwhm[j]); put(G.spot_tcen[j]);
+    }
+    if (G.uespot) {
+        put(G.ue_dir[0]); put(G.ue_dir[1]); put(G.ue_dir[2]);
+        put(G.ue_longhw); put(G.ue_lathw); put(G.ue_taper); put(G.ue_temp);
+    }
+    return c;
+}
+
 GeoSig geom_sig(const GridPlan &gp) {
     const GridDev &G = gp.dev;
     GeoSig k;
@@ -583,6 +612,7 @@ struct GeoCache {
     struct Rec {
         std::vector<RingHost> rings;
         Entry e;
+        ContSig cont;  // continuum of the fluxes kept with the faces
     };
     size_t cap = 0;  // faces (0: no reuse across calls)
     size_t used = 0;
@@ -600,8 +630,8 @@ struct GeoCache {
         std::memset(&pool, 0, sizeof(pool));
     }
     // bytes per cached face: positions and normals (6 x 8), area, gravity,
-    // eclipse flag (3 x 4), ingress and egress (2 x 8)
-    static constexpr size_t BYTES_PER_FACE = 6 * 8 + 3 * 4 + 2 * 8;
+    // eclipse flag, flux (4 x 4), ingress and egress (2 x 8)
+    static constexpr size_t BYTES_PER_FACE = 6 * 8 + 4 * 4 + 2 * 8;
     // false when the pool cannot be allocated (reuse across calls is then off)
     bool ensure_pool() {
         if (mem) return true;
@@ -630,7 +660,7 @@ struct GeoCache {
         pool.ecl = reinterpret_cast<int *>(take(cap * 4));
         pool.ein = reinterpret_cast<double *>(take(cap * 8));
         pool.eout = reinterpret_cast<double *>(take(cap * 8));
-        pool.flux = nullptr;
+        pool.flux = reinterpret_cast<float *>(take(cap * 4));
         return true;
     }
 };
@@ -817,8 +847,13 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     // copied; the rest are computed (and offered to the cache).
     const int ngrid = (int)grids.size();
     std::vector<int> dup_of(ngrid, -1), hit_off(ngrid, -1);
+    std::vector<char> hit_flux(ngrid, 0);  // the cache record also holds this grid's fluxes
     std::vector<GeoSig> sigs(ngrid);
-    parallel_for((size_t)ngrid, [&](size_t g) { sigs[g] = geom_sig(grids[g]); });
+    std::vector<ContSig> csig(ngrid);
+    parallel_for((size_t)ngrid, [&](size_t g) {
+        sigs[g] = geom_sig(grids[g]);
+        csig[g] = cont_sig(grids[g].dev);
+    });
     {
         std::unordered_map<GeoSig, std::vector<int>, GeoSigHash> first;  // computed grids by signature
         first.reserve((size_t)ngrid);
@@ -839,6 +874,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
                 for (const auto &rec : c->second)
                     if (rec.e.n == grids[g].dev.nface && same_rings(rec.rings, grids[g].rings)) {
                         hit_off[g] = rec.e.off;
+                        hit_flux[g] = rec.cont == csig[g];
                         break;
                     }
         }
@@ -911,6 +947,28 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             inserts.clear();
         }
     }
+
+    // continuum reuse: a grid whose faces are copied and whose continuum
+    // inputs equal its source's takes the source's fluxes too -- from the
+    // cache, or from a grid of this call after k_continuum computed it; new
+    // grids offered to the cache take their fluxes along
+    std::vector<FaceCopy> flux_hit, flux_dup, flux_out;
+    for (int g = 0; g < ngrid; g++)
+        if (hit_off[g] >= 0 && hit_flux[g]) {
+            flux_hit.push_back(FaceCopy{hit_off[g], gdev[g].face_off, gdev[g].nface, 0});
+            gdev[g].cont_copy = 1;
+        }
+    for (int g = 0; g < ngrid; g++) {
+        const int h = dup_of[g];
+        if (h < 0 || !(csig[g] == csig[h])) continue;
+        if (gdev[h].cont_copy)  // the source takes cached fluxes: so does this copy
+            flux_hit.push_back(FaceCopy{hit_off[h], gdev[g].face_off, gdev[g].nface, 0});
+        else
+            flux_dup.push_back(FaceCopy{gdev[h].face_off, gdev[g].face_off, gdev[g].nface, 0});
+        gdev[g].cont_copy = 1;
+    }
+    for (const auto &ins : inserts)
+        flux_out.push_back(FaceCopy{gdev[ins.first].face_off, ins.second.off, ins.second.n, 0});
 
     // Eclipse searches that may stop early: grids used by one instance only
     // (no in-batch copies) and offered to no cache, whose ranges are then
@@ -1090,6 +1148,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<FaceCopy>(std::max<size_t>(copy_hit.size(), 1));
     sz.add<FaceCopy>(std::max<size_t>(copy_dup.size(), 1));
     sz.add<FaceCopy>(std::max<size_t>(copy_out.size(), 1));
+    sz.add<FaceCopy>(std::max<size_t>(flux_hit.size(), 1));
+    sz.add<FaceCopy>(std::max<size_t>(flux_dup.size(), 1));
+    sz.add<FaceCopy>(std::max<size_t>(flux_out.size(), 1));
     sz.add<int2>(std::max<size_t>(gaps.size(), 1));
     for (int k = 0; k < 5; k++) sz.add<double>(std::max<size_t>(n, 1));
     sz.add<int>(std::max<size_t>(n, 1));
@@ -1128,6 +1189,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     FaceCopy *d_copy_hit = A.take<FaceCopy>(std::max<size_t>(copy_hit.size(), 1));
     FaceCopy *d_copy_dup = A.take<FaceCopy>(std::max<size_t>(copy_dup.size(), 1));
     FaceCopy *d_copy_out = A.take<FaceCopy>(std::max<size_t>(copy_out.size(), 1));
+    FaceCopy *d_flux_hit = A.take<FaceCopy>(std::max<size_t>(flux_hit.size(), 1));
+    FaceCopy *d_flux_dup = A.take<FaceCopy>(std::max<size_t>(flux_dup.size(), 1));
+    FaceCopy *d_flux_out = A.take<FaceCopy>(std::max<size_t>(flux_out.size(), 1));
     int2 *d_gaps = A.take<int2>(std::max<size_t>(gaps.size(), 1));
     double *d_time = A.take<double>(std::max<size_t>(n, 1));
     double *d_expose = A.take<double>(std::max<size_t>(n, 1));
@@ -1235,6 +1299,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         {d_copy_hit, copy_hit.data(), copy_hit.size() * sizeof(FaceCopy)},
         {d_copy_dup, copy_dup.data(), copy_dup.size() * sizeof(FaceCopy)},
         {d_copy_out, copy_out.data(), copy_out.size() * sizeof(FaceCopy)},
+        {d_flux_hit, flux_hit.data(), flux_hit.size() * sizeof(FaceCopy)},
+        {d_flux_dup, flux_dup.data(), flux_dup.size() * sizeof(FaceCopy)},
+        {d_flux_out, flux_out.data(), flux_out.size() * sizeof(FaceCopy)},
         {d_gaps, gaps.data(), gaps.size() * sizeof(int2)},
     };
     size_t up_end = 0;  // the uploaded arrays lead the arena
@@ -1310,6 +1377,12 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         if (!copy_out.empty()) launch_copy_faces(s, d_copy_out, (int)copy_out.size(), fa, ctx->geo.pool);
         mark(ev[2]);
         launch_continuum(s, d_grids, ngrid, nface_total, fa);
+        // fluxes of grids with a known continuum: cached ones first (the
+        // pool slots new grids take below may be ones read here), then
+        // copies within the call, then the new grids' fluxes into the cache
+        if (!flux_hit.empty()) launch_copy_flux(s, d_flux_hit, (int)flux_hit.size(), ctx->geo.pool.flux, fa.flux);
+        if (!flux_dup.empty()) launch_copy_flux(s, d_flux_dup, (int)flux_dup.size(), fa.flux, fa.flux);
+        if (!flux_out.empty()) launch_copy_flux(s, d_flux_out, (int)flux_out.size(), fa.flux, ctx->geo.pool.flux);
         mark(ev[3]);
         if (any_route) {
             launch_point_sums(s, d_grids, d_insts, (int)M, fa, psum_chunks, d_gsums);
@@ -1340,13 +1413,14 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         auto put = [&key](const void *p, size_t nb) { key.append(static_cast<const char *>(p), nb); };
         const long long sizes[] = {(long long)M, (long long)n, S, ngrid, nring, nhalf_total, nface_total,
                                    (long long)copy_hit.size(), (long long)copy_dup.size(), (long long)copy_out.size(),
+                                   (long long)flux_hit.size(), (long long)flux_dup.size(), (long long)flux_out.size(),
                                    any_route, any_ecl_stop, ecl_rounds, bits_lb, psum_chunks,
                                    (long long)fsegs.size(), (long long)task_order.size(), ntask, R, fp64,
                                    (long long)batches.size(), TILE, (long long)diag_idx.size()};
         put(sizes, sizeof(sizes));
         for (int v = 0; v < 16; v++) put(&var_range[v], sizeof(var_range[v]));
         const void *ptrs[] = {d_grids, d_rings, d_insts, d_status, d_cand, d_cand_count, d_copy_hit, d_copy_dup,
-                              d_copy_out, d_gsums, d_fsegs, d_seg_cnt, d_lists, d_counts, d_task_first, d_task_order,
+                              d_copy_out, d_flux_hit, d_flux_dup, d_flux_out, d_gsums, d_fsegs, d_seg_cnt, d_lists, d_counts, d_task_first, d_task_order,
                               d_batches, d_tasks, d_partial, d_out, d_calc, d_scal, d_scat_part, d_diag_idx, d_diag,
                               d_bits};
         put(ptrs, sizeof(ptrs));
@@ -1454,13 +1528,15 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     for (const auto &ins : inserts) {
         const int inst = gdev[ins.first].inst;
         if (ip[inst].status == LC_OK && !h_status[inst])
-            ctx->geo.map[sigs[ins.first]].push_back(GeoCache::Rec{grids[ins.first].rings, ins.second});
+            ctx->geo.map[sigs[ins.first]].push_back(GeoCache::Rec{grids[ins.first].rings, ins.second,
+                                                                  csig[ins.first]});
     }
     for (int g = 0; g < ngrid; g++) {
         if (computed(g))
             ctx->prof.grids_computed += 1;
         else
             ctx->prof.grids_reused += 1;
+        if (gdev[g].cont_copy) ctx->prof.continua_reused += 1;
     }
 
     // ---------------- finalise per instance (on the planning threads)

@@ -411,12 +411,20 @@ __global__ __launch_bounds__(256) void k_copy_faces(const FaceCopy *__restrict__
     }
 }
 
+__global__ __launch_bounds__(256) void k_copy_flux(const FaceCopy *__restrict__ items, const float *__restrict__ src,
+                                                   float *__restrict__ dst) {
+    const FaceCopy c = items[blockIdx.y];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += gridDim.x * blockDim.x)
+        dst[c.dst + i] = src[c.src + i];
+}
+
 // Continuum brightness * area of one face (set_star_continuum.cpp:103-296)
 __global__ __launch_bounds__(256) void k_continuum(const GridDev *__restrict__ grids, int ngrid, int nface_total,
                                                   FaceArrays fa) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nface_total) return;
     const GridDev &G = grids[wave_lookup(ngrid, f, [&](int g) { return grids[g].face_off; })];
+    if (G.cont_copy) return;  // fluxes copied from an identical grid afterwards
     const V3 posn = v3(fa.px[f], fa.py[f], fa.pz[f]);
     const V3 dirn = v3(fa.dx[f], fa.dy[f], fa.dz[f]);
     const V3 cofm_other = G.star == PRIMARY ? v3(1., 0., 0.) : v3(0., 0., 0.);
@@ -792,6 +800,11 @@ void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays f
 void launch_copy_faces(hipStream_t s, const FaceCopy *items, int nitem, FaceArrays src, FaceArrays dst) {
     for (int k = 0; k < nitem; k += 65535)
         hipLaunchKernelGGL(k_copy_faces, dim3(32, std::min(65535, nitem - k)), dim3(256), 0, s, items + k, src, dst);
+}
+
+void launch_copy_flux(hipStream_t s, const FaceCopy *items, int nitem, const float *src, float *dst) {
+    for (int k = 0; k < nitem; k += 65535)
+        hipLaunchKernelGGL(k_copy_flux, dim3(16, std::min(65535, nitem - k)), dim3(256), 0, s, items + k, src, dst);
 }
 
 void launch_fill_gaps(hipStream_t s, const int2 *gaps, int ngap, FaceArrays fa) {

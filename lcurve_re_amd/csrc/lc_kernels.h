@@ -33,7 +33,11 @@ struct GridDev {
     int ltype, glens;
     // 1: the eclipse search may end once its bracket holds none of the
     // instance's phases (a grid used by one instance and kept by no cache)
-    int ecl_stop, pad_;
+    int ecl_stop;
+    // 1: the fluxes equal those of another grid of the call or of the cache
+    // (same faces, same continuum inputs) and are copied after k_continuum,
+    // which skips this grid
+    int cont_copy;
 };
 
 // One latitude ring (set_star_grid.cpp:262-320).
@@ -151,6 +155,8 @@ void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays f
 // grids (not the fluxes, which the continuum kernel recomputes).
 void launch_copy_faces(hipStream_t s, const FaceCopy *items, int nitem, FaceArrays src, FaceArrays dst);
 void launch_continuum(hipStream_t s, const GridDev *grids, int ngrid, int nface_total, FaceArrays fa);
+// Copies the fluxes of whole grids (FaceCopy ranges).
+void launch_copy_flux(hipStream_t s, const FaceCopy *items, int nitem, const float *src, float *dst);
 // Exclusive prefix count of a 0/1 flag over the workgroup in thread order
 // (NT a multiple of 64), from wave ballots; the total through `total`.
 // lds_w holds NT / 64 ints.

@@ -45,6 +45,9 @@ def test_batch_dedupe_is_bit_identical(cold):
     prof = cold.profile()
     # each instance has two grids (fine = coarse): 4 geometries are copies
     assert prof["grids_reused"] == 6 and prof["grids_computed"] == 4, prof
+    # continua: both grids of the t0 model, star 1 of the gravity_dark2
+    # model (t1 changes both stars: star 2 is irradiated by T1)
+    assert prof["continua_reused"] == 3, prof
     for i, m in enumerate(models):
         c1, _, s1 = cold.light_curve_comp_batch([m], data, scale=True)
         assert s1[0] == 0
@@ -72,6 +75,59 @@ def test_cross_call_reuse(cold):
         assert e.profile()["grids_computed"] == 4
     finally:
         e.close()
+
+
+def test_continuum_reuse_across_calls(cold):
+    """A cached grid keeps its fluxes: a later call with the same faces and
+    continuum inputs copies them (continua_reused), one with another
+    temperature recomputes them; every curve equals a cold evaluation."""
+    m = _model(q=0.4, iangle=86.0, r1=0.1, r2=0.25)
+    data = _data()
+    e = L.Engine(0)
+    try:
+        e.set_stage_cache(1 << 20)
+        print("continuum reuse across calls", flush=True)
+        steps = [m, m.copy().set("t0", 0.004), m.copy().set("t2", 4500.0), m.copy().set("ldc1_1", 0.55)]
+        want = [0, 2, 0, 2]   # t2 changes star 2's continuum and star 1's irradiation input
+        for mm, w in zip(steps, want):
+            e.reset_profile()
+            got, _, st = e.light_curve_comp_batch([mm], data)
+            ref, _, _ = cold.light_curve_comp_batch([mm], data)
+            assert st[0] == 0 and np.array_equal(got, ref)
+            assert e.profile()["continua_reused"] == w, (mm, e.profile())
+    finally:
+        e.close()
+
+
+def test_fit_batch_reuses_continua(cold):
+    """The LM Jacobian batch: the t0 and limb-darkening columns share the
+    base point's faces and continuum; the Jacobian equals the one computed
+    with every grid built on its own (LCURVE_AMD_DEDUPE=0)."""
+    import os
+
+    from lcurve_re_amd import fit as F
+    cfg = W.model_json(vary=W.FIT_PARAMS, comp={"nlat1f": 24, "nlat2f": 24, "nlat1c": 24, "nlat2c": 24})
+    m = L.Model.from_json(json.dumps(cfg))
+    data = L.make_data(np.arange(800) / 800.0, ferr=1.5e-3, weight=1.0, flux=1.0)
+    print("continuum reuse in the fit batch", flush=True)
+    cold.reset_profile()
+    a = F.FitBatch(m, data, F.engine_evaluator(cold), scale=True).jacobian()
+    p = cold.profile()
+    # t0 (2 instances), the limb-darkening columns (6: two are one-sided), 2
+    # grids each, and star 1 of the two gravity_dark2 instances
+    assert p["continua_reused"] == 18, p
+    os.environ["LCURVE_AMD_DEDUPE"] = "0"
+    try:
+        e = L.Engine(0)
+    finally:
+        del os.environ["LCURVE_AMD_DEDUPE"]
+    try:
+        e.set_stage_cache(0)
+        b = F.FitBatch(m, data, F.engine_evaluator(e), scale=True).jacobian()
+        assert e.profile()["continua_reused"] == 0
+    finally:
+        e.close()
+    assert np.array_equal(a["J"], b["J"]) and a["chisq0"] == b["chisq0"]
 
 
 def test_cache_smaller_than_a_grid(cold):
