@@ -7,6 +7,10 @@ uncertainty marks a point to be ignored: ferr becomes 0 and weight 0
 (lcurve.cpp:35-44): time with 8 decimals in 17 columns, expose with 8,
 flux with 5 decimals in 10 columns, ferr and weight with 5, ndiv.
 """
+import json
+import math
+import sys
+
 import numpy as np
 
 from . import make_data, data_view
@@ -97,3 +101,83 @@ def write_chain(path, names, pars, chisq, log_prior=None, thin=1):
         for i in range(0, len(pars), thin):
             f.write("%d%s,%s,%s\n" % (i, "".join("," + format(v, "g") for v in pars[i]), format(chisq[i], "g"),
                                       format(lp[i], "g")))
+
+
+# ------------------------------------------------------------ plot frames
+def normalized_plot_device(device):
+    """Plot device name trimmed of whitespace and quotes, lower case
+    (new_helpers.h:35-46)."""
+    return str(device).strip(" \t\n\r\v\f\"'").lower()
+
+
+def plotting_streamed(device):
+    """True for the machine-readable stream backend (new_helpers.h:52-56)."""
+    return normalized_plot_device(device) in ("stream", "json-stream", "astra")
+
+
+def plotting_disabled(device):
+    """True when the device switches plotting off (new_helpers.h:62-69)."""
+    return normalized_plot_device(device) in ("", "none", "null", "disable", "disabled", "off", "false", "no")
+
+
+def _json_number(x):
+    x = float(x)
+    return x if math.isfinite(x) else None   # nlohmann dumps NaN and inf as null
+
+
+def model_frame(data, fit, no_file=False, copy=None, metadata=None):
+    """The JSON text of one "lcurve.plot.v1" frame (new_helpers.h:71-109):
+    x (time minus an origin when the span is below 1% of the mean time),
+    flux, error (ferr of `copy` unless no_file; non-positive or non-finite
+    ones become 1e-3), model, residual, and meta when given; keys sorted and
+    no spaces, as nlohmann::json::dump() writes them.  None when there is
+    nothing to plot."""
+    v = data_view(data)
+    fit = np.asarray(fit, np.float64)
+    n = len(v)
+    if n == 0 or fit.size < n:
+        return None
+    t = v["time"]
+    x1, x2 = float(np.min(t)), float(np.max(t))
+    origin = x1 if x2 - x1 < 0.01 * abs((x1 + x2) / 2.0) else 0.0
+    src = v if (no_file or copy is None) else data_view(copy)
+    frame = {"protocol": "lcurve.plot.v1", "x": [], "flux": [], "error": [], "model": [], "residual": []}
+    for i in range(n):
+        sigma = float(src["ferr"][i])
+        if not (sigma > 0.0) or not math.isfinite(sigma):
+            sigma = 1e-3
+        f = float(v["flux"][i])
+        frame["x"].append(_json_number(float(t[i]) - origin))
+        frame["flux"].append(_json_number(f))
+        frame["error"].append(_json_number(sigma))
+        frame["model"].append(_json_number(fit[i]))
+        frame["residual"].append(_json_number((f - float(fit[i])) / sigma))
+    if metadata:
+        frame["meta"] = metadata
+    return json.dumps(frame, sort_keys=True, separators=(",", ":"), allow_nan=False)
+
+
+def stream_model_frame(data, fit, no_file=False, copy=None, metadata=None, out=None):
+    """Writes "@@LCURVE_PLOT@@" + the frame + a newline to `out` (stdout)
+    and flushes, as Helpers::stream_model_frame does for the stream plot
+    device."""
+    text = model_frame(data, fit, no_file, copy, metadata)
+    if text is None:
+        return
+    out = sys.stdout if out is None else out
+    out.write("@@LCURVE_PLOT@@" + text + "\n")
+    out.flush()
+
+
+def plot_model(data, fit, no_file, copy, device, out=None):
+    """Helpers::plot_model (new_helpers.h:111-122) for the devices that do
+    not open gnuplot: nothing when the device disables plotting, one frame
+    with meta {"phase": "final"} for the stream device.  Returns True when a
+    frame was written; gnuplot devices (an external program) return False
+    without plotting."""
+    if plotting_disabled(device):
+        return False
+    if plotting_streamed(device):
+        stream_model_frame(data, fit, no_file, copy, {"phase": "final"}, out=out)
+        return True
+    return False

@@ -94,3 +94,54 @@ def test_chain_file(tmp_path, cxx_format):
     assert lines[1] == f"0,{g(2)},{g(3)},{g(12)},0"
     assert lines[2] == f"2,{g(9)},{g(10)},{g(8)},0"
     assert len(lines) == 3
+
+
+# ---------------------------------------------------------- plot frames
+FRAME_EXE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cxx", "_build", "json_frame_check")
+
+
+@pytest.mark.skipif(not os.path.exists(FRAME_EXE), reason="tests/cxx/json_frame_check not built (no json.hpp)")
+@pytest.mark.parametrize("seed,base_time", [(1, 0.0), (2, 58000.25), (3, -3.5)])
+def test_plot_frame_matches_nlohmann(seed, base_time):
+    """The stream plot device's frame (new_helpers.h:71-109) byte for byte
+    against the same frame built with nlohmann::json: sorted keys, shortest
+    round-trip numbers, the time origin rule, the error floor and meta."""
+    import subprocess
+
+    import lcurve_re_amd as L
+    from lcurve_re_amd import data_io as D
+    rng = np.random.default_rng(seed)
+    n = 40
+    t = base_time + np.sort(rng.uniform(0, 0.2, n))
+    flux = np.concatenate([rng.normal(1, 0.1, n - 4), [0.0, 1e-12, 3e20, -2.5]])
+    ferr = np.concatenate([rng.uniform(1e-4, 1e-2, n - 3), [0.0, -1.0, np.inf]])
+    fit = flux * (1 + rng.normal(0, 1e-3, n))
+    meta = {"iteration": int(seed), "chisq": float(rng.uniform(1, 1e4)), "label": "lm"}
+    data = L.make_data(t, ferr=ferr, flux=flux)
+    got = "@@LCURVE_PLOT@@" + D.model_frame(data, fit, no_file=True, metadata=meta) + "\n"
+    text = f"{n}\n" + "".join(f"{float(a)!r} {float(b)!r} {float(c)!r} {float(d)!r}\n"
+                              for a, b, c, d in zip(t, flux, ferr, fit)) + json.dumps(meta) + "\n"
+    r = subprocess.run([FRAME_EXE], input=text, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert got == r.stdout
+    # the writer adds the marker and flushes
+    import io
+    buf = io.StringIO()
+    D.stream_model_frame(data, fit, no_file=True, metadata=meta, out=buf)
+    assert buf.getvalue() == r.stdout
+    assert D.plotting_streamed(' "Stream" ') and D.plotting_disabled("'None'") and not D.plotting_disabled("x11")
+
+
+def test_plot_model_devices():
+    import io
+
+    import lcurve_re_amd as L
+    from lcurve_re_amd import data_io as D
+    data = L.make_data(np.array([0.1, 0.2]), ferr=np.array([0.01, 0.02]), flux=np.array([1.0, 1.1]))
+    buf = io.StringIO()
+    assert D.plot_model(data, [1.0, 1.0], False, data, " Stream ", out=buf)
+    frame = json.loads(buf.getvalue()[len("@@LCURVE_PLOT@@"):])
+    assert frame["meta"] == {"phase": "final"} and frame["protocol"] == "lcurve.plot.v1"
+    assert frame["residual"] == [0.0, (1.1 - 1.0) / 0.02]
+    buf = io.StringIO()
+    assert not D.plot_model(data, [1.0, 1.0], False, data, "NULL", out=buf) and buf.getvalue() == ""
