@@ -34,6 +34,7 @@
 #include <unistd.h>
 
 #include "../../include/lcurve_amd.h"
+#include "lc_comp.h"
 #include "lc_host.h"
 #include "lc_kernels.h"
 #include "lc_math.h"
@@ -759,14 +760,17 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     std::vector<InstPlan> ip(M);
     std::vector<InstDev> insts(M);
     std::vector<std::vector<GridPlan>> local(M);
+    std::vector<CompHost> chost(M);  // disc, edge and bright spot elements
     parallel_for(M, [&](size_t i) {
         const lc_model &m = rq.models[i];
         InstPlan &P = ip[i];
         InstDev &I = insts[i];
         std::memset(&I, 0, sizeof(I));
         try {
-            if (m.add_disc || m.add_spot)
-                throw EngineError(LC_E_UNSUPPORTED, "disc and bright spot components are not supported");
+            if (m.add_disc && m.opaque)
+                throw EngineError(LC_E_UNSUPPORTED, "opaque discs (the disc eclipsing the stars) are not supported");
+            if ((m.add_disc || m.add_spot) && m.iscale)
+                throw EngineError(LC_E_UNSUPPORTED, "iscale with disc or bright spot components is not supported");
             double r1, r2;
             get_r1r2(m, r1, r2);
             double rl2 = 1. - l1(2, m.q.value, m.spin2.value);
@@ -811,6 +815,12 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             I.copy2 = P.copy2;
             I.scale = o.scale;
             I.rdata = o.rdata;
+            if (m.add_disc || m.add_spot) {
+                std::string cerr;
+                const int cst = plan_components(m, chost[i], cerr);
+                if (cst) throw EngineError(cst, cerr);
+                I.has_comp = chost[i].size() > 0 ? 1 : 0;
+            }
             I.active = 1;
         } catch (const EngineError &e) {
             P.status = e.code;
@@ -970,6 +980,83 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     for (const auto &ins : inserts)
         flux_out.push_back(FaceCopy{gdev[ins.first].face_off, ins.second.off, ins.second.n, 0});
 
+    // disc, edge and bright spot: their elements in one table (instance
+    // order), each in two slots of the component face arrays -- [off, off+n)
+    // searched for eclipses by star 1, [NC + off, ...) by star 2 -- through
+    // pseudo-grids that name that star as the companion
+    std::vector<CompDev> comps;
+    int NC = 0;
+    for (size_t i = 0; i < M; i++) {
+        if (ip[i].status != LC_OK || !insts[i].has_comp) continue;
+        const CompHost &c = chost[i];
+        CompDev C;
+        std::memset(&C, 0, sizeof(C));
+        C.inst = (int)i;
+        C.off = NC;
+        C.n_disc = c.n_disc;
+        C.n_edge = c.n_edge;
+        C.n_spot = c.n_spot;
+        C.opaque = c.opaque;
+        C.lin = c.lin;
+        C.quad = c.quad;
+        C.rdisc1 = c.rdisc1;
+        C.rdisc2 = c.rdisc2;
+        C.beta = c.beta;
+        C.height = c.height;
+        C.iangle = c.iangle;
+        C.sum_off = (long long)comps.size() * 3 * S;
+        insts[i].comp_off = C.sum_off;
+        comps.push_back(C);
+        NC += c.size();
+    }
+    const int ncomp = (int)comps.size();
+    std::vector<GridDev> cgrids;
+    std::vector<double> cpx, cpy, cpz, cdx, cdy, cdz;
+    std::vector<float> cflux;
+    if (ncomp) {
+        cgrids.resize(2 * ncomp);
+        for (int by = 0; by < 2; by++)
+            for (int k = 0; k < ncomp; k++) {
+                const CompHost &c = chost[comps[k].inst];
+                GridDev &G = cgrids[by * ncomp + k];
+                std::memset(&G, 0, sizeof(G));
+                G.inst = comps[k].inst;
+                G.nface = c.size();
+                G.face_off = by * NC + comps[k].off;
+                // by star 1: the element plays a star-2 face (companion star
+                // 1), by star 2 a star-1 face (star_eclipse.cpp:15-25)
+                G.star = by == 0 ? 1 : 0;
+                G.eclipse = by == 0 ? c.eclipse1 : c.eclipse2;
+                G.roche_other = by == 0 ? c.roche1 : c.roche2;
+                G.q = c.q;
+                G.spin_other = by == 0 ? c.spin1 : c.spin2;
+                G.rref_other = by == 0 ? c.rref1 : c.rref2;
+                G.pref_other = by == 0 ? c.pref1 : c.pref2;
+                G.r_other = by == 0 ? c.r1 : c.r2;
+                G.cosi = c.cosi;
+                G.sini = c.sini;
+                G.delta = c.delta;
+                G.acc = c.delta / 10.;
+                G.cont_copy = 1;
+            }
+        cpx.resize(2 * (size_t)NC); cpy.resize(2 * (size_t)NC); cpz.resize(2 * (size_t)NC);
+        cdx.resize(NC); cdy.resize(NC); cdz.resize(NC); cflux.resize(NC);
+        for (const CompDev &C : comps) {
+            const CompHost &c = chost[C.inst];
+            for (int k = 0; k < c.size(); k++) {
+                const size_t f = (size_t)C.off + k;
+                cpx[f] = cpx[NC + f] = c.px[k];
+                cpy[f] = cpy[NC + f] = c.py[k];
+                cpz[f] = cpz[NC + f] = c.pz[k];
+                cdx[f] = c.dx[k];
+                cdy[f] = c.dy[k];
+                cdz[f] = c.dz[k];
+                cflux[f] = c.flux[k];
+            }
+        }
+    }
+    const size_t CN = (size_t)std::max(NC, 1), CN2 = 2 * CN;
+
     // Eclipse searches that may stop early: grids used by one instance only
     // (no in-batch copies) and offered to no cache, whose ranges are then
     // tested only at that instance's phases.
@@ -1126,6 +1213,27 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<double>(FN);
     sz.add<double>(FN);
     sz.add<int2>(FN);  // cand
+    // components: uploaded tables, slots, their eclipse search, records, sums
+    sz.add<CompDev>(std::max(ncomp, 1));
+    sz.add<GridDev>(std::max(2 * ncomp, 1));
+    for (int k = 0; k < 3; k++) sz.add<double>(CN2);
+    for (int k = 0; k < 3; k++) sz.add<double>(CN);
+    sz.add<float>(CN);
+    sz.add<int>(CN2);
+    sz.add<double>(CN2);
+    sz.add<double>(CN2);
+    sz.add<int2>(CN2);
+    sz.add<int>(1);                                     // candidate count
+    sz.add<int>(1);                                     // hit count
+    for (int k = 0; k < 2; k++) sz.add<int>(CN2);       // face, grid
+    for (int k = 0; k < 2; k++) sz.add<int>(2 * CN2);   // round lists
+    for (int k = 0; k < 6; k++) sz.add<double>(2 * CN2);  // phi, lim, acc, res, b_in, b_out
+    sz.add<int>(ECL_COUNTS);
+    for (int k = 0; k < 3; k++) sz.add<double>(CN);
+    sz.add<float>(CN);
+    sz.add<int>(CN);
+    sz.add<double2>(CN * COMP_MAX_RANGES);
+    sz.add<double>(std::max<size_t>((size_t)ncomp * 3 * S, 1));
     sz.add<int>(1);
     sz.add<int>(1);      // eclipse work lists
     sz.add<int>(FN);
@@ -1206,6 +1314,13 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     int *d_diag_idx = A.take<int>(std::max<size_t>(diag_idx.size(), 1));
     FlatSeg *d_fsegs = A.take<FlatSeg>(fsegs.size());
     int *d_seg_cnt = A.take<int>(fsegs.size());
+    CompDev *d_comps = A.take<CompDev>(std::max(ncomp, 1));
+    GridDev *d_cgrids = A.take<GridDev>(std::max(2 * ncomp, 1));
+    FaceArrays cs;  // component slots: positions (both halves), normals and fluxes (first half)
+    std::memset(&cs, 0, sizeof(cs));
+    cs.px = A.take<double>(CN2); cs.py = A.take<double>(CN2); cs.pz = A.take<double>(CN2);
+    cs.dx = A.take<double>(CN); cs.dy = A.take<double>(CN); cs.dz = A.take<double>(CN);
+    cs.flux = A.take<float>(CN);
     FaceArrays fa;
     fa.px = A.take<double>(FN); fa.py = A.take<double>(FN); fa.pz = A.take<double>(FN);
     fa.dx = A.take<double>(FN); fa.dy = A.take<double>(FN); fa.dz = A.take<double>(FN);
@@ -1262,6 +1377,31 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     double *d_gsums = A.take<double>(std::max<size_t>(M, 1) * 8 * psum_chunks);
     double *d_scat_part = A.take<double>(scat_part);
     double *d_diag = A.take<double>(std::max<size_t>(diag_idx.size(), 1) * 8);
+    cs.ecl = A.take<int>(CN2);
+    cs.ein = A.take<double>(CN2);
+    cs.eout = A.take<double>(CN2);
+    int2 *d_ccand = A.take<int2>(CN2);
+    int *d_ccount = A.take<int>(1);
+    EclipseWork cew;
+    std::memset(&cew, 0, sizeof(cew));
+    cew.hit_count = A.take<int>(1);
+    cew.face = A.take<int>(CN2);
+    cew.grid = A.take<int>(CN2);
+    cew.alist[0] = A.take<int>(2 * CN2);
+    cew.alist[1] = A.take<int>(2 * CN2);
+    cew.phi = A.take<double>(CN2 * 2);
+    cew.lim = A.take<double>(2 * CN2);
+    cew.acc = A.take<double>(2 * CN2);
+    cew.res = A.take<double>(2 * CN2);
+    cew.b_in = A.take<double>(2 * CN2);
+    cew.b_out = A.take<double>(2 * CN2);
+    cew.acount = A.take<int>(ECL_COUNTS);
+    CompArrays ca;
+    ca.dx = A.take<double>(CN); ca.dy = A.take<double>(CN); ca.dz = A.take<double>(CN);
+    ca.flux = A.take<float>(CN);
+    ca.nrange = A.take<int>(CN);
+    ca.range = A.take<double2>(CN * COMP_MAX_RANGES);
+    ca.sums = A.take<double>(std::max<size_t>((size_t)ncomp * 3 * S, 1));
 
     // ---------------- uploads through one pinned staging block
     std::vector<int> ndiv(n);
@@ -1303,6 +1443,11 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         {d_flux_dup, flux_dup.data(), flux_dup.size() * sizeof(FaceCopy)},
         {d_flux_out, flux_out.data(), flux_out.size() * sizeof(FaceCopy)},
         {d_gaps, gaps.data(), gaps.size() * sizeof(int2)},
+        {d_comps, comps.data(), comps.size() * sizeof(CompDev)},
+        {d_cgrids, cgrids.data(), cgrids.size() * sizeof(GridDev)},
+        {cs.px, cpx.data(), cpx.size() * 8}, {cs.py, cpy.data(), cpy.size() * 8}, {cs.pz, cpz.data(), cpz.size() * 8},
+        {cs.dx, cdx.data(), cdx.size() * 8}, {cs.dy, cdy.data(), cdy.size() * 8}, {cs.dz, cdz.data(), cdz.size() * 8},
+        {cs.flux, cflux.data(), cflux.size() * sizeof(float)},
     };
     size_t up_end = 0;  // the uploaded arrays lead the arena
     for (auto &u : ups)
@@ -1343,6 +1488,11 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     HIPCHK(hipMemsetAsync(d_cand_count, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(ew.hit_count, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(ew.acount, 0, ECL_COUNTS * sizeof(int), s));
+    if (ncomp) {
+        HIPCHK(hipMemsetAsync(d_ccount, 0, sizeof(int), s));
+        HIPCHK(hipMemsetAsync(cew.hit_count, 0, sizeof(int), s));
+        HIPCHK(hipMemsetAsync(cew.acount, 0, ECL_COUNTS * sizeof(int), s));
+    }
     launch_fill_gaps(s, d_gaps, (int)gaps.size(), fa);  // one launch for all gaps
 
     DataDev dd;
@@ -1370,6 +1520,11 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         launch_faces(s, d_grids, d_rings, nring, nhalf_total, fa, d_cand, d_cand_count, d_status);
         mark(ev[1]);
         launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, ew, d_status, ecl_rounds);
+        if (ncomp) {  // the components' eclipses by the stars, then their ranges
+            launch_comp_pretest(s, d_cgrids, 2 * ncomp, 2 * NC, cs, d_ccand, d_ccount);
+            launch_eclipse(s, d_cgrids, 2 * ncomp, cs, d_ccand, d_ccount, 2 * NC, cew, d_status, false);
+            launch_comp_build(s, d_comps, ncomp, NC, cs, ca, d_status);
+        }
         // reused grids: cache hits first (a duplicate may copy from one), then
         // duplicates within the batch, then the new grids into the cache
         if (!copy_hit.empty()) launch_copy_faces(s, d_copy_hit, (int)copy_hit.size(), ctx->geo.pool, fa);
@@ -1390,6 +1545,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         }
         launch_flat_build(s, d_grids, d_fsegs, (int)fsegs.size(), d_seg_cnt, fa, fl, fp64 ? 1 : 0);
         launch_phases(s, d_insts, (int)M, dd, pa);
+        if (ncomp) launch_comp_sums(s, d_comps, ncomp, S, pa, ca);
         if (any_route) launch_route(s, d_insts, (int)M, S, pa, d_lists, d_counts);
         launch_make_tasks(s, d_task_first, d_task_order, (int)task_order.size(), d_batches, ntask, d_tasks);
         mark(ev[4]);
@@ -1402,7 +1558,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         }
         mark(ev[5]);
         launch_flux_reduce(s, d_batches, (int)batches.size(), S, TILE, d_lists, d_counts, d_partial, d_out);
-        launch_scatter(s, d_insts, (int)M, dd, pa, d_out, d_calc, d_scal, d_scat_part);
+        launch_scatter(s, d_insts, (int)M, dd, pa, d_out, ca.sums, d_calc, d_scal, d_scat_part);
         if (!diag_idx.empty()) launch_diag(s, d_grids, d_diag_idx, (int)diag_idx.size(), fa, d_diag);
     };
     // A graph replays the exact launches (grids, blocks, arguments) of its
@@ -1414,13 +1570,14 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         const long long sizes[] = {(long long)M, (long long)n, S, ngrid, nring, nhalf_total, nface_total,
                                    (long long)copy_hit.size(), (long long)copy_dup.size(), (long long)copy_out.size(),
                                    (long long)flux_hit.size(), (long long)flux_dup.size(), (long long)flux_out.size(),
+                                   ncomp, NC,
                                    any_route, any_ecl_stop, ecl_rounds, bits_lb, psum_chunks,
                                    (long long)fsegs.size(), (long long)task_order.size(), ntask, R, fp64,
                                    (long long)batches.size(), TILE, (long long)diag_idx.size()};
         put(sizes, sizeof(sizes));
         for (int v = 0; v < 16; v++) put(&var_range[v], sizeof(var_range[v]));
         const void *ptrs[] = {d_grids, d_rings, d_insts, d_status, d_cand, d_cand_count, d_copy_hit, d_copy_dup,
-                              d_copy_out, d_flux_hit, d_flux_dup, d_flux_out, d_gsums, d_fsegs, d_seg_cnt, d_lists, d_counts, d_task_first, d_task_order,
+                              d_copy_out, d_flux_hit, d_flux_dup, d_flux_out, d_comps, d_cgrids, d_ccand, d_ccount, d_gsums, d_fsegs, d_seg_cnt, d_lists, d_counts, d_task_first, d_task_order,
                               d_batches, d_tasks, d_partial, d_out, d_calc, d_scal, d_scat_part, d_diag_idx, d_diag,
                               d_bits};
         put(ptrs, sizeof(ptrs));
@@ -1430,6 +1587,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         put(&fl, sizeof(fl));
         put(&pa, sizeof(pa));
         put(&ew, sizeof(ew));
+        put(&cs, sizeof(cs));
+        put(&cew, sizeof(cew));
+        put(&ca, sizeof(ca));
         if (ctx->g_exec && key == ctx->g_key) {
             graphed = true;
         } else if (key == ctx->g_seen) {  // second call of this shape: capture it

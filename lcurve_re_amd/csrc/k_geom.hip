@@ -10,6 +10,7 @@
 // (ingress_egress.cpp:35-97) over that compacted list only.
 #include <algorithm>
 
+#include "lc_comp.h"
 #include "lc_kernels.h"
 #include "lc_math.h"
 
@@ -166,6 +167,34 @@ __global__ __launch_bounds__(LC_FACES_NT) void k_faces(const GridDev *__restrict
     for (int v = 0; v < w; v++) k += s_w[v];
     if (c1) cand[k++] = make_int2(f1, gidx);  // the search needs the face's grid
     if (c2) cand[k] = make_int2(f2, gidx);
+}
+
+// Eclipse pre-test of component elements (disc, edge, bright spot): every
+// element sits in two slots, one per eclipsing star, whose pseudo-grids
+// name that star as the companion; candidates go to the search as star
+// faces do (star_eclipse.cpp:15-25 for each star in turn).
+__global__ __launch_bounds__(256) void k_comp_pretest(const GridDev *__restrict__ grids, int ngrid, int nface,
+                                                      FaceArrays fa, int2 *cand, int *cand_count) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    bool c = false;
+    int g = 0;
+    if (f < nface) {
+        g = wave_lookup(ngrid, f, [&](int k) { return grids[k].face_off; });
+        c = eclipse_pretest(grids[g], v3(fa.px[f], fa.py[f], fa.pz[f]), f, fa);
+    }
+    __shared__ int s_w[4], s_base;
+    const unsigned long long b = __ballot(c);
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    if (lane == 0) s_w[w] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        s_base = tot ? atomicAdd(cand_count, tot) : 0;
+    }
+    __syncthreads();
+    int k = s_base + __popcll(b & ((1ull << lane) - 1ull));
+    for (int v = 0; v < w; v++) k += s_w[v];
+    if (c) cand[k] = make_int2(f, g);
 }
 
 // FP64 ingress/egress search (ingress_egress.cpp:35-97) over the compacted
@@ -805,6 +834,13 @@ void launch_copy_faces(hipStream_t s, const FaceCopy *items, int nitem, FaceArra
 void launch_copy_flux(hipStream_t s, const FaceCopy *items, int nitem, const float *src, float *dst) {
     for (int k = 0; k < nitem; k += 65535)
         hipLaunchKernelGGL(k_copy_flux, dim3(16, std::min(65535, nitem - k)), dim3(256), 0, s, items + k, src, dst);
+}
+
+void launch_comp_pretest(hipStream_t s, const GridDev *grids, int ngrid, int nface, FaceArrays fa, int2 *cand,
+                         int *cand_count) {
+    if (nface <= 0 || ngrid <= 0) return;
+    hipLaunchKernelGGL(k_comp_pretest, dim3((nface + 255) / 256), dim3(256), 0, s, grids, ngrid, nface, fa, cand,
+                       cand_count);
 }
 
 void launch_fill_gaps(hipStream_t s, const int2 *gaps, int ngap, FaceArrays fa) {

@@ -170,9 +170,31 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
 // star 2 fine, star 2 coarse, plus third light (light_curve_comp.cpp:
 // 476-498); with SPLIT the two stars apart and no third light, the iscale
 // components (light_curve_comp.cpp:378-416).
+// The disc, edge and bright spot at one data point: each one's
+// sub-exposure sums with the trapezoid weights over max(1, ndiv - 1), their
+// total times the slope factor (comp_light.cpp:322-407,
+// light_curve_comp.cpp:483-497).
+__device__ __forceinline__ double comp_point(const InstDev &I, const DataDev &d, const double *comp, int np) {
+    const double frac = (d.time[np] - d.middle) / d.range;
+    const double slfac = 1. + frac * (I.slope + frac * (I.quad + frac * I.cube));
+    const int j0 = d.sub_off[np], ndiv = d.ndiv[np];
+    double c[3];
+    for (int kind = 0; kind < 3; kind++) {
+        const double *sk = comp + I.comp_off + (size_t)kind * d.S + j0;
+        double sum = 0.0;
+        for (int nd = 0; nd < ndiv; nd++) {
+            const double wgt = (ndiv == 1) ? 1.0 : ((nd == 0 || nd == ndiv - 1) ? 0.5 : 1.0);
+            sum += wgt * sk[nd];
+        }
+        c[kind] = sum / max(1, ndiv - 1);
+    }
+    return slfac * (c[0] + c[1] + c[2]);
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void point_sums(const InstDev &I, const DataDev &d, const PhaseArrays &pa, size_t base,
-                                           const double *o1, const double *o2, int np, double &v1, double &v2) {
+                                           const double *o1, const double *o2, const double *comp, int np,
+                                           double &v1, double &v2) {
     const int j0 = d.sub_off[np], nd = d.ndiv[np];
     double v = 0.;
     for (int k = 0; k < nd; k++) {
@@ -197,14 +219,17 @@ __device__ __forceinline__ void point_sums(const InstDev &I, const DataDev &d, c
             size_t j = j0 + k;
             if (pa.type[base + j] != 3) v += pa.w2[base + j] * o2[j];
         }
-    if (SPLIT)
+    if (SPLIT) {
         v2 = I.star2 ? v : 0.;  // star 2 is a component only when t2 > 0
-    else
+    } else {
+        if (I.has_comp) v += comp_point(I, d, comp, np);
         v1 = v + I.third;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__ insts, DataDev d, PhaseArrays pa,
-                                                    const double *__restrict__ out, double *calc, double *part) {
+                                                    const double *__restrict__ out, const double *__restrict__ comp,
+                                                    double *calc, double *part) {
     __shared__ double lds[256];
     const int inst = blockIdx.y, nblk = gridDim.x;
     const InstDev &I = insts[inst];
@@ -220,7 +245,7 @@ __global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__
             // rows of the weighted design matrix: weight > 0 and
             // sigma = ferr / sqrt(weight) > 0 (svdfit.cpp:34-53)
             double v1, v2;
-            point_sums<true>(I, d, pa, base, o1, o2, np, v1, v2);
+            point_sums<true>(I, d, pa, base, o1, o2, comp, np, v1, v2);
             c[np] = v1;
             if (d.weight[np] > 0.) {
                 wn += d.weight[np];
@@ -237,7 +262,7 @@ __global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__
             continue;
         }
         double v, unused;
-        point_sums<false>(I, d, pa, base, o1, o2, np, v, unused);
+        point_sums<false>(I, d, pa, base, o1, o2, comp, np, v, unused);
         c[np] = v;
         if (d.weight[np] > 0.) {
             double wgt = d.weight[np] / sqr(d.ferr[np]);
@@ -329,7 +354,8 @@ __device__ __forceinline__ void inst_scale(const InstDev &I, const double *pi, i
 }
 
 __global__ __launch_bounds__(256) void k_scatter_scale(const InstDev *__restrict__ insts, DataDev d, PhaseArrays pa,
-                                                      const double *__restrict__ out, double *calc, double *part) {
+                                                      const double *__restrict__ out, const double *__restrict__ comp,
+                                                      double *calc, double *part) {
     __shared__ double lds[256];
     const int inst = blockIdx.y, nblk = gridDim.x;
     const InstDev &I = insts[inst];
@@ -351,7 +377,7 @@ __global__ __launch_bounds__(256) void k_scatter_scale(const InstDev *__restrict
         double v;
         if (I.iscale) {
             double v1, v2;
-            point_sums<true>(I, d, pa, base, o1, o2, np, v1, v2);
+            point_sums<true>(I, d, pa, base, o1, o2, comp, np, v1, v2);
             v = a0 * v1 + zc;
             if (I.star2) v += a4 * v2;
         } else {
@@ -417,11 +443,11 @@ void launch_route(hipStream_t s, const InstDev *insts, int ninst, int S, PhaseAr
 }
 
 void launch_scatter(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa, const double *out,
-                    double *calc, double *scal, double *part) {
+                    const double *comp, double *calc, double *scal, double *part) {
     if (ninst <= 0) return;
     const int nblk = std::max(1, (d.n + SCAT_PTS - 1) / SCAT_PTS);
-    hipLaunchKernelGGL(k_scatter_sum, dim3(nblk, ninst), dim3(256), 0, s, insts, d, pa, out, calc, part);
-    hipLaunchKernelGGL(k_scatter_scale, dim3(nblk, ninst), dim3(256), 0, s, insts, d, pa, out, calc, part);
+    hipLaunchKernelGGL(k_scatter_sum, dim3(nblk, ninst), dim3(256), 0, s, insts, d, pa, out, comp, calc, part);
+    hipLaunchKernelGGL(k_scatter_scale, dim3(nblk, ninst), dim3(256), 0, s, insts, d, pa, out, comp, calc, part);
     hipLaunchKernelGGL(k_scatter_final, dim3(ninst), dim3(64), 0, s, insts, nblk, part, scal);
 }
 

@@ -54,7 +54,9 @@ struct InstDev {
     double zcols;         // iscale, scale off: sfac[1..3] times the disc, edge and spot columns (0)
     int copy1, copy2, grid1f, grid2f, grid1c, grid2c;
     int scale, rdata, active, iscale;
-    int star2, pad[3];  // iscale: star 2 is a component (t2 > 0)
+    int star2;           // iscale: star 2 is a component (t2 > 0)
+    int has_comp;        // disc, edge or bright spot present
+    long long comp_off;  // first of its disc, edge and spot sums (3 x S)
 };
 
 // Phases at which light_curve_comp.cpp:220-254 samples the Point-path sums.
@@ -218,8 +220,9 @@ void launch_flux_reduce(hipStream_t s, const BatchDev *batches, int nbatch, int 
                         const int *lists, const int *counts, const double *partial, double *out);
 // part: workspace of ninst * ceil(n / SCAT_PTS) * SCAT_PARTS doubles
 constexpr int SCAT_PTS = 1024, SCAT_PARTS = 8;
+// comp: the disc, edge and spot sums of instances with has_comp (else unused)
 void launch_scatter(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa, const double *out,
-                    double *calc, double *scal, double *part);
+                    const double *comp, double *calc, double *scal, double *part);
 void launch_diag(hipStream_t s, const GridDev *grids, const int *gidx, int ng, FaceArrays fa, double *sums);
 
 }  // namespace lck

@@ -11,6 +11,8 @@ configuration, see oracle/ref_harness.cpp):
   into 1, 3 or 5 sub-phases, some zero weights): unscaled with diagnostics
   (wdwarf, logg1/2, rv1/2), scaled, the fast path, and iscale;
 * comp_gravity on the case grids with the continuum fluxes;
+* the disc, disc edge and bright spot of cataclysmic-variable-like binaries
+  (light_curve_comp.cpp:272-333, 483-497), unscaled, scaled and fast;
 * the detached workload exactly (nlat 40, 1000 phases k/1000), the fit
   workload's base curve (8000 phases) and a reduced semi-detached workload
   (coarse nlat 30, fine nlat 60 with the infill strip, 2000 x 5 sub-phases).
@@ -48,6 +50,25 @@ VARIANTS = {
     "spots": (SPOTS, {}),
     "all": (dict(SPOTS, gravity_dark1=0.2, gravity_dark2=0.3, absorb=1.0, t2=6000.0, beam_factor1=1.3,
                  beam_factor2=1.6), {"gdark_bolom1": 0, "gdark_bolom2": 1, "mirror": 1}),
+}
+
+
+# A cataclysmic-variable-like binary: white dwarf, Roche-filling donor, a
+# flared disc with its outer edge, and the bright spot where the stream meets
+# the disc's rim.
+CV_PHYS = {"q": 0.15, "iangle": 82.0, "r1": 0.015, "r2": 0.99 * (1 - W.xl12(0.15)), "t1": 15000.0, "t2": 3000.0,
+           "rdisc1": 0.0, "rdisc2": 0.3, "height_disc": 0.01, "beta_disc": 1.5, "temp_disc": 5000.0,
+           "texp_disc": -0.75, "lin_limb_disc": 0.4, "quad_limb_disc": 0.1, "temp_edge": 4000.0, "absorb_edge": 0.5,
+           "radius_spot": 0.3, "length_spot": 0.02, "height_spot": 0.01, "expon_spot": 2.0, "epow_spot": 1.0,
+           "angle_spot": 140.0, "yaw_spot": 10.0, "temp_spot": 12000.0, "tilt_spot": 60.0, "cfrac_spot": 0.3}
+CV_COMP = {"nlat1f": 16, "nlat1c": 16, "nlat2f": 24, "nlat2c": 24, "nrad": 30, "nspot": 80}
+COMPONENT_CASES = {
+    "cv_disc_spot": (CV_PHYS, dict(CV_COMP, add_disc=1, add_spot=1)),
+    "cv_disc": (dict(CV_PHYS, iangle=86.0, rdisc1=0.03, texp_disc=-1.2), dict(CV_COMP, add_disc=1, nrad=25)),
+    "cv_spot_sphere": (dict(CV_PHYS, iangle=78.0, beam_factor2=1.4), dict(CV_COMP, add_spot=1, roche1=0, nspot=50)),
+    "cv_thick_no_eclipse1": (dict(CV_PHYS, height_disc=0.05, beta_disc=1.1, absorb_edge=0.0, q=0.3,
+                                  r2=0.95 * (1 - W.xl12(0.3))),
+                             dict(CV_COMP, add_disc=1, add_spot=1, eclipse1=0, nrad=20, nspot=40)),
 }
 
 
@@ -109,6 +130,27 @@ def main():
                 out[f"{key}_iscale_io"] = np.array([*io.sfac, io.wdwarf, io.chisq, io.wnok, io.logg1, io.logg2,
                                                     io.rv1, io.rv2])
             print("case", key, "done", flush=True)
+    # disc, disc edge and bright spot (light_curve_comp.cpp:272-333, 483-497)
+    for key, (phys, comp) in COMPONENT_CASES.items():
+        cfg = W.model_json(phys, comp)
+        m = L.Model.from_json(cfg)
+        out[f"{key}_model_json"] = np.frombuffer(json.dumps(cfg).encode(), np.uint8)
+        t, expose, ndiv, weight, ferr, rng = points()
+        data = L.make_data(t, expose=expose, ndiv=ndiv, ferr=ferr, weight=weight, flux=np.zeros(len(t)))
+        st, c0, _ = P.ref_light_curve(m.pod, data, scale=False, rdata=True, diagnostics=True)
+        assert st == 0, R.ref_last_error()
+        flux = 2.5 * c0 / np.max(np.abs(c0)) * (1.0 + ferr * rng.standard_normal(len(t)))
+        data = L.make_data(t, expose=expose, ndiv=ndiv, ferr=ferr, weight=weight, flux=flux)
+        out[f"{key}_dflux"] = flux
+        for mode, kw in {"plain": dict(scale=False, rdata=True, diagnostics=True, sfac=1.7),
+                         "scaled": dict(scale=True, rdata=True, diagnostics=True),
+                         "fast": dict(scale=True, rdata=True, diagnostics=False, max_model_points=60)}.items():
+            st, c, io = P.ref_light_curve(m.pod, data, **kw)
+            assert st == 0, R.ref_last_error()
+            out[f"{key}_{mode}_calc"] = c
+            out[f"{key}_{mode}_io"] = np.array([*io.sfac, io.wdwarf, io.chisq, io.wnok, io.logg1, io.logg2,
+                                                io.rv1, io.rv2])
+        print("case", key, "done", flush=True)
     # the workloads at (or near) their own shapes
     w = W.detached_small()
     m = L.Model.from_json(w["model"])
