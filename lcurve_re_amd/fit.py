@@ -81,8 +81,15 @@ class FitBatch:
         return out
 
     def _run(self, vectors):
+        """Evaluates the vectors; a curve with a non-finite value counts as a
+        failed evaluation, as compute_residuals rejects it
+        (levmarq_solver.cpp:697)."""
         calc, status = self.evaluate(self._models(vectors), self.data, self.opts)
-        return calc, np.asarray(status)
+        status = np.array(status, copy=True)
+        if len(status):
+            bad = (status == 0) & ~np.all(np.isfinite(calc), axis=1)
+            status[bad] = -1
+        return calc, status
 
     def plan(self, pars, h):
         """(kind, vectors) per column: kind 'c' central (+h, -h), 'f'/'b'
@@ -104,6 +111,18 @@ class FitBatch:
                         continue
             cols.append(("f" if sign > 0 else "b", hj))
         return cols
+
+    def _one_sided(self, pars, j, h):
+        """The one-sided step of levmarq_solver.cpp:834-848 for column j:
+        forward, backward past the upper limit, half the range forward when
+        both sides leave the limits, (None, 0) when the range is empty."""
+        lo, hi = self.limits[j]
+        if pars[j] + h <= hi:
+            return "f", h
+        if pars[j] - h >= lo:
+            return "b", h
+        h = 0.5 * (hi - lo)
+        return ("f", h) if h >= 1e-15 else (None, 0.0)
 
     def _vectors(self, pars, cols, which):
         vecs, index = [], []
@@ -182,15 +201,38 @@ class FitBatch:
                                        vecs[a], None if b is None else vecs[b], prior)
             else:
                 pending.append(j)
-        # failed columns: halve the step and retry, batched; a central
-        # difference that keeps failing falls back to a one-sided one
-        tries = {j: 0 for j in pending}
+        # failed columns, retried in batches as levmarq_solver.cpp:800-866
+        # does one by one: a central difference at h/2 .. h/16, then the
+        # one-sided plan from the first step at h .. h/16, and a forward one
+        # last tries backward at the smallest step; a column that never
+        # evaluates stays zero
         first = {j: cols[j][1] for j in pending}
-        while pending:
-            for j in pending:
-                kind, hj = cols[j]
-                cols[j] = (kind, 0.5 * hj)
+        tries = {j: 0 for j in pending}
+        back = {j: False for j in pending}
+
+        def advance(j):
+            kind, hj = cols[j]
+            if tries[j] < 4:
                 tries[j] += 1
+                cols[j] = (kind, 0.5 * hj)
+                return True
+            if kind == "c":
+                okind, oh = self._one_sided(pars, j, first[j])
+                if okind is None:
+                    return False
+                cols[j] = (okind, oh)
+                tries[j] = 0
+                return True
+            if kind == "f" and not back[j]:
+                back[j] = True
+                cols[j] = ("b", hj)
+                return True
+            return False
+
+        while pending:
+            pending = [j for j in pending if advance(j)]
+            if not pending:
+                break
             vecs, index = self._vectors(pars, cols, pending)
             c2, s2 = yield vecs
             s2 = np.asarray(s2)
@@ -201,11 +243,7 @@ class FitBatch:
                 if s2[a] == 0 and (b is None or s2[b] == 0):
                     J[:, j] = self._column(kind, hj, c2[a], None if b is None else c2[b], r0,
                                            vecs[a], None if b is None else vecs[b], prior)
-                elif tries[j] < 4:
-                    nxt.append(j)
-                elif kind == "c":
-                    cols[j] = ("f", 2.0 * first[j])   # halved again before the next try
-                    tries[j] = 0
+                else:
                     nxt.append(j)
             pending = nxt
         nw = len(walkers)

@@ -120,3 +120,73 @@ def test_jacobian_gpu_vs_oracle(precision, engine):
         ref = o["J"][:, j]
         err = np.linalg.norm(g["J"][:, j] - ref) / max(np.linalg.norm(ref), 1e-300)
         assert err < 1e-3, (n, err)
+
+
+def _fake_problem(fail):
+    """A linear 'light curve' calc = A p over the fit model's parameters,
+    with an evaluator that fails (status 2) or returns a non-finite curve
+    wherever fail(j, delta) says so for the one perturbed parameter j."""
+    cfg = W.fit_batch(nphase=50)["model"]
+    m = L.Model.from_json(json.dumps(cfg))
+    p0 = m.get_param()
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((50, len(p0)))
+    data = L.make_data(np.arange(50) / 50.0, ferr=0.01, weight=1.0, flux=A @ p0 + 0.01)
+    tried = []
+
+    def evaluate(models, d, opts):
+        calc = np.zeros((len(models), 50))
+        st = np.zeros(len(models), np.int32)
+        for k, mm in enumerate(models):
+            p = mm.get_param()
+            diff = np.nonzero(p != p0)[0]
+            calc[k] = A @ p
+            if len(diff) == 1:
+                j = int(diff[0])
+                tried.append((j, float(p[j] - p0[j])))
+                f = fail(j, p[j] - p0[j])
+                if f == "status":
+                    st[k] = 2
+                elif f == "nan":
+                    calc[k, 7] = np.nan
+        return calc, st
+    return m, data, evaluate, tried, p0, A
+
+
+@pytest.mark.parametrize("how", ["status", "nan"])
+def test_jacobian_retries_like_the_reference(how):
+    """levmarq_solver.cpp:800-866: a failing central column retries at
+    h/2 .. h/16, then the one-sided step at h .. h/16, then backward at
+    -h/16; a non-finite curve counts as a failed evaluation (:697)."""
+    j_fail = 0   # q, central by the plan
+    seen = []
+
+    def fail(j, dlt):
+        # every attempt on column j_fail fails but the 16th, the backward step
+        if j != j_fail:
+            return None
+        seen.append(dlt)
+        return None if len(seen) == 16 else how
+    m, data, evaluate, tried, p0, A = _fake_problem(fail)
+    fb = F.FitBatch(m, data, evaluate, scale=False)
+    out = fb.jacobian()
+    h = F.fd_steps(p0, m.get_limit())[j_fail]
+    seq = [d for j, d in tried if j == j_fail]
+    want = []
+    for k in range(5):
+        want += [h / 2 ** k, -h / 2 ** k]
+    want += [h / 2 ** k for k in range(5)] + [-h / 16]
+    assert len(seq) == len(want) and np.allclose(seq, want, rtol=1e-12, atol=0), (seq, want)
+    # the last, backward step succeeded: for r = (f - A p) / sigma the
+    # column is -A_j / sigma, up to the rounding of a step of h/16
+    np.testing.assert_allclose(out["J"][:, j_fail], -A[:, j_fail] / 0.01, rtol=1e-3)
+    assert out["kinds"][j_fail] == "b"
+
+
+def test_jacobian_column_stays_zero_when_nothing_evaluates():
+    m, data, evaluate, tried, p0, A = _fake_problem(lambda j, dlt: "status" if j == 1 else None)
+    out = F.FitBatch(m, data, evaluate, scale=False).jacobian()
+    assert np.all(out["J"][:, 1] == 0.0)
+    assert np.any(out["J"][:, 0] != 0.0)
+    # 5 central pairs, 5 forward steps, 1 backward step
+    assert len([d for j, d in tried if j == 1]) == 10 + 5 + 1
