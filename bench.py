@@ -605,7 +605,8 @@ def run_fit_batch(L, eng, args):
     lm["modes"] = {"s": t_modes, "engine_calls": mode_calls, "modes": len(modes),
                    "weights": [round(m["weight"], 6) for m in modes]}
     lm["error_mcmc"] = {"s": t_emc, "engine_calls": emc["calls"], "kept": [n for _, n, _ in emc["per_mode"]],
-                        "acceptance": emc["acceptance"], "ok": emc["ok"], "chain_steps_per_s": sum(
+                        "acceptance": emc["acceptance"], "ok": emc["ok"], "converged": emc["converged"],
+                        "chain_steps_per_s": sum(
                             n + max(300, n // 5) for _, n, _ in emc["per_mode"]) / t_emc}
     # the LM with physical priors (K1 and R1 at 1% around the solution's
     # implied values, R1 moved by 2%): continuation over 7 prior scales

@@ -241,7 +241,7 @@ def _reflect(prop, lims):
 
 
 def error_mcmc(fb, modes, err_s2=None, log_prior=None, dsteps=None, steps=8000, min_steps=500,
-               mode_min_weight=0.005, seed=43, reanchor_rounds=2, reanchor_tol=0.5):
+               mode_min_weight=0.005, seed=43, reanchor_rounds=2, reanchor_tol=0.5, cov=None):
     """The LM program's error sampling over the posterior modes
     (levmarq_solver.cpp:2090-2500), on a fit.FitBatch and the modes of
     fit.laplace_modes.
@@ -250,28 +250,37 @@ def error_mcmc(fb, modes, err_s2=None, log_prior=None, dsteps=None, steps=8000, 
     is), each by one chain of max(min_steps, steps w / w_sampled) kept
     steps after max(300, that / 5) burn-in steps, from the mode's optimum:
     proposal p + exp(s) (2.38 / sqrt(d)) L z with L the Cholesky factor of
-    the mode's covariance (dsteps times z when it has none), reflected off
+    the mode's covariance (else of `cov`, the LM covariance of the fit,
+    else dsteps times z), reflected off
     the limits (at most 20 bounces, then clamped); a proposal with log
     prior <= -1e29 is rejected unevaluated, an evaluation failure is a
     rejection; log alpha = (-0.5 (chi2' - chi2) + (lp' - lp)) / err_s2
     with chi2 the LM's light-curve chi^2 (a zero weight counting as 1) and
     err_s2 by default chi2_LC / max(1, N - k) of the heaviest mode;
     Robbins-Monro during burn-in: every 50 steps s += (1 + batch)^-0.6
-    (rate - 0.234), s in [log 1e-3, log 1e3].  The best state met
-    (largest -0.5 chi2 + lp) is the MAP, starting from the heaviest mode's
-    optimum.  Re-anchoring: when a round raises the MAP's -chi2 + 2 lp by
-    more than reanchor_tol, the sampling restarts (samples cleared, the
-    chain that found the MAP starting from it), at most reanchor_rounds
-    times.
+    (rate - 0.234), s in [log 1e-3, log 1e3].  A mode whose start does
+    not evaluate is skipped for the round (0 samples).  The best state met
+    (largest -0.5 chi2 + lp) is tracked from the heaviest mode's optimum,
+    the first anchor.  Re-anchoring: when a round raises the best
+    -chi2 + 2 lp by more than reanchor_tol, that state is adopted as the
+    anchor (a callable `cov` is evaluated again there) and the sampling
+    restarts (samples cleared, the chain that found it starting from it),
+    at most reanchor_rounds times; when the last allowed round still gains
+    more, the anchor is kept and the fit is reported as not converged
+    (levmarq_solver.cpp:2435-2458).
 
     MI355X-side change: the chains of all modes advance in lockstep, one
     batched engine call per step for every chain still running, and each
     chain draws from its own PCG64 generator (seed + mode index) instead of
     one generator shared by chains run one after another.  Returns dict
     with samples (kept states of all chains, mode order), per_mode (mode
-    index, kept, accepted), acceptance, map_pars, map_chisq, map_lp, ok
-    (more than 100 samples and acceptance above 1%), calls, rounds,
-    reanchored and the gain of each round.
+    index, kept, accepted; kept 0 for a skipped mode), acceptance,
+    map_pars / map_chisq / map_lp (the adopted anchor, the reported
+    solution), best_pars / best_chisq / best_lp (the best state met,
+    adopted or not), converged (the last round gained at most
+    reanchor_tol), ok (more than 100 samples and acceptance above 1%),
+    calls, rounds, reanchored and the gain of each round.  cov: None, a
+    matrix, or p -> matrix (or None).
     """
     npar = fb.npar
     lims = fb.limits
@@ -287,48 +296,61 @@ def error_mcmc(fb, modes, err_s2=None, log_prior=None, dsteps=None, steps=8000, 
     prop_sd = 2.38 / math.sqrt(npar)
     lp = (lambda v: log_prior(v)) if log_prior else (lambda v: 0.0)
 
+    def chol(c):
+        if c is None:
+            return None
+        try:
+            return np.linalg.cholesky(np.asarray(c, float))
+        except np.linalg.LinAlgError:
+            return None
+
+    def global_chol(p):
+        return chol(cov(p) if callable(cov) else cov)
+
     chains = []
     for k in pick:
         m = modes[k]
         nst = max(min_steps, int(steps * m["weight"] / max(wincl, 1e-300)))
-        L = None
-        if m.get("cov") is not None:
-            try:
-                L = np.linalg.cholesky(m["cov"])
-            except np.linalg.LinAlgError:
-                L = None
-        chains.append({"mode": k, "p": np.array(m["pars"], float), "L": L, "nst": nst,
+        chains.append({"mode": k, "p": np.array(m["pars"], float), "L": chol(m.get("cov")), "nst": nst,
                        "nbr": max(300, nst // 5), "step": 0, "rng": np.random.default_rng(seed + k),
                        "s": 0.0, "batch": 0, "bacc": 0, "bn": 0, "kept": [], "acc": 0})
-    # the MAP starts at the heaviest mode's optimum (the adopted LM fit)
-    calc, status = fb._run([chains[0]["p"]])
+    # the first anchor is the heaviest mode's optimum (the LM fit); a start
+    # that does not evaluate leaves the best state unset
+    anchor = np.array(modes[0]["pars"], float)
+    calc, status = fb._run([anchor])
     calls = 1
-    if status[0] != 0:
-        raise RuntimeError("the heaviest mode's optimum does not evaluate")
-    map_chisq = float(np.sum(residuals_lc(data, calc[0]) ** 2))
-    map_pars, map_lp, map_k = chains[0]["p"].copy(), -0.5 * map_chisq + lp(chains[0]["p"]), -1
-    gains, reanchored, rnd = [], False, 0
+    if status[0] == 0:
+        anc_chisq = float(np.sum(residuals_lc(data, calc[0]) ** 2))
+        anc_lp = -0.5 * anc_chisq + lp(anchor)
+    else:
+        anc_chisq, anc_lp = math.inf, -1e300
+    Lg = global_chol(anchor) if cov is not None else None
+    map_pars, map_chisq, map_lp, map_k = anchor.copy(), anc_chisq, anc_lp, -1
+    gains, reanchored, converged, rnd = [], False, False, 0
     while True:
         round_start_lp = map_lp
         for k, c in enumerate(chains):
-            c.update(p=(map_pars if reanchored and k == map_k else modes[c["mode"]]["pars"]).copy(), step=0, s=0.0,
+            c.update(p=(anchor if reanchored and k == map_k else modes[c["mode"]]["pars"]).copy(), step=0, s=0.0,
                      batch=0, bacc=0, bn=0, kept=[], acc=0)
         calc, status = fb._run([c["p"] for c in chains])
         calls += 1
         for k, c in enumerate(chains):
-            if status[k] != 0:
-                raise RuntimeError("a chain's starting point does not evaluate")
+            # a mode whose start does not evaluate is skipped this round
+            c["skip"] = status[k] != 0
+            if c["skip"]:
+                continue
             c["chi"] = float(np.sum(residuals_lc(data, calc[k]) ** 2))
             c["lp"] = lp(c["p"])
         while True:
-            live = [c for c in chains if c["step"] < c["nst"] + c["nbr"]]
+            live = [c for c in chains if not c["skip"] and c["step"] < c["nst"] + c["nbr"]]
             if not live:
                 break
             props, lps = [], []
             for c in live:
                 z = c["rng"].standard_normal(npar)
                 sf = math.exp(c["s"])
-                prop = c["p"] + sf * prop_sd * (c["L"] @ z) if c["L"] is not None else c["p"] + sf * prop_sd * diag * z
+                Lc = c["L"] if c["L"] is not None else Lg
+                prop = c["p"] + sf * prop_sd * (Lc @ z) if Lc is not None else c["p"] + sf * prop_sd * diag * z
                 props.append(_reflect(prop, lims))
                 lps.append(lp(props[-1]))
             ev = [i for i, v in enumerate(lps) if v > -1e29]
@@ -362,15 +384,24 @@ def error_mcmc(fb, modes, err_s2=None, log_prior=None, dsteps=None, steps=8000, 
                     c["kept"].append(c["p"].copy())
                 c["step"] += 1
         gains.append(2.0 * (map_lp - round_start_lp))
-        if gains[-1] <= reanchor_tol or rnd >= reanchor_rounds:
+        if gains[-1] <= reanchor_tol:
+            converged = True  # the round started at the optimum
             break
-        reanchored = True  # the MAP's chain restarts there (adopt_map)
+        if rnd >= reanchor_rounds:
+            break  # still descending: the anchor is kept, not converged
+        # adopt the best state (adopt_map): the anchor, its chain's start
+        # and the global proposal covariance move there
+        reanchored = True
+        anchor, anc_chisq, anc_lp = map_pars.copy(), map_chisq, map_lp
+        if callable(cov):
+            Lg = global_chol(anchor)
         rnd += 1
     samples = np.array([v for c in chains for v in c["kept"]]).reshape(-1, npar)
     drawn = sum(len(c["kept"]) for c in chains)
     acc = sum(c["acc"] for c in chains) / drawn if drawn else 0.0
     return {"samples": samples, "per_mode": [(c["mode"], len(c["kept"]), c["acc"]) for c in chains],
-            "acceptance": acc, "map_pars": map_pars, "map_chisq": map_chisq, "map_lp": map_lp,
+            "acceptance": acc, "map_pars": anchor, "map_chisq": anc_chisq, "map_lp": anc_lp,
+            "best_pars": map_pars, "best_chisq": map_chisq, "best_lp": map_lp, "converged": converged,
             "ok": len(samples) > 100 and acc > 0.01, "calls": calls, "err_s2": err_s2, "rounds": rnd + 1,
             "reanchored": reanchored, "gains": gains}
 

@@ -371,8 +371,66 @@ def test_error_mcmc_reanchors_on_a_better_state():
     assert r["map_chisq"] < chi_off and r["map_chisq"] - chi_min < 5.0
     sd = np.sqrt(np.diag(C))
     assert np.all(np.abs(r["samples"].mean(axis=0) - phat) < 0.2 * sd)
+    assert r["converged"]
     one = M.error_mcmc(fb, [mode], dsteps=[0.01, 0.01], steps=6000, seed=2, reanchor_rounds=0)
     assert one["rounds"] == 1 and not one["reanchored"]
+    # the one round allowed still gained: the anchor is kept and the fit is
+    # reported as not converged (levmarq_solver.cpp:2438-2458)
+    assert not one["converged"] and one["gains"][0] > 0.5
+    assert np.array_equal(one["map_pars"], off) and abs(one["map_chisq"] - chi_off) <= 1e-9 * chi_off
+    assert one["best_chisq"] < one["map_chisq"]
+
+
+def _linear_posterior(n=150, seed=5):
+    from lcurve_re_amd import mcmc as M
+    X = np.linspace(0.0, 1.0, n)
+    A = np.stack([np.ones_like(X), X], axis=1)
+    rng = np.random.default_rng(seed)
+    sig = 0.05 * np.ones_like(X)
+    y = A @ np.array([0.3, -0.7]) + sig * rng.standard_normal(len(X))
+    data = L.make_data(X, ferr=sig, flux=y, weight=1.0)
+    W = 1.0 / sig ** 2
+    Fm = A.T @ (A * W[:, None])
+    phat = np.linalg.solve(Fm, A.T @ (W * y))
+    chi_min = float(np.sum(W * (y - A @ phat) ** 2))
+    C = chi_min / (n - 2) * np.linalg.inv(Fm)
+    return M, A, data, phat, chi_min, C
+
+
+def test_error_mcmc_skips_a_mode_that_does_not_evaluate():
+    """levmarq_solver.cpp:2412-2418: a mode whose start fails is skipped
+    with no samples, the others are sampled."""
+    M, A, data, phat, chi_min, C = _linear_posterior()
+    bad = phat + np.array([3.0, 0.0])
+
+    def ev(models, d, o):
+        st = np.array([1 if m.p[0] > 2.0 else 0 for m in models])
+        return np.array([A @ m.p for m in models]), st
+    fb = F.FitBatch(ToyModel(list(phat), [(-10.0, 10.0)] * 2), data, ev, scale=False)
+    modes = [{"pars": phat, "weight": 0.7, "cov": C, "chisq_lc": chi_min},
+             {"pars": bad, "weight": 0.3, "cov": C, "chisq_lc": chi_min}]
+    r = M.error_mcmc(fb, modes, dsteps=[0.01, 0.01], steps=2000, seed=3)
+    kept = {k: n for k, n, _ in r["per_mode"]}
+    assert kept[1] == 0 and kept[0] == 1400
+    assert r["converged"] and r["ok"]
+    assert np.array_equal(r["map_pars"], phat)
+
+
+def test_error_mcmc_global_covariance_is_the_fallback_proposal():
+    """A mode without its own covariance proposes from the LM covariance
+    (pcov) rather than from the diagonal steps: with steps 1e-4 sigma the
+    diagonal chain stays narrow even with its scale adapted to 1e3, the
+    covariance chain samples the closed-form posterior."""
+    M, A, data, phat, chi_min, C = _linear_posterior()
+    ev = lambda models, d, o: (np.array([A @ m.p for m in models]), np.zeros(len(models), int))
+    fb = F.FitBatch(ToyModel(list(phat), [(-10.0, 10.0)] * 2), data, ev, scale=False)
+    sd = np.sqrt(np.diag(C))
+    mode = {"pars": phat, "weight": 1.0, "cov": None, "chisq_lc": chi_min}
+    tiny = M.error_mcmc(fb, [mode], dsteps=list(1e-4 * sd), steps=3000, seed=6)
+    full = M.error_mcmc(fb, [mode], dsteps=list(1e-4 * sd), steps=3000, seed=6, cov=lambda p: C)
+    assert np.all(tiny["samples"].std(axis=0) < 0.5 * sd)
+    assert np.all(np.abs(full["samples"].std(axis=0) / sd - 1.0) < 0.2)
+    assert 0.1 < full["acceptance"] < 0.6
 
 
 def test_multistart_survives_starts_that_do_not_evaluate():
