@@ -54,6 +54,22 @@ def residuals(data, calc):
     return np.sqrt(w) * (v["flux"] - calc) / sigma
 
 
+class BaseEvalError(RuntimeError):
+    """The base point of a Jacobian did not evaluate; status is its status
+    (-1: the curve holds a non-finite value)."""
+
+    def __init__(self, status):
+        super().__init__(f"base evaluation failed with status {int(status)}")
+        self.status = int(status)
+
+
+def counted(status):
+    """Evaluations that levmarq_solver.cpp:693 counts in fev_count: those
+    that returned a curve, finite or not (status 0 or -1), not those that
+    threw (an engine status > 0)."""
+    return np.isin(np.asarray(status), (0, -1))
+
+
 class FitBatch:
     """Jacobian + walker evaluation of one model against one data set.
 
@@ -185,13 +201,15 @@ class FitBatch:
         status = np.asarray(status)
         n = calc.shape[1]
         if status[0] != 0:
-            raise RuntimeError(f"base evaluation failed with status {int(status[0])}")
+            raise BaseEvalError(status[0])
         chisq_lc0 = float(np.sum(residuals(self.data, calc[0]) ** 2))
         if prior is not None and callable(prior[1]):
             prior = (prior[0], prior[1](chisq_lc0))
         r0 = self._resid(calc[0], pars, prior)
         J = np.zeros((len(r0), self.npar))
         calls = [len(batch)]
+        # column evaluations that returned a curve (the reference's count)
+        fev = int(np.sum(counted(status[1:1 + len(vecs)])))
         pending = []
         for (j, a, b) in index:
             kind, hj = cols[j]
@@ -237,6 +255,7 @@ class FitBatch:
             c2, s2 = yield vecs
             s2 = np.asarray(s2)
             calls.append(len(vecs))
+            fev += int(np.sum(counted(s2)))
             nxt = []
             for (j, a, b) in index:
                 kind, hj = cols[j]
@@ -256,19 +275,20 @@ class FitBatch:
         return {"J": J, "resid0": r0, "chisq0": float(np.sum(r0 ** 2)), "chisq_lc0": chisq_lc0,
                 "prior_eff": None if prior is None else prior[1], "calc0": calc[0], "steps": steps,
                 "kinds": [k for k, _ in cols], "walker_chisq": wchi, "walker_status": wst,
-                "instances_per_call": calls}
+                "instances_per_call": calls, "fev_columns": fev}
 
     def sum_squares(self, vectors, prior=None):
         """sum r_i^2 at each vector (nan where the evaluation failed)."""
         return self.drive(self.sum_squares_requests(vectors, prior))
 
-    def sum_squares_requests(self, vectors, prior=None):
+    def sum_squares_requests(self, vectors, prior=None, with_status=False):
         if not len(vectors):
-            return np.zeros(0)
+            return (np.zeros(0), np.zeros(0, int)) if with_status else np.zeros(0)
         vectors = [np.asarray(v, float) for v in vectors]
         calc, status = yield vectors
-        return np.array([float(np.sum(self._resid(calc[k], vectors[k], prior) ** 2)) if status[k] == 0 else np.nan
-                         for k in range(len(vectors))])
+        ssq = np.array([float(np.sum(self._resid(calc[k], vectors[k], prior) ** 2)) if status[k] == 0 else np.nan
+                        for k in range(len(vectors))])
+        return (ssq, np.asarray(status)) if with_status else ssq
 
     def _column(self, kind, hj, ca, cb, r0, va, vb, prior):
         ra = self._resid(ca, va, prior)
@@ -298,7 +318,7 @@ SQRT_EPS = math.sqrt(np.finfo(np.float64).eps)
 
 
 def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, prior=None, continuation=True,
-                     stages=6, auto_balance=True, balance_target=1.0, **kw):
+                     stages=6, auto_balance=True, balance_target=1.0, max_fev=None, **kw):
     """Levenberg-Marquardt fit of fb.model's free parameters to fb.data
     (levmarq_solver.cpp:985-1530), as a request generator: it yields lists
     of parameter vectors to evaluate and receives (calc, status) for them
@@ -317,10 +337,23 @@ def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, 
     first Jacobian call instead of an evaluation of its own (the same
     engine result).  Iterations add up over the stages; calls counts
     batched engine calls.
+
+    max_fev (default 200 (npar + 1), lm_max_fev of
+    levmarq_solver.cpp:431): the fit ends with stop "max function
+    evaluations" once the evaluations the one-at-a-time solver would have
+    made since the start reach it, tested where it tests
+    (levmarq_solver.cpp:1494: after an accepted, rejected or recovering
+    iteration).  Counted as there (fev, returned): the start point, each
+    stage's starting point and auto-balance point, each trial taken in
+    order, each Jacobian column evaluation (retries included); evaluations
+    that returned a curve, not those that failed.  The batched extras
+    (trials past the accepted one, the Jacobian around a rejected first
+    trial) are not counted.
     """
     p = np.array(fb.model.get_param() if pars is None else pars, float)
+    fev = {"n": 1, "max": 200 * (fb.npar + 1) if max_fev is None else int(max_fev)}   # the start point
     if prior is None or len(prior) == 0:
-        return (yield from lm_stage_requests(fb, p, max_iter=max_iter, ftol=ftol, xtol=xtol, **kw))
+        return (yield from lm_stage_requests(fb, p, max_iter=max_iter, ftol=ftol, xtol=xtol, fev=fev, **kw))
     sched = [s / stages for s in range(stages + 1)] if continuation else [1.0]
     total = {"iterations": 0, "calls": 0, "history": [], "stages": []}
     res = None
@@ -335,9 +368,11 @@ def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, 
                 b = 1.0 if scale > 0.0 else 0.0
             return math.sqrt(scale * b)
         start = p
+        if auto_balance and scale > 0.0:
+            fev["n"] += 1   # the chi2_LC evaluation that sets the balance
         res = yield from lm_stage_requests(
             fb, p, prior=(prior, eff), max_iter=max_iter if final else max(20, max_iter // len(sched)),
-            ftol=ftol if final else max(ftol, 1e-4), xtol=xtol if final else max(xtol, 1e-4), **kw)
+            ftol=ftol if final else max(ftol, 1e-4), xtol=xtol if final else max(xtol, 1e-4), fev=fev, **kw)
         p = res["pars"]
         total["iterations"] += res["iterations"]
         total["calls"] += res["calls"]
@@ -345,13 +380,15 @@ def levmarq_requests(fb, pars=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, 
                              for h in res["history"]]
         total["stages"].append({"scale": scale, "start": start, "eff": res["prior_eff"], "iterations": res["iterations"],
                                 "stop": res["stop"], "sum_sq": res["chisq"]})
+        if res["stop"] == "max function evaluations":
+            break   # goto lm_done: no further stage
     return dict(res, iterations=total["iterations"], calls=total["calls"], history=total["history"],
                 stages=total["stages"])
 
 
 def lm_stage_requests(fb, p, prior=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_EPS, gtol=0.0, tau0=1e-3,
                       lambda_conv=1.0, lambda_max=1e8, max_recoveries=3, fd_shrink_factor=0.1, candidates=4,
-                      log=None):
+                      log=None, fev=None):
     """One pass of the LM inner loop (levmarq_solver.cpp:1225-1500) from p,
     with the prior residuals (fn, eff) when given (eff may be a function of
     the starting point's chi2_LC; see FitBatch.jacobian_requests).
@@ -374,14 +411,18 @@ def lm_stage_requests(fb, p, prior=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_
     first accepted one ends the batch; if it is not the first, its Jacobian
     is one more call).  The iterates are those of the sequential loop
     (candidates=1).  Returns dict with pars, chisq (= |r|^2), iterations
-    (trials, as the reference counts them), calls, stop reason, lambda and
-    the iteration log.
+    (trials, as the reference counts them), calls, stop reason, lambda,
+    the iteration log and fev (the evaluation count; see
+    levmarq_requests, whose dict {"n", "max"} fev is, shared over stages).
     """
     npar = fb.npar
     lims = fb.limits
     p = np.array(p, float)
+    if fev is None:
+        fev = {"n": 0, "max": 200 * (npar + 1)}
     shrink = 1.0
     jac = yield from fb.jacobian_requests(p, shrink=shrink, prior=prior)
+    fev["n"] += 1 + jac["fev_columns"]   # the stage's starting point and its Jacobian
     calls = len(jac["instances_per_call"])
     if prior is not None:
         prior = (prior[0], jac["prior_eff"])
@@ -446,13 +487,17 @@ def lm_stage_requests(fb, p, prior=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_
         trials = [e[3] for e in plan if e[0] == "trial"]
         spec = None
         ssq = np.zeros(0)
+        tst = np.zeros(0, int)
         if trials:
             try:
                 spec = yield from fb.jacobian_requests(trials[0], walkers=trials[1:], shrink=shrink, prior=prior)
                 calls += len(spec["instances_per_call"])
                 ssq = np.concatenate([[spec["chisq0"]], spec["walker_chisq"]])
-            except RuntimeError:  # the first trial failed to evaluate
-                ssq = np.concatenate([[np.nan], (yield from fb.sum_squares_requests(trials[1:], prior))])
+                tst = np.concatenate([[0], spec["walker_status"]])
+            except BaseEvalError as e:  # the first trial failed to evaluate
+                rest, rst = yield from fb.sum_squares_requests(trials[1:], prior, with_status=True)
+                ssq = np.concatenate([[np.nan], rest])
+                tst = np.concatenate([[e.status], rst])
                 calls += 1 if len(trials) > 1 else 0
         ti = 0
         accepted = False
@@ -471,6 +516,7 @@ def lm_stage_requests(fb, p, prior=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_
                 continue
             s_t = ssq[ti]
             first = ti == 0
+            fev["n"] += int(counted(tst[ti]))
             ti += 1
             if not np.isfinite(s_t):
                 lam, nu = l * n_, n_ * 2.0
@@ -493,6 +539,7 @@ def lm_stage_requests(fb, p, prior=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_
                 else:
                     jac = yield from fb.jacobian_requests(p, shrink=shrink, prior=prior)
                     calls += len(jac["instances_per_call"])
+                fev["n"] += jac["fev_columns"]
                 J = jac["J"]
                 r = jac["resid0"]
                 D = scaling(J, D)
@@ -506,6 +553,8 @@ def lm_stage_requests(fb, p, prior=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_
                 lam = min(max(lam, 1e-20), lambda_max)
                 hist.append((it, sum_sq, lam, status))
                 accepted = True
+                if not done and fev["n"] >= fev["max"]:
+                    done = "max function evaluations"
                 break
             lam, nu = l * n_, n_ * 2.0
             fails += 1
@@ -517,6 +566,7 @@ def lm_stage_requests(fb, p, prior=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_
                     shrink *= fd_shrink_factor
                     jac = yield from fb.jacobian_requests(p, shrink=shrink, prior=prior)
                     calls += len(jac["instances_per_call"])
+                    fev["n"] += jac["fev_columns"]
                     J = jac["J"]
                     r = jac["resid0"]
                     D = scaling(J, None)
@@ -524,8 +574,13 @@ def lm_stage_requests(fb, p, prior=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_
                     lam, nu = tau0, 2.0
                     fails = 0
                     hist.append((it, sum_sq, lam, "recover"))
+                    if fev["n"] >= fev["max"]:
+                        done = "max function evaluations"
                 else:
                     done = "noise floor"
+                break
+            if fev["n"] >= fev["max"]:
+                done = "max function evaluations"
                 break
         if log:
             log(f"levmarq: iteration {it} sum_sq {sum_sq:.8g} lambda {lam:.3g} calls {calls}")
@@ -534,7 +589,7 @@ def lm_stage_requests(fb, p, prior=None, max_iter=200, ftol=SQRT_EPS, xtol=SQRT_
             break
     return {"pars": p, "chisq": sum_sq, "chisq_lc": jac["chisq_lc0"],
             "iterations": it, "calls": calls, "stop": stop, "lambda": lam, "history": hist,
-            "prior_eff": None if prior is None else prior[1]}
+            "prior_eff": None if prior is None else prior[1], "fev": fev["n"]}
 
 
 def levmarq(fb, pars=None, **kw):

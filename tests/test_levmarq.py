@@ -90,6 +90,38 @@ def test_batched_candidates_give_the_sequential_iterates(k):
     assert bat["calls"] < seq["calls"]
 
 
+def test_max_fev_counts_and_stops_like_the_reference():
+    """lm_max_fev (levmarq_solver.cpp:431, 649-693, 1050, 1494): every
+    returned evaluation counts (the start point, the stage's starting point,
+    2 per central column, each trial, the 8 column evaluations after an
+    accepted trial); the test after an accepted or rejected iteration ends
+    the fit.  Batched candidates count what the one-at-a-time loop counts,
+    not what they evaluate."""
+    data, lims, *_ = toy_problem()
+    start = np.array([8.0, 5.0, -3.0, 3.0])
+    fb = lambda: F.FitBatch(ToyModel(start, lims), data, toy_eval, scale=False)
+    full = F.levmarq(fb(), tau0=1e-6, candidates=1, max_fev=10 ** 6)
+    kinds = [h[3] for h in full["history"]]
+    assert "reject" in kinds and kinds.count("accept") >= 3
+    fev = 1 + 1 + 2 * 4
+    after = []
+    for kd in kinds:
+        fev += 1 + (8 if kd in ("accept", "ftol") else 0)
+        after.append(fev)
+    assert full["fev"] == after[-1]
+    assert fev < 200 * 5 and F.levmarq(fb(), tau0=1e-6)["stop"] == full["stop"]   # the default never binds here
+    for k in (0, 3, 5):
+        lim = after[k]
+        for cand in (1, 4):
+            r = F.levmarq(fb(), tau0=1e-6, candidates=cand, max_fev=lim)
+            assert r["stop"] == "max function evaluations" and r["iterations"] == k + 1, (k, cand, r["stop"])
+            assert r["history"] == full["history"][:k + 1] and r["fev"] == lim
+        # one below: the same iteration already passes it only if the
+        # previous count did not
+        r = F.levmarq(fb(), tau0=1e-6, candidates=4, max_fev=lim - 1)
+        assert r["iterations"] == (k + 1 if k == 0 or after[k - 1] < lim - 1 else k)
+
+
 def test_levmarq_fits_an_oracle_light_curve():
     from lcurve_re_amd import workloads as W
     from oracle import pyoracle as P
