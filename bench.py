@@ -324,6 +324,29 @@ def roofline_from_profile(prof, steps, precision, workload, R=None):
             "continua_reused_per_step": prof["continua_reused"] / steps}
 
 
+def plain_gpu(L, eng, models, data, precision, batched, batched_rate, passes=2):
+    """The same instances evaluated one engine call each (no batching across
+    instances, the way a caller of light_curve_comp evaluates them), timed
+    over `passes` passes after one untimed pass; light curves/s, the
+    batched rate over it, and the largest difference from the batched
+    curves relative to each curve's largest value."""
+    packs = [L.ModelBatch([m]) for m in models]
+    out = np.zeros((1, len(data)), np.float64)
+    worst = 0.0
+    for k, p in enumerate(packs):
+        eng.light_curve_comp_batch(p, data, precision=precision, out=out)
+        worst = max(worst, float(np.max(np.abs(out[0] - batched[k])) / max(np.max(np.abs(batched[k])), 1e-300)))
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        for p in packs:
+            eng.light_curve_comp_batch(p, data, precision=precision, out=out)
+    el = time.perf_counter() - t0
+    rate = len(models) * passes / el
+    log(f"bench: plain GPU, one call per instance: {rate:.1f} light curves/s")
+    return {"value": rate, "unit": "light curves/s", "calls_per_pass": len(models), "passes": passes,
+            "batched_over_plain": batched_rate / rate, "max_rel_diff_vs_batched": worst}
+
+
 def eager_profile(eng, step, nsteps):
     """Profile of nsteps calls with graph replay off: a replayed graph is
     timed as a whole, so per-kernel and per-stage times come from eager
@@ -387,6 +410,7 @@ def run_batch_workload(L, eng, args):
     prof = eager_profile(eng, lambda: eng.light_curve_comp_batch(packed, data, precision=args.precision, out=calc),
                          nprof)
     roofline = roofline_from_profile(prof, nprof, args.precision, cfg["workload"])
+    plain = plain_gpu(L, eng, models, data, args.precision, calc, M * args.steps / el)
     cpu, err, port = (None, None, None)
     if not args.no_cpu:
         cpu, err, port = cpu_baseline(models, data, calc, args.cpu_budget)
@@ -397,7 +421,7 @@ def run_batch_workload(L, eng, args):
             "dtype": "fp64 per face and accumulation" if args.precision else
             "fp32 per face, fp64 geometry/accumulation",
             "data": "synthetic", "config": cfg, "instances_ok": nok, "roofline": roofline,
-            "cpu_baseline": cpu, "cpu_oracle_port": port,
+            "cpu_baseline": cpu, "cpu_oracle_port": port, "plain_gpu": plain,
             "max_rel_err": err["max_normalised"] if err else None, "rel_err": err}
 
 
