@@ -115,22 +115,31 @@ def cpu_baseline(models, data, gpu_calc, budget_s=15.0):
         done += 1
         if time.perf_counter() - t0 > budget_s and done >= 2:
             break
+    # a workload of few instances is evaluated again until the budget is used
+    passes = 1
+    while sample and done == len(sample) * passes and time.perf_counter() - t0 < budget_s:
+        for m in sample:
+            run(m.pod, data, scale=False, rdata=True, diagnostics=False)
+            done += 1
+        passes += 1
     el = time.perf_counter() - t0
     base = {"value": done / el, "unit": "light curves/s", "cores": threads,
             "kind": "reference" if use_ref else "port", "cpu_model": cpu_model(), "host_cores": os.cpu_count(),
-            "sample": f"first {done} of {len(models)} instances of the same workload"
-                      + (f" ({skipped} with nlat 153 passed over)" if skipped else "") + ", "
+            "sample": f"first {len(sample)} of {len(models)} instances of the same workload"
+                      + (f" ({skipped} with nlat 153 passed over)" if skipped else "")
+                      + (f", {passes} passes ({done} curves)" if passes > 1 else "") + ", "
                       + ("the reference's light_curve_comp compiled from its sources (oracle/_ref, -O3, OpenMP, "
                          "grid cache off)" if use_ref else "FP64 oracle (OpenMP, -O3 -ffp-contract=off)")
                       + f", {el:.1f} s"}
     port = None
     if use_ref:
         t1 = time.perf_counter()
-        for m in sample:
-            P.light_curve(m.pod, data, scale=False, rdata=True, diagnostics=False)
+        for _ in range(passes):
+            for m in sample:
+                P.light_curve(m.pod, data, scale=False, rdata=True, diagnostics=False)
         elp = time.perf_counter() - t1
         port = {"value": done / elp, "unit": "light curves/s", "cores": threads,
-                "sample": f"the same {done} instances through the FP64 oracle, {elp:.1f} s"}
+                "sample": f"the same {done} curves through the FP64 oracle, {elp:.1f} s"}
     return base, {"max_normalised": e_max, "pointwise": e_pt,
                   "against": "reference" if use_ref else "oracle", "instances": done}, port
 
