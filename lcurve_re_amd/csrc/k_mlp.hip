@@ -181,6 +181,187 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, c
     }
 }
 
+// ------------------------------------------------------------ GEMM, large
+// C[M,N] = A[M,K] B[N,K]^T with both operands K-contiguous (row strides lda,
+// ldb), the layout every training GEMM is given: the forward pass reads the
+// transposed weight copy, the weight gradients read transposed activations
+// and gradients that their producers write, the input gradients read W as
+// it is.  Workgroup tile BM x BN x BK (BM, BN = 64 or 128; BK = 64, or 128
+// for the 64 x 64 tile, whose K stages are then half as many), four waves of
+// BM/2 x BN/2, each a BM/64 x BN/64 block of 32x32x16 MFMAs.  LDS holds two K
+// stages of A[m][k] and B[n][k] (rows padded by 16 bytes, so the 16-byte
+// operand reads of 8 consecutive rows fall in distinct banks); a thread
+// stages 16-byte chunks (8 consecutive k of one row): one global load and
+// one LDS write each, zero-filled past M, N or the K range, element loads
+// where a chunk is cut by the K range or unaligned.  The next stage's
+// chunks are loaded into registers while the current stage's MFMAs run and
+// written to the other LDS buffer after them: one barrier per K stage.
+// blockIdx.z splits K as in k_gemm.  Epilogue: bias, SiLU (EPI_SILU), or
+// the SiLU gradient of the pre-activation Zin (EPI_DSILU); outputs Z
+// (pre-activation), Cf (FP32), Cb (BF16) and CbT (BF16, transposed:
+// CbT[j * ldt + i], four consecutive rows per 8-byte store).
+constexpr int NBK = 64;  // K per split is a multiple of this
+enum { EPI_DSILU = 4 };
+
+__device__ __forceinline__ uint4 load_chunk(const uint16_t *p, bool row_ok, int valid, bool vec) {
+    if (row_ok && vec && valid >= 8) return *reinterpret_cast<const uint4 *>(p);
+    uint16_t e[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) e[i] = (row_ok && i < valid) ? p[i] : (uint16_t)0;
+    return make_uint4(e[0] | ((uint32_t)e[1] << 16), e[2] | ((uint32_t)e[3] << 16), e[4] | ((uint32_t)e[5] << 16),
+                      e[6] | ((uint32_t)e[7] << 16));
+}
+
+template <int BM, int BN, int BK, int EPI>
+__global__ __launch_bounds__(256, 2) void k_gemm_nt(int M, int N, int K, int kchunk, const uint16_t *__restrict__ A,
+                                                    int lda, const uint16_t *__restrict__ B, int ldb, bool vec,
+                                                    const float *__restrict__ bias, const float *__restrict__ Zin,
+                                                    float *__restrict__ Z, float *__restrict__ Cf,
+                                                    uint16_t *__restrict__ Cb, int ldc, uint16_t *__restrict__ CbT,
+                                                    int ldt) {
+    constexpr int NI = BM / 64, NJ = BN / 64;           // 32x32 tiles per wave
+    constexpr int KC = BK / 8, NLDK = BK + 8;             // 16-byte chunks per row; LDS row length
+    constexpr int CA = BM * KC / 256, CB = BN * KC / 256;  // chunks per thread and stage
+    __shared__ __attribute__((aligned(16))) __bf16 As[2][BM * NLDK];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][BN * NLDK];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (w >> 1) * (BM / 2), wn = (w & 1) * (BN / 2);
+    const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+    f32x16 acc[NI][NJ];
+#pragma unroll
+    for (int i = 0; i < NI; i++)
+#pragma unroll
+        for (int j = 0; j < NJ; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+
+    // chunk c of a stage: row c / KC, k offset 8 (c % KC); thread t takes
+    // chunks t, t + 256, ...
+    uint4 ra[CA], rb[CB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < CA; i++) {
+            const int c = t + 256 * i, row = c / KC, k = k0 + 8 * (c % KC);
+            ra[i] = load_chunk(A + (size_t)(m0 + row) * lda + k, m0 + row < M, ke - k, vec);
+        }
+#pragma unroll
+        for (int i = 0; i < CB; i++) {
+            const int c = t + 256 * i, row = c / KC, k = k0 + 8 * (c % KC);
+            rb[i] = load_chunk(B + (size_t)(n0 + row) * ldb + k, n0 + row < N, ke - k, vec);
+        }
+    };
+    auto stage = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < CA; i++) {
+            const int c = t + 256 * i;
+            *reinterpret_cast<uint4 *>(&As[buf][(c / KC) * NLDK + 8 * (c % KC)]) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < CB; i++) {
+            const int c = t + 256 * i;
+            *reinterpret_cast<uint4 *>(&Bs[buf][(c / KC) * NLDK + 8 * (c % KC)]) = rb[i];
+        }
+    };
+
+    if (kb < ke) {
+        fetch(kb);
+        stage(0);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+        const bool more = k0 + BK < ke;
+        if (more) fetch(k0 + BK);  // in flight during the MFMAs
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            // lane l: row (l % 32), k block 8 * (l / 32) of this 16-wide step
+            const int ko = kk + 8 * (lane >> 5);
+            bf16x8 a[NI], b[NJ];
+#pragma unroll
+            for (int i = 0; i < NI; i++)
+                a[i] = *reinterpret_cast<const bf16x8 *>(&As[buf][(wm + 32 * i + (lane & 31)) * NLDK + ko]);
+#pragma unroll
+            for (int j = 0; j < NJ; j++)
+                b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[buf][(wn + 32 * j + (lane & 31)) * NLDK + ko]);
+#pragma unroll
+            for (int i = 0; i < NI; i++)
+#pragma unroll
+                for (int j = 0; j < NJ; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) stage(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    // accumulator r of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
+    float *Cs = Cf ? Cf + (size_t)blockIdx.z * M * ldc : nullptr;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int col = n0 + wn + 32 * j + (lane & 31);
+        if (col >= N) continue;
+        const float bj = (EPI & EPI_BIAS) ? bias[col] : 0.f;
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int row0 = m0 + wm + 32 * i + 8 * g + 4 * (lane >> 5);
+                uint16_t tb[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int row = row0 + q;
+                    tb[q] = 0;
+                    if (row >= M) continue;
+                    const size_t o = (size_t)row * ldc + col;
+                    float z = acc[i][j][4 * g + q] + bj;
+                    if (EPI & EPI_DSILU) z *= silu_grad(Zin[o]);
+                    if (Z) Z[o] = z;
+                    const float v = (EPI & EPI_SILU) ? silu(z) : z;
+                    if (Cs) Cs[o] = v;
+                    const uint16_t h = f2bf(v);
+                    if (Cb) Cb[o] = h;
+                    tb[q] = h;
+                }
+                if (CbT) {
+                    uint16_t *dst = CbT + (size_t)col * ldt + row0;
+                    if (row0 + 4 <= M && (ldt & 3) == 0) {
+                        *reinterpret_cast<uint2 *>(dst) =
+                            make_uint2(tb[0] | ((uint32_t)tb[1] << 16), tb[2] | ((uint32_t)tb[3] << 16));
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; q++)
+                            if (row0 + q < M) dst[q] = tb[q];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// dst[j * ldd + i] = src[i * lds + j] for a rows x cols BF16 matrix, in
+// 64 x 64 tiles through LDS (both sides coalesced); blockIdx.z selects one
+// of up to 8 matrices of a TransposeSet
+struct TransposeSet {
+    const uint16_t *src[8];
+    uint16_t *dst[8];
+    int rows[8], cols[8], ldd[8];  // source rows are cols long; destination rows ldd apart
+};
+__global__ __launch_bounds__(256) void k_transpose_bf16(TransposeSet ts) {
+    __shared__ uint16_t tile[64][65];
+    const int z = blockIdx.z;
+    const int rows = ts.rows[z], cols = ts.cols[z];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    if (r0 >= rows || c0 >= cols) return;
+    const uint16_t *src = ts.src[z];
+    uint16_t *dst = ts.dst[z];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4)
+        if (r0 + r < rows && c0 + tx < cols) tile[r][tx] = src[(size_t)(r0 + r) * cols + c0 + tx];
+    __syncthreads();
+    for (int c = ty; c < 64; c += 4)
+        if (c0 + c < cols && r0 + tx < rows) dst[(size_t)(c0 + c) * ts.ldd[z] + r0 + tx] = tile[tx][c];
+}
+
 // out[o] = sum over splits of part[s * n + o] (in split order)
 __global__ void k_sum_splits(size_t n, int splits, const float *__restrict__ part, float *__restrict__ out) {
     for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n; o += (size_t)gridDim.x * blockDim.x) {
@@ -218,7 +399,8 @@ __global__ void k_colsum_part(int M, int N, const float *__restrict__ G, float *
     part[(size_t)blockIdx.y * N + j] = s;
 }
 
-// MSE over M x N: dY = 2 (Y - T) / (M N); sum of squares into *loss
+// MSE over M x N: dY = 2 (Y - T) / (M N); the sum of squares of each
+// workgroup into loss_part[blockIdx.x] (k_sum_loss adds them in order)
 __global__ __launch_bounds__(256) void k_mse(size_t n, float scale, const float *__restrict__ Y,
                                              const float *__restrict__ T, float *__restrict__ dY,
                                              uint16_t *__restrict__ dYb, double *__restrict__ loss) {
@@ -239,7 +421,21 @@ __global__ __launch_bounds__(256) void k_mse(size_t n, float scale, const float 
         if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss, part[0]);
+    if (threadIdx.x == 0) loss[blockIdx.x] = part[0];
+}
+
+// *out = sum of part[0 .. n) in a fixed order (one workgroup)
+__global__ __launch_bounds__(256) void k_sum_loss(int n, const double *__restrict__ part, double *__restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
 }
 
 // Adam (bias-corrected) on FP32 masters, refreshing the BF16 copies
@@ -368,6 +564,62 @@ inline int split_count(int M, int N, int K) {
     return std::max(1, std::min(512 / std::max(tiles, 1), K / 256));
 }
 
+// Large-tile GEMM (k_gemm_nt): the largest tile (128 x 128, 128 x 64,
+// 64 x 64) that still gives two workgroups per CU (512).  K per split is a
+// multiple of the 64-wide K stage.  Returns the number of splits used.
+template <int EPI>
+int gemm_nt(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
+            const float *bias, const float *Zin, float *Z, float *Cf, uint16_t *Cb, int ldc, uint16_t *CbT, int ldt,
+            int splits = 1) {
+    if (M <= 0 || N <= 0 || K <= 0) return 0;
+    const bool vec = (lda % 8 == 0) && (ldb % 8 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0);
+    const int c = (K + std::max(splits, 1) - 1) / std::max(splits, 1);
+    const int kchunk = std::max(NBK, (c + NBK - 1) / NBK * NBK);
+    const int ns = (K + kchunk - 1) / kchunk;
+    auto tiles = [&](int bm, int bn) { return ((M + bm - 1) / bm) * ((N + bn - 1) / bn) * ns; };
+#define LC_GEMM_NT(BM_, BN_, BK_)                                                                                     \
+    hipLaunchKernelGGL((k_gemm_nt<BM_, BN_, BK_, EPI>), dim3((N + BN_ - 1) / BN_, (M + BM_ - 1) / BM_, ns), dim3(256), 0, \
+                       s, M, N, K, kchunk, A, lda, B, ldb, vec, bias, Zin, Z, Cf, Cb, ldc, CbT, ldt)
+    if (tiles(128, 128) >= 512)
+        LC_GEMM_NT(128, 128, 64);
+    else if (tiles(128, 64) >= 512)
+        LC_GEMM_NT(128, 64, 64);
+    else
+        LC_GEMM_NT(64, 64, 128);
+#undef LC_GEMM_NT
+    return ns;
+}
+
+// splits of K for k_gemm_nt: about 512 workgroups of 64 x 64, at least
+// 256 elements of K per split
+inline int split_count_nt(int M, int N, int K) {
+    const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+    return std::max(1, std::min(512 / std::max(tiles, 1), K / 256));
+}
+
+// Batches of BF16 matrix transposes (at most 8 per launch)
+struct Transposer {
+    TransposeSet ts{};
+    int n = 0, rmax = 0, cmax = 0;
+    void add(hipStream_t s, const uint16_t *src, uint16_t *dst, int rows, int cols, int ldd = 0) {
+        if (rows <= 0 || cols <= 0) return;
+        ts.src[n] = src;
+        ts.dst[n] = dst;
+        ts.rows[n] = rows;
+        ts.cols[n] = cols;
+        ts.ldd[n] = ldd > 0 ? ldd : rows;
+        rmax = std::max(rmax, rows);
+        cmax = std::max(cmax, cols);
+        if (++n == 8) flush(s);
+    }
+    void flush(hipStream_t s) {
+        if (!n) return;
+        dim3 grid((cmax + 63) / 64, (rmax + 63) / 64, n);
+        hipLaunchKernelGGL(k_transpose_bf16, grid, dim3(256), 0, s, ts);
+        n = rmax = cmax = 0;
+    }
+};
+
 }  // namespace
 
 // ================================================================== MLP
@@ -382,18 +634,24 @@ struct lc_mlp {
     // device: parameters and optimiser state (one block each, same layout)
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     uint16_t *pb = nullptr;     // BF16 copy of the parameters
+    uint16_t *pbT = nullptr;    // the same with each W_l transposed ([fan_out x fan_in] at woff[l])
     // activations for max_batch rows
     uint16_t *xb = nullptr;                  // inputs, BF16
+    uint16_t *xbT = nullptr;                 // inputs, BF16, transposed [inputs x rows]
     float *x32 = nullptr;                    // inputs, FP32 (for the skip)
     float *skip = nullptr;                   // optional FP32 linear skip [inputs x outputs]
     std::vector<float *> z;                  // pre-activations per layer (1..L)
     std::vector<uint16_t *> ab;              // BF16 activations per layer (1..L-1)
+    std::vector<uint16_t *> abT;             // the same, transposed [width x rows] (training)
     float *y = nullptr, *t = nullptr;        // output and target
     float *ga = nullptr, *gz = nullptr;      // gradient scratch (fp32)
     float *part = nullptr;                   // split-K partial products of the weight gradients
     float *cpart = nullptr;                  // partial column sums of the bias gradients
     uint16_t *gzb = nullptr;                 // gradient, BF16
-    double *loss = nullptr;
+    uint16_t *gzb2 = nullptr;                // second BF16 gradient buffer (large-tile backward)
+    uint16_t *gzbT = nullptr;                // BF16 gradient, transposed [width x rows]
+    double *loss = nullptr;                  // the last loss (sum of squares)
+    double *loss_part = nullptr;             // per-workgroup sums of k_mse
     // data set on the device
     float *dx = nullptr, *dt = nullptr;
     // likelihood for lc_mlp_chisq: data and output de-standardisation
@@ -425,20 +683,49 @@ struct lc_mlp {
     }
     float *W(int l) { return p + woff[l]; }
     uint16_t *Wb(int l) { return pb + woff[l]; }
+    uint16_t *WbT(int l) { return pbT + woff[l]; }
     float *bvec(int l) { return p + boff[l]; }
+
+    // Batches of at least this many rows run on the large-tile GEMM
+    // (k_gemm_nt); smaller ones on k_gemm, whose 64-row tiles waste less.
+    static constexpr int NT_ROWS = 128;
+
+    // the transposed BF16 weights, after every change of pb
+    void refresh_wt() {
+        Transposer tr;
+        for (int l = 1; l <= L; l++) tr.add(stream, Wb(l), WbT(l), width[l - 1], width[l]);
+        tr.flush(stream);
+    }
+    // the transposed BF16 inputs of a training batch
+    void transpose_inputs(int B) {
+        if (B < NT_ROWS) return;
+        Transposer tr;
+        tr.add(stream, xb, xbT, B, width[0], max_batch);
+        tr.flush(stream);
+    }
 
     // forward of B rows whose BF16 inputs are in xb; output in y.  with_skip:
     // add the linear skip of the FP32 inputs in x32 (inference only; the
-    // network is trained on what the skip leaves)
-    void forward(int B, bool with_skip = false) {
+    // network is trained on what the skip leaves).  train: keep what the
+    // backward pass reads (transposed activations on the large-tile path)
+    void forward(int B, bool with_skip = false, bool train = false) {
         hipStream_t s = stream;
         const uint16_t *in = xb;
+        const bool nt = B >= NT_ROWS;
         for (int l = 1; l <= L; l++) {
             const int K = width[l - 1], N = width[l];
-            if (l < L)
+            if (nt) {
+                if (l < L)
+                    gemm_nt<EPI_BIAS | EPI_SILU>(s, B, N, K, in, K, WbT(l), K, bvec(l), nullptr, z[l], nullptr, ab[l],
+                                                 N, train ? abT[l] : nullptr, max_batch);
+                else
+                    gemm_nt<EPI_BIAS>(s, B, N, K, in, K, WbT(l), K, bvec(l), nullptr, nullptr, y, nullptr, N, nullptr,
+                                      0);
+            } else if (l < L) {
                 gemm<false, false, EPI_BIAS | EPI_SILU>(s, B, N, K, in, K, Wb(l), N, bvec(l), z[l], nullptr, ab[l], N);
-            else
+            } else {
                 gemm<false, false, EPI_BIAS>(s, B, N, K, in, K, Wb(l), N, bvec(l), nullptr, y, nullptr, N);
+            }
             gemm_flop += 2.0 * B * N * K;
             in = ab[l];
         }
@@ -453,8 +740,13 @@ struct lc_mlp {
         hipStream_t s = stream;
         const int NL = width[L];
         const size_t n = (size_t)B * NL;
-        MLPCHK(hipMemsetAsync(loss, 0, sizeof(double), s));
-        hipLaunchKernelGGL(k_mse, dim3(nblocks(n)), dim3(256), 0, s, n, 2.f / (float)n, y, t, gz, gzb, loss);
+        const int nb = nblocks(n);
+        hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, n, 2.f / (float)n, y, t, gz, gzb, loss_part);
+        hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, s, nb, loss_part, loss);
+        if (B >= NT_ROWS) {
+            backward_nt(B);
+            return;
+        }
         for (int l = L; l >= 1; l--) {
             const int K = width[l - 1], N = width[l];
             const uint16_t *in = l == 1 ? xb : ab[l - 1];
@@ -481,12 +773,52 @@ struct lc_mlp {
         }
     }
 
+    // backward on the large-tile GEMM: dW_l = A_{l-1}^T G_l from the
+    // transposed activations and gradients (rows max_batch apart), whose
+    // extra last row of ones makes the GEMM's last output row the bias
+    // gradient sum_i G_l[i] (b_l follows W_l in the parameter vector);
+    // dA_{l-1} = G_l W_l^T with the SiLU gradient applied in its epilogue,
+    // which also writes the next layer's transposed gradient (the BF16
+    // gradient alternates between two buffers: the GEMM reads one and
+    // writes the other)
+    void backward_nt(int B) {
+        hipStream_t s = stream;
+        {
+            Transposer tr;
+            tr.add(s, gzb, gzbT, B, width[L], max_batch);
+            tr.flush(s);
+        }
+        uint16_t *cur = gzb, *nxt = gzb2;
+        for (int l = L; l >= 1; l--) {
+            const int K = width[l - 1], N = width[l];
+            const uint16_t *inT = l == 1 ? xbT : abT[l - 1];
+            const int sp = split_count_nt(K + 1, N, B);
+            if (sp > 1) {
+                const int ns = gemm_nt<0>(s, K + 1, N, B, inT, max_batch, gzbT, max_batch, nullptr, nullptr, nullptr,
+                                          part, nullptr, N, nullptr, 0, sp);
+                const size_t nw_ = (size_t)(K + 1) * N;
+                hipLaunchKernelGGL(k_sum_splits, dim3(nblocks(nw_)), dim3(256), 0, s, nw_, ns, part, g + woff[l]);
+            } else {
+                gemm_nt<0>(s, K + 1, N, B, inT, max_batch, gzbT, max_batch, nullptr, nullptr, nullptr, g + woff[l],
+                           nullptr, N, nullptr, 0);
+            }
+            gemm_flop += 2.0 * B * N * K;
+            if (l > 1) {
+                gemm_nt<EPI_DSILU>(s, B, K, N, cur, N, Wb(l), N, nullptr, z[l - 1], nullptr, gz, nxt, K, gzbT,
+                                   max_batch);
+                gemm_flop += 2.0 * B * N * K;
+                std::swap(cur, nxt);
+            }
+        }
+    }
+
     void adam(float lr) {
         adam_t++;
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
         const float c1 = 1.f - (float)std::pow(0.9, (double)adam_t), c2 = 1.f - (float)std::pow(0.999, (double)adam_t);
         hipLaunchKernelGGL(k_adam, dim3(nblocks(nparam)), dim3(256), 0, stream, nparam, p, g, m, v, pb, lr, b1, b2,
                            eps, c1, c2);
+        refresh_wt();
     }
 
     void upload_inputs(const float *x, int B) {
@@ -550,33 +882,53 @@ lc_status lc_mlp_create(int device, const int32_t *widths, int nwidth, int max_b
         mm->m = mm->alloc<float>(off);
         mm->v = mm->alloc<float>(off);
         mm->pb = mm->alloc<uint16_t>(off);
+        mm->pbT = mm->alloc<uint16_t>(off);
         MLPCHK(hipMemset(mm->p, 0, off * sizeof(float)));
         MLPCHK(hipMemset(mm->g, 0, off * sizeof(float)));
         MLPCHK(hipMemset(mm->m, 0, off * sizeof(float)));
         MLPCHK(hipMemset(mm->v, 0, off * sizeof(float)));
         MLPCHK(hipMemset(mm->pb, 0, off * sizeof(uint16_t)));
+        MLPCHK(hipMemset(mm->pbT, 0, off * sizeof(uint16_t)));
         const size_t B = (size_t)max_batch;
         mm->xb = mm->alloc<uint16_t>(B * std::max(widths[0], 1));
         mm->x32 = mm->alloc<float>(B * std::max(widths[0], 1));
+        // transposed inputs and activations: width + 1 rows of max_batch,
+        // the last one all ones (the bias column of the weight gradients)
+        auto ones_row = [&](uint16_t *q, int rows) {
+            std::vector<uint16_t> one(B, (uint16_t)0x3f80);
+            MLPCHK(hipMemset(q, 0, (size_t)rows * B * sizeof(uint16_t)));
+            MLPCHK(hipMemcpy(q + (size_t)rows * B, one.data(), B * sizeof(uint16_t), hipMemcpyHostToDevice));
+        };
+        mm->xbT = mm->alloc<uint16_t>(B * (std::max(widths[0], 1) + 1));
+        ones_row(mm->xbT, widths[0]);
         mm->z.assign(nwidth, nullptr);
         mm->ab.assign(nwidth, nullptr);
+        mm->abT.assign(nwidth, nullptr);
         for (int l = 1; l <= mm->L; l++) {
             mm->z[l] = mm->alloc<float>(B * widths[l]);
-            if (l < mm->L) mm->ab[l] = mm->alloc<uint16_t>(B * widths[l]);
+            if (l < mm->L) {
+                mm->ab[l] = mm->alloc<uint16_t>(B * widths[l]);
+                mm->abT[l] = mm->alloc<uint16_t>(B * (widths[l] + 1));
+                ones_row(mm->abT[l], widths[l]);
+            }
         }
         mm->y = mm->alloc<float>(B * widths[mm->L]);
         mm->t = mm->alloc<float>(B * widths[mm->L]);
         mm->ga = mm->alloc<float>(B * wmax);
         size_t part_n = 1;
         for (int l = 1; l <= mm->L; l++) {
-            const int sp = split_count(widths[l - 1], widths[l], max_batch);
-            if (sp > 1) part_n = std::max(part_n, (size_t)sp * widths[l - 1] * widths[l]);  // splits <= sp
+            const int sp = std::max(split_count(widths[l - 1], widths[l], max_batch),
+                                    split_count_nt(widths[l - 1] + 1, widths[l], max_batch));
+            if (sp > 1) part_n = std::max(part_n, (size_t)sp * (widths[l - 1] + 1) * widths[l]);  // splits <= sp
         }
         mm->part = mm->alloc<float>(part_n);
         mm->cpart = mm->alloc<float>((size_t)CS_PART * wmax);
         mm->gz = mm->alloc<float>(B * wmax);
         mm->gzb = mm->alloc<uint16_t>(B * wmax);
+        mm->gzb2 = mm->alloc<uint16_t>(B * wmax);
+        mm->gzbT = mm->alloc<uint16_t>(B * wmax);
         mm->loss = mm->alloc<double>(1);
+        mm->loss_part = mm->alloc<double>(4096);   // nblocks() caps grids at 4096
         *out = mm.release();
         return LC_OK;
     });
@@ -602,6 +954,7 @@ lc_status lc_mlp_set_params(lc_mlp *mm, const float *params, size_t n) {
         MLPCHK(hipMemsetAsync(mm->m, 0, n * sizeof(float), mm->stream));
         MLPCHK(hipMemsetAsync(mm->v, 0, n * sizeof(float), mm->stream));
         hipLaunchKernelGGL(k_cast_bf16, dim3(nblocks(n)), dim3(256), 0, mm->stream, n, mm->p, mm->pb);
+        mm->refresh_wt();
         MLPCHK(hipStreamSynchronize(mm->stream));
         mm->adam_t = 0;
         return LC_OK;
@@ -644,8 +997,9 @@ lc_status lc_mlp_grad(lc_mlp *mm, const float *x, const float *target, size_t ba
         MLPCHK(hipSetDevice(mm->device));
         const int B = (int)batch, nout = mm->width[mm->L];
         mm->upload_inputs(x, B);
+        mm->transpose_inputs(B);
         MLPCHK(hipMemcpyAsync(mm->t, target, (size_t)B * nout * sizeof(float), hipMemcpyHostToDevice, mm->stream));
-        mm->forward(B);
+        mm->forward(B, false, true);
         mm->backward(B);
         MLPCHK(hipGetLastError());
         MLPCHK(hipMemcpyAsync(grad, mm->g, mm->nparam * sizeof(float), hipMemcpyDeviceToHost, mm->stream));
@@ -699,7 +1053,8 @@ lc_status lc_mlp_train(lc_mlp *mm, int steps, int batch, float lr, uint64_t seed
         for (int s = 0; s < steps; s++) {
             hipLaunchKernelGGL(k_gather, dim3(batch), dim3(256), 0, mm->stream, batch, nin, nout, mm->nrows, seed,
                                (uint64_t)(mm->adam_t + 1), mm->dx, mm->dt, mm->xb, mm->t);
-            mm->forward(batch);
+            mm->transpose_inputs(batch);
+            mm->forward(batch, false, true);
             mm->backward(batch);
             mm->adam(lr);
         }
@@ -725,9 +1080,10 @@ lc_status lc_mlp_eval_dataset(lc_mlp *mm, size_t row0, size_t count, double *mse
             hipLaunchKernelGGL(k_cast_bf16, dim3(nblocks(n)), dim3(256), 0, mm->stream, n, mm->dx + r0 * nin, mm->xb);
             mm->forward(B);
             const size_t no = (size_t)B * nout;
-            MLPCHK(hipMemsetAsync(mm->loss, 0, sizeof(double), mm->stream));
-            hipLaunchKernelGGL(k_mse, dim3(nblocks(no)), dim3(256), 0, mm->stream, no, 0.f, mm->y,
-                               mm->dt + r0 * nout, nullptr, nullptr, mm->loss);
+            const int nb = nblocks(no);
+            hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, mm->stream, no, 0.f, mm->y, mm->dt + r0 * nout, nullptr,
+                               nullptr, mm->loss_part);
+            hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, mm->stream, nb, mm->loss_part, mm->loss);
             double l = 0.;
             MLPCHK(hipMemcpyAsync(&l, mm->loss, sizeof(double), hipMemcpyDeviceToHost, mm->stream));
             MLPCHK(hipStreamSynchronize(mm->stream));

@@ -134,17 +134,18 @@ def test_fit_workload_full_shape(engine):
     print(f"fit workload: worst Jacobian column {worst:.3e}", flush=True)
 
 
-def test_surrogate_network_full_shape():
+@pytest.mark.parametrize("batch", [32, 320])
+def test_surrogate_network_full_shape(batch):
     from lcurve_re_amd import surrogate as S
     from oracle import mlp_ref as R
     widths = S.WIDTHS
     assert widths == [12, 512, 512, 512, 512, 2048]
     rng = np.random.default_rng(21)
     p = S.init_params(widths, 9)
-    x = rng.uniform(-1, 1, (32, 12)).astype(np.float32)
-    t = rng.standard_normal((32, 2048)).astype(np.float32)
-    print("surrogate network 12-4x512-2048, batch 32: forward and gradient", flush=True)
-    mlp = S.MLP(widths, max_batch=64)
+    x = rng.uniform(-1, 1, (batch, 12)).astype(np.float32)
+    t = rng.standard_normal((batch, 2048)).astype(np.float32)
+    print(f"surrogate network 12-4x512-2048, batch {batch}: forward and gradient", flush=True)
+    mlp = S.MLP(widths, max_batch=max(64, batch))
     try:
         mlp.set_params(p)
         y = mlp.forward(x)

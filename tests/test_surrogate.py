@@ -144,15 +144,20 @@ def test_gemm_layer_matches_numpy():
 
 
 @pytest.mark.gpu
-def test_forward_backward_match_cpu_reference():
+@pytest.mark.parametrize("batch", [70, 256, 300])
+def test_forward_backward_match_cpu_reference(batch):
+    """Batches under 128 rows run on the 64 x 64 GEMM, larger ones on the
+    128-row tiles with transposed operands (256: 16-byte loads where the
+    widths allow; 300: rows that are not a multiple of 8, element loads and
+    partial tiles)."""
     from oracle import mlp_ref as R
     widths = [12, 96, 80, 130]
     rng = np.random.default_rng(12)
     p = S.init_params(widths, 5)
-    x = rng.uniform(-1, 1, (70, 12)).astype(np.float32)
-    t = rng.standard_normal((70, 130)).astype(np.float32)
-    print("surrogate: forward/backward 12-96-80-130, batch 70", flush=True)
-    mlp = S.MLP(widths, max_batch=128)
+    x = rng.uniform(-1, 1, (batch, 12)).astype(np.float32)
+    t = rng.standard_normal((batch, 130)).astype(np.float32)
+    print(f"surrogate: forward/backward 12-96-80-130, batch {batch}", flush=True)
+    mlp = S.MLP(widths, max_batch=384)
     try:
         mlp.set_params(p)
         y = mlp.forward(x)
