@@ -221,6 +221,32 @@ def copy_calibration(mib=256, reps=4):
                                            "checksum": float(y[::4096].sum().item())}}), flush=True)
 
 
+def mlp_train_timing(steps=300, batch=4096, rows=65536):
+    """Adam steps of the surrogate network (12 -> 4 x 512 -> 2048, BF16 on the
+    matrix cores) on a synthetic data set resident on the device: time per
+    step and the GEMM work per second of training wall time."""
+    from lcurve_re_amd import surrogate as S
+    widths = S.WIDTHS
+    mlp = S.MLP(widths, max_batch=batch)
+    try:
+        mlp.set_params(S.init_params(widths, 3))
+        rng = np.random.default_rng(1)
+        mlp.set_dataset(rows)
+        mlp.load_dataset(0, rng.uniform(-1, 1, (rows, widths[0])).astype(np.float32),
+                         0.1 * rng.standard_normal((rows, widths[-1])).astype(np.float32))
+        mlp.train(5, batch, 1e-3, 7)
+        st0 = mlp.stats()
+        t0 = time.perf_counter()
+        loss = mlp.train(steps, batch, 1e-3, 8)
+        el = time.perf_counter() - t0
+        st = mlp.stats()
+    finally:
+        mlp.close()
+    flop = st["gemm_flop"] - st0["gemm_flop"]
+    return {"network": widths, "batch": batch, "steps": steps, "ms_per_step": el / steps * 1e3,
+            "gemm_flop_per_step": flop / steps, "gemm_tflops_over_wall": flop / el / 1e12, "final_loss": loss}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,6 +267,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default flux tile shape")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
+    ap.add_argument("--mlp-train", action="store_true",
+                    help="run only Adam steps of the surrogate network on synthetic data (GEMM timing)")
     ap.add_argument("--valu-calibration", action="store_true",
                     help="run only the VALU issue-rate kernels of known instruction count")
     args = ap.parse_args()
@@ -251,6 +279,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.mlp_train:
+        print(json.dumps({"mlp_train": mlp_train_timing()}), flush=True)
         return
     if args.valu_calibration:
         print(json.dumps({"valu_calibration": valu_calibration()}), flush=True)
