@@ -260,6 +260,8 @@ def main():
     ap.add_argument("--local-curves", type=int, default=65536, help="surrogate_mcmc: curves for the local network")
     ap.add_argument("--refine-steps", type=int, default=2000, help="surrogate_mcmc: Adam steps of the local network")
     ap.add_argument("--n-sub", type=int, default=5, help="surrogate_mcmc: surrogate steps per engine evaluation")
+    ap.add_argument("--k-try", type=int, default=6,
+                    help="surrogate_mcmc: first-stage trials screened per walker and round (batched sampler)")
     ap.add_argument("--precision", type=int, default=None,
                     help="0: FP32 per face (default), 1: FP64 per face (default for fit_batch)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -522,16 +524,29 @@ def run_surrogate_mcmc(L, eng, args):
         pilot[Tc] = stp["accepted"] / stp["wall_s"]
     T = max(pilot, key=pilot.get)
     cal["pilot_accepted_per_s"] = {str(k): v for k, v in pilot.items()}
+    # the same for the sampler with batched first-stage trials
+    pilot_b = {}
+    for Tc in sorted({1.0, 2.0, 4.0, round(T_cal, 3)}):
+        _, stp = S.run_chains_batched(post.loglike, post.legal, x0, cov, 60, seed=43,
+                                      sur_loglike=post.surrogate_loglike(sur, Tc), k_try=args.k_try)
+        pilot_b[Tc] = stp["accepted"] / stp["wall_s"]
+    T_b = max(pilot_b, key=pilot_b.get)
+    cal["pilot_accepted_per_s_batched"] = {str(k): v for k, v in pilot_b.items()}
     log(f"bench: local surrogate {linfo}; stage-1 temperature {T:.3g} {cal}")
     n_iter = max(args.mcmc_steps // nw, 20)
     burn = n_iter // 10
     res = {}
     sl = post.surrogate_loglike(sur, T)
     for name, f, nsub in (("plain", None, 0), ("delayed_acceptance", sl, 1),
-                          ("surrogate_transition", sl, args.n_sub)):
+                          ("surrogate_transition", sl, args.n_sub),
+                          ("delayed_acceptance_batched", post.surrogate_loglike(sur, T_b), -1)):
         eng.reset_profile()
-        chain, st = S.run_chains(post.loglike, post.legal, x0, cov, n_iter, seed=11, sur_loglike=f,
-                                 n_sub=max(nsub, 1))
+        if nsub < 0:
+            chain, st = S.run_chains_batched(post.loglike, post.legal, x0, cov, n_iter, seed=11, sur_loglike=f,
+                                             k_try=args.k_try)
+        else:
+            chain, st = S.run_chains(post.loglike, post.legal, x0, cov, n_iter, seed=11, sur_loglike=f,
+                                     n_sub=max(nsub, 1))
         e = S.ess(chain, burn)
         res[name] = {"chain": chain, "stats": st, "ess_min": float(e.min()), "ess": e,
                      "ess_per_s": float(e.min()) / st["wall_s"], "accept": st["accepted"] / st["proposals"],
@@ -541,19 +556,22 @@ def run_surrogate_mcmc(L, eng, args):
     a = res["plain"]["chain"][burn:].reshape(-1, d)
     ks, shift = {}, {}
     sd = a.std(axis=0)
-    for other in ("delayed_acceptance", "surrogate_transition"):
+    others = ("delayed_acceptance", "surrogate_transition", "delayed_acceptance_batched")
+    for other in others:
         b = res[other]["chain"][burn:].reshape(-1, d)
         kk = {}
         for j, name in enumerate(S.NAMES):
-            # thin each chain by its integrated autocorrelation time
-            ta = max(1, int((n_iter - burn) * nw / res["plain"]["ess"][j]))
-            tb = max(1, int((n_iter - burn) * nw / res[other]["ess"][j]))
+            # thin each walker's chain by twice its integrated
+            # autocorrelation time (at one time samples still correlate
+            # by ~1/e, which a two-sample KS test reads as a difference)
+            ta = max(1, int(2 * (n_iter - burn) * nw / res["plain"]["ess"][j]))
+            tb = max(1, int(2 * (n_iter - burn) * nw / res[other]["ess"][j]))
             ca = res["plain"]["chain"][burn::ta, :, j].ravel()
             cb = res[other]["chain"][burn::tb, :, j].ravel()
             kk[name] = float(SS.ks_2samp(ca, cb).pvalue)
         ks[other] = kk
         shift[other] = (np.abs(a.mean(axis=0) - b.mean(axis=0)) / np.where(sd > 0, sd, 1)).tolist()
-    best = max(("delayed_acceptance", "surrogate_transition"), key=lambda k: res[k]["ess_per_s"])
+    best = max(others, key=lambda k: res[k]["ess_per_s"])
     gain = res[best]["ess_per_s"] / res["plain"]["ess_per_s"]
     out_stats = {k: {kk: vv for kk, vv in v.items() if kk not in ("chain", "ess")} for k, v in res.items()}
     return {"metric": "effective samples/sec of surrogate-screened MCMC vs plain GPU MCMC",
@@ -564,8 +582,9 @@ def run_surrogate_mcmc(L, eng, args):
             "data": "synthetic", "config": {"workload": "surrogate_mcmc", "walkers": nw, "walker_steps": n_iter * nw,
                                             "phases": len(w["time"]), "network": S.WIDTHS,
                                             "training_curves": args.curves, "prior": S.PRIOR,
-                                            "n_sub": args.n_sub},
-            "surrogate": info, "stage1_temperature": T, "calibration": cal, "chains": out_stats,
+                                            "n_sub": args.n_sub, "k_try": args.k_try},
+            "surrogate": info, "stage1_temperature": T, "stage1_temperature_batched": T_b, "calibration": cal,
+            "chains": out_stats,
             "best": best, "gain_ess_per_s": gain, "posterior_ks_pvalue": ks, "posterior_mean_shift_in_sd": shift,
             "roofline": {"bound": "mfma", "kernel": "k_gemm (training steps: all kernels of a step)",
                          "achieved": info["gemm_tflops"], "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",

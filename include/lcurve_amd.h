@@ -247,6 +247,11 @@ lc_status lc_model_get_dstep(const lc_model *m, double *v, size_t n);
 lc_status lc_model_get_limit(const lc_model *m, double *lo, double *hi, size_t n);
 /* Model::is_not_legal (model.cpp:788): returns 1 if illegal, 0 if legal */
 int lc_model_is_not_legal(const lc_model *m, const double *vpar, size_t n);
+/* Model::is_not_legal over nvec parameter vectors of n values each (rows n
+   apart): out[k] = 1 if vector k is illegal.  No counterpart in the
+   reference, which tests one vector per call; for samplers screening many
+   proposals at once. */
+lc_status lc_model_is_not_legal_batch(const lc_model *m, const double *vpar, size_t nvec, size_t n, int *out);
 /* Model::get_name (model.cpp:562): writes NUL-terminated name */
 lc_status lc_model_get_name(const lc_model *m, int i, char *buf, size_t buflen);
 /* Lcurve::numface via set_star_grid's face count (set_star_grid.cpp:110-162) */

@@ -124,6 +124,17 @@ class Model:
         v = np.ascontiguousarray(vpar, np.float64)
         return bool(_lib().lc_model_is_not_legal(C.byref(self.pod), _dp(v), len(v)))
 
+    def is_not_legal_batch(self, P):
+        """is_not_legal of each row of P: a bool array."""
+        P = np.ascontiguousarray(np.atleast_2d(P), np.float64)
+        out = np.zeros(P.shape[0], np.int32)
+        if P.shape[0]:
+            st = _lib().lc_model_is_not_legal_batch(C.byref(self.pod), _dp(P), P.shape[0], P.shape[1],
+                                                    out.ctypes.data_as(C.POINTER(C.c_int)))
+            if st != LC_OK:
+                raise LcurveError(st, "lc_model_is_not_legal_batch")
+        return out.astype(bool)
+
     def get_name(self, i):
         buf = C.create_string_buffer(64)
         _lib().lc_model_get_name(C.byref(self.pod), int(i), buf, 64)

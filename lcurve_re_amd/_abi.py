@@ -147,6 +147,7 @@ SIGNATURES = {
     "lc_model_get_dstep": (_c.c_int, [_c.POINTER(lc_model), _DP, _c.c_size_t]),
     "lc_model_get_limit": (_c.c_int, [_c.POINTER(lc_model), _DP, _DP, _c.c_size_t]),
     "lc_model_is_not_legal": (_c.c_int, [_c.POINTER(lc_model), _DP, _c.c_size_t]),
+    "lc_model_is_not_legal_batch": (_c.c_int, [_c.POINTER(lc_model), _DP, _c.c_size_t, _c.c_size_t, _IP]),
     "lc_model_get_name": (_c.c_int, [_c.POINTER(lc_model), _c.c_int, _c.c_char_p, _c.c_size_t]),
     "lc_star_grid_size": (_c.c_int, [_c.POINTER(lc_model), _c.c_int, _c.c_int, _c.POINTER(_c.c_size_t)]),
     "lc_light_curve_comp": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_model), _c.POINTER(lc_datum), _c.c_size_t,

@@ -583,6 +583,33 @@ int lc_model_is_not_legal(const lc_model *m, const double *vpar, size_t n) {
     return 0;
 }
 
+lc_status lc_model_is_not_legal_batch(const lc_model *m, const double *vpar, size_t nvec, size_t n, int *out) {
+    if (!m || (!vpar && nvec) || (!out && nvec)) return LC_E_ARG;
+    const size_t nv = (size_t)lc_model_nvary(m);
+    if (nv > n) {
+        for (size_t k = 0; k < nvec; k++) out[k] = 1;
+        return LC_OK;
+    }
+    int st = 0;
+    // the limits depend on the current model only: computed once
+    const double xl11 = lcx_host_l1(1, m->q.value, m->spin1.value, &st);
+    const double xl12 = 1. - lcx_host_l1(2, m->q.value, m->spin2.value, &st);
+    for (size_t r = 0; r < nvec; r++) {
+        const double *vp = vpar + r * n;
+        int bad = 0;
+        size_t k = 0;
+        for (size_t i = 0; i < NPTAB && !bad; i++) {
+            const PDesc &d = PTAB[i];
+            if (!active(m, d)) continue;
+            const double v = vp[k++];
+            bad = (d.lo_kind == 1 && v < d.lo) || (d.lo_kind == 2 && v <= d.lo) || (d.hi_kind == 1 && v > d.hi) ||
+                  (d.hi_kind == 2 && v >= d.hi) || (d.hi_kind == 3 && v >= xl11) || (d.hi_kind == 4 && v > xl12);
+        }
+        out[r] = bad;
+    }
+    return LC_OK;
+}
+
 lc_status lc_model_get_name(const lc_model *m, int i, char *buf, size_t buflen) {
     if (!buf || buflen == 0) return LC_E_ARG;
     int k = 0;

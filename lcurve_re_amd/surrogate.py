@@ -393,7 +393,10 @@ class Posterior:
         self.evaluations = 0
 
     def legal(self, P):
-        return np.array([not self.base.is_not_legal(p) for p in np.asarray(P)], bool)
+        P = np.asarray(P, np.float64)
+        if P.shape[0] == 0:
+            return np.zeros(0, bool)
+        return ~self.base.is_not_legal_batch(P)
 
     def loglike(self, P):
         P = np.asarray(P, np.float64)
@@ -475,6 +478,73 @@ def run_chains(loglike, legal, x0, cov, n_iter, seed, sur_loglike=None, n_sub=1)
                    "surrogate_s": t_sur, "wall_s": time.perf_counter() - t0}
 
 
+def run_chains_batched(loglike, legal, x0, cov, n_iter, seed, sur_loglike, k_try=4):
+    """Delayed-acceptance Metropolis (Christen & Fox 2005, as run_chains with
+    n_sub = 1) with the first-stage trials of a walker batched: until a
+    walker's first stage accepts, every trial starts from the same state, so
+    its next k_try trials y_1..y_k = x + N(0, cov) are drawn at once and
+    screened in one surrogate call for all walkers; the first y_j that the
+    first stage accepts goes to the engine (one call per round for all
+    walkers), and y_1..y_{j-1} are the chain's first-stage rejections (the
+    walker stays at x for those steps).  The chain is the one the step-by-
+    step delayed-acceptance sampler draws from the same proposals; a round
+    advances a walker by up to k_try steps for one engine evaluation.
+    Returns the chain (n_iter, W, d) and counters (rounds = engine calls)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = np.array(x0, np.float64)
+    W, d = x.shape
+    Lc = np.linalg.cholesky(cov)
+    lx = loglike(x)
+    ls = sur_loglike(x)
+    chain = np.empty((n_iter, W, d))
+    step = np.zeros(W, int)
+    n_eval, n_sur, n_sub_acc, n_acc, rounds, t_sur = W, W, 0, 0, 0, 0.0
+    t0 = time.perf_counter()
+    while True:
+        act = np.nonzero(step < n_iter)[0]
+        if len(act) == 0:
+            break
+        ts = time.perf_counter()
+        z = x[act, None, :] + rng.standard_normal((len(act), k_try, d)) @ Lc.T
+        u1 = np.log(rng.random((len(act), k_try)))
+        u2 = np.log(rng.random(len(act)))
+        flat = z.reshape(-1, d)
+        ok = legal(flat)
+        lz = np.full(len(flat), -np.inf)
+        lz[ok] = sur_loglike(flat[ok])
+        n_sur += int(ok.sum())
+        lz = lz.reshape(len(act), k_try)
+        t_sur += time.perf_counter() - ts
+        room = np.minimum(k_try, n_iter - step[act])
+        with np.errstate(invalid="ignore"):
+            ok1 = (u1 < lz - ls[act, None]) & (np.arange(k_try)[None, :] < room[:, None])
+        has = ok1.any(axis=1)
+        j = np.where(has, np.argmax(ok1, axis=1), room)   # first-stage rejections before the first acceptance
+        for t in range(k_try):                            # the walker stays at x for those steps
+            rows = act[t < j]
+            chain[step[rows] + t, rows] = x[rows]
+        step[act] += j
+        r = np.nonzero(has)[0]
+        if len(r):
+            pend = act[r]
+            y = z[r, j[r]]
+            lsy = lz[r, j[r]]
+            n_sub_acc += len(r)
+            ly = loglike(y)
+            n_eval += len(r)
+            rounds += 1
+            with np.errstate(invalid="ignore"):
+                acc = u2[r] < (ly - lx[pend]) - (lsy - ls[pend])
+            a = pend[acc]
+            x[a], lx[a], ls[a] = y[acc], ly[acc], lsy[acc]
+            n_acc += int(acc.sum())
+            chain[step[pend], pend] = x[pend]
+            step[pend] += 1
+    return chain, {"evaluations": n_eval, "surrogate_evaluations": n_sur, "surrogate_steps_accepted": n_sub_acc,
+                   "accepted": n_acc, "proposals": n_iter * W, "n_sub": 1, "k_try": k_try, "rounds": rounds,
+                   "surrogate_s": t_sur, "wall_s": time.perf_counter() - t0}
+
+
 def calibrate_temperature(post, sur, x, cov, n, seed):
     """Stage-1 temperature from pilot proposals around the walkers x: the
     spread of (surrogate - engine) log-likelihood differences, at least 1.
@@ -539,8 +609,7 @@ def train_local_surrogate(engine, base, samples, cov, n, steps, seed, batch=2048
         while got < cnt:
             P = samples[rng.integers(0, samples.shape[0], 4096)] + rng.standard_normal((4096, samples.shape[1])) @ Lc.T
             P[:, 10] = center[10]
-            ok = np.array([not base.is_not_legal(p) for p in P], bool)
-            P = P[ok]
+            P = P[~base.is_not_legal_batch(P)]
             if P.shape[0] == 0:
                 continue
             Pz = P.copy()

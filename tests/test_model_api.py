@@ -153,3 +153,26 @@ def test_is_not_legal_radii_use_current_roche_lobes():
     assert m.is_not_legal([xl11 * (1 + 1e-9), 0.1])       # r1 >= xl11
     assert m.is_not_legal([0.1, rl2 * (1 + 1e-9)])        # r2 > 1 - xl12
     assert m.is_not_legal([0.0, 0.1])
+
+
+def test_is_not_legal_batch_is_the_single_test_per_row():
+    """lc_model_is_not_legal_batch: the per-vector test on every row, at the
+    edges of each limit and at random points inside and outside."""
+    m, _ = _model(vary=VARY_ORDER)
+    rng = np.random.default_rng(4)
+    p = m.get_param()
+    rows = [p]
+    for j, (lo, hi) in enumerate(m.get_limit()):
+        for v in (lo, hi, lo - 1e-9 * max(1.0, abs(lo)), hi + 1e-9 * max(1.0, abs(hi))):
+            if np.isfinite(v):
+                q = p.copy()
+                q[j] = v
+                rows.append(q)
+    rows += list(p * (1 + 0.3 * rng.standard_normal((400, len(p)))))
+    P = np.array(rows)
+    want = np.array([m.is_not_legal(r) for r in P])
+    got = m.is_not_legal_batch(P)
+    assert want.any() and not want.all()
+    np.testing.assert_array_equal(got, want)
+    assert m.is_not_legal_batch(P[:, :-1]).all()   # too few values
+    assert m.is_not_legal_batch(np.zeros((0, len(p)))).shape == (0,)

@@ -114,6 +114,51 @@ def test_delayed_acceptance_keeps_the_target(n_sub):
         assert sd["accepted"] / sd["evaluations"] > sp["accepted"] / sp["evaluations"]
 
 
+@pytest.mark.parametrize("k_try", [1, 4])
+def test_batched_first_stage_keeps_the_target(k_try):
+    """run_chains_batched: the delayed-acceptance chain with each walker's
+    first-stage trials screened k_try at a time; same target, and one
+    expensive evaluation per round covers several chain steps."""
+    mean, cov, loglike, legal = _gauss_problem()
+    prec_s = np.linalg.inv(cov * 1.5)
+    calls = []
+
+    def counted(P):
+        calls.append(len(P))
+        return loglike(P)
+
+    def sur(P):
+        d = np.asarray(P) - (mean + 0.3)
+        return -0.5 * np.einsum("ij,jk,ik->i", d, prec_s, d)
+
+    x0 = np.random.default_rng(1).standard_normal((32, 2)) + mean
+    ch, st = S.run_chains_batched(counted, legal, x0, 2.0 * cov, 6000, seed=9, sur_loglike=sur, k_try=k_try)
+    assert ch.shape == (6000, 32, 2) and np.all(np.isfinite(ch))
+    s = ch[500:].reshape(-1, 2)
+    n_eff = S.ess(ch, 500).min()
+    se = np.sqrt(np.diag(cov) / n_eff)
+    assert np.all(np.abs(s.mean(axis=0) - mean) < 5 * se)
+    assert np.allclose(np.cov(s.T), cov, rtol=0.12, atol=0.05)
+    assert st["evaluations"] == sum(calls) and st["rounds"] == len(calls) - 1
+    if k_try > 1:
+        assert st["rounds"] < 0.6 * 6000
+
+
+def test_batched_first_stage_is_the_stepwise_chain():
+    """With an exact first stage (surrogate = target), the second stage
+    accepts every first-stage acceptance; the chain's steps are then those
+    of plain Metropolis driven by the first-stage draws: check the accepted
+    fraction matches plain Metropolis with the same proposal."""
+    mean, cov, loglike, legal = _gauss_problem()
+    x0 = np.random.default_rng(2).standard_normal((64, 2)) + mean
+    plain, sp = S.run_chains(loglike, legal, x0, 2.0 * cov, 3000, seed=3)
+    ch, st = S.run_chains_batched(loglike, legal, x0, 2.0 * cov, 3000, seed=4, sur_loglike=loglike, k_try=6)
+    assert st["accepted"] == st["surrogate_steps_accepted"] == st["evaluations"] - 64
+    a0 = sp["accepted"] / sp["proposals"]
+    a1 = st["accepted"] / st["proposals"]
+    assert abs(a1 - a0) < 0.02, (a0, a1)
+
+
 def test_autocorr_time_of_ar1():
     rng = np.random.default_rng(2)
     rho, n, W_ = 0.9, 20000, 8
