@@ -455,6 +455,27 @@ __global__ __launch_bounds__(256) void k_comp_build(const CompDev *__restrict__ 
     if (bad) atomicCAS(&inst_status[C.inst], 0, LC_E_ECLIPSE_SEARCH);
 }
 
+// The opaque disc's ranges of each star face of the grids gidx[]
+// (blockIdx.y): disc_eclipse at the face position, as light_curve_comp.cpp:
+// 286-326 appends them to the star's own.
+__global__ __launch_bounds__(256) void k_disc_star_ranges(const GridDev *__restrict__ grids, const int *__restrict__ gidx,
+                                                         const CompDev *__restrict__ comps, FaceArrays fa,
+                                                         int *inst_status) {
+    const GridDev &G = grids[gidx[blockIdx.y]];
+    const CompDev &C = comps[G.disc_comp];
+    const double ang = PI / 180. * C.iangle;  // disc_eclipse.cpp:62-66
+    const double si = sin(ang), co = cos(ang);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G.nface; k += gridDim.x * blockDim.x) {
+        const int f = G.face_off + k;
+        int bad = 0;
+        const int nr =
+            disc_eclipse(si, co, C.rdisc1, C.rdisc2, C.beta, C.height, v3(fa.px[f], fa.py[f], fa.pz[f]), fa.dr + 2 * f,
+                         bad);
+        fa.dn[f] = nr;
+        if (bad) atomicCAS(&inst_status[G.inst], 0, LC_E_ECLIPSE_SEARCH);
+    }
+}
+
 constexpr int CT = 256;  // threads (sub-phases) per workgroup and faces per LDS stage
 
 // Disc, edge and spot sums at the sub-phases of one instance
@@ -522,6 +543,14 @@ void launch_comp_build(hipStream_t s, const CompDev *comps, int ncomp, int nc, F
     if (ncomp <= 0 || nc <= 0) return;
     hipLaunchKernelGGL(k_comp_build, dim3((nc + 255) / 256), dim3(256), 0, s, comps, ncomp, nc, slots, ca,
                        inst_status);
+}
+
+void launch_disc_star_ranges(hipStream_t s, const GridDev *grids, const int *gidx, int ng, int max_face,
+                             const CompDev *comps, FaceArrays fa, int *inst_status) {
+    if (ng <= 0 || max_face <= 0) return;
+    for (int k = 0; k < ng; k += 65535)
+        hipLaunchKernelGGL(k_disc_star_ranges, dim3(std::min((max_face + 255) / 256, 64), std::min(65535, ng - k)),
+                           dim3(256), 0, s, grids, gidx + k, comps, fa, inst_status);
 }
 
 void launch_comp_sums(hipStream_t s, const CompDev *comps, int ncomp, int S, PhaseArrays pa, CompArrays ca) {

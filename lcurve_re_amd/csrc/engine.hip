@@ -358,6 +358,7 @@ GridPlan plan_grid(const lc_model &m, int star, bool fine) {
     GridPlan gp;
     GridDev &G = gp.dev;
     std::memset(&G, 0, sizeof(G));
+    G.disc_comp = -1;
     const double q = m.q.value;
     double r1, r2;
     get_r1r2(m, r1, r2);
@@ -767,8 +768,6 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         InstDev &I = insts[i];
         std::memset(&I, 0, sizeof(I));
         try {
-            if (m.add_disc && m.opaque)
-                throw EngineError(LC_E_UNSUPPORTED, "opaque discs (the disc eclipsing the stars) are not supported");
             if ((m.add_disc || m.add_spot) && m.iscale)
                 throw EngineError(LC_E_UNSUPPORTED, "iscale with disc or bright spot components is not supported");
             double r1, r2;
@@ -1010,6 +1009,24 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         NC += c.size();
     }
     const int ncomp = (int)comps.size();
+    // an opaque disc hides parts of both stars: their grids get the disc's
+    // ranges (k_disc_star_ranges) and ragged flux sums (k_flux_ragged)
+    std::vector<int> dgrids;
+    int dmax_face = 0;
+    for (int k = 0; k < ncomp; k++) {
+        if (!comps[k].opaque) continue;
+        const InstPlan &P = ip[comps[k].inst];
+        for (int q = 0; q < 4; q++) {
+            const int g = P.g[q];
+            if (g < 0 || gdev[g].disc_comp >= 0) continue;
+            gdev[g].disc_comp = k;
+            if (gdev[g].nface > 0) {
+                dgrids.push_back(g);
+                dmax_face = std::max(dmax_face, gdev[g].nface);
+            }
+        }
+    }
+    const bool any_disc = !dgrids.empty();
     std::vector<GridDev> cgrids;
     std::vector<double> cpx, cpy, cpz, cdx, cdy, cdz;
     std::vector<float> cflux;
@@ -1020,6 +1037,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
                 const CompHost &c = chost[comps[k].inst];
                 GridDev &G = cgrids[by * ncomp + k];
                 std::memset(&G, 0, sizeof(G));
+                G.disc_comp = -1;
                 G.inst = comps[k].inst;
                 G.nface = c.size();
                 G.face_off = by * NC + comps[k].off;
@@ -1125,7 +1143,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         return p.status == LC_OK && (!p.copy1 || !p.copy2);
     });
     std::vector<BatchDev> batches;
-    std::vector<std::vector<int>> batches_by_var(16);
+    std::vector<std::vector<int>> batches_by_var(16), ragged_by_var(8);
     long long part_total = 0;
     double alg_flops = 0., alg_elems = 0., alg_bytes = 0.;
     double var_flops[16] = {0.};
@@ -1148,9 +1166,17 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
                 B.ntile = ntile;
                 B.part_off = part_total;
                 part_total += (long long)B.nchunk * ntile * TILE;
+                int var = flux_variant(G);
+                if (G.disc_comp >= 0) {  // one partial row, k_flux_ragged
+                    part_total -= (long long)B.nchunk * ntile * TILE;
+                    B.nchunk = 1;
+                    part_total += (long long)ntile * TILE;
+                    ragged_by_var[var & 7].push_back((int)batches.size());
+                    batches.push_back(B);
+                    continue;
+                }
                 int bi = (int)batches.size();
                 batches.push_back(B);
-                int var = flux_variant(G);
                 batches_by_var[var].push_back(bi);
                 if (copy) {  // algorithmic work: every face x every sub-phase of the grid
                     alg_elems += (double)G.nface * S;
@@ -1176,6 +1202,12 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         var_range[v] = {v0, ntask - v0};
     }
     task_first.push_back(ntask);
+    std::vector<int> ragged_order;
+    std::vector<std::pair<int, int>> ragged_range(8, {0, 0});
+    for (int v = 0; v < 8; v++) {
+        ragged_range[v] = {(int)ragged_order.size(), (int)ragged_by_var[v].size()};
+        ragged_order.insert(ragged_order.end(), ragged_by_var[v].begin(), ragged_by_var[v].end());
+    }
 
     // diagnostics grids: star 1 fine, star 2 fine, and the wdwarf grid
     std::vector<int> diag_idx;
@@ -1285,6 +1317,12 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     sz.add<double>(scat_part);
     sz.add<int>(std::max<size_t>(diag_idx.size(), 1));
     sz.add<double>(std::max<size_t>(diag_idx.size(), 1) * 8);
+    sz.add<int>(std::max<size_t>(ragged_order.size(), 1));
+    sz.add<int>(std::max<size_t>(dgrids.size(), 1));
+    if (any_disc) {
+        sz.add<int>(FN);
+        sz.add<double2>(2 * FN);
+    }
     ctx->dev.reserve(sz.total + 4096);
     ctx->dev.reset();
     DevArena &A = ctx->dev;
@@ -1312,6 +1350,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     int *d_task_first = A.take<int>(task_first.size());
     int *d_task_order = A.take<int>(std::max<size_t>(task_order.size(), 1));
     int *d_diag_idx = A.take<int>(std::max<size_t>(diag_idx.size(), 1));
+    int *d_ragged = A.take<int>(std::max<size_t>(ragged_order.size(), 1));
+    int *d_dgrids = A.take<int>(std::max<size_t>(dgrids.size(), 1));
     FlatSeg *d_fsegs = A.take<FlatSeg>(fsegs.size());
     int *d_seg_cnt = A.take<int>(fsegs.size());
     CompDev *d_comps = A.take<CompDev>(std::max(ncomp, 1));
@@ -1322,12 +1362,17 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     cs.dx = A.take<double>(CN); cs.dy = A.take<double>(CN); cs.dz = A.take<double>(CN);
     cs.flux = A.take<float>(CN);
     FaceArrays fa;
+    std::memset(&fa, 0, sizeof(fa));
     fa.px = A.take<double>(FN); fa.py = A.take<double>(FN); fa.pz = A.take<double>(FN);
     fa.dx = A.take<double>(FN); fa.dy = A.take<double>(FN); fa.dz = A.take<double>(FN);
     fa.area = A.take<float>(FN); fa.grav = A.take<float>(FN); fa.flux = A.take<float>(FN);
     fa.ecl = A.take<int>(FN);
     fa.ein = A.take<double>(FN);
     fa.eout = A.take<double>(FN);
+    if (any_disc) {
+        fa.dn = A.take<int>(FN);
+        fa.dr = A.take<double2>(2 * FN);
+    }
     int2 *d_cand = A.take<int2>(FN);
     int *d_cand_count = A.take<int>(1);
     EclipseWork ew;
@@ -1435,6 +1480,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         {d_task_first, task_first.data(), task_first.size() * sizeof(int)},
         {d_task_order, task_order.data(), task_order.size() * sizeof(int)},
         {d_diag_idx, diag_idx.data(), diag_idx.size() * sizeof(int)},
+        {d_ragged, ragged_order.data(), ragged_order.size() * sizeof(int)},
+        {d_dgrids, dgrids.data(), dgrids.size() * sizeof(int)},
         {d_fsegs, fsegs.data(), fsegs.size() * sizeof(FlatSeg)},
         {d_copy_hit, copy_hit.data(), copy_hit.size() * sizeof(FaceCopy)},
         {d_copy_dup, copy_dup.data(), copy_dup.size() * sizeof(FaceCopy)},
@@ -1530,6 +1577,8 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         if (!copy_hit.empty()) launch_copy_faces(s, d_copy_hit, (int)copy_hit.size(), ctx->geo.pool, fa);
         if (!copy_dup.empty()) launch_copy_faces(s, d_copy_dup, (int)copy_dup.size(), fa, fa);
         if (!copy_out.empty()) launch_copy_faces(s, d_copy_out, (int)copy_out.size(), fa, ctx->geo.pool);
+        if (any_disc)  // the opaque disc's ranges of the star faces, copied grids included
+            launch_disc_star_ranges(s, d_grids, d_dgrids, (int)dgrids.size(), dmax_face, d_comps, fa, d_status);
         mark(ev[2]);
         launch_continuum(s, d_grids, ngrid, nface_total, fa);
         // fluxes of grids with a known continuum: cached ones first (the
@@ -1556,6 +1605,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
                         fl, pa, S, d_lists, d_counts, d_partial);
             mark(ctx->ev_var[k + 1]);
         }
+        for (int v = 0; v < 8; v++)
+            launch_flux_ragged(s, v, d_ragged + ragged_range[v].first, ragged_range[v].second, S, d_batches, d_grids,
+                               fa, pa, S, d_lists, d_counts, d_partial);
         mark(ev[5]);
         launch_flux_reduce(s, d_batches, (int)batches.size(), S, TILE, d_lists, d_counts, d_partial, d_out);
         launch_scatter(s, d_insts, (int)M, dd, pa, d_out, ca.sums, d_calc, d_scal, d_scat_part);
@@ -1573,13 +1625,15 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
                                    ncomp, NC,
                                    any_route, any_ecl_stop, ecl_rounds, bits_lb, psum_chunks,
                                    (long long)fsegs.size(), (long long)task_order.size(), ntask, R, fp64,
-                                   (long long)batches.size(), TILE, (long long)diag_idx.size()};
+                                   (long long)batches.size(), TILE, (long long)diag_idx.size(),
+                                   (long long)ragged_order.size(), (long long)dgrids.size(), dmax_face};
         put(sizes, sizeof(sizes));
         for (int v = 0; v < 16; v++) put(&var_range[v], sizeof(var_range[v]));
+        for (int v = 0; v < 8; v++) put(&ragged_range[v], sizeof(ragged_range[v]));
         const void *ptrs[] = {d_grids, d_rings, d_insts, d_status, d_cand, d_cand_count, d_copy_hit, d_copy_dup,
                               d_copy_out, d_flux_hit, d_flux_dup, d_flux_out, d_comps, d_cgrids, d_ccand, d_ccount, d_gsums, d_fsegs, d_seg_cnt, d_lists, d_counts, d_task_first, d_task_order,
                               d_batches, d_tasks, d_partial, d_out, d_calc, d_scal, d_scat_part, d_diag_idx, d_diag,
-                              d_bits};
+                              d_bits, d_ragged, d_dgrids};
         put(ptrs, sizeof(ptrs));
         put(&ctx->geo.pool, sizeof(ctx->geo.pool));
         put(&dd, sizeof(dd));
@@ -2093,6 +2147,7 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
     const bool fp64 = precision == LC_PREC_FP64;
     GridDev G;
     std::memset(&G, 0, sizeof(G));
+    G.disc_comp = -1;
     G.star = star;
     G.nface = (int)nf;
     G.beam = beam;
@@ -2303,6 +2358,7 @@ int flat_sum_noalloc_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const l
     }
     GridDev G;
     std::memset(&G, 0, sizeof(G));
+    G.disc_comp = -1;
     G.star = star;
     G.nface = (int)nf;
     G.beam = beam;
@@ -2385,6 +2441,7 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     DevArena &A = ctx->dev;
     hipStream_t s = ctx->stream;
     FaceArrays fa;
+    std::memset(&fa, 0, sizeof(fa));
     fa.px = A.take<double>(FN); fa.py = A.take<double>(FN); fa.pz = A.take<double>(FN);
     fa.dx = A.take<double>(FN); fa.dy = A.take<double>(FN); fa.dz = A.take<double>(FN);
     fa.area = A.take<float>(FN); fa.grav = A.take<float>(FN); fa.flux = A.take<float>(FN);
@@ -2450,6 +2507,7 @@ int continuum_stage(lc_ctx *ctx, const lc_model *m, lc_star_faces *s1, lc_star_f
     GridDev G[2];
     for (int k = 0; k < 2; k++) {
         std::memset(&G[k], 0, sizeof(GridDev));
+        G[k].disc_comp = -1;
         G[k].star = k;
         fill_physics(*m, G[k]);
     }

@@ -685,6 +685,53 @@ __global__ __launch_bounds__(NT) void k_flux_fp64(const TaskDev *__restrict__ ta
     }
 }
 
+// Flux sums of a grid with an opaque disc's ranges: one thread per routed
+// sub-phase, the grid's faces in the reference's FlatGrid order (faces with
+// no eclipse range, with one, with more; each class in index order,
+// comp_light_flat.cpp:26-65), FP64 elements summed in 1024-face blocks that
+// are merged in order (comp_light_flat.cpp:242-307): the reference's sums
+// bit for bit.  A face is hidden when any of its ranges (the star's, then
+// the disc's) holds the phase, as its multi-range test does.  Every thread
+// of a workgroup reads the same face at the same time.  One partial row.
+template <int LT, bool BEAM, bool GLENS>
+__global__ __launch_bounds__(256) void k_flux_ragged(const int *__restrict__ order, const BatchDev *__restrict__ batches,
+                                                    const GridDev *__restrict__ grids, FaceArrays fa, PhaseArrays pa,
+                                                    int S, const int *__restrict__ lists,
+                                                    const int *__restrict__ counts, double *__restrict__ partial) {
+    const BatchDev B = batches[order[blockIdx.y]];
+    const GridDev &G = grids[B.grid];
+    const int count = B.count_idx < 0 ? S : counts[B.count_idx];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    const int j = B.list_off < 0 ? e : lists[B.list_off + e];
+    const size_t idx = (size_t)B.inst * S + j;
+    const double ex = pa.ex[idx], ey = pa.ey[idx], ez = pa.ez[idx], ph = pa.phin[idx];
+    const double base = 1.0 - (G.ldc[0] + G.ldc[1] + G.ldc[2] + G.ldc[3]);
+    double total = 0., blk = 0.;
+    int pos = 0;
+    for (int cls = 0; cls < 3; cls++) {
+        for (int k = 0; k < G.nface; k++) {
+            const int f = G.face_off + k;
+            const int se = fa.ecl[f] ? 1 : 0, nd = fa.dn[f];
+            if (min(se + nd, 2) != cls) continue;
+            bool hid = se && ecl_range(ph, fa.ein[f], fa.eout[f]);
+            for (int r = 0; r < nd && !hid; r++) {
+                const double2 d = fa.dr[2 * f + r];
+                hid = ecl_range(ph, d.x, d.y);
+            }
+            if (!hid)
+                blk += elem64<LT, BEAM, GLENS>(ex, ey, ez, fa.dx[f], fa.dy[f], fa.dz[f], fa.px[f], fa.py[f], fa.pz[f],
+                                               (double)fa.flux[f], G, base);
+            if (++pos == 1024) {
+                total += blk;
+                blk = 0.;
+                pos = 0;
+            }
+        }
+    }
+    partial[B.part_off + e] = total + blk;
+}
+
 // Merge chunk partials in chunk order; out[(inst*2 + star)*S + j].
 __global__ void k_flux_reduce(const BatchDev *__restrict__ batches, int S, int tile_len, const int *__restrict__ lists,
                               const int *__restrict__ counts, const double *__restrict__ partial,
@@ -774,6 +821,29 @@ void launch_make_tasks(hipStream_t s, const int *first, const int *order, int nb
     if (ntask <= 0 || nb <= 0) return;
     hipLaunchKernelGGL(k_make_tasks, dim3((ntask + 255) / 256), dim3(256), 0, s, first, order, nb, batches, ntask,
                        tasks);
+}
+
+void launch_flux_ragged(hipStream_t s, int variant, const int *order, int nb, int max_count, const BatchDev *batches,
+                        const GridDev *grids, FaceArrays fa, PhaseArrays pa, int S, const int *lists,
+                        const int *counts, double *partial) {
+    if (nb <= 0 || max_count <= 0) return;
+    const dim3 grid((max_count + 255) / 256, nb);
+#define LC_RAGGED_CASE(V, LT, BM, GL)                                                                            \
+    case V:                                                                                                    \
+        hipLaunchKernelGGL((k_flux_ragged<LT, BM, GL>), grid, dim3(256), 0, s, order, batches, grids, fa, pa, S, \
+                           lists, counts, partial);                                                            \
+        break;
+    switch (variant & 7) {
+        LC_RAGGED_CASE(0, 0, false, false)
+        LC_RAGGED_CASE(1, 1, false, false)
+        LC_RAGGED_CASE(2, 0, true, false)
+        LC_RAGGED_CASE(3, 1, true, false)
+        LC_RAGGED_CASE(4, 0, false, true)
+        LC_RAGGED_CASE(5, 1, false, true)
+        LC_RAGGED_CASE(6, 0, true, true)
+        LC_RAGGED_CASE(7, 1, true, true)
+    }
+#undef LC_RAGGED_CASE
 }
 
 void launch_flux_reduce(hipStream_t s, const BatchDev *batches, int nbatch, int S, int tile_len, const int *lists,

@@ -619,10 +619,24 @@ __device__ __forceinline__ double ldc_imu(double mu, const GridDev &G) {
     return im;
 }
 
+// hidden at the normalised phase ph by one of the opaque disc's ranges of
+// the face (Point::visible over the ranges appended after the star's own)
+__device__ __forceinline__ bool disc_hidden(const GridDev &G, const FaceArrays &fa, int f, double ph) {
+    if (G.disc_comp < 0) return false;
+    for (int r = 0; r < fa.dn[f]; r++) {
+        const double2 d = fa.dr[2 * f + r];
+        if ((ph >= d.x && ph <= d.y) || ph <= d.y - 1.0) return true;
+    }
+    return false;
+}
+
 // Point-path element at one phase with spin = 1 for beaming
 // (comp_light.cpp:21-92 as called by comp_star1/comp_star2)
 __device__ double point_elem(const GridDev &G, FaceArrays fa, int f, const V3 &earth, double phi) {
     double ph = phi - floor(phi);
+    // the star's own range only: the reference sets the grid-switching
+    // scales before it appends an opaque disc's ranges
+    // (light_curve_comp.cpp:205-250 before 286-326)
     if (fa.ecl[f] && ((ph >= fa.ein[f] && ph <= fa.eout[f]) || ph <= fa.eout[f] - 1.0)) return 0.0;
     V3 dirn = v3(fa.dx[f], fa.dy[f], fa.dz[f]);
     double mu = dot(earth, dirn);
@@ -757,7 +771,8 @@ __global__ __launch_bounds__(1024) void k_diag(const GridDev *__restrict__ grids
         if (G.star == PRIMARY) {
             // flat element (comp_light_flat.cpp:110-127), eclipse by range
             double mu = earth.x * fa.dx[f] + earth.y * fa.dy[f] + earth.z * fa.dz[f];
-            bool hidden = fa.ecl[f] && ((0.5 >= fa.ein[f] && 0.5 <= fa.eout[f]) || 0.5 <= fa.eout[f] - 1.0);
+            bool hidden = (fa.ecl[f] && ((0.5 >= fa.ein[f] && 0.5 <= fa.eout[f]) || 0.5 <= fa.eout[f] - 1.0)) ||
+                          disc_hidden(G, fa, f, 0.5);
             if (mu > G.mucrit && !hidden) {
                 double m = dmin(mu, 1.0), im;
                 if (G.ltype == 0) {

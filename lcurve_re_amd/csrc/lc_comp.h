@@ -62,5 +62,9 @@ void launch_comp_build(hipStream_t s, const CompDev *comps, int ncomp, int nc, F
 // Sums of the disc, edge and spot elements at every sub-phase of the
 // instances that have components (comp_light.cpp:322-407, FP64).
 void launch_comp_sums(hipStream_t s, const CompDev *comps, int ncomp, int S, PhaseArrays pa, CompArrays ca);
+// An opaque disc's eclipse ranges of the star faces (light_curve_comp.cpp:
+// 286-326): for each grid gidx[k] (with disc_comp >= 0), dn/dr of its faces.
+void launch_disc_star_ranges(hipStream_t s, const GridDev *grids, const int *gidx, int ng, int max_face,
+                             const CompDev *comps, FaceArrays fa, int *inst_status);
 
 }  // namespace lck

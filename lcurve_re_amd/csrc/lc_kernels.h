@@ -38,6 +38,10 @@ struct GridDev {
     // (same faces, same continuum inputs) and are copied after k_continuum,
     // which skips this grid
     int cont_copy;
+    // the instance's opaque disc hides parts of this star (index of its
+    // CompDev, -1 without): the disc's ranges are in FaceArrays dn/dr and
+    // the grid's flux sums run in k_flux_ragged
+    int disc_comp;
 };
 
 // One latitude ring (set_star_grid.cpp:262-320).
@@ -78,6 +82,11 @@ struct FaceArrays {
     float *area, *grav, *flux;
     int *ecl;
     double *ein, *eout;
+    // an opaque disc's eclipse ranges of the face (disc_eclipse: 0 to 2,
+    // appended after the star's own range, light_curve_comp.cpp:286-326);
+    // null when no grid of the call has one
+    int *dn;
+    double2 *dr;  // [2 * face]
 };
 
 // Flux-stage copy of every grid, in the reference's FlatGrid order
@@ -218,6 +227,12 @@ void launch_make_tasks(hipStream_t s, const int *first, const int *order, int nb
                        int ntask, TaskDev *tasks);
 void launch_flux_reduce(hipStream_t s, const BatchDev *batches, int nbatch, int S, int threads_per_tile,
                         const int *lists, const int *counts, const double *partial, double *out);
+// Flux sums of batches whose grid has an opaque disc's ranges (several
+// eclipse ranges per face): batches order[0 .. nb), FP64 in the reference's
+// order (faces with 0, 1, more ranges; 1024-face blocks), one partial row.
+void launch_flux_ragged(hipStream_t s, int variant, const int *order, int nb, int max_count, const BatchDev *batches,
+                        const GridDev *grids, FaceArrays fa, PhaseArrays pa, int S, const int *lists,
+                        const int *counts, double *partial);
 // part: workspace of ninst * ceil(n / SCAT_PTS) * SCAT_PARTS doubles
 constexpr int SCAT_PTS = 1024, SCAT_PARTS = 8;
 // comp: the disc, edge and spot sums of instances with has_comp (else unused)

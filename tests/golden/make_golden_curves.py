@@ -12,7 +12,8 @@ configuration, see oracle/ref_harness.cpp):
   (wdwarf, logg1/2, rv1/2), scaled, the fast path, and iscale;
 * comp_gravity on the case grids with the continuum fluxes;
 * the disc, disc edge and bright spot of cataclysmic-variable-like binaries
-  (light_curve_comp.cpp:272-333, 483-497), unscaled, scaled and fast;
+  (light_curve_comp.cpp:272-333, 483-497), with transparent and opaque
+  discs, unscaled, scaled and fast;
 * the detached workload exactly (nlat 40, 1000 phases k/1000), the fit
   workload's base curve (8000 phases) and a reduced semi-detached workload
   (coarse nlat 30, fine nlat 60 with the infill strip, 2000 x 5 sub-phases).
@@ -69,6 +70,11 @@ COMPONENT_CASES = {
     "cv_thick_no_eclipse1": (dict(CV_PHYS, height_disc=0.05, beta_disc=1.1, absorb_edge=0.0, q=0.3,
                                   r2=0.95 * (1 - W.xl12(0.3))),
                              dict(CV_COMP, add_disc=1, add_spot=1, eclipse1=0, nrad=20, nspot=40)),
+    # opaque discs: the disc hides parts of both stars (light_curve_comp.cpp:286-326)
+    "cv_disc_opaque": (dict(CV_PHYS, iangle=86.0, rdisc1=0.03, texp_disc=-1.2, height_disc=0.03, beta_disc=1.2),
+                       dict(CV_COMP, add_disc=1, nrad=25, opaque=1)),
+    "cv_disc_spot_opaque": (dict(CV_PHYS, iangle=88.0, height_disc=0.04, beta_disc=1.1, r1=0.02),
+                            dict(CV_COMP, add_disc=1, add_spot=1, opaque=1, nlat1c=12)),
 }
 
 

@@ -3,7 +3,9 @@ on the GPU against golden curves of the reference compiled from its sources
 (tests/golden/curves.npz, tests/golden/make_golden_curves.py):
 cataclysmic-variable-like binaries with a flared disc, its outer edge
 irradiated by the donor, and the bright spot, including a spherical white
-dwarf, a thick disc and a case without eclipses by star 1, each unscaled
+dwarf, a thick disc, a case without eclipses by star 1 and two opaque discs
+that hide parts of both stars (their ranges join the stars' own, and those
+grids' flux sums take several ranges per face), each unscaled
 with diagnostics, scaled and through light_curve_comp_fast, in both
 precisions.  Errors are max |gpu - ref| / max |ref| and the pointwise
 max |gpu - ref| / |ref|, both within 1e-5.  The bright spot's position
@@ -19,7 +21,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "curves.npz")
-CASES = ["cv_disc_spot", "cv_disc", "cv_spot_sphere", "cv_thick_no_eclipse1"]
+CASES = ["cv_disc_spot", "cv_disc", "cv_spot_sphere", "cv_thick_no_eclipse1", "cv_disc_opaque", "cv_disc_spot_opaque"]
 
 
 def errors(calc, ref):
@@ -70,9 +72,9 @@ def test_components_in_a_batch_match_single_calls(engine, gold):
     import lcurve_re_amd as L
     ms = [L.Model.from_json(json.loads(bytes(gold[f"{c}_model_json"]).decode())) for c in CASES]
     plain = L.Model.from_json(json.loads(bytes(gold["detached_base_model_json"]).decode()))
-    models = [ms[0], plain, ms[1], ms[2], plain, ms[3]]
+    models = [ms[0], plain, ms[1], ms[2], plain, ms[3], ms[4], ms[5], ms[4]]
     data = L.make_data(np.arange(700) / 700.0, ferr=1e-3, weight=1.0, flux=1.0)
-    print("components: mixed batch of 6", flush=True)
+    print("components: mixed batch of 9", flush=True)
     calc, _, st = engine.light_curve_comp_batch(models, data, scale=False, rdata=True, precision=L.LC_PREC_FP64)
     assert not st.any()
     for i, m in enumerate(models):
@@ -85,7 +87,7 @@ def test_unsupported_component_options_raise(engine, gold):
     import lcurve_re_amd as L
     cfg = json.loads(bytes(gold["cv_disc_spot_model_json"]).decode())
     data = L.make_data(np.arange(50) / 50.0)
-    for key in ("opaque", "iscale"):
+    for key in ("iscale",):
         c = json.loads(json.dumps(cfg))
         c["model_parameters"][key] = "1"
         m = L.Model.from_json(c)
