@@ -30,7 +30,7 @@ typedef enum lc_status {
                                 (pot_min.cpp:98,271,291,401; dbrent.cpp:649)         */
     LC_E_PARAM = 4,          /* malformed or illegal parameter (model.cpp:10-360)     */
     LC_E_DEVICE = 5,         /* HIP runtime error                                     */
-    LC_E_UNSUPPORTED = 6,    /* opaque disc, iscale with disc/spot: outside this engine */
+    LC_E_UNSUPPORTED = 6,    /* an option outside this engine (none at present)        */
     LC_E_ARG = 7,            /* invalid argument (null pointer, bad size)             */
     LC_E_LENSING = 8         /* lensing configuration rejected (set_star_grid.cpp:80-100) */
 } lc_status;

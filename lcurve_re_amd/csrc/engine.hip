@@ -768,8 +768,6 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         InstDev &I = insts[i];
         std::memset(&I, 0, sizeof(I));
         try {
-            if ((m.add_disc || m.add_spot) && m.iscale)
-                throw EngineError(LC_E_UNSUPPORTED, "iscale with disc or bright spot components is not supported");
             double r1, r2;
             get_r1r2(m, r1, r2);
             double rl2 = 1. - l1(2, m.q.value, m.spin2.value);
@@ -810,6 +808,9 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             I.star2 = m.t2.value > 0. ? 1 : 0;
             I.sfac4 = rq.io[i].sfac[4];
             I.zcols = rq.io[i].sfac[1] * 0. + rq.io[i].sfac[2] * 0. + rq.io[i].sfac[3] * 0.;
+            I.sfac1 = rq.io[i].sfac[1];
+            I.sfac2 = rq.io[i].sfac[2];
+            I.sfac3 = rq.io[i].sfac[3];
             I.copy1 = P.copy1;
             I.copy2 = P.copy2;
             I.scale = o.scale;
@@ -1766,9 +1767,11 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         lc_scalars &io = rq.io[i];
         const double *sc = h_scal + i * 8;
         if (o.scale) {
-            if (rq.models[i].iscale) {  // star 1, disc, edge, spot (zero columns), star 2
+            if (rq.models[i].iscale) {  // star 1, disc, edge, spot (zero columns without them), star 2
                 io.sfac[0] = sc[0];
-                io.sfac[1] = io.sfac[2] = io.sfac[3] = 0.;
+                io.sfac[1] = sc[5];
+                io.sfac[2] = sc[6];
+                io.sfac[3] = sc[7];
                 io.sfac[4] = sc[4];
             } else {
                 for (int k = 0; k < 5; k++) io.sfac[k] = sc[0];

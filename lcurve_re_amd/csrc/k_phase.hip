@@ -191,6 +191,24 @@ __device__ __forceinline__ double comp_point(const InstDev &I, const DataDev &d,
     return slfac * (c[0] + c[1] + c[2]);
 }
 
+// The three component columns of the iscale fit at one data point: each
+// one's sub-exposure sum as in comp_point, times the slope factor
+// (light_curve_comp.cpp:390-403)
+__device__ __forceinline__ void comp_cols(const InstDev &I, const DataDev &d, const double *comp, int np, double *c) {
+    const double frac = (d.time[np] - d.middle) / d.range;
+    const double slfac = 1. + frac * (I.slope + frac * (I.quad + frac * I.cube));
+    const int j0 = d.sub_off[np], ndiv = d.ndiv[np];
+    for (int kind = 0; kind < 3; kind++) {
+        const double *sk = comp + I.comp_off + (size_t)kind * d.S + j0;
+        double sum = 0.0;
+        for (int nd = 0; nd < ndiv; nd++) {
+            const double wgt = (ndiv == 1) ? 1.0 : ((nd == 0 || nd == ndiv - 1) ? 0.5 : 1.0);
+            sum += wgt * sk[nd];
+        }
+        c[kind] = slfac * (sum / max(1, ndiv - 1));
+    }
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void point_sums(const InstDev &I, const DataDev &d, const PhaseArrays &pa, size_t base,
                                            const double *o1, const double *o2, const double *comp, int np,
@@ -239,6 +257,10 @@ __global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__
     const double *o2 = out + ((size_t)inst * 2 + 1) * d.S;
     double *c = calc + (size_t)inst * d.n;
     double sdy = 0., syy = 0., wn = 0., s12 = 0., s22 = 0., b2 = 0.;
+    const bool five = I.iscale && I.has_comp;  // star 1, disc, edge, spot, star 2
+    double g5[20];
+#pragma unroll
+    for (int q = 0; q < 20; q++) g5[q] = 0.;
     const int p1 = min(d.n, (int)(blockIdx.x + 1) * SCAT_PTS);
     for (int np = blockIdx.x * SCAT_PTS + threadIdx.x; np < p1; np += 256) {
         if (I.iscale) {
@@ -257,6 +279,18 @@ __global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__
                     s12 += u1 * u2;
                     s22 += u2 * u2;
                     b2 += u2 * yb;
+                    if (five) {
+                        double cc[3];
+                        comp_cols(I, d, comp, np, cc);
+                        const double u[5] = {u1, is * cc[0], is * cc[1], is * cc[2], u2};
+                        int q = 0;
+#pragma unroll
+                        for (int a = 0; a < 5; a++)
+#pragma unroll
+                            for (int b = a; b < 5; b++) g5[q++] += u[a] * u[b];
+#pragma unroll
+                        for (int a = 0; a < 5; a++) g5[15 + a] += u[a] * yb;
+                    }
                 }
             }
             continue;
@@ -285,6 +319,11 @@ __global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__
             pp[6] = s22;
             pp[7] = b2;
         }
+        if (five)
+            for (int q = 0; q < 20; q++) {
+                const double t = block_sum<256>(g5[q], lds);
+                if (threadIdx.x == 0) pp[8 + q] = t;
+            }
     }
     if (threadIdx.x == 0) {
         pp[0] = sdy;
@@ -323,12 +362,73 @@ __device__ __forceinline__ void iscale_factors(double s11, double s12, double s2
     a4 = v1 * f;
 }
 
-// The instance's factors (star 1 / all, star 2) from the block partials,
-// summed in block order.
+// Least squares for the five iscale columns (star 1, disc, edge, spot,
+// star 2) through their Gram matrix G (packed upper triangle) and g: cyclic
+// Jacobi for its eigenvalues (the squared singular values of the weighted
+// design matrix) and eigenvectors; directions with an eigenvalue below
+// 1e-10 of the largest (singular value below 1e-5, svdfit.cpp:58-64) are
+// dropped, the factors are the sum of v (v . g) / lambda over the rest.
+__device__ void iscale_factors5(const double *G, const double *g, double *a) {
+    double A[5][5], V[5][5];
+    int q = 0;
+    for (int i = 0; i < 5; i++)
+        for (int j = i; j < 5; j++) A[i][j] = A[j][i] = G[q++];
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) V[i][j] = i == j ? 1. : 0.;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0., dia = 0.;
+        for (int i = 0; i < 5; i++) {
+            dia += A[i][i] * A[i][i];
+            for (int j = i + 1; j < 5; j++) off += A[i][j] * A[i][j];
+        }
+        if (!(off > 1e-32 * dia)) break;
+        for (int p = 0; p < 4; p++)
+            for (int r = p + 1; r < 5; r++) {
+                if (A[p][r] == 0.) continue;
+                const double th = (A[r][r] - A[p][p]) / (2. * A[p][r]);
+                const double t = (th >= 0. ? 1. : -1.) / (fabs(th) + sqrt(th * th + 1.));
+                const double c = 1. / sqrt(t * t + 1.), s = t * c;
+                for (int k = 0; k < 5; k++) {  // A <- J^T A J
+                    const double akp = A[k][p], akr = A[k][r];
+                    A[k][p] = c * akp - s * akr;
+                    A[k][r] = s * akp + c * akr;
+                }
+                for (int k = 0; k < 5; k++) {
+                    const double apk = A[p][k], ark = A[r][k];
+                    A[p][k] = c * apk - s * ark;
+                    A[r][k] = s * apk + c * ark;
+                }
+                for (int k = 0; k < 5; k++) {
+                    const double vkp = V[k][p], vkr = V[k][r];
+                    V[k][p] = c * vkp - s * vkr;
+                    V[k][r] = s * vkp + c * vkr;
+                }
+            }
+    }
+    double lmax = 0.;
+    for (int i = 0; i < 5; i++) lmax = fmax(lmax, A[i][i]);
+    for (int k = 0; k < 5; k++) a[k] = 0.;
+    if (!(lmax > 0.)) return;
+    for (int i = 0; i < 5; i++) {
+        const double l = A[i][i];
+        if (!(l > 0.) || l < 1e-10 * lmax) continue;
+        double vg = 0.;
+        for (int k = 0; k < 5; k++) vg += V[k][i] * g[k];
+        const double f = vg / l;
+        for (int k = 0; k < 5; k++) a[k] += V[k][i] * f;
+    }
+}
+
+// The instance's factors (star 1 / all, star 2; with iscale and
+// components also the disc, edge and spot factors a13) from the block
+// partials, summed in block order.
 __device__ __forceinline__ void inst_scale(const InstDev &I, const double *pi, int nblk, double &a0, double &a4,
-                                           double &wn, bool &want_chi) {
+                                           double *a13, double &wn, bool &want_chi) {
     a0 = I.sfac0;
     a4 = I.sfac4;
+    a13[0] = I.sfac1;
+    a13[1] = I.sfac2;
+    a13[2] = I.sfac3;
     want_chi = I.rdata != 0;
     wn = 0.;
     if (!I.scale) return;
@@ -344,8 +444,23 @@ __device__ __forceinline__ void inst_scale(const InstDev &I, const double *pi, i
             b2 += q[7];
         }
     }
+    if (I.iscale && I.has_comp) {
+        double G[20], a[5];
+        for (int q = 0; q < 20; q++) G[q] = 0.;
+        for (int b = 0; b < nblk; b++)
+            for (int q = 0; q < 20; q++) G[q] += pi[b * SCAT_PARTS + 8 + q];
+        iscale_factors5(G, G + 15, a);
+        a0 = a[0];
+        a13[0] = a[1];
+        a13[1] = a[2];
+        a13[2] = a[3];
+        a4 = a[4];
+        want_chi = true;
+        return;
+    }
     if (I.iscale) {
         iscale_factors(syy, s12, s22, sdy, b2, a0, a4);
+        a13[0] = a13[1] = a13[2] = 0.;
         want_chi = true;  // light_curve_comp.cpp:539-547
         return;
     }
@@ -361,9 +476,9 @@ __global__ __launch_bounds__(256) void k_scatter_scale(const InstDev *__restrict
     const InstDev &I = insts[inst];
     if (!I.active) return;
     double *pi = part + (size_t)inst * nblk * SCAT_PARTS;
-    double a0, a4, wn;
+    double a0, a4, wn, a13[3];
     bool want_chi;
-    inst_scale(I, pi, nblk, a0, a4, wn, want_chi);
+    inst_scale(I, pi, nblk, a0, a4, a13, wn, want_chi);
     const size_t base = (size_t)inst * d.S;
     const double *o1 = out + ((size_t)inst * 2 + 0) * d.S;
     const double *o2 = out + ((size_t)inst * 2 + 1) * d.S;
@@ -375,7 +490,15 @@ __global__ __launch_bounds__(256) void k_scatter_scale(const InstDev *__restrict
     const int p1 = min(d.n, (int)(blockIdx.x + 1) * SCAT_PTS);
     for (int np = blockIdx.x * SCAT_PTS + threadIdx.x; np < p1; np += 256) {
         double v;
-        if (I.iscale) {
+        if (I.iscale && I.has_comp) {
+            // sfac[0] f0 + sfac[1] f1 + sfac[2] f2 + sfac[3] f3 (+ sfac[4] f4),
+            // light_curve_comp.cpp:555-558
+            double v1, v2, cc[3];
+            point_sums<true>(I, d, pa, base, o1, o2, comp, np, v1, v2);
+            comp_cols(I, d, comp, np, cc);
+            v = a0 * v1 + a13[0] * cc[0] + a13[1] * cc[1] + a13[2] * cc[2];
+            if (I.star2) v += a4 * v2;
+        } else if (I.iscale) {
             double v1, v2;
             point_sums<true>(I, d, pa, base, o1, o2, comp, np, v1, v2);
             v = a0 * v1 + zc;
@@ -403,9 +526,9 @@ __global__ void k_scatter_final(const InstDev *__restrict__ insts, int nblk, con
     const InstDev &I = insts[inst];
     if (!I.active || threadIdx.x != 0) return;
     const double *pi = part + (size_t)inst * nblk * SCAT_PARTS;
-    double a0, a4, wn;
+    double a0, a4, wn, a13[3];
     bool want_chi;
-    inst_scale(I, pi, nblk, a0, a4, wn, want_chi);
+    inst_scale(I, pi, nblk, a0, a4, a13, wn, want_chi);
     double chi = 0., wn2 = 0.;
     for (int b = 0; b < nblk; b++) {
         chi += pi[b * SCAT_PARTS + 3];
@@ -414,6 +537,9 @@ __global__ void k_scatter_final(const InstDev *__restrict__ insts, int nblk, con
     double *s = scal + (size_t)inst * 8;
     s[0] = a0;
     s[4] = a4;
+    s[5] = a13[0];
+    s[6] = a13[1];
+    s[7] = a13[2];
     if (I.scale && !I.iscale) {
         s[1] = chi;
         s[2] = wn;

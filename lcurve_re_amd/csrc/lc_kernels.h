@@ -55,6 +55,7 @@ struct InstDev {
     double t0, period, pdot, deltat, slope, quad, cube, third, cosi, sini;
     double phase1, phase2, scale11, scale12, scale21, scale22;
     double sfac0, sfac4;  // the caller's factors (scale off): star 1 / all, star 2 (iscale)
+    double sfac1, sfac2, sfac3;  // iscale, scale off: the caller's disc, edge and spot factors
     double zcols;         // iscale, scale off: sfac[1..3] times the disc, edge and spot columns (0)
     int copy1, copy2, grid1f, grid2f, grid1c, grid2c;
     int scale, rdata, active, iscale;
@@ -234,7 +235,9 @@ void launch_flux_ragged(hipStream_t s, int variant, const int *order, int nb, in
                         const GridDev *grids, FaceArrays fa, PhaseArrays pa, int S, const int *lists,
                         const int *counts, double *partial);
 // part: workspace of ninst * ceil(n / SCAT_PTS) * SCAT_PARTS doubles
-constexpr int SCAT_PTS = 1024, SCAT_PARTS = 8;
+// slots 0-7: scale sums, chi**2, the two-star iscale sums; 8-27: the
+// five-column iscale Gram matrix (15) and right-hand side (5)
+constexpr int SCAT_PTS = 1024, SCAT_PARTS = 28;
 // comp: the disc, edge and spot sums of instances with has_comp (else unused)
 void launch_scatter(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa, const double *out,
                     const double *comp, double *calc, double *scal, double *part);

@@ -289,14 +289,15 @@ def light_curve(model, data, scale=False, rdata=True, diagnostics=False, max_mod
 
 def ref_light_curve(model, data, scale=False, rdata=True, diagnostics=False, max_model_points=0, sfac=1.0):
     """The compiled reference's light_curve_comp(_fast) (oracle/_ref);
-    returns (status, calc, scalars) like light_curve()."""
+    returns (status, calc, scalars) like light_curve().  sfac: one factor
+    for all five, or five (iscale)."""
     abi = _abi()
     lib = ref()
     n = len(data)
     calc = np.zeros(n, np.float64)
     io = abi.lc_scalars()
     for k in range(5):
-        io.sfac[k] = sfac
+        io.sfac[k] = sfac[k] if hasattr(sfac, "__len__") else sfac
     o = abi.lc_eval_opts(int(scale), int(rdata), 0, int(diagnostics), int(max_model_points), 1)
     st = lib.ref_light_curve_comp(C.byref(model), data, n, C.byref(o), dptr(calc), C.byref(io))
     return st, calc, io

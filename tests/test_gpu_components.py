@@ -83,14 +83,35 @@ def test_components_in_a_batch_match_single_calls(engine, gold):
         assert np.array_equal(c1, calc[i]), i
 
 
-def test_unsupported_component_options_raise(engine, gold):
+@pytest.mark.parametrize("case", CASES)
+def test_components_scaled_on_their_own(engine, gold, case):
+    """iscale with components (light_curve_comp.cpp:378-416, 511-573): five
+    columns (star 1, disc, edge, spot, star 2) fitted by least squares with
+    svdfit's singular-value cut, and the caller's factors when scale is
+    off.  The curve and chi**2 are compared; factors of nearly collinear
+    columns are ill-determined (the reference's own SVD lands ~1e-9 off the
+    optimum), so they are printed, and checked where a column is alone."""
     import lcurve_re_amd as L
-    cfg = json.loads(bytes(gold["cv_disc_spot_model_json"]).decode())
-    data = L.make_data(np.arange(50) / 50.0)
-    for key in ("iscale",):
-        c = json.loads(json.dumps(cfg))
-        c["model_parameters"][key] = "1"
-        m = L.Model.from_json(c)
-        with pytest.raises(L.LcurveError) as e:
-            engine.light_curve_comp(m, data)
-        assert e.value.status == 6   # LC_E_UNSUPPORTED
+    cfg = json.loads(bytes(gold[f"{case}_model_json"]).decode())
+    cfg["model_parameters"]["iscale"] = "1"
+    m = L.Model.from_json(cfg)
+    data = L.make_data(gold["pts_time"], expose=gold["pts_expose"], ndiv=gold["pts_ndiv"], ferr=gold["pts_ferr"],
+                       weight=gold["pts_weight"], flux=gold[f"{case}_dflux"])
+    for prec in (L.LC_PREC_FP64, L.LC_PREC_MIXED):
+        for mode, kw in (("iscale_scaled", dict(scale=True)), ("iscale_given", dict(scale=False,
+                                                                                     sfac=[1.7, 0.8, 1.3, 0.6, 1.1]))):
+            print(f"components iscale {case} {mode} precision {prec}", flush=True)
+            calc, io = engine.light_curve_comp(m, data, rdata=True, diagnostics=True, precision=prec, **kw)
+            ref = gold[f"{case}_{mode}_calc"]
+            rio = gold[f"{case}_{mode}_io"]
+            e_max, e_pt = errors(calc, ref)
+            print(f"   max-normalised {e_max:.3e} pointwise {e_pt:.3e}; sfac {list(io.sfac)} vs {list(rio[:5])}",
+                  flush=True)
+            assert e_max <= 1e-5 and e_pt <= 1e-5, (case, mode, prec, e_max, e_pt)
+            assert abs(io.chisq / rio[6] - 1) <= 1e-4
+            if mode == "iscale_given":
+                assert list(io.sfac) == [1.7, 0.8, 1.3, 0.6, 1.1]
+            else:
+                for k in range(5):   # a column the model lacks has factor 0 in both
+                    if rio[k] == 0.0:
+                        assert io.sfac[k] == 0.0, (k, io.sfac[k])
