@@ -1324,6 +1324,16 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         sz.add<int>(FN);
         sz.add<double2>(2 * FN);
     }
+    // results gathered into one block for a single download: calc, scalars,
+    // diagnostics sums, instance records, statuses, the eclipse counts, the
+    // evaluated-element counters and the routed sub-phase counts
+    auto up8 = [](size_t b) { return (b + 7) & ~(size_t)7; };
+    const size_t o_calc = 0, o_scal = o_calc + M * n * 8, o_diag = o_scal + M * 8 * 8,
+                 o_insts = up8(o_diag + diag_idx.size() * 8 * 8), o_status = up8(o_insts + M * sizeof(InstDev)),
+                 o_counts = up8(o_status + M * sizeof(int)), o_evald = o_counts + 8,
+                 o_route = o_evald + 16 * 64 * sizeof(unsigned long long),
+                 down_total = up8(o_route + (any_route ? M * 4 * sizeof(int) : 0));
+    sz.add<char>(down_total);
     ctx->dev.reserve(sz.total + 4096);
     ctx->dev.reset();
     DevArena &A = ctx->dev;
@@ -1448,6 +1458,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     ca.nrange = A.take<int>(CN);
     ca.range = A.take<double2>(CN * COMP_MAX_RANGES);
     ca.sums = A.take<double>(std::max<size_t>((size_t)ncomp * 3 * S, 1));
+    char *d_down = A.take<char>(down_total);
 
     // ---------------- uploads through one pinned staging block
     std::vector<int> ndiv(n);
@@ -1500,8 +1511,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     size_t up_end = 0;  // the uploaded arrays lead the arena
     for (auto &u : ups)
         if (u.bytes) up_end = std::max(up_end, (size_t)(static_cast<char *>(u.dst) - A.base) + u.bytes);
-    const size_t down_bytes = M * n * 8 + M * 8 * 8 + M * sizeof(int) + diag_idx.size() * 8 * 8 + M * sizeof(InstDev);
-    ctx->pin.reserve(std::max(up_end, down_bytes) + 4096);
+    ctx->pin.reserve(std::max(up_end, down_total) + 4096);
     hipStream_t s = ctx->stream;
     ctx->prof.host_plan_ms_total +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call0).count();
@@ -1533,14 +1543,6 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             4);
         HIPCHK(hipMemcpyAsync(A.base, ctx->pin.base, up_end, hipMemcpyHostToDevice, s));
     }
-    HIPCHK(hipMemsetAsync(d_cand_count, 0, sizeof(int), s));
-    HIPCHK(hipMemsetAsync(ew.hit_count, 0, sizeof(int), s));
-    HIPCHK(hipMemsetAsync(ew.acount, 0, ECL_COUNTS * sizeof(int), s));
-    if (ncomp) {
-        HIPCHK(hipMemsetAsync(d_ccount, 0, sizeof(int), s));
-        HIPCHK(hipMemsetAsync(cew.hit_count, 0, sizeof(int), s));
-        HIPCHK(hipMemsetAsync(cew.acount, 0, ECL_COUNTS * sizeof(int), s));
-    }
     launch_fill_gaps(s, d_gaps, (int)gaps.size(), fa);  // one launch for all gaps
 
     DataDev dd;
@@ -1549,10 +1551,42 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
 
     // ---------------- launches
     hipEvent_t *ev = ctx->ev;
-    // clears for the kernel sequence, on the stream ahead of it
-    HIPCHK(hipMemsetAsync(d_out, 0, SM * 2 * sizeof(double), s));
-    HIPCHK(hipMemsetAsync(fl.evald, 0, 16 * 64 * sizeof(unsigned long long), s));
-    if (any_ecl_stop) HIPCHK(hipMemsetAsync(d_bits, 0, bits_words * sizeof(unsigned), s));
+    // the clears of the kernel sequence (its first kernel, so a captured
+    // graph holds them too) and the gather of its results (its last)
+    ByteRegions zero, gather;
+    std::memset(&zero, 0, sizeof(zero));
+    std::memset(&gather, 0, sizeof(gather));
+    int nzero = 0, ngather = 0;
+    auto add_zero = [&](void *p, size_t b) {
+        zero.dst[nzero] = p;
+        zero.n[nzero++] = b;
+    };
+    auto add_gather = [&](size_t off, const void *src, size_t b) {
+        if (!b) return;
+        gather.dst[ngather] = d_down + off;
+        gather.src[ngather] = src;
+        gather.n[ngather++] = b;
+    };
+    add_zero(d_cand_count, sizeof(int));
+    add_zero(ew.hit_count, sizeof(int));
+    add_zero(ew.acount, ECL_COUNTS * sizeof(int));
+    if (ncomp) {
+        add_zero(d_ccount, sizeof(int));
+        add_zero(cew.hit_count, sizeof(int));
+        add_zero(cew.acount, ECL_COUNTS * sizeof(int));
+    }
+    add_zero(d_out, SM * 2 * sizeof(double));
+    add_zero(fl.evald, 16 * 64 * sizeof(unsigned long long));
+    if (any_ecl_stop) add_zero(d_bits, bits_words * sizeof(unsigned));
+    add_gather(o_calc, d_calc, M * n * 8);
+    add_gather(o_scal, d_scal, M * 8 * 8);
+    add_gather(o_diag, d_diag, diag_idx.size() * 8 * 8);
+    add_gather(o_insts, d_insts, M * sizeof(InstDev));
+    add_gather(o_status, d_status, M * sizeof(int));
+    add_gather(o_counts, d_cand_count, sizeof(int));
+    add_gather(o_counts + 4, ew.hit_count, sizeof(int));
+    add_gather(o_evald, fl.evald, 16 * 64 * sizeof(unsigned long long));
+    if (any_route) add_gather(o_route, d_counts, M * 4 * sizeof(int));
     const bool ecl_rounds = any_ecl_stop && 2LL * nface_total >= ctx->ecl_round_min;
 
     // ---------------- launches: kernels only, so the sequence can be a graph
@@ -1563,6 +1597,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         auto mark = [&](hipEvent_t e) {
             if (timed) HIPCHK(hipEventRecord(e, s));
         };
+        launch_regions(s, zero, nzero);
         if (any_ecl_stop) launch_phase_bits(s, d_insts, (int)M, dd, d_bits, bits_lb);
         mark(ev[0]);
         launch_faces(s, d_grids, d_rings, nring, nhalf_total, fa, d_cand, d_cand_count, d_status);
@@ -1613,6 +1648,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         launch_flux_reduce(s, d_batches, (int)batches.size(), S, TILE, d_lists, d_counts, d_partial, d_out);
         launch_scatter(s, d_insts, (int)M, dd, pa, d_out, ca.sums, d_calc, d_scal, d_scat_part);
         if (!diag_idx.empty()) launch_diag(s, d_grids, d_diag_idx, (int)diag_idx.size(), fa, d_diag);
+        launch_regions(s, gather, ngather);
     };
     // A graph replays the exact launches (grids, blocks, arguments) of its
     // capture: the key holds every size, flag and device pointer they use.
@@ -1634,7 +1670,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         const void *ptrs[] = {d_grids, d_rings, d_insts, d_status, d_cand, d_cand_count, d_copy_hit, d_copy_dup,
                               d_copy_out, d_flux_hit, d_flux_dup, d_flux_out, d_comps, d_cgrids, d_ccand, d_ccount, d_gsums, d_fsegs, d_seg_cnt, d_lists, d_counts, d_task_first, d_task_order,
                               d_batches, d_tasks, d_partial, d_out, d_calc, d_scal, d_scat_part, d_diag_idx, d_diag,
-                              d_bits, d_ragged, d_dgrids};
+                              d_bits, d_ragged, d_dgrids, d_down};
         put(ptrs, sizeof(ptrs));
         put(&ctx->geo.pool, sizeof(ctx->geo.pool));
         put(&dd, sizeof(dd));
@@ -1676,28 +1712,19 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     }
     HIPCHK(hipGetLastError());
 
-    // ---------------- downloads
+    // ---------------- the download: one copy of the gathered block
     char *pb = ctx->pin.base;
-    double *h_calc = reinterpret_cast<double *>(pb);
-    double *h_scal = h_calc + M * n;
-    double *h_diag = h_scal + M * 8;
-    InstDev *h_insts = reinterpret_cast<InstDev *>(h_diag + diag_idx.size() * 8);
-    int *h_status = reinterpret_cast<int *>(h_insts + M);
-    HIPCHK(hipMemcpyAsync(h_calc, d_calc, M * n * 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(h_scal, d_scal, M * 8 * 8, hipMemcpyDeviceToHost, s));
-    if (!diag_idx.empty())
-        HIPCHK(hipMemcpyAsync(h_diag, d_diag, diag_idx.size() * 8 * 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(h_insts, d_insts, M * sizeof(InstDev), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(h_status, d_status, M * sizeof(int), hipMemcpyDeviceToHost, s));
-    int h_counts[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(&h_counts[0], d_cand_count, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&h_counts[1], ew.hit_count, sizeof(int), hipMemcpyDeviceToHost, s));
-    unsigned long long h_evald[16 * 64];
-    HIPCHK(hipMemcpyAsync(h_evald, fl.evald, sizeof(h_evald), hipMemcpyDeviceToHost, s));
-    std::vector<int> cnt(any_route ? M * 4 : 0);  // routed sub-phase counts
-    if (any_route) HIPCHK(hipMemcpyAsync(cnt.data(), d_counts, M * 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(pb, d_down, down_total, hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(ev[6], s));
     HIPCHK(hipStreamSynchronize(s));
+    const double *h_calc = reinterpret_cast<const double *>(pb + o_calc);
+    const double *h_scal = reinterpret_cast<const double *>(pb + o_scal);
+    const double *h_diag = reinterpret_cast<const double *>(pb + o_diag);
+    const InstDev *h_insts = reinterpret_cast<const InstDev *>(pb + o_insts);
+    int *h_status = reinterpret_cast<int *>(pb + o_status);
+    const int *h_counts = reinterpret_cast<const int *>(pb + o_counts);
+    const unsigned long long *h_evald = reinterpret_cast<const unsigned long long *>(pb + o_evald);
+    const int *cnt = reinterpret_cast<const int *>(pb + o_route);  // routed sub-phase counts
     ctx->prof.eclipse_candidates += h_counts[0];
     ctx->prof.eclipse_hits += h_counts[1];
 
@@ -1789,7 +1816,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             double s1 = 1.;
             if (!(pn <= I.phase1 || pn >= 1. - I.phase1))
                 s1 = (I.scale11 * (1. - I.phase1 - pn) + I.scale12 * (pn - I.phase1)) / (1 - 2. * I.phase1);
-            double wd = (0.0 + 1.0 * s1 * h_diag[diag_of[i][2] * 8 + 4]) / 1;
+            const double wd = s1 * h_diag[diag_of[i][2] * 8 + 4];
             io.wdwarf = wd * (o.scale ? sc[0] : io.sfac[0]);
             // comp_gravity1/2 (comp_gravity.cpp) and comp_radius1/2
             double gm = std::pow(1000. * m.velocity_scale.value, 3.0) * m.tperiod * lcx::DAY / TWOPI;

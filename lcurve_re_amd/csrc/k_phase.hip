@@ -1,6 +1,8 @@
 // k_phase.hip -- orbital phases and sub-exposures, fine/coarse routing,
 // scatter of weighted sums back to data points, scale factor and chi**2
 // (light_curve_comp.cpp:350-586, re_scale.cpp:16-43).
+#include <algorithm>
+
 #include "lc_kernels.h"
 #include "lc_math.h"
 
@@ -551,7 +553,24 @@ __global__ void k_scatter_final(const InstDev *__restrict__ insts, int nblk, con
     }
 }
 
+__global__ __launch_bounds__(256) void k_regions(ByteRegions R) {
+    const int r = blockIdx.y;
+    const size_t n4 = R.n[r] >> 2;
+    unsigned *d = static_cast<unsigned *>(R.dst[r]);
+    const unsigned *src = static_cast<const unsigned *>(R.src[r]);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = src ? src[i] : 0u;
+}
+
 }  // namespace
+
+void launch_regions(hipStream_t s, const ByteRegions &r, int nreg) {
+    if (nreg <= 0) return;
+    size_t mx = 0;
+    for (int k = 0; k < nreg; k++) mx = std::max(mx, r.n[k]);
+    const int bx = (int)std::max<size_t>(1, std::min<size_t>(4096, (mx / 4 + 255) / 256));
+    hipLaunchKernelGGL(k_regions, dim3(bx, nreg), dim3(256), 0, s, r);
+}
 
 void launch_phases(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa) {
     if (ninst <= 0 || d.n <= 0) return;

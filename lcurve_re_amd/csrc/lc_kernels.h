@@ -211,6 +211,16 @@ void launch_point_sums(hipStream_t s, const GridDev *grids, const InstDev *insts
                        int nchunk, double *part);
 void launch_gint_scales(hipStream_t s, InstDev *insts, int ninst, int nchunk, const double *part);
 void launch_phases(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa);
+// Byte regions (sizes multiples of 4) copied from src, or zeroed where src
+// is null, in one launch: the clears ahead of a call's kernels and the
+// gather of its results into one block for a single download.
+constexpr int MAX_REGIONS = 12;
+struct ByteRegions {
+    void *dst[MAX_REGIONS];
+    const void *src[MAX_REGIONS];
+    size_t n[MAX_REGIONS];
+};
+void launch_regions(hipStream_t s, const ByteRegions &r, int nreg);
 // Marks in bits (ninst x 2^lb bits, cleared by the caller) the bin of every
 // phase at which an instance's eclipse ranges are tested: its sub-phases,
 // the eight Ginterp phases and phase 0.5 (diagnostics).
