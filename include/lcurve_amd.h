@@ -286,7 +286,8 @@ lc_status lc_light_curve_comp_batch(lc_ctx *ctx, const lc_model *models,
  * 64 MiB per GPU (needs about 64 B per face + 8 B per 1024 faces per
  * sub-phase); larger inputs return LC_E_ARG with the sizes.  The call
  * allocates nothing, runs on ctx's stream and returns when out is written.
- * Calls on one GPU share that array: calls that overlap on one GPU (other
+ * Grids with ragged faces (n1 < n) return LC_E_UNSUPPORTED here; the host-
+ * array entry points take them.  Calls on one GPU share that array: calls that overlap on one GPU (other
  * threads, other contexts) race. */
 lc_status lc_flat_sum_multi_noalloc(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_batch *pb,
                                     const lc_ldc *ldc, double beam, double spin, double VFAC, double XCOFM,

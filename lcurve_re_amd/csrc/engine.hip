@@ -2165,9 +2165,16 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
                    double beam, double spin, double VFAC, double XCOFM, int glens1, double rlens1, int precision,
                    double *out) {
     if (!g || !pb || !ldc || !out) throw EngineError(LC_E_ARG, "flat_sum: null argument");
-    if (g->n1 != g->n) throw EngineError(LC_E_UNSUPPORTED, "flat_sum: faces with several eclipse ranges");
-    if (g->n0 > g->n1) throw EngineError(LC_E_ARG, "flat_sum: n0 > n1");
-    const size_t nf = g->n, S = pb->n;
+    if (g->n0 > g->n1 || g->n1 > g->n) throw EngineError(LC_E_ARG, "flat_sum: need n0 <= n1 <= n");
+    const size_t nf = g->n, S = pb->n, n1 = g->n1, nr = nf - n1;  // faces [n1, n) have several ranges
+    size_t nrange = 0;
+    if (nr) {
+        if (!g->moff || g->moff[0] != 0) throw EngineError(LC_E_ARG, "flat_sum: moff must start at 0");
+        for (size_t r = 0; r < nr; r++)
+            if (g->moff[r + 1] < g->moff[r]) throw EngineError(LC_E_ARG, "flat_sum: moff must not decrease");
+        nrange = (size_t)g->moff[nr];
+        if (nrange && (!g->min_ || !g->mout_)) throw EngineError(LC_E_ARG, "flat_sum: null min_/mout_");
+    }
     if (S == 0) return LC_OK;
     if (nf == 0) {
         std::fill(out, out + S, 0.0);
@@ -2179,7 +2186,7 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
     std::memset(&G, 0, sizeof(G));
     G.disc_comp = -1;
     G.star = star;
-    G.nface = (int)nf;
+    G.nface = (int)n1;  // the chunked kernels sum faces [0, n1); the rest after them
     G.beam = beam;
     G.spin_beam = spin;
     G.VFAC = VFAC;
@@ -2198,7 +2205,7 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
         double vn = g->dx[i] * vx + g->dy[i] * vy;
         r1[i] = make_float4((float)vx, (float)vy, (float)vn, 0.f);
         r2[i] = make_float4((float)g->px[i], (float)g->py[i], (float)g->pz[i], 0.f);
-        ec[i] = i >= g->n0 ? make_double2(g->in1[i - g->n0], g->out1[i - g->n0]) : make_double2(0., 0.);
+        ec[i] = i >= g->n0 && i < n1 ? make_double2(g->in1[i - g->n0], g->out1[i - g->n0]) : make_double2(0., 0.);
     }
     const int R = ctx->flux_rphase > 0 ? ctx->flux_rphase : (beam != 0.0 ? 2 : 1), TILE = 256 * R;
     const int chunk = fp64 ? 1024 : (ctx->flux_chunk > 0 ? ctx->flux_chunk : 1024);
@@ -2208,11 +2215,15 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
     B.nchunk = (int)((nf + chunk - 1) / chunk);
     B.ntile = (int)((S + TILE - 1) / TILE);
     B.part_off = 0;
+    const int nchunk_main = (int)((n1 + chunk - 1) / chunk);
     std::vector<TaskDev> tasks;
     for (int t = 0; t < B.ntile; t++)
-        for (int c = 0; c < B.nchunk; c++) tasks.push_back(TaskDev{0, t, c, 0});
+        for (int c = 0; c < nchunk_main; c++) tasks.push_back(TaskDev{0, t, c, 0});
     const size_t part = (size_t)B.nchunk * B.ntile * TILE;
-    ctx->dev.reserve(nf * (3 * 16 + 16 + 7 * 8 + 8) + S * 8 * 10 + part * 8 + tasks.size() * 16 + 65536);
+    std::vector<double2> rng(nrange);
+    for (size_t j = 0; j < nrange; j++) rng[j] = make_double2(g->min_[j], g->mout_[j]);
+    ctx->dev.reserve(nf * (3 * 16 + 16 + 7 * 8 + 8) + S * 8 * 10 + part * 8 + tasks.size() * 16 + nr * 4 +
+                     nrange * 16 + 65536);
     ctx->dev.reset();
     DevArena &A = ctx->dev;
     hipStream_t s = ctx->stream;
@@ -2222,7 +2233,7 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
     fl.rec1 = dev_upload(A, r1.data(), nf, s);
     fl.rec2 = dev_upload(A, r2.data(), nf, s);
     fl.ecl = dev_upload(A, ec.data(), nf, s);
-    if (fp64) {
+    if (fp64 || nr) {
         fl.fdx = dev_upload(A, g->dx, nf, s); fl.fdy = dev_upload(A, g->dy, nf, s); fl.fdz = dev_upload(A, g->dz, nf, s);
         fl.fpx = dev_upload(A, g->px, nf, s); fl.fpy = dev_upload(A, g->py, nf, s); fl.fpz = dev_upload(A, g->pz, nf, s);
         fl.fflux = dev_upload(A, g->flux, nf, s);
@@ -2242,10 +2253,15 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
     TaskDev *d_tasks = dev_upload(A, tasks.data(), tasks.size(), s);
     double *d_partial = A.take<double>(part);
     double *d_out = A.take<double>(2 * S);
+    const int *d_moff = nr ? dev_upload(A, g->moff, nr + 1, s) : nullptr;
+    const double2 *d_rng = nrange ? dev_upload(A, rng.data(), nrange, s) : nullptr;
     HIPCHK(hipMemsetAsync(d_out, 0, 2 * S * sizeof(double), s));
     HIPCHK(hipEventRecord(ctx->ev[4], s));
     launch_flux(s, flux_variant(G), fp64 ? 1 : 0, R, d_tasks, (int)tasks.size(), d_batch, d_grid, fl, pa, (int)S,
                 nullptr, nullptr, d_partial);
+    if (nr)
+        launch_flux_ragged_tail(s, flux_variant(G), d_grid, d_batch, fl, d_moff, d_rng, pa, (int)S, (int)n1, (int)nf,
+                                chunk, (int)(n1 / chunk), B.nchunk, nchunk_main, (size_t)B.ntile * TILE, d_partial);
     HIPCHK(hipEventRecord(ctx->ev[5], s));
     launch_flux_reduce(s, d_batch, 1, (int)S, TILE, nullptr, nullptr, d_partial, d_out);
     HIPCHK(hipGetLastError());

@@ -732,6 +732,35 @@ __global__ __launch_bounds__(256) void k_flux_ragged(const int *__restrict__ ord
     partial[B.part_off + e] = total + blk;
 }
 
+// Faces with several eclipse ranges at the end of a flat grid (the flat-sum
+// entry points): one thread per sub-phase and chunk, continuing the chunk's
+// partial in face order as the reference's block loop does.
+template <int LT, bool BEAM, bool GLENS>
+__global__ __launch_bounds__(256) void k_flux_ragged_tail(const GridDev *__restrict__ grid,
+                                                         const BatchDev *__restrict__ batch, FlatArrays fl,
+                                                         const int *__restrict__ moff,
+                                                         const double2 *__restrict__ ranges, PhaseArrays pa, int S,
+                                                         int n1, int nface, int chunk, int c_first, int nchunk_main,
+                                                         size_t stride, double *__restrict__ partial) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= S) return;
+    const GridDev &G = *grid;
+    const int c = c_first + (int)blockIdx.y;
+    const double ex = pa.ex[e], ey = pa.ey[e], ez = pa.ez[e], ph = pa.phin[e];
+    const double base = 1.0 - (G.ldc[0] + G.ldc[1] + G.ldc[2] + G.ldc[3]);
+    double *out = partial + batch->part_off + (size_t)c * stride + e;
+    double acc = c < nchunk_main ? *out : 0.0;
+    const int lo = max(c * chunk, n1), hi = min((c + 1) * chunk, nface);
+    for (int i = lo; i < hi; i++) {
+        bool hid = false;
+        for (int j = moff[i - n1]; j < moff[i - n1 + 1] && !hid; j++) hid = ecl_range(ph, ranges[j].x, ranges[j].y);
+        if (!hid)
+            acc += elem64<LT, BEAM, GLENS>(ex, ey, ez, fl.fdx[i], fl.fdy[i], fl.fdz[i], fl.fpx[i], fl.fpy[i],
+                                           fl.fpz[i], fl.fflux[i], G, base);
+    }
+    *out = acc;
+}
+
 // Merge chunk partials in chunk order; out[(inst*2 + star)*S + j].
 __global__ void k_flux_reduce(const BatchDev *__restrict__ batches, int S, int tile_len, const int *__restrict__ lists,
                               const int *__restrict__ counts, const double *__restrict__ partial,
@@ -821,6 +850,30 @@ void launch_make_tasks(hipStream_t s, const int *first, const int *order, int nb
     if (ntask <= 0 || nb <= 0) return;
     hipLaunchKernelGGL(k_make_tasks, dim3((ntask + 255) / 256), dim3(256), 0, s, first, order, nb, batches, ntask,
                        tasks);
+}
+
+void launch_flux_ragged_tail(hipStream_t s, int variant, const GridDev *grid, const BatchDev *batch, FlatArrays fl,
+                             const int *moff, const double2 *ranges, PhaseArrays pa, int S, int n1, int nface,
+                             int chunk, int c_first, int nchunk, int nchunk_main, size_t stride, double *partial) {
+    if (S <= 0 || nchunk <= c_first) return;
+    const dim3 g((S + 255) / 256, nchunk - c_first);
+#define LC_TAIL_CASE(V, LT, BM, GL)                                                                               \
+    case V:                                                                                                     \
+        hipLaunchKernelGGL((k_flux_ragged_tail<LT, BM, GL>), g, dim3(256), 0, s, grid, batch, fl, moff, ranges, pa, \
+                           S, n1, nface, chunk, c_first, nchunk_main, stride, partial);                           \
+        break;
+    switch (variant & 7) {
+        LC_TAIL_CASE(0, 0, false, false)
+        LC_TAIL_CASE(1, 1, false, false)
+        LC_TAIL_CASE(2, 0, true, false)
+        LC_TAIL_CASE(3, 1, true, false)
+        LC_TAIL_CASE(4, 0, false, true)
+        LC_TAIL_CASE(5, 1, false, true)
+        LC_TAIL_CASE(6, 0, true, true)
+        LC_TAIL_CASE(7, 1, true, true)
+    default: break;
+    }
+#undef LC_TAIL_CASE
 }
 
 void launch_flux_ragged(hipStream_t s, int variant, const int *order, int nb, int max_count, const BatchDev *batches,

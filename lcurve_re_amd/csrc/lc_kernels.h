@@ -244,6 +244,15 @@ void launch_flux_reduce(hipStream_t s, const BatchDev *batches, int nbatch, int 
 void launch_flux_ragged(hipStream_t s, int variant, const int *order, int nb, int max_count, const BatchDev *batches,
                         const GridDev *grids, FaceArrays fa, PhaseArrays pa, int S, const int *lists,
                         const int *counts, double *partial);
+// Continues the chunk partials of one flat grid (batch 0, no routing) over
+// its faces [n1, nface) that have several eclipse ranges: chunks
+// c_first .. nchunk-1 of `chunk` faces; a chunk below nchunk_main starts
+// from its partial, a later one from zero; faces in index order, FP64
+// elements, hidden when any of ranges[moff[i-n1] .. moff[i-n1+1]) holds
+// the phase (comp_light_flat.cpp:294-307).  stride = doubles per chunk row.
+void launch_flux_ragged_tail(hipStream_t s, int variant, const GridDev *grid, const BatchDev *batch, FlatArrays fl,
+                             const int *moff, const double2 *ranges, PhaseArrays pa, int S, int n1, int nface,
+                             int chunk, int c_first, int nchunk, int nchunk_main, size_t stride, double *partial);
 // part: workspace of ninst * ceil(n / SCAT_PTS) * SCAT_PARTS doubles
 // slots 0-7: scale sums, chi**2, the two-star iscale sums; 8-27: the
 // five-column iscale Gram matrix (15) and right-hand side (5)

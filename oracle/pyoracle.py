@@ -122,6 +122,9 @@ def ref():
                               _DP, _DP, _DP, _DP, _DP, _DP, _FP, _FP, _IP, _DP, _DP]),
         "ref_flat_sum_multi": (None, [I, Z, Z, Z, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, Z, _DP, _DP,
                                       _DP, _DP, D, D, D, D, D, I, D, D, D, D, I, D, _DP]),
+        "ref_flat_sum_multi_ragged": (None, [I, Z, Z, Z, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _IP, _DP,
+                                             _DP, Z, _DP, _DP, _DP, _DP, D, D, D, D, D, I, D, D, D, D, I, D,
+                                             _DP]),
         "ref_comp_star": (D, [I, Z, _DP, _DP, _DP, _DP, _DP, _DP, _FP, _IP, _DP, _DP, D, D, D, D, D, I, D, D,
                               D, I, D, D, C.c_float, I, D, D, D]),
         "ref_re_scale": (D, [Z, _DP, _DP, _DP, _DP, _DP, _DP, _DP]),
@@ -188,26 +191,39 @@ class Faces:
 
 
 class Flat:
-    """Lcurve::FlatGrid in numpy: stable partition, never-eclipsed first
-    (comp_light_flat.cpp:26-65)."""
+    """Lcurve::FlatGrid in numpy: stable partition into faces with no
+    eclipse range, with one, with more; each class in index order
+    (comp_light_flat.cpp:26-65).  `extra` optionally maps a face index to
+    further (ingress, egress) ranges after its own, as an opaque disc's
+    ranges follow the star's (light_curve_comp.cpp:286-326)."""
 
-    def __init__(self, faces):
-        order = np.concatenate([np.nonzero(faces.necl == 0)[0], np.nonzero(faces.necl == 1)[0]])
+    def __init__(self, faces, extra=None):
+        extra = extra or {}
+        ranges = []
+        for i in range(faces.n):
+            r = [(float(faces.ingress[i]), float(faces.egress[i]))] if faces.necl[i] else []
+            ranges.append(r + [(float(a), float(b)) for a, b in extra.get(i, ())])
+        cls = np.array([min(len(r), 2) for r in ranges], np.int64)
+        order = np.concatenate([np.nonzero(cls == c)[0] for c in (0, 1, 2)])
         self.n = faces.n
-        self.n0 = int(np.count_nonzero(faces.necl == 0))
-        self.n1 = self.n
+        self.n0 = int(np.count_nonzero(cls == 0))
+        self.n1 = self.n0 + int(np.count_nonzero(cls == 1))
         for k in ("dx", "dy", "dz", "px", "py", "pz"):
             setattr(self, k, np.ascontiguousarray(getattr(faces, k)[order]))
         self.flux = faces.flux[order].astype(np.float64)
-        self.in1 = np.ascontiguousarray(faces.ingress[order][self.n0:])
-        self.out1 = np.ascontiguousarray(faces.egress[order][self.n0:])
-        self.moff = np.zeros(1, np.int32)
+        one = [ranges[i][0] for i in order[self.n0:self.n1]]
+        self.in1 = np.array([a for a, _ in one], np.float64)
+        self.out1 = np.array([b for _, b in one], np.float64)
+        rag = [ranges[i] for i in order[self.n1:]]
+        self.moff = np.concatenate([[0], np.cumsum([len(r) for r in rag])]).astype(np.int32)
+        self.min_ = np.array([a for r in rag for a, _ in r] or [0.0], np.float64)
+        self.mout_ = np.array([b for r in rag for _, b in r] or [0.0], np.float64)
 
     def view(self):
         abi = _abi()
         v = abi.lc_flat_grid()
         v.n, v.n0, v.n1 = self.n, self.n0, self.n1
-        for k in ("dx", "dy", "dz", "px", "py", "pz", "flux", "in1", "out1"):
+        for k in ("dx", "dy", "dz", "px", "py", "pz", "flux", "in1", "out1", "min_", "mout_"):
             setattr(v, k, dptr(getattr(self, k)))
         v.moff = iptr(self.moff)
         return v

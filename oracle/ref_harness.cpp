@@ -189,6 +189,29 @@ void ref_flat_sum_multi(int star, size_t n, size_t n0, size_t n1, const double *
         Lcurve::flat_sum_star2_multi(g, pb, ldc, beam, spin, VFAC, XCOFM, glens1 != 0, rlens1, out);
 }
 
+/* The same on a FlatGrid with ragged faces [n1, n): face n1 + r has the
+ * ranges min_[moff[r] .. moff[r+1]) / mout_[...] (comp_light_flat.cpp:26-65). */
+void ref_flat_sum_multi_ragged(int star, size_t n, size_t n0, size_t n1, const double *dx, const double *dy,
+                               const double *dz, const double *px, const double *py, const double *pz,
+                               const double *flux, const double *in1, const double *out1, const int *moff,
+                               const double *min_, const double *mout_, size_t np, const double *ex,
+                               const double *ey, const double *ez, const double *phin, double c1, double c2,
+                               double c3, double c4, double mucrit, int ltype, double beam, double spin, double VFAC,
+                               double XCOFM, int glens1, double rlens1, double *out) {
+    Lcurve::FlatGrid g;
+    fill_flat(g, n, n0, n1, dx, dy, dz, px, py, pz, flux, in1, out1);
+    g.moff.assign(moff, moff + (n - n1 + 1));
+    g.min_.assign(min_, min_ + moff[n - n1]);
+    g.mout_.assign(mout_, mout_ + moff[n - n1]);
+    Lcurve::PhaseBatch pb;
+    for (size_t k = 0; k < np; k++) pb.push(ex[k], ey[k], ez[k], phin[k], 1.0, (int)k);
+    Lcurve::LDC ldc(c1, c2, c3, c4, mucrit, ltype == 0 ? Lcurve::LDC::POLY : Lcurve::LDC::CLARET);
+    if (star == 0)
+        Lcurve::flat_sum_star1_multi(g, pb, ldc, beam, spin, VFAC, XCOFM, out);
+    else
+        Lcurve::flat_sum_star2_multi(g, pb, ldc, beam, spin, VFAC, XCOFM, glens1 != 0, rlens1, out);
+}
+
 static void fill_points(std::vector<Lcurve::Point> &v, size_t n, const double *px, const double *py,
                         const double *pz, const double *dx, const double *dy, const double *dz, const float *flux,
                         const int *necl, const double *ingress, const double *egress) {

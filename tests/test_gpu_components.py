@@ -107,7 +107,8 @@ def test_components_scaled_on_their_own(engine, gold, case):
             e_max, e_pt = errors(calc, ref)
             print(f"   max-normalised {e_max:.3e} pointwise {e_pt:.3e}; sfac {list(io.sfac)} vs {list(rio[:5])}",
                   flush=True)
-            assert e_max <= 1e-5 and e_pt <= 1e-5, (case, mode, prec, e_max, e_pt)
+            tol = 2e-9 if prec == L.LC_PREC_FP64 else 1e-5   # FP64 measured <= 3.3e-10 pointwise
+            assert e_max <= tol and e_pt <= tol, (case, mode, prec, e_max, e_pt)
             assert abs(io.chisq / rio[6] - 1) <= 1e-4
             if mode == "iscale_given":
                 assert list(io.sfac) == [1.7, 0.8, 1.3, 0.6, 1.1]

@@ -56,6 +56,33 @@ def test_flux_fp64_bit_exact_vs_reference(case, engine):
         i += 1
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_flux_ragged_faces_vs_reference(precision, engine):
+    """Faces with several eclipse ranges (the FlatGrid's last class, as an
+    opaque disc's ranges make them) through lc_flat_sum_star{1,2}_multi:
+    FP64 bit-identical to the compiled reference's sums, mixed <= 1e-5; the
+    classes straddle the 1024-face blocks, beaming, lensing and mucrit."""
+    import lcurve_re_amd as L
+    from test_oracle_golden import ragged_inputs
+    g = load_golden("flat_ragged")
+    VFAC, XCOFM = g["VFAC_XCOFM"]
+    prec = L.LC_PREC_FP64 if precision == "fp64" else L.LC_PREC_MIXED
+    i = 0
+    while f"sum{i}" in g:
+        star, flat, ph, ldc, beam, spin, glens, rl = ragged_inputs(g, i)
+        print(f"ragged sum={i} star={star} n={flat.n} n1={flat.n1} {precision}", flush=True)
+        out = engine.flat_sum(star, flat.view(), ph.view(), ldc, beam, spin, VFAC, XCOFM, glens, rl,
+                              precision=prec)
+        ref = g[f"sum{i}"]
+        if prec == L.LC_PREC_FP64:
+            bad = np.nonzero(out != ref)[0]
+            assert bad.size == 0, (i, bad[:5], out[bad[:5]], ref[bad[:5]])
+        else:
+            assert rel_err(out, ref) <= 1e-5, (i, rel_err(out, ref))
+        i += 1
+    assert i == 6
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_flux_mixed_vs_reference(case, engine):
     import lcurve_re_amd as L

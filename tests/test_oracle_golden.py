@@ -95,3 +95,37 @@ def test_re_scale_matches_reference(oracle_lib):
     s = oracle_lib.orc_re_scale(data, len(fit), P.dptr(fit), C.byref(chisq), C.byref(wnok))
     assert np.array_equal(fit, g["rs_fit_out"])
     assert (s, chisq.value, wnok.value) == tuple(g["rs_scale"])
+
+
+def ragged_inputs(g, i):
+    """Inputs of flat_ragged.npz sum i: the FlatGrid with the extra ranges,
+    sub-phases, LDC and the element parameters."""
+    from oracle import pyoracle as P
+    from lcurve_re_amd import _abi
+    star, lt, beam, glens, rl, spin, c1, c2, c3, c4, muc = g[f"sum{i}_spec"]
+    star, lt, glens = int(star), int(lt), int(glens)
+    key = f"s{star}"
+    faces = golden_faces(g, key)
+    extra = {}
+    for f, a, b in zip(g[f"{key}_extra_face"], g[f"{key}_extra_in"], g[f"{key}_extra_out"]):
+        extra.setdefault(int(f), []).append((float(a), float(b)))
+    ldc = _abi.lc_ldc(c1, c2, c3, c4, muc, lt, 0)
+    ph = P.Phases(g["ph_ex"], g["ph_ey"], g["ph_ez"], g["ph_phin"])
+    return star, P.Flat(faces, extra), ph, ldc, beam, spin, glens, rl
+
+
+def test_flat_sums_with_ragged_faces_match_reference(oracle_lib):
+    """Faces with several eclipse ranges (the FlatGrid's last class): the
+    oracle's blocked sums against flat_sum_star{1,2}_multi of the compiled
+    reference, bit for bit; the classes straddle the 1024-face blocks."""
+    from oracle import pyoracle as P
+    g = load_golden("flat_ragged")
+    VFAC, XCOFM = g["VFAC_XCOFM"]
+    i = 0
+    while f"sum{i}" in g:
+        star, flat, ph, ldc, beam, spin, glens, rl = ragged_inputs(g, i)
+        assert flat.n1 < flat.n and flat.n1 // 1024 < (flat.n - 1) // 1024
+        out = P.flat_sum(star, flat, ph, ldc, beam, spin, VFAC, XCOFM, glens, rl)
+        assert np.array_equal(out, g[f"sum{i}"]), (i, np.max(np.abs(out - g[f"sum{i}"])))
+        i += 1
+    assert i == 6
