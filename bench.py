@@ -386,7 +386,8 @@ def plain_gpu(L, eng, models, data, precision, batched, batched_rate, passes=2):
     rate = len(models) * passes / el
     log(f"bench: plain GPU, one call per instance: {rate:.1f} light curves/s")
     return {"value": rate, "unit": "light curves/s", "calls_per_pass": len(models), "passes": passes,
-            "batched_over_plain": batched_rate / rate, "max_rel_diff_vs_batched": worst}
+            "batched_over_plain": batched_rate / rate, "batched_output": "host memory",
+            "max_rel_diff_vs_batched": worst}
 
 
 def eager_profile(eng, step, nsteps):
@@ -427,32 +428,56 @@ def run_batch_workload(L, eng, args):
         R, chunk = eng.autotune(models, data, precision=args.precision)
         cfg["flux_tile"] = {"subphases_per_lane": R, "face_chunk": chunk, "autotuned": True}
     log(f"bench: workload {cfg}")
-    # the models packed once into the C layout and one output array, as a
-    # caller evaluating the same instances repeatedly would keep them
+    # the models packed once into the C layout and the output in GPU memory
+    # (lc_light_curve_comp_batch_dev), as a caller evaluating the same
+    # instances repeatedly on the device would keep them; the same steps with
+    # the curves copied back to host memory are timed after them
     packed = L.ModelBatch(models)
     calc = np.zeros((len(models), len(data)), np.float64)
+    dcalc = L.DeviceBuffer((len(models), len(data)))
+
+    def step_dev():
+        return eng.light_curve_comp_batch_dev(packed, data, dcalc.ptr, precision=args.precision)
+
+    def step_host():
+        return eng.light_curve_comp_batch(packed, data, precision=args.precision, out=calc)[1:]
+
     # one untimed call of the workload's launch shape before the warm-up, so
     # that the engine's graph capture (on a shape's second call) is not in
     # the timed steps whatever --warmup is
-    eng.light_curve_comp_batch(packed, data, precision=args.precision, out=calc)
+    step_dev()
     cfg["graph_prime_calls"] = 1
     for i in range(args.warmup):
-        _, io, status = eng.light_curve_comp_batch(packed, data, precision=args.precision, out=calc)
+        io, status = step_dev()
         log(f"bench: warmup {i} done, failed instances: {int(np.count_nonzero(status))}")
     eng.reset_profile()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        _, io, status = eng.light_curve_comp_batch(packed, data, precision=args.precision, out=calc)
+        io, status = step_dev()
     el = time.perf_counter() - t0
     timed = eng.profile()
     nok = int(np.count_nonzero(status == 0))
-    log(f"bench: {args.steps} steps in {el:.3f} s; profile {timed}")
+    log(f"bench: {args.steps} steps in {el:.3f} s (curves left in GPU memory); profile {timed}")
     cfg["graph_replays_per_step"] = timed["graph_launches"] / args.steps
+    cfg["output"] = "device memory (lc_light_curve_comp_batch_dev)"
+    step_host()   # the host-output launch shape's own capture
+    for i in range(max(1, args.warmup)):
+        step_host()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step_host()
+    el_host = time.perf_counter() - t0
+    host_out = {"value": M * args.steps / el_host, "unit": "light curves/s", "ms_per_step": el_host / args.steps * 1e3,
+                "what": "the same steps with the curves copied to host memory (lc_light_curve_comp_batch; "
+                        "the PCIe download is inside the step)"}
+    log(f"bench: host output {host_out}")
+    assert np.array_equal(dcalc.to_host()[status == 0], calc[status == 0]), "device and host curves differ"
     nprof = max(1, min(args.steps, 3))
-    prof = eager_profile(eng, lambda: eng.light_curve_comp_batch(packed, data, precision=args.precision, out=calc),
-                         nprof)
+    prof = eager_profile(eng, step_dev, nprof)
     roofline = roofline_from_profile(prof, nprof, args.precision, cfg["workload"])
-    plain = plain_gpu(L, eng, models, data, args.precision, calc, M * args.steps / el)
+    # one call per instance returns host curves, so it is set against the
+    # batched steps with host output
+    plain = plain_gpu(L, eng, models, data, args.precision, calc, host_out["value"])
     cpu, err, port = (None, None, None)
     if not args.no_cpu:
         cpu, err, port = cpu_baseline(models, data, calc, args.cpu_budget)
@@ -463,7 +488,7 @@ def run_batch_workload(L, eng, args):
             "dtype": "fp64 per face and accumulation" if args.precision else
             "fp32 per face, fp64 geometry/accumulation",
             "data": "synthetic", "config": cfg, "instances_ok": nok, "roofline": roofline,
-            "cpu_baseline": cpu, "cpu_oracle_port": port, "plain_gpu": plain,
+            "cpu_baseline": cpu, "cpu_oracle_port": port, "plain_gpu": plain, "host_output": host_out,
             "max_rel_err": err["max_normalised"] if err else None, "rel_err": err}
 
 

@@ -275,6 +275,19 @@ lc_status lc_light_curve_comp_batch(lc_ctx *ctx, const lc_model *models,
                                     const lc_eval_opts *opts, double *calc,
                                     lc_scalars *io, int32_t *status);
 
+/* No counterpart in the reference (its calc is a host vector).  The batched
+ * form with the curves left in GPU memory, for callers that use them there
+ * (a sampler or a training set on the device): dev_calc is device memory
+ * of ctx's GPU, m x n doubles; rows of failed instances are left as they
+ * were.  Scalars and statuses come back in host memory as above, and the
+ * curves are written when the call returns.  light_curve_comp_fast sampling
+ * (max_model_points below n) is refused (its interpolation runs on the
+ * host). */
+lc_status lc_light_curve_comp_batch_dev(lc_ctx *ctx, const lc_model *models,
+                                        size_t m, const lc_datum *data, size_t n,
+                                        const lc_eval_opts *opts, double *dev_calc,
+                                        lc_scalars *io, int32_t *status);
+
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): flat_sum_star1_multi (star 0) / flat_sum_star2_multi

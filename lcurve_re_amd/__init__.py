@@ -23,10 +23,10 @@ import json
 import numpy as np
 
 from . import _abi
-from ._abi import (LC_OK, LC_PREC_FP64, LC_PREC_MIXED, LC_LDC_CLARET, LC_LDC_POLY,
+from ._abi import (LC_OK, LC_E_DEVICE, LC_PREC_FP64, LC_PREC_MIXED, LC_LDC_CLARET, LC_LDC_POLY,
                    lc_datum, lc_eval_opts, lc_ldc, lc_model, lc_scalars)
 
-__all__ = ["Model", "ModelBatch", "Engine", "LcurveError", "make_data", "read_data", "write_data", "lc_model", "lc_datum", "lc_ldc",
+__all__ = ["Model", "ModelBatch", "Engine", "DeviceBuffer", "LcurveError", "make_data", "read_data", "write_data", "lc_model", "lc_datum", "lc_ldc",
            "LC_PREC_MIXED", "LC_PREC_FP64", "LC_LDC_POLY", "LC_LDC_CLARET"]
 
 
@@ -47,6 +47,65 @@ _DP = C.POINTER(C.c_double)
 
 def _dp(a):
     return a.ctypes.data_as(_DP)
+
+
+_hip_rt = None
+
+
+def _hip():
+    """The HIP runtime the library is linked against (loaded with it), for
+    the few device-memory calls DeviceBuffer needs."""
+    global _hip_rt
+    if _hip_rt is None:
+        _lib()
+        rt = C.CDLL("libamdhip64.so.7")
+        rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        rt.hipFree.argtypes = [C.c_void_p]
+        rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        rt.hipSetDevice.argtypes = [C.c_int]
+        for f in (rt.hipMalloc, rt.hipFree, rt.hipMemcpy, rt.hipSetDevice):
+            f.restype = C.c_int
+        _hip_rt = rt
+    return _hip_rt
+
+
+class DeviceBuffer:
+    """A float64 array of `shape` in GPU memory (hipMalloc on `device`), for
+    lc_light_curve_comp_batch_dev: `ptr` is its device address; `fill`
+    writes a host array or scalar into it, `to_host` copies it back."""
+
+    def __init__(self, shape, device=0):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        self.nbytes = int(np.prod(self.shape)) * 8
+        self.device = int(device)
+        rt = _hip()
+        p = C.c_void_p()
+        if rt.hipSetDevice(self.device) or rt.hipMalloc(C.byref(p), max(self.nbytes, 8)):
+            raise LcurveError(LC_E_DEVICE, "DeviceBuffer: hipMalloc of %d bytes failed" % self.nbytes)
+        self.ptr = p.value
+
+    def fill(self, values):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, np.float64), self.shape))
+        if _hip().hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), self.nbytes, 1):
+            raise LcurveError(LC_E_DEVICE, "DeviceBuffer: copy to the device failed")
+        return self
+
+    def to_host(self):
+        a = np.empty(self.shape, np.float64)
+        if _hip().hipMemcpy(a.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes, 2):
+            raise LcurveError(LC_E_DEVICE, "DeviceBuffer: copy to the host failed")
+        return a
+
+    def close(self):
+        if self.ptr:
+            _hip().hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Model:
@@ -262,6 +321,24 @@ class Engine:
         if st != LC_OK:
             self._err(st)
         return calc, io, status
+
+    def light_curve_comp_batch_dev(self, models, data, dev_calc, scale=False, rdata=True, diagnostics=False,
+                                   sfac=1.0, precision=LC_PREC_MIXED, max_model_points=0):
+        """As light_curve_comp_batch, with calc written to GPU memory: dev_calc
+        is the device address (an int, e.g. a tensor's data_ptr()) of m x n
+        doubles on this engine's GPU.  Returns (scalars, status)."""
+        pods, m = _packed(models)
+        n = len(data)
+        io = (lc_scalars * m)()
+        if m:
+            np.ctypeslib.as_array(io)["sfac"][:] = sfac
+        status = np.zeros(m, np.int32)
+        o = self._opts(scale, rdata, 0, diagnostics, max_model_points, precision)
+        st = _lib().lc_light_curve_comp_batch_dev(self._ctx, pods, m, data, n, C.byref(o), C.c_void_p(int(dev_calc)),
+                                                  io, status.ctypes.data_as(C.POINTER(C.c_int32)))
+        if st != LC_OK:
+            self._err(st)
+        return io, status
 
     def flat_sum(self, star, grid_view, phase_view, ldc, beam=0.0, spin=1.0, VFAC=0.0, XCOFM=0.0, glens1=0,
                  rlens1=0.0, precision=LC_PREC_MIXED):

@@ -155,6 +155,9 @@ SIGNATURES = {
     "lc_light_curve_comp_batch": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_model), _c.c_size_t,
                                              _c.POINTER(lc_datum), _c.c_size_t, _c.POINTER(lc_eval_opts), _DP,
                                              _c.POINTER(lc_scalars), _IP]),
+    "lc_light_curve_comp_batch_dev": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_model), _c.c_size_t,
+                                                 _c.POINTER(lc_datum), _c.c_size_t, _c.POINTER(lc_eval_opts),
+                                                 _c.c_void_p, _c.POINTER(lc_scalars), _IP]),
     "lc_flat_sum_star1_multi": (_c.c_int, [_c.c_void_p, _c.POINTER(lc_flat_grid), _c.POINTER(lc_phase_batch),
                                            _c.POINTER(lc_ldc), _c.c_double, _c.c_double, _c.c_double,
                                            _c.c_double, _c.c_int, _DP]),

@@ -732,9 +732,10 @@ struct EvalRequest {
     const lc_datum *data;
     size_t n;
     lc_eval_opts opts;
-    double *calc;       // m x n
+    double *calc;       // m x n (host)
     lc_scalars *io;     // m
     int32_t *status;    // m
+    double *dev_calc = nullptr;  // m x n on the device instead of calc
 };
 
 // Runs the whole pipeline for a batch of instances sharing the data set.
@@ -1324,15 +1325,18 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         sz.add<int>(FN);
         sz.add<double2>(2 * FN);
     }
-    // results gathered into one block for a single download: calc, scalars,
+    // results gathered into one block for a single download: scalars,
     // diagnostics sums, instance records, statuses, the eclipse counts, the
-    // evaluated-element counters and the routed sub-phase counts
+    // evaluated-element counters, the routed sub-phase counts and, last (left
+    // out when the curves stay on the device), calc
     auto up8 = [](size_t b) { return (b + 7) & ~(size_t)7; };
-    const size_t o_calc = 0, o_scal = o_calc + M * n * 8, o_diag = o_scal + M * 8 * 8,
+    const bool dev_out = rq.dev_calc != nullptr;
+    const size_t o_scal = 0, o_diag = o_scal + M * 8 * 8,
                  o_insts = up8(o_diag + diag_idx.size() * 8 * 8), o_status = up8(o_insts + M * sizeof(InstDev)),
                  o_counts = up8(o_status + M * sizeof(int)), o_evald = o_counts + 8,
                  o_route = o_evald + 16 * 64 * sizeof(unsigned long long),
-                 down_total = up8(o_route + (any_route ? M * 4 * sizeof(int) : 0));
+                 o_calc = up8(o_route + (any_route ? M * 4 * sizeof(int) : 0)),
+                 down_total = dev_out ? o_calc : o_calc + M * n * 8;
     sz.add<char>(down_total);
     ctx->dev.reserve(sz.total + 4096);
     ctx->dev.reset();
@@ -1428,7 +1432,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     double *d_out = A.take<double>(SM * 2);
     double *d_partial = A.take<double>((size_t)std::max<long long>(part_total, 1));
     TaskDev *d_tasks = A.take<TaskDev>(std::max(ntask, 1));
-    double *d_calc = A.take<double>(std::max<size_t>(M, 1) * n);
+    double *d_calc = dev_out ? rq.dev_calc : A.take<double>(std::max<size_t>(M, 1) * n);
     double *d_scal = A.take<double>(std::max<size_t>(M, 1) * 8);
     double *d_gsums = A.take<double>(std::max<size_t>(M, 1) * 8 * psum_chunks);
     double *d_scat_part = A.take<double>(scat_part);
@@ -1578,7 +1582,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     add_zero(d_out, SM * 2 * sizeof(double));
     add_zero(fl.evald, 16 * 64 * sizeof(unsigned long long));
     if (any_ecl_stop) add_zero(d_bits, bits_words * sizeof(unsigned));
-    add_gather(o_calc, d_calc, M * n * 8);
+    if (!dev_out) add_gather(o_calc, d_calc, M * n * 8);
     add_gather(o_scal, d_scal, M * 8 * 8);
     add_gather(o_diag, d_diag, diag_idx.size() * 8 * 8);
     add_gather(o_insts, d_insts, M * sizeof(InstDev));
@@ -1790,7 +1794,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
             if (ip[i].msg.empty()) ip[i].msg = status_msg(st);
             return;
         }
-        std::memcpy(rq.calc + i * n, h_calc + i * n, n * 8);
+        if (!dev_out) std::memcpy(rq.calc + i * n, h_calc + i * n, n * 8);
         lc_scalars &io = rq.io[i];
         const double *sc = h_scal + i * 8;
         if (o.scale) {
@@ -2132,6 +2136,34 @@ lc_status lc_light_curve_comp_batch(lc_ctx *ctx, const lc_model *models, size_t 
         HIPCHK(hipSetDevice(ctx->device));
         EvalRequest rq{models, m, data, n, *opts, calc, io, status};
         run_eval(ctx, rq);
+        return LC_OK;
+    });
+}
+
+lc_status lc_light_curve_comp_batch_dev(lc_ctx *ctx, const lc_model *models, size_t m, const lc_datum *data,
+                                        size_t n, const lc_eval_opts *opts, double *dev_calc, lc_scalars *io,
+                                        int32_t *status) {
+    if (!ctx) return LC_E_ARG;
+    if (!models || !data || !opts || !dev_calc || !io || !status) {
+        ctx->err = "light_curve_comp_batch_dev: null argument";
+        return LC_E_ARG;
+    }
+    if (m == 0) return LC_OK;
+    return guarded(ctx, [&]() -> int {
+        if (n == 0) throw EngineError(LC_E_ARG, "light_curve_comp: empty data");
+        if (opts->max_model_points > 0 && n > (size_t)opts->max_model_points)
+            throw EngineError(LC_E_ARG, "light_curve_comp_batch_dev: no light_curve_comp_fast sampling "
+                                        "(its interpolation runs on the host)");
+        HIPCHK(hipSetDevice(ctx->device));
+        hipPointerAttribute_t attr;
+        const hipError_t qa = hipPointerGetAttributes(&attr, dev_calc);
+        if (qa != hipSuccess) (void)hipGetLastError();  // a host pointer: clear the query's error
+        if (qa != hipSuccess || attr.type != hipMemoryTypeDevice || attr.device != ctx->device)
+            throw EngineError(LC_E_ARG, "light_curve_comp_batch_dev: dev_calc is not device memory of this "
+                                        "context's GPU");
+        EvalRequest rq{models, m, data, n, *opts, nullptr, io, status};
+        rq.dev_calc = dev_calc;
+        run_batch(ctx, rq);
         return LC_OK;
     });
 }

@@ -265,6 +265,42 @@ def test_packed_batch_and_out_array(engine):
         engine.light_curve_comp_batch(models, data, out=np.zeros((len(models), len(data) + 1)))
 
 
+def test_batch_with_curves_left_on_the_device(engine):
+    """lc_light_curve_comp_batch_dev writes the curves of the host-array
+    entry point, bit for bit, into device memory; twice in a row (the second
+    call smaller, then the first shape again: captured graphs are replayed
+    with the caller's buffer), failed rows untouched, scalars identical;
+    host pointers and _fast sampling are refused."""
+    import lcurve_re_amd as L
+    from lcurve_re_amd import workloads as W
+    for m, nphase in ((5, 700), (3, 250), (5, 700)):
+        w = W.batch_prior(m=m, nphase=nphase, nlat_lo=20, nlat_hi=36)
+        models = [L.Model.from_json(c) for c in w["models"]]
+        models.insert(1, L.Model.from_json(W.model_json(phys={"r2": 0.6})))  # Roche overflow
+        data = L.make_data(w["time"], flux=np.linspace(0.5, 1.5, nphase))
+        print(f"device output: {len(models)} instances x {nphase} phases", flush=True)
+        ref, io_ref, st_ref = engine.light_curve_comp_batch(models, data, scale=True, rdata=True, diagnostics=True)
+        dev = L.DeviceBuffer((len(models), nphase)).fill(-7.0)
+        io, st = engine.light_curve_comp_batch_dev(models, data, dev.ptr, scale=True, rdata=True, diagnostics=True)
+        got = dev.to_host()
+        assert list(st) == list(st_ref) and st[1] == 1
+        for i in range(len(models)):
+            if i == 1:
+                assert (got[i] == -7.0).all()
+                continue
+            assert np.array_equal(got[i], ref[i]), i
+            assert io[i].chisq == io_ref[i].chisq and list(io[i].sfac) == list(io_ref[i].sfac)
+            assert io[i].wdwarf == io_ref[i].wdwarf and io[i].logg1 == io_ref[i].logg1
+    host = np.zeros((len(models), nphase))
+    with pytest.raises(L.LcurveError):
+        engine.light_curve_comp_batch_dev(models, data, host.ctypes.data)
+    with pytest.raises(L.LcurveError):
+        engine.light_curve_comp_batch_dev(models, data, dev.ptr, max_model_points=100)
+    # the ordinary entry point still works after the refusals
+    again, _, _ = engine.light_curve_comp_batch(models, data, scale=True, rdata=True, diagnostics=True)
+    assert np.array_equal(again[0], ref[0])
+
+
 def test_roche_overflow_raises(engine):
     import lcurve_re_amd as L
     from lcurve_re_amd import workloads as W
