@@ -47,6 +47,7 @@ struct TileInfo {
     double cx, cy, cz;  // centre earth vector
     float rad;          // max |e - e_c| over the tile
     double pmin, pmax;  // phin range of the tile
+    bool zflat;         // every sub-phase has e_z == cz exactly (one inclination)
 };
 
 // Tile statistics for the visibility and eclipse shortcuts.
@@ -55,8 +56,10 @@ __device__ __forceinline__ TileInfo tile_info(const double *ex, const double *ey
                                               const bool *ok, double cx, double cy, double cz) {
     __shared__ float s_rad[NT / 64];
     __shared__ double s_min[NT / 64], s_max[NT / 64];
+    __shared__ int s_zd[NT / 64];
     float rad = 0.f;
     double pmin = 2.0, pmax = -1.0;
+    bool zd = false;
 #pragma unroll
     for (int r = 0; r < R; r++) {
         if (ok[r]) {
@@ -64,28 +67,34 @@ __device__ __forceinline__ TileInfo tile_info(const double *ex, const double *ey
             rad = fmaxf(rad, (float)sqrt(a * a + b * b + c * c));
             pmin = fmin(pmin, ph[r]);
             pmax = fmax(pmax, ph[r]);
+            zd = zd || ez[r] != cz;
         }
     }
     rad = wave_max(rad);
     pmin = wave_min_d(pmin);
     pmax = wave_max_d(pmax);
+    const bool wzd = __ballot(zd) != 0ull;
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         s_rad[w] = rad;
         s_min[w] = pmin;
         s_max[w] = pmax;
+        s_zd[w] = wzd ? 1 : 0;
     }
     __syncthreads();
     TileInfo t;
     t.rad = s_rad[0];
     t.pmin = s_min[0];
     t.pmax = s_max[0];
+    int zdiff = s_zd[0];
 #pragma unroll
     for (int k = 1; k < NT / 64; k++) {
         t.rad = fmaxf(t.rad, s_rad[k]);
         t.pmin = fmin(t.pmin, s_min[k]);
         t.pmax = fmax(t.pmax, s_max[k]);
+        zdiff |= s_zd[k];
     }
+    t.zflat = zdiff == 0;
     t.cx = cx;
     t.cy = cy;
     t.cz = cz;
@@ -129,6 +138,41 @@ __device__ __forceinline__ int tile_eclipse_class(const TileInfo &t, double in, 
     if (!any) return 0;
     bool all = (t.pmin >= in && t.pmax <= out) || (t.pmax <= out - 1.0);
     return all ? 2 : 1;
+}
+
+// The per-tile decision for one face record (both kernels of the Claret
+// law make it with this function, so they agree face by face): act = it
+// can contribute to some sub-phase (d.e_c above cull), cls = its eclipse
+// class for the tile, full = seen at every sub-phase and not eclipsed in
+// the tile (d.e_c above full_cut), to the moment sums.
+struct FaceDecision {
+    bool act, full;
+    int cls;
+    float dc;
+};
+__device__ __forceinline__ FaceDecision decide_face(const float4 &a0, const double2 &io, bool in, bool ecl,
+                                                    float ecx, float ecy, float ecz, float cull, float full_cut,
+                                                    const TileInfo &ti, bool mom) {
+    FaceDecision r;
+    r.dc = fmaf(a0.z, ecz, fmaf(a0.y, ecy, a0.x * ecx));
+    r.act = in && r.dc > cull;
+    r.cls = 0;
+    if (r.act && ecl) {
+        r.cls = tile_eclipse_class(ti, io.x, io.y);
+        r.act = r.cls != 2;
+    }
+    r.full = mom && r.act && r.cls == 0 && r.dc > full_cut;
+    return r;
+}
+
+// Full faces of a Claret-law tile that go to the Taylor moments: the tile has
+// one inclination (e - e_c has no z part) and |d.(e - e_c)| <= rad |d_xy|
+// stays below TAY_RHO mu_c, where the Taylor series of mu^(3/2) and
+// mu^(5/2) truncated after degree 9 is within 1.3e-8 of the terms.
+constexpr float TAY_RHO = 0.3f;
+__device__ __forceinline__ bool taylor_face(const float4 &a0, float dc, const TileInfo &ti) {
+    const float dxy2 = fmaf(a0.y, a0.y, a0.x * a0.x);
+    return ti.zflat && ti.rad * ti.rad * dxy2 <= (TAY_RHO * TAY_RHO) * (dc * dc);
 }
 
 // Order-preserving compaction of the active faces of a stage: returns the
@@ -272,9 +316,9 @@ __device__ __forceinline__ void sweep(const float4 *s_a0, const float4 *s_a1, co
 // over the 55 monomials of degree 1..5.  The moments M are FP64 sums over
 // the face chunk (each wave owns the monomials k = wave + 4t); the tile's
 // sub-phases then evaluate the polynomial in FP64.  Terms reach ~2e4 times
-// the result, so FP64 keeps the evaluation error near 1e-11 relative.  For
-// the Claret law only the integer powers of mu are a polynomial (degree 3:
-// the first 19 monomials); sweep_half adds the rest face by face.
+// the result, so FP64 keeps the evaluation error near 1e-11 relative.  The
+// Claret law, with half-integer powers of mu, goes through the Taylor
+// moments below instead.
 constexpr int NMOM = 55, MOM_PER_WAVE = 14;
 constexpr unsigned char MOM_A[NMOM] = {1,0,0,2,1,1,0,0,0,3,2,2,1,1,1,0,0,0,0,4,3,3,2,2,2,1,1,1,1,0,0,0,0,0,
                                         5,4,4,3,3,3,2,2,2,2,1,1,1,1,1,0,0,0,0,0,0};
@@ -313,49 +357,84 @@ __device__ __forceinline__ void mom_sweep(const float4 *s_m, int n, double (&m)[
     for (int j = (int)(threadIdx.x & 63); j < n; j += 64) mom_add<W, NM>(s_m[j], m);
 }
 
-// Claret law: mu I(mu) = base mu + c2 mu^2 + c4 mu^3 + mu^(3/2) (c1 + c3 mu).
-// The integer powers go through the moments of degree 1..3 (the first 19
-// monomials); this sweep adds the half-integer part of the faces seen at
-// every sub-phase (mu in (0, 1], no clip or mask needed), 8 VALU per element.
-template <int R>
-__device__ __forceinline__ void sweep_half(const float4 *s_m, int n_, const float *ex, const float *ey,
-                                           const float *ez, const Coef32 &K, double *acc) {
-#pragma clang fp contract(fast)
-    const int n = __builtin_amdgcn_readfirstlane(n_);
-    for (int k0 = 0; k0 < n; k0 += 32) {
-        float a32[R];
+
+// Claret law without beaming or lensing, the full faces that taylor_face
+// selects.  With delta = d.(e - e_c) = d_x dX + d_y dY (the tile's
+// sub-phases share e_z), mu I(mu) = H(mu) = base mu + c1 mu^(3/2) + c2 mu^2 +
+// c3 mu^(5/2) + c4 mu^3 is expanded about mu_c = d.e_c:
+//   H(mu_c + delta) = sum_k T_k(mu_c) delta^k,  T_k = H^(k)(mu_c) / k!,
+// exact in its integer powers and truncated after k = 9 in the half-integer
+// ones (|delta| <= 0.3 mu_c: relative error below 1.3e-8 of those terms).
+// Summed over the faces,
+//   sum_i f_i H(mu_i) = sum_{k,j} C(k,j) M_kj dX^(k-j) dY^j,
+//   M_kj = sum_i f_i T_k(mu_ci) d_xi^(k-j) d_yi^j,
+// 55 moments in FP64 (wave w owns monomials w + 4t) over the chunk's
+// selected faces, reduced over the lanes and evaluated in FP64 at every
+// sub-phase of the tile.  The other full faces of the tile (|delta| above
+// 0.3 mu_c, or several inclinations in the tile) are summed face by face.
+// Tried: a separate kernel for these moments (its per-task set-up and
+// reductions were latency bound: 6.8 against 6.7 ms per batched step for
+// the two kernels together).
+constexpr int TAY_K = 9, NTAY = 55, TAY_PER_WAVE = 14;
+constexpr unsigned char TAY_KK[NTAY] = {0,1,1,2,2,2,3,3,3,3,4,4,4,4,4,5,5,5,5,5,5,6,6,6,6,6,6,6,7,7,7,7,7,7,7,7,
+                                        8,8,8,8,8,8,8,8,8,9,9,9,9,9,9,9,9,9,9};
+constexpr unsigned char TAY_JJ[NTAY] = {0,0,1,0,1,2,0,1,2,3,0,1,2,3,4,0,1,2,3,4,5,0,1,2,3,4,5,6,0,1,2,3,4,5,6,7,
+                                        0,1,2,3,4,5,6,7,8,0,1,2,3,4,5,6,7,8,9};
+__constant__ double c_tay_binom[NTAY] = {1.,1.,1.,1.,2.,1.,1.,3.,3.,1.,1.,4.,6.,4.,1.,1.,5.,10.,10.,5.,1.,1.,6.,
+                                         15.,20.,15.,6.,1.,1.,7.,21.,35.,35.,21.,7.,1.,1.,8.,28.,56.,70.,56.,28.,
+                                         8.,1.,1.,9.,36.,84.,126.,126.,84.,36.,9.,1.};
+
+// m[t] += f T_k(mu_c) d_x^(k-j) d_y^j for this wave's monomials (k, j) =
+// (TAY_KK, TAY_JJ)[W + 4t], one face, k ascending: the mu^(3/2) and
+// mu^(5/2) terms by the binomial recurrence
+//   C(p, k+1) mu^(p-k-1) = C(p, k) mu^(p-k) (p - k) / ((k + 1) mu),
+// the integer powers' coefficients I_k added for k <= 3.  1/sqrt(mu_c)
+// from the hardware estimate and one Newton step (relative error ~1e-14).
+template <int W>
+__device__ __forceinline__ void tay_add(const float4 &rec, double cx, double cy, double cz, const double *cl,
+                                        double (&m)[TAY_PER_WAVE]) {
+    const double x = rec.x, y = rec.y, z = rec.z, f = rec.w;
+    const double mu = fma(z, cz, fma(y, cy, x * cx));
+    double q = __builtin_amdgcn_rsq(mu);
+    q = q * fma(-0.5 * mu, q * q, 1.5);
+    const double r = q * q;
+    const double base = cl[4], c1 = cl[0], c2 = cl[1], c3 = cl[2], c4 = cl[3];
+    const double fm15 = f * mu * (mu * q);  // f mu^(3/2)
+    double p15 = c1 * fm15, p25 = c3 * (fm15 * mu);
+    double xp[TAY_K + 1], yp[TAY_K + 1];
+    xp[0] = yp[0] = 1.0;
 #pragma unroll
-        for (int r = 0; r < R; r++) a32[r] = 0.f;
-        const int k1 = min(k0 + 32, n);
-        int k = k0;
-        for (; k + 4 <= k1; k += 4) {
+    for (int i = 1; i <= TAY_K; i++) {
+        xp[i] = xp[i - 1] * x;
+        yp[i] = yp[i - 1] * y;
+    }
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const float4 a = s_m[k + u];
+    for (int k = 0; k <= TAY_K; k++) {
+        double T = p15 + p25;
+        if (k == 0) T = fma(f * mu, fma(mu, fma(mu, c4, c2), base), T);
+        if (k == 1) T = fma(f, fma(mu, fma(3.0 * c4, mu, 2.0 * c2), base), T);
+        if (k == 2) T = fma(f, fma(3.0 * c4, mu, c2), T);
+        if (k == 3) T = fma(f, c4, T);
 #pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const float mu = fmaf(ez[r], a.z, fmaf(ey[r], a.y, ex[r] * a.x));
-                    const float v = mu * __builtin_amdgcn_sqrtf(mu) * fmaf(K.c3, mu, K.c1);
-                    a32[r] = fmaf(a.w, v, a32[r]);
-                }
+        for (int t = 0; t < TAY_PER_WAVE; t++) {
+            if (W + 4 * t < NTAY && TAY_KK[W + 4 * t] == k) {
+                const int j = TAY_JJ[W + 4 * t];
+                m[t] = fma(T, xp[k - j] * yp[j], m[t]);
             }
         }
-        for (; k < k1; k++) {
-            const float4 a = s_m[k];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                const float mu = fmaf(ez[r], a.z, fmaf(ey[r], a.y, ex[r] * a.x));
-                const float v = mu * __builtin_amdgcn_sqrtf(mu) * fmaf(K.c3, mu, K.c1);
-                a32[r] = fmaf(a.w, v, a32[r]);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < R; r++) acc[r] += (double)a32[r];
+        p15 *= r * ((1.5 - k) / (k + 1));
+        p25 *= r * ((2.5 - k) / (k + 1));
     }
 }
 
+template <int W>
+__device__ __forceinline__ void tay_sweep(const float4 *s_t, int n, double cx, double cy, double cz, const double *cl,
+                                          double (&m)[TAY_PER_WAVE]) {
+    for (int j = (int)(threadIdx.x & 63); j < n; j += 64) tay_add<W>(s_t[j], cx, cy, cz, cl, m);
+}
+
 template <int LT, bool BEAM, bool GLENS, bool MUC, int R>
-__global__ __launch_bounds__(NT, (LT == 1 && !BEAM && !GLENS) ? 5 : 4) void k_flux_mixed(const TaskDev *__restrict__ tasks,
+__global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict__ tasks,
                                                    const BatchDev *__restrict__ batches,
                                                    const GridDev *__restrict__ grids, FlatArrays fl, PhaseArrays pa,
                                                    int S, const int *__restrict__ lists,
@@ -368,7 +447,10 @@ __global__ __launch_bounds__(NT, (LT == 1 && !BEAM && !GLENS) ? 5 : 4) void k_fl
     __shared__ double2 s_io[FB];
     __shared__ int s_cnt[3][NT / 64];
     constexpr bool MOM = !BEAM && !GLENS;  // moment sums (above)
-    constexpr int NM = LT == 0 ? NMOM : 19;  // monomials: degree 5 (polynomial law) or 3 (Claret)
+    // Claret law: the full faces taylor_face selects go through the Taylor
+    // moments (the others of the full class through the direct sweep)
+    constexpr bool TAYF = LT == 1 && MOM;
+    constexpr int NM = LT == 0 ? NMOM : 0;  // 3-variable monomials: degree 5 (polynomial law)
     __shared__ float4 s_m[MOM ? FB : 1];
     __shared__ double s_coef[MOM ? NMOM : 1];
 
@@ -432,6 +514,9 @@ __global__ __launch_bounds__(NT, (LT == 1 && !BEAM && !GLENS) ? 5 : 4) void k_fl
     for (int t = 0; t < MOM_PER_WAVE; t++) mom[t] = 0.;
     // d.e_c above this: mu > max(0, mucrit) at every sub-phase, no clip or mask
     const float full_cut = fmaxf(0.f, (float)G.mucrit) + 1.e-5f + ti.rad;
+    // Claret coefficients for the Taylor moments: c1..c4, base
+    const double tcl[5] = {G.ldc[0], G.ldc[1], G.ldc[2], G.ldc[3],
+                           1.0 - (G.ldc[0] + G.ldc[1] + G.ldc[2] + G.ldc[3])};
 
     // stage prefetch registers (clamped index keeps the loads unconditional)
     int f = f_lo + (int)threadIdx.x;
@@ -454,19 +539,16 @@ __global__ __launch_bounds__(NT, (LT == 1 && !BEAM && !GLENS) ? 5 : 4) void k_fl
             if (GLENS) pa2 = rec2[fc];
             pio = eclr[fc];
         }
-        const float dc = fmaf(a0.z, ecz, fmaf(a0.y, ecy, a0.x * ecx));
-        bool act = in && dc > cull;
-        int cls = 0;
-        if (act && ecl) {
-            cls = tile_eclipse_class(ti, io.x, io.y);
-            act = cls != 2;
-        }
-        const bool full = MOM && act && cls == 0 && dc > full_cut;  // to the moment sums
+        const FaceDecision fd = decide_face(a0, io, in, ecl, ecx, ecy, ecz, cull, full_cut, ti, MOM);
+        const bool act = fd.act, full = fd.full;  // full: to the moment sums
+        const int cls = fd.cls;
+        const bool tay = TAYF && full && taylor_face(a0, fd.dc, ti);
+        const bool tomom = TAYF ? tay : full;  // to the moment list s_m
         int tot0, tot1, totm = 0;
-        const int slot0 = compact_slot(act && cls == 0 && !full, tot0, s_cnt[0]);
+        const int slot0 = compact_slot(act && cls == 0 && !tomom, tot0, s_cnt[0]);
         const int slot1 = compact_slot(act && cls == 1, tot1, s_cnt[1]);
-        const int slotm = MOM ? compact_slot(full, totm, s_cnt[2]) : 0;
-        if (full) {
+        const int slotm = MOM ? compact_slot(tomom, totm, s_cnt[2]) : 0;
+        if (tomom) {
             s_m[slotm] = a0;
         } else if (act) {
             const int slot = cls == 0 ? slot0 : FB + slot1;
@@ -476,8 +558,16 @@ __global__ __launch_bounds__(NT, (LT == 1 && !BEAM && !GLENS) ? 5 : 4) void k_fl
             if (cls) s_io[slot1] = io;
         }
         __syncthreads();
-        evc += (unsigned long long)(tot0 + tot1 + totm) * (unsigned long long)live;
-        if (MOM && totm) {
+        evc += (unsigned long long)(tot0 + tot1 + (TAYF ? 0 : totm)) * (unsigned long long)live;
+        if (TAYF && totm) {
+            const int nm = __builtin_amdgcn_readfirstlane(totm);
+            switch (threadIdx.x >> 6) {
+            case 0: tay_sweep<0>(s_m, nm, ti.cx, ti.cy, ti.cz, tcl, mom); break;
+            case 1: tay_sweep<1>(s_m, nm, ti.cx, ti.cy, ti.cz, tcl, mom); break;
+            case 2: tay_sweep<2>(s_m, nm, ti.cx, ti.cy, ti.cz, tcl, mom); break;
+            default: tay_sweep<3>(s_m, nm, ti.cx, ti.cy, ti.cz, tcl, mom); break;
+            }
+        } else if (MOM && totm) {
             const int nm = __builtin_amdgcn_readfirstlane(totm);
             switch (threadIdx.x >> 6) {
             case 0: mom_sweep<0, NM>(s_m, nm, mom); break;
@@ -485,7 +575,6 @@ __global__ __launch_bounds__(NT, (LT == 1 && !BEAM && !GLENS) ? 5 : 4) void k_fl
             case 2: mom_sweep<2, NM>(s_m, nm, mom); break;
             default: mom_sweep<3, NM>(s_m, nm, mom); break;
             }
-            if (LT == 1) sweep_half<R>(s_m, nm, ex, ey, ez, K, acc);
         }
         sweep<LT, BEAM, GLENS, MUC, false, R>(s_a0, s_a1, s_a2, s_io, tot0, ex, ey, ez, ph, K, acc);
         if (tot1)
@@ -495,7 +584,38 @@ __global__ __launch_bounds__(NT, (LT == 1 && !BEAM && !GLENS) ? 5 : 4) void k_fl
         f = fn;
     }
 
-    if (MOM) {
+    if (TAYF) {
+        // the Taylor moments summed over each wave's lanes, times C(k, j);
+        // then every sub-phase evaluates the polynomial in dX, dY
+        const int w = (int)(threadIdx.x >> 6);
+#pragma unroll
+        for (int t = 0; t < TAY_PER_WAVE; t++) {
+            double v = mom[t];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            const int k = w + 4 * t;
+            if ((threadIdx.x & 63) == 0 && k < NTAY) s_coef[k] = v * c_tay_binom[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            // the FP64 direction is loaded again (registers, as below)
+            const int el = tile0 + r * NT + (int)threadIdx.x;
+            const int j = ok[r] ? (B.list_off < 0 ? el : lists[B.list_off + el]) : 0;
+            const size_t idx = (size_t)B.inst * S + j;
+            const double X = ok[r] ? pa.ex[idx] - ti.cx : 0., Y = ok[r] ? pa.ey[idx] - ti.cy : 0.;
+            double xp[TAY_K + 1], yp[TAY_K + 1];
+            xp[0] = yp[0] = 1.0;
+#pragma unroll
+            for (int i = 1; i <= TAY_K; i++) {
+                xp[i] = xp[i - 1] * X;
+                yp[i] = yp[i - 1] * Y;
+            }
+            double e = 0.;
+#pragma unroll
+            for (int k = 0; k < NTAY; k++) e = fma(s_coef[k], xp[TAY_KK[k] - TAY_JJ[k]] * yp[TAY_JJ[k]], e);
+            acc[r] += e;
+        }
+    } else if (MOM) {
         // each wave's moments summed over its lanes, scaled into the
         // polynomial's coefficients; then every sub-phase evaluates it
         const double c1 = G.ldc[0], c2 = G.ldc[1], c3 = G.ldc[2], c4 = G.ldc[3];
