@@ -363,7 +363,8 @@ def roofline_from_profile(prof, steps, precision, workload, R=None):
             "eclipse_hits_per_step": prof["eclipse_hits"] / steps,
             "grids_computed_per_step": prof["grids_computed"] / steps,
             "grids_reused_per_step": prof["grids_reused"] / steps,
-            "continua_reused_per_step": prof["continua_reused"] / steps}
+            "continua_reused_per_step": prof["continua_reused"] / steps,
+            "eclipse_searches_bisected_per_step": prof["eclipse_bisected"] / steps}
 
 
 def plain_gpu(L, eng, models, data, precision, batched, batched_rate, passes=2):

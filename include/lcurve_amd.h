@@ -183,6 +183,10 @@ typedef struct lc_profile {
      * and continuum inputs instead of recomputed (ContKey reuse,
      * light_curve_comp.cpp:43-52, 205-268) */
     int64_t continua_reused;
+    /* of the 2 x eclipse_hits ingress/egress searches, the ones bisected:
+     * the others were ended ahead of the bisection because their face is
+     * not seen over their first bracket (calls with early stops only) */
+    double eclipse_bisected;
 } lc_profile;
 
 typedef struct lc_ctx lc_ctx;

@@ -121,7 +121,8 @@ class lc_profile(C.Structure):
                 ("flux_var_launches", C.c_int64 * 16), ("flux_var_evaluated", C.c_double * 16),
                 ("grids_computed", C.c_int64), ("grids_reused", C.c_int64),
                 ("graph_launches", C.c_int64), ("graph_captures", C.c_int64), ("graph_ms_total", C.c_double),
-                ("last_rphase", C.c_int64), ("last_chunk", C.c_int64), ("continua_reused", C.c_int64)]
+                ("last_rphase", C.c_int64), ("last_chunk", C.c_int64), ("continua_reused", C.c_int64),
+                ("eclipse_bisected", C.c_double)]
 
 
 # Every entry point include/lcurve_amd.h declares, with its signature.

@@ -1333,7 +1333,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     const bool dev_out = rq.dev_calc != nullptr;
     const size_t o_scal = 0, o_diag = o_scal + M * 8 * 8,
                  o_insts = up8(o_diag + diag_idx.size() * 8 * 8), o_status = up8(o_insts + M * sizeof(InstDev)),
-                 o_counts = up8(o_status + M * sizeof(int)), o_evald = o_counts + 8,
+                 o_counts = up8(o_status + M * sizeof(int)), o_evald = o_counts + 16,
                  o_route = o_evald + 16 * 64 * sizeof(unsigned long long),
                  o_calc = up8(o_route + (any_route ? M * 4 * sizeof(int) : 0)),
                  down_total = dev_out ? o_calc : o_calc + M * n * 8;
@@ -1562,11 +1562,13 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     std::memset(&gather, 0, sizeof(gather));
     int nzero = 0, ngather = 0;
     auto add_zero = [&](void *p, size_t b) {
+        if (nzero == MAX_REGIONS) throw EngineError(LC_E_DEVICE, "internal: too many clear regions");
         zero.dst[nzero] = p;
         zero.n[nzero++] = b;
     };
     auto add_gather = [&](size_t off, const void *src, size_t b) {
         if (!b) return;
+        if (ngather == MAX_REGIONS) throw EngineError(LC_E_DEVICE, "internal: too many gather regions");
         gather.dst[ngather] = d_down + off;
         gather.src[ngather] = src;
         gather.n[ngather++] = b;
@@ -1589,6 +1591,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     add_gather(o_status, d_status, M * sizeof(int));
     add_gather(o_counts, d_cand_count, sizeof(int));
     add_gather(o_counts + 4, ew.hit_count, sizeof(int));
+    if (any_ecl_stop) add_gather(o_counts + 8, ew.acount + ECL_ROUNDS + 1, sizeof(int));
     add_gather(o_evald, fl.evald, 16 * 64 * sizeof(unsigned long long));
     if (any_route) add_gather(o_route, d_counts, M * 4 * sizeof(int));
     const bool ecl_rounds = any_ecl_stop && 2LL * nface_total >= ctx->ecl_round_min;
@@ -1606,10 +1609,11 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         mark(ev[0]);
         launch_faces(s, d_grids, d_rings, nring, nhalf_total, fa, d_cand, d_cand_count, d_status);
         mark(ev[1]);
-        launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, ew, d_status, ecl_rounds);
+        launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, ew, d_status, ecl_rounds,
+                       any_ecl_stop);
         if (ncomp) {  // the components' eclipses by the stars, then their ranges
             launch_comp_pretest(s, d_cgrids, 2 * ncomp, 2 * NC, cs, d_ccand, d_ccount);
-            launch_eclipse(s, d_cgrids, 2 * ncomp, cs, d_ccand, d_ccount, 2 * NC, cew, d_status, false);
+            launch_eclipse(s, d_cgrids, 2 * ncomp, cs, d_ccand, d_ccount, 2 * NC, cew, d_status, false, false);
             launch_comp_build(s, d_comps, ncomp, NC, cs, ca, d_status);
         }
         // reused grids: cache hits first (a duplicate may copy from one), then
@@ -1731,6 +1735,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     const int *cnt = reinterpret_cast<const int *>(pb + o_route);  // routed sub-phase counts
     ctx->prof.eclipse_candidates += h_counts[0];
     ctx->prof.eclipse_hits += h_counts[1];
+    ctx->prof.eclipse_bisected += any_ecl_stop ? h_counts[2] : 2.0 * h_counts[1];
 
     if (any_route) {  // algorithmic work of the fine/coarse batches
         for (const BatchDev &B : batches) {
@@ -2560,7 +2565,7 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     launch_faces(s, d_grid, d_rings, (int)rings.size(), ho, fa, d_cand, d_count, d_status);
     HIPCHK(hipEventRecord(ctx->ev[1], s));
     // full searches gain nothing from rounds; LCURVE_AMD_ECL_ROUNDS_MIN=0 forces them (tests)
-    launch_eclipse(s, d_grid, 1, fa, d_cand, d_count, nf, ew, d_status, ctx->ecl_round_min <= 0);
+    launch_eclipse(s, d_grid, 1, fa, d_cand, d_count, nf, ew, d_status, ctx->ecl_round_min <= 0, false);
     HIPCHK(hipEventRecord(ctx->ev[2], s));
     HIPCHK(hipGetLastError());
     int st = 0;

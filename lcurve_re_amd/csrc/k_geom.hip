@@ -325,6 +325,64 @@ __device__ __forceinline__ bool bracket_clear(const unsigned *bits, int lb, doub
     return true;
 }
 
+// The visibility test of an early-stopping search (see k_eclipse_bisect):
+// FP32 direction, the phase of its largest mu and the threshold.
+struct SeenTest {
+    float dx, dy, dz, ph_top, thr;
+    bool never;  // never seen at any phase
+    __device__ void init(const FaceArrays &fa, int f, const GridDev &G) {
+        const double ddx = fa.dx[f], ddy = fa.dy[f], ddz = fa.dz[f];
+        thr = (float)G.mucrit - 1e-5f;
+        never = G.sini * sqrt(ddx * ddx + ddy * ddy) + ddz * G.cosi <= (double)thr;
+        ph_top = (float)(atan2(-ddy, ddx) / TWOPI);
+        dx = (float)ddx;
+        dy = (float)ddy;
+        dz = (float)ddz;
+    }
+    __device__ float mu(const V3 &e) const { return fmaf(dz, (float)e.z, fmaf(dy, (float)e.y, dx * (float)e.x)); }
+    // no phase of [min(a, b), max(a, b)] shows the face (mu_a, mu_b: mu at a, b)
+    __device__ bool unseen(double a, double b, float mu_a, float mu_b) const {
+        if (never) return true;
+        const double lo = fmin(a, b), hi = fmax(a, b);
+        if (fmaxf(mu_a, mu_b) > thr) return false;
+        return (double)ph_top + ceil(lo - (double)ph_top) > hi;  // the maximum is not inside
+    }
+};
+
+// Ahead of round 0 of calls with early stops: every search (items
+// 2*hit + side) whose first bracket [phi, phi1 or phi2] never shows its face
+// ends here with the bracket's midpoint; the others go to the list round 0
+// runs over (alist[1], count acount[ECL_ROUNDS + 1]; one atomic per wave).
+__global__ __launch_bounds__(256) void k_eclipse_open(const GridDev *__restrict__ grids, FaceArrays fa,
+                                                      EclipseWork ew) {
+    const int n = *ew.hit_count * 2;
+    for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const int i = i0 + (int)threadIdx.x;
+        bool open = false;
+        if (i < n) {
+            const int k = i >> 1, f = ew.face[k];
+            const GridDev &G = grids[ew.grid[k]];
+            open = true;
+            if (G.ecl_stop) {
+                SeenTest v;
+                v.init(fa, f, G);
+                const double a = ew.phi[k], b = ew.lim[i];
+                if (v.unseen(a, b, v.mu(set_earth(G.cosi, G.sini, a)), v.mu(set_earth(G.cosi, G.sini, b)))) {
+                    const double x = (a + b) / 2.;
+                    ew.res[i] = x - floor(x);
+                    open = false;
+                }
+            }
+        }
+        const unsigned long long m = __ballot(open);
+        const int lane = (int)(threadIdx.x & 63);
+        int base = 0;
+        if (lane == 0 && m) base = atomicAdd(&ew.acount[ECL_ROUNDS + 1], __popcll(m));
+        base = __shfl(base, 0, 64);
+        if (open) ew.alist[1][base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+    }
+}
+
 #ifndef LC_FIRST_NT
 #define LC_FIRST_NT 512  // workgroup size of the first pot_min pass (4x fewer list atomics than 128)
 #endif
@@ -339,9 +397,11 @@ __device__ __forceinline__ bool bracket_clear(const unsigned *bits, int lb, doub
 // their wave ends.
 __global__ __launch_bounds__(128, LC_ECL_WAVES) void k_eclipse_bisect(const GridDev *__restrict__ grids, int ngrid,
                                                                      FaceArrays fa, EclipseWork ew,
-                                                                     int *inst_status, int round, int nstep) {
-    const int *in = round == 0 ? nullptr : ew.alist[(round - 1) & 1];
-    const int n = round == 0 ? *ew.hit_count * 2 : ew.acount[round - 1];
+                                                                     int *inst_status, int round, int nstep,
+                                                                     int open_list) {
+    // round 0 runs over every search, or over the list k_eclipse_open left
+    const int *in = round == 0 ? (open_list ? ew.alist[1] : nullptr) : ew.alist[(round - 1) & 1];
+    const int n = round == 0 ? (open_list ? ew.acount[ECL_ROUNDS + 1] : *ew.hit_count * 2) : ew.acount[round - 1];
     int *out = ew.alist[round & 1];
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const int i = in ? in[t] : t;
@@ -354,21 +414,47 @@ __global__ __launch_bounds__(128, LC_ECL_WAVES) void k_eclipse_bisect(const Grid
         const double rref = G.rref_other, pref = G.pref_other;
         const V3 r = v3(fa.px[f], fa.py[f], fa.pz[f]);
         const double ACC = ew.acc[k];
-        double pin = in ? ew.b_in[i] : ew.phi[k], pout = in ? ew.b_out[i] : ew.lim[i];
+        double pin = round > 0 ? ew.b_in[i] : ew.phi[k], pout = round > 0 ? ew.b_out[i] : ew.lim[i];
         const unsigned *bits = G.ecl_stop ? ew.phase_bits + ((size_t)G.inst << (ew.bits_lb - 5)) : nullptr;
+        // The face's visibility mu(phase) = d.e(phase) (the same early-stop
+        // scope): where mu <= mucrit the reference's elements are zero
+        // whatever the ranges say (comp_light_flat.cpp:110-168, Point path
+        // likewise), so a search over whose bracket the face is never seen
+        // can stop with any value inside it: the eclipsed phases (the
+        // projection of [ingress, egress] onto [0, 1), which the
+        // normalisation and the egress + 1 shift represent) then differ from
+        // the reference's only inside the bracket.  mu = sini |d_xy|
+        // cos(2 pi (phase - ph_top)) + d_z cosi has one maximum per orbit,
+        // at ph_top; over a bracket without it the largest mu is at an end.
+        // Kept in FP32: the 1e-5 margin covers that rounding, the FP32 mu of
+        // the mixed flux kernels, and a maximum misplaced by the FP32 ph_top
+        // (mu is flat there: ~1e-13).
+        SeenTest vis;
+        float mu_in = 0.f, mu_out = 0.f;
+        if (bits) {
+            vis.init(fa, f, G);
+            mu_in = vis.mu(set_earth(cosi, sini, pin));
+            mu_out = vis.mu(set_earth(cosi, sini, pout));
+        }
         int st = ST_OK;
         bool done = false;
         for (int s = 0; s < nstep; s++) {
             if (!(fabs(pin - pout) > delta)) break;
             const double pmid = (pin + pout) / 2.;
+            const V3 em = set_earth(cosi, sini, pmid);
             bool h;
-            st = fblink(q, other, spin, ACC, set_earth(cosi, sini, pmid), r, rref, pref, h);
+            st = fblink(q, other, spin, ACC, em, r, rref, pref, h);
             if (st) break;
-            if (h)
+            const float mu_mid = bits ? vis.mu(em) : 0.f;
+            if (h) {
                 pin = pmid;
-            else
+                mu_in = mu_mid;
+            } else {
                 pout = pmid;
-            if (bits && bracket_clear(bits, ew.bits_lb, fmin(pin, pout), fmax(pin, pout))) {
+                mu_out = mu_mid;
+            }
+            if (bits && (bracket_clear(bits, ew.bits_lb, fmin(pin, pout), fmax(pin, pout)) ||
+                         vis.unseen(pin, pout, mu_in, mu_out))) {
                 done = true;
                 break;
             }
@@ -824,7 +910,8 @@ void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int
 }
 
 void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int2 *cand,
-                    const int *cand_count, int max_cand, EclipseWork ew, int *inst_status, bool rounds) {
+                    const int *cand_count, int max_cand, EclipseWork ew, int *inst_status, bool rounds,
+                    bool stop) {
     if (max_cand <= 0 || ngrid <= 0) return;
     const int blocks = std::min((max_cand + 127) / 128, 16384);
     hipLaunchKernelGGL((k_eclipse_potmin<true, LC_FIRST_NT>), dim3(std::min((max_cand + LC_FIRST_NT - 1) / LC_FIRST_NT, 16384)),
@@ -832,11 +919,14 @@ void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays f
     hipLaunchKernelGGL((k_eclipse_potmin<false, 128>), dim3(blocks), dim3(128), 0, s, grids, fa, cand, cand_count, ew,
                        inst_status);
     const int blocks2 = std::min((2 * max_cand + 127) / 128, 16384);
+    if (stop)
+        hipLaunchKernelGGL(k_eclipse_open, dim3(std::min((2 * max_cand + 255) / 256, 16384)), dim3(256), 0, s, grids, fa,
+                           ew);
     const int nround = rounds ? ECL_ROUNDS : 1;
     for (int round = 0; round < nround; round++) {
         const int nstep = round == nround - 1 ? (1 << 30) : (round == 0 ? ECL_FIRST_STEPS : ECL_STEPS);
         hipLaunchKernelGGL(k_eclipse_bisect, dim3(blocks2), dim3(128), 0, s, grids, ngrid, fa, ew, inst_status, round,
-                           nstep);
+                           nstep, stop ? 1 : 0);
     }
     hipLaunchKernelGGL(k_eclipse_final, dim3(std::min((max_cand + 255) / 256, 4096)), dim3(256), 0, s, fa, ew);
 }

@@ -115,7 +115,8 @@ struct EclipseWork {
     // the item lists of consecutive rounds (alternating) and their counts
     double *b_in, *b_out;
     int *alist[2];   // alist[1] first holds the candidates pot_min has to search (k_eclipse_first)
-    int *acount;     // [ECL_COUNTS]: open searches after each round, then the pot_min list count
+    int *acount;     // [ECL_COUNTS]: open searches after each round, the pot_min list count, the
+                     // searches left open by the visibility test ahead of round 0
     // per instance, a bitmap of the 2^bits_lb bins of [0, 1) that hold a
     // phase at which eclipse ranges are tested (bin = floor(phase * 2^lb))
     const unsigned *phase_bits;
@@ -126,7 +127,7 @@ struct EclipseWork {
 // ECL_FIRST_STEPS steps of every search, the second ECL_STEPS more on the
 // searches still open (compacted), the last one runs them to the end.
 constexpr int ECL_ROUNDS = 3, ECL_FIRST_STEPS = 10, ECL_STEPS = 3;
-constexpr int ECL_COUNTS = ECL_ROUNDS + 1;
+constexpr int ECL_COUNTS = ECL_ROUNDS + 2;
 
 struct DataDev {
     const double *time, *expose, *flux, *ferr, *weight;
@@ -161,8 +162,12 @@ struct FaceCopy {
 void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int nring, int nhalf_total,
                   FaceArrays fa, int2 *cand, int *cand_count, int *inst_status);
 // rounds: run the bisections in ECL_ROUNDS launches (else one)
+// stop: some grids may end their searches early (GridDev::ecl_stop); their
+// searches over a face that is never seen in the first bracket end before
+// round 0, which then runs over the list of the others.
 void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int2 *cand,
-                    const int *cand_count, int max_cand, EclipseWork ew, int *inst_status, bool rounds);
+                    const int *cand_count, int max_cand, EclipseWork ew, int *inst_status, bool rounds,
+                    bool stop);
 // Copies positions, normals, areas, gravities and eclipse ranges of whole
 // grids (not the fluxes, which the continuum kernel recomputes).
 void launch_copy_faces(hipStream_t s, const FaceCopy *items, int nitem, FaceArrays src, FaceArrays dst);

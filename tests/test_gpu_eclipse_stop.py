@@ -1,12 +1,17 @@
 """Eclipse searches that end once no tested phase can tell (GPU).
 
 The batched engine stops an ingress/egress bisection as soon as its bracket
-holds none of the phases at which that instance's eclipse ranges are ever
-compared (its sub-phases, the Ginterp phases, phase 0.5).  Every comparison
-(ecl_range, comp_light_flat.cpp:171-173) then has the same answer as with
-the reference's full bisection to delta_phase, so light curves and scalars
-must be bit-identical to an engine that always bisects to the end
-(LCURVE_AMD_FULL_ECLIPSE).
+lies within one integer and either holds none of the phases at which that
+instance's eclipse ranges are ever compared (its sub-phases, the Ginterp
+phases, phase 0.5), or spans only phases at which the face is not seen
+(mu <= mucrit, where the reference's element is zero whatever the range
+says).  Every comparison (ecl_range, comp_light_flat.cpp:171-173) then has
+the same answer as with the reference's full bisection to delta_phase, or
+multiplies a zero, so in FP64 mode light curves and scalars must be
+bit-identical to an engine that always bisects to the end
+(LCURVE_AMD_FULL_ECLIPSE).  In mixed mode a face whose stopped range puts a
+tested phase on the other side joins another of the flux kernel's compacted
+lists, which changes only the grouping of the FP32 partial sums.
 """
 import json
 
@@ -61,7 +66,7 @@ def _models():
 
 @pytest.mark.parametrize("prec", [L.LC_PREC_FP64, L.LC_PREC_MIXED])
 @pytest.mark.parametrize("nphase,expose,ndiv", [(2000, 0.0, 1), (500, 0.004, 5), (3, 0.0, 1)])
-def test_early_stop_is_bit_identical(engines, prec, nphase, expose, ndiv):
+def test_early_stop_matches_full_searches(engines, prec, nphase, expose, ndiv):
     full = engines[0]
     models = _models()
     t = np.sort(np.random.default_rng(nphase).uniform(-0.3, 1.7, nphase))
@@ -73,10 +78,20 @@ def test_early_stop_is_bit_identical(engines, prec, nphase, expose, ndiv):
     for e in engines[1:]:
         cb, ib, sb = e.light_curve_comp_batch(models, data, **kw)
         assert list(sa) == list(sb), (sa, sb)
-        assert np.array_equal(ca, cb)
-        for a, b in zip(ia, ib):
-            assert (a.chisq, a.wnok, a.wdwarf, a.logg1, a.logg2, a.rv1, a.rv2, a.sfac[0]) == \
-                   (b.chisq, b.wnok, b.wdwarf, b.logg1, b.logg2, b.rv1, b.rv2, b.sfac[0])
+        if prec == L.LC_PREC_FP64:
+            assert np.array_equal(ca, cb)
+            for a, b in zip(ia, ib):
+                assert (a.chisq, a.wnok, a.wdwarf, a.logg1, a.logg2, a.rv1, a.rv2, a.sfac[0]) == \
+                       (b.chisq, b.wnok, b.wdwarf, b.logg1, b.logg2, b.rv1, b.rv2, b.sfac[0])
+        else:
+            rel = float(np.max(np.abs(ca - cb) / np.max(np.abs(ca), axis=1, keepdims=True)))
+            print(f"   mixed: largest difference {rel:.3e} of each curve's maximum", flush=True)
+            assert rel <= 1e-7
+            for a, b in zip(ia, ib):
+                # logg and rv do not depend on the ranges; wdwarf carries the fitted scale
+                assert (a.wnok, a.logg1, a.logg2, a.rv1, a.rv2) == (b.wnok, b.logg1, b.logg2, b.rv1, b.rv2)
+                for x, y in ((a.chisq, b.chisq), (a.sfac[0], b.sfac[0]), (a.wdwarf, b.wdwarf)):
+                    assert abs(x - y) <= 1e-6 * abs(x), (x, y)
 
 
 def test_rounds_give_the_same_grid(engines):
