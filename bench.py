@@ -325,8 +325,12 @@ def roofline_from_profile(prof, steps, precision, workload, R=None):
     alg = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     if R is None:
         R = int(prof["last_rphase"])  # the shape the engine used
-    kname = (f"k_flux_{'fp64' if precision else 'mixed'}<{v & 1}, {bool(v & 2)}, {bool(v & 4)}"
-             + ("" if precision else f", {bool(v & 8)}") + f", {R}>")
+    if not precision and (v & 6) == 0 and os.environ.get("LCURVE_AMD_FLUX_WAVE", "1") != "0":
+        # no beaming, no lensing: the per-wave tile kernel
+        kname = f"k_flux_wave<{v & 1}, {bool(v & 8)}, {R}>"
+    else:
+        kname = (f"k_flux_{'fp64' if precision else 'mixed'}<{v & 1}, {bool(v & 2)}, {bool(v & 4)}"
+                 + ("" if precision else f", {bool(v & 8)}") + f", {R}>")
     c_elem = {0: 24, 1: 26, 2: 39, 3: 41}[v & 3]
     evald = prof["flux_var_evaluated"][v] / launches
     traffic = measured_traffic(workload, kname)

@@ -682,6 +682,9 @@ struct lc_ctx {
     int flux_rphase = 0;
     int flux_chunk = 0;
     int last_rphase = 1, last_chunk = 1024;
+    // mixed-precision sums without beaming or lensing: per-wave phase tiles
+    // (k_flux_wave); 0: the workgroup-tile kernel k_flux_mixed
+    int flux_wave = 1;
     bool dedupe = true;  // build grids equal within one batch once
     bool ecl_stop = true;  // eclipse searches end once no data phase can tell
     // below this many faces per call the searches are latency bound: the
@@ -1645,7 +1648,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         for (size_t k = 0; k < var_used.size(); k++) {
             const int v = var_used[k];
             if (k == 0) mark(ctx->ev_var[0]);
-            launch_flux(s, v, fp64 ? 1 : 0, R, d_tasks + var_range[v].first, var_range[v].second, d_batches, d_grids,
+            launch_flux(s, v, fp64 ? 1 : 0, R, ctx->flux_wave, d_tasks + var_range[v].first, var_range[v].second, d_batches, d_grids,
                         fl, pa, S, d_lists, d_counts, d_partial);
             mark(ctx->ev_var[k + 1]);
         }
@@ -1982,6 +1985,7 @@ lc_status lc_ctx_create(int device, lc_ctx **out) {
         if (const char *r = std::getenv("LCURVE_AMD_FLUX_R")) c->flux_rphase = std::atoi(r);
         if (c->flux_rphase != 1 && c->flux_rphase != 2 && c->flux_rphase != 4) c->flux_rphase = 0;
         if (const char *ch = std::getenv("LCURVE_AMD_FLUX_CHUNK")) c->flux_chunk = std::max(32, std::atoi(ch));
+        if (const char *fw = std::getenv("LCURVE_AMD_FLUX_WAVE")) c->flux_wave = std::atoi(fw) != 0;
         // grids kept across calls, as the reference's per-thread cache
         // (disabled by the same environment variable)
         c->geo.cap = std::getenv("LCURVE_NO_GRID_CACHE") ? 0 : DEFAULT_GEO_CACHE_FACES;
@@ -2294,7 +2298,7 @@ int flat_sum_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const lc_phase_
     const double2 *d_rng = nrange ? dev_upload(A, rng.data(), nrange, s) : nullptr;
     HIPCHK(hipMemsetAsync(d_out, 0, 2 * S * sizeof(double), s));
     HIPCHK(hipEventRecord(ctx->ev[4], s));
-    launch_flux(s, flux_variant(G), fp64 ? 1 : 0, R, d_tasks, (int)tasks.size(), d_batch, d_grid, fl, pa, (int)S,
+    launch_flux(s, flux_variant(G), fp64 ? 1 : 0, R, ctx->flux_wave, d_tasks, (int)tasks.size(), d_batch, d_grid, fl, pa, (int)S,
                 nullptr, nullptr, d_partial);
     if (nr)
         launch_flux_ragged_tail(s, flux_variant(G), d_grid, d_batch, fl, d_moff, d_rng, pa, (int)S, (int)n1, (int)nf,
@@ -2489,7 +2493,7 @@ int flat_sum_noalloc_stage(lc_ctx *ctx, int star, const lc_flat_grid *g, const l
     hipLaunchKernelGGL(k_pack_flat_dev, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, *g, star, spin,
                        VFAC, XCOFM, (int)ntask, (int)nchunk, G, B, fl.rec0, fl.rec1, fl.rec2, fl.ecl, d_tasks,
                        fl.evald, d_grid, d_batch, d_n0);
-    launch_flux(s, flux_variant(G), fp64 ? 1 : 0, R, d_tasks, (int)ntask, d_batch, d_grid, fl, pa, (int)S, nullptr,
+    launch_flux(s, flux_variant(G), fp64 ? 1 : 0, R, ctx->flux_wave, d_tasks, (int)ntask, d_batch, d_grid, fl, pa, (int)S, nullptr,
                 nullptr, d_partial);
     launch_flux_reduce(s, d_batch, 1, (int)S, TILE, nullptr, nullptr, d_partial, out);
     HIPCHK(hipGetLastError());

@@ -284,7 +284,7 @@ __device__ __forceinline__ void face_elems(const float4 *s_a0, const float4 *s_a
 // the FP64 accumulators.  n is uniform across the workgroup (it comes from
 // the LDS counters), so the loop bounds live in scalar registers and the
 // face loop is unrolled by 4 to batch the LDS reads.
-template <int LT, bool BEAM, bool GLENS, bool MUC, bool ECL, int R>
+template <int LT, bool BEAM, bool GLENS, bool MUC, bool ECL, int R, int UNR = 4>
 __device__ __forceinline__ void sweep(const float4 *s_a0, const float4 *s_a1, const float4 *s_a2, const double2 *s_io,
                                       int n_, const float *ex, const float *ey, const float *ez, const double *ph,
                                       const Coef32 &K, double *acc) {
@@ -295,9 +295,9 @@ __device__ __forceinline__ void sweep(const float4 *s_a0, const float4 *s_a1, co
         for (int r = 0; r < R; r++) a32[r] = 0.f;
         const int k1 = min(k0 + 32, n);
         int k = k0;
-        for (; k + 4 <= k1; k += 4) {
+        for (; k + UNR <= k1; k += UNR) {
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < UNR; u++)
                 face_elems<LT, BEAM, GLENS, MUC, ECL, R>(s_a0, s_a1, s_a2, s_io, k + u, ex, ey, ez, ph, K, a32);
         }
         for (; k < k1; k++)
@@ -671,6 +671,363 @@ __global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict_
         atomicAdd(fl.evald + (LT | (BEAM ? 2 : 0) | (GLENS ? 4 : 0) | (MUC ? 8 : 0)) * 64 + (blockIdx.x & 63), evc);
 }
 
+// ----------------------------------------------------------- per-wave tiles
+// k_flux_wave: the flux sums of grids without beaming or lensing (the
+// moment variants of k_flux_mixed), with the task's phase tile split
+// between its waves.  Wave w owns the R*64 consecutive sub-phases
+// [tile0 + w R 64, tile0 + (w + 1) R 64) and makes every per-tile decision
+// (cull, eclipse class, moments or direct sum) against its own centre and
+// radius, a quarter of the workgroup tile's.  A face crossing the limb then
+// costs direct elements only in the wave whose sub-phases it crosses in,
+// and the moment sums accept faces four times closer to the limb, so the
+// direct elements of a batched launch drop by about half (1/4 per tile
+// width, twice as many moment faces).  There is no workgroup barrier: each
+// wave streams the chunk's records itself (64 faces per step, one per
+// lane; the four waves of a workgroup read the same lines, which the CU's
+// cache serves), compacts the faces it sweeps into its own LDS lists and
+// queues its moment faces in an LDS ring, from which it sums them a full
+// wave at a time (Claret: 32 faces, two lanes each; polynomial law: 64).
+//
+// Moments of the Claret law: Taylor moments as in k_flux_mixed, in delta =
+// d_x dX + d_y dY, here of degree 8 with |delta| <= 0.27 mu_c (the same
+// accuracy; a ninth degree would not fit the registers of four waves per
+// SIMD).  The lower half of the wave accumulates the 25 monomials (k, j)
+// with j <= k - j, the upper half
+// the same code with d_x and d_y exchanged, which gives the mirrored
+// monomials (k, k - j); the self-mirrored ones (2j = k) are taken from the
+// lower half only.  Polynomial law: mu I(mu) = Q(mu) of degree 5, so its
+// Taylor series about mu_c = d.e_c ends at degree 5 and is exact:
+//   sum_i f_i Q(d_i.e) = sum_{k<=5, j<=k} C(k, j) M_kj dX^(k-j) dY^j,
+//   M_kj = sum_i f_i Q^(k)(mu_ci)/k! d_xi^(k-j) d_yi^j,
+// 21 FP64 moments per lane, one lane per face.  Both need the wave's
+// sub-phases to share e_z (one inclination); otherwise those faces are
+// summed directly.
+constexpr int WQ = 128;                     // moment queue entries per wave
+// Claret Taylor moments of the per-wave tiles: degree 8 with |delta| <=
+// 0.27 mu_c (remainder of the mu^(3/2) series below 2.3e-8 of the term,
+// mu^(5/2) below 9e-9), 25 monomials per lane
+constexpr int WK = 8;
+constexpr float TAY_RHO_W = 0.27f;
+constexpr int NTA = 25;
+constexpr unsigned char TA_K[NTA] = {0,1,2,2,3,3,4,4,4,5,5,5,6,6,6,6,7,7,7,7,8,8,8,8,8};
+constexpr unsigned char TA_J[NTA] = {0,0,0,1,0,1,0,1,2,0,1,2,0,1,2,3,0,1,2,3,0,1,2,3,4};
+constexpr int NPM = 21;                     // polynomial-law moments, k <= 5
+// binomial C(n, k) and Taylor coefficients of mu^(3/2), mu^(5/2):
+// C(3/2, k), C(5/2, k)
+constexpr double BIN15[TAY_K + 1] = {1., 1.5, 0.375, -0.0625, 0.0234375, -0.01171875, 0.0068359375,
+                                     -0.00439453125, 0.0030212402343750, -0.0021820068359375};
+constexpr double BIN25[TAY_K + 1] = {1., 2.5, 1.875, 0.3125, -0.0390625, 0.01171875, -0.00488281250,
+                                     0.00244140625, -0.0013732910156250, 0.0008392333984375};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// a value equal in every active lane, moved to scalar registers
+__device__ __forceinline__ double uni(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffull));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// set bits of m below this lane
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+__device__ __forceinline__ float unif(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
+// m[t] += f T_k(mu_c) u^(k-j) v^j for the 25 lower-half monomials
+// (k, j) = (TA_K, TA_J)[t], T_k = C(3/2, k) c1 mu^(3/2-k) + C(5/2, k) c3
+// mu^(5/2-k) + the integer powers' coefficients (k <= 3).  (u, v) = (d_x,
+// d_y), or (d_y, d_x) in the upper half of the wave.  The binomials are
+// dyadic fractions with short mantissas (FP64 literal operands).
+__device__ __forceinline__ void tay_add_half(const float4 &rec, bool swap, double cx, double cy, double cz,
+                                             double c1, double c2, double c3, double c4, double base,
+                                             double (&m)[NTA]) {
+    const double x = rec.x, y = rec.y, z = rec.z, f = rec.w;
+    const double mu = fma(z, cz, fma(y, cy, x * cx));
+    double q = __builtin_amdgcn_rsq(mu);
+    q = q * fma(-0.5 * mu, q * q, 1.5);
+    const double r = q * q;
+    const double u = swap ? y : x, v = swap ? x : y;
+    // u^(k-j) v^j = u^k (v/u)^j: one ratio instead of two power tables
+    // (registers); with u = 0 every monomial of the half but k = 0 is zero
+    double ru = __builtin_amdgcn_rcp(u);
+    ru = ru * fma(-u, ru, 2.0);
+    const double wr = u == 0.0 ? 0.0 : v * ru;
+    const double fm15 = f * mu * (mu * q);  // f mu^(3/2)
+    double s1 = fm15 * c1, s3 = fm15 * mu * c3;  // f c1 mu^(3/2 - k), f c3 mu^(5/2 - k)
+    double uk = 1.0;  // u^k
+#pragma unroll
+    for (int k = 0; k <= WK; k++) {
+        double T = fma(BIN15[k], s1, BIN25[k] * s3);
+        if (k == 0) T = fma(f * mu, fma(mu, fma(mu, c4, c2), base), T);
+        if (k == 1) T = fma(f, fma(mu, fma(3.0 * c4, mu, 2.0 * c2), base), T);
+        if (k == 2) T = fma(f, fma(3.0 * c4, mu, c2), T);
+        if (k == 3) T = fma(f, c4, T);
+        double t = k == 0 ? T : T * uk;
+#pragma unroll
+        for (int tt = 0; tt < NTA; tt++) {
+            if (TA_K[tt] == k) {
+                if (TA_J[tt] > 0) t *= wr;
+                m[tt] += t;
+            }
+        }
+        uk *= u;
+        s1 *= r;
+        s3 *= r;
+    }
+}
+
+// m[k(k+1)/2 + j] += f Q^(k)(mu_c)/k! d_x^(k-j) d_y^j, Q(mu) = sum_n q_n mu^n
+// (n = 1..5, qc[n-1]); the Taylor shift by synthetic division.
+__device__ __forceinline__ void poly_add(const float4 &rec, double cx, double cy, double cz, const double *qc,
+                                         double (&m)[NPM]) {
+    const double x = rec.x, y = rec.y, z = rec.z, f = rec.w;
+    const double mu = fma(z, cz, fma(y, cy, x * cx));
+    double b[6] = {0., qc[0], qc[1], qc[2], qc[3], qc[4]};
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+#pragma unroll
+        for (int k = 4; k >= i; k--) b[k] = fma(mu, b[k + 1], b[k]);
+    double up[6], vp[6];
+    up[0] = f;
+    vp[0] = 1.0;
+    vp[1] = y;
+#pragma unroll
+    for (int i = 1; i < 6; i++) up[i] = up[i - 1] * x;
+#pragma unroll
+    for (int i = 2; i < 6; i++) vp[i] = vp[i - 1] * y;
+#pragma unroll
+    for (int k = 0; k <= 5; k++)
+#pragma unroll
+        for (int j = 0; j <= k; j++) {
+            const int t = k * (k + 1) / 2 + j;
+            m[t] = j == 0 ? fma(b[k], up[k], m[t]) : fma(b[k] * vp[j], up[k - j], m[t]);
+        }
+}
+
+template <int LT, bool MUC, int R>
+__global__ __launch_bounds__(NT, 4) void k_flux_wave(const TaskDev *__restrict__ tasks,
+                                                  const BatchDev *__restrict__ batches,
+                                                  const GridDev *__restrict__ grids, FlatArrays fl, PhaseArrays pa,
+                                                  int S, const int *__restrict__ lists,
+                                                  const int *__restrict__ counts, double *__restrict__ partial) {
+    constexpr int NW = NT / 64;
+    constexpr int NMW = LT == 1 ? NTA : NPM;    // moments per lane
+    constexpr int QB = LT == 1 ? 32 : 64;       // moment faces per queue pass
+    // faces per unrolled sweep step: the Claret moments hold 50 registers
+    // through the face loop, so its sweeps batch fewer LDS reads
+    constexpr int SWU = LT == 1 && R > 1 ? 2 : 4;
+    __shared__ float4 s_d0[NW][64];             // direct faces, no eclipse edge in the wave's tile
+    __shared__ float4 s_d1[NW][64];             // faces whose eclipse starts or ends inside it
+    __shared__ double2 s_io[NW][64];            // their ranges
+    __shared__ float4 s_q[NW][WQ];              // moment queue (ring)
+    __shared__ double s_cf[NW][NTAY];           // reduced moments x binomials
+
+    // wave-uniform values are kept in scalar registers (readfirstlane)
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63);
+    const TaskDev T = tasks[blockIdx.x];
+    const BatchDev B = batches[T.batch];
+    const GridDev &G = grids[B.grid];
+    const int count = B.count_idx < 0 ? S : counts[B.count_idx];
+    const int TILE = R * NT;
+    const int sub0 = T.tile * TILE + w * (R * 64);  // this wave's first element
+    if (sub0 >= count) return;                      // no workgroup barrier below
+    const int live = min(count - sub0, R * 64);
+
+    // this wave's sub-phases: element sub0 + r*64 + lane
+    float ex[R], ey[R], ez[R];
+    double ph[R];
+    bool ok[R];
+    TileInfo ti;
+    {
+        const int em = sub0 + (live - 1) / 2;
+        const int jm = B.list_off < 0 ? em : lists[B.list_off + em];
+        const size_t im = (size_t)B.inst * S + jm;
+        ti.cx = uni(pa.ex[im]);
+        ti.cy = uni(pa.ey[im]);
+        ti.cz = uni(pa.ez[im]);
+        float rad = 0.f;
+        double pmin = 2.0, pmax = -1.0;
+        bool zd = false;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int el = sub0 + r * 64 + lane;
+            ok[r] = el < count;
+            const int j = ok[r] ? (B.list_off < 0 ? el : lists[B.list_off + el]) : 0;
+            const size_t idx = (size_t)B.inst * S + j;
+            const double dx = ok[r] ? pa.ex[idx] : 0., dy = ok[r] ? pa.ey[idx] : 0., dz = ok[r] ? pa.ez[idx] : 0.;
+            ph[r] = ok[r] ? pa.phin[idx] : 0.5;
+            ex[r] = (float)dx;
+            ey[r] = (float)dy;
+            ez[r] = (float)dz;
+            if (ok[r]) {
+                const double a = dx - ti.cx, b = dy - ti.cy, c = dz - ti.cz;
+                rad = fmaxf(rad, (float)sqrt(a * a + b * b + c * c));
+                pmin = fmin(pmin, ph[r]);
+                pmax = fmax(pmax, ph[r]);
+                zd = zd || dz != ti.cz;
+            }
+        }
+        ti.rad = unif(wave_max(rad));
+        ti.pmin = uni(wave_min_d(pmin));
+        ti.pmax = uni(wave_max_d(pmax));
+        ti.zflat = __ballot(zd) == 0ull;
+    }
+    Coef32 K;
+    K.c1 = (float)G.ldc[0];
+    K.c2 = (float)G.ldc[1];
+    K.c3 = (float)G.ldc[2];
+    K.c4 = (float)G.ldc[3];
+    K.base = (float)(1.0 - (G.ldc[0] + G.ldc[1] + G.ldc[2] + G.ldc[3]));
+    const double c1 = G.ldc[0], c2 = G.ldc[1], c3 = G.ldc[2], c4 = G.ldc[3];
+    const double base = uni(1.0 - (c1 + c2 + c3 + c4));
+    // mu I(mu) = mu - sum_j c_j mu (1 - mu)^j expanded in powers of mu
+    const double qc[5] = {base, uni(c1 + 2.0 * c2 + 3.0 * c3 + 4.0 * c4), uni(-(c2 + 3.0 * c3 + 6.0 * c4)),
+                          uni(c3 + 4.0 * c4), -c4};
+    K.q1 = (float)qc[0];
+    K.q2 = (float)qc[1];
+    K.q3 = (float)qc[2];
+    K.q4 = (float)qc[3];
+    K.q5 = (float)qc[4];
+    K.mucrit = (float)G.mucrit;
+    K.beam = 0.f;
+    K.rlens = 0.f;
+    const float ecx = (float)ti.cx, ecy = (float)ti.cy, ecz = (float)ti.cz;
+    const float cull = unif((float)G.mucrit - 1.e-5f - ti.rad);
+    const float full_cut = unif(fmaxf(0.f, (float)G.mucrit) + 1.e-5f + ti.rad);
+    // the polynomial law's moments need one inclination in the tile
+    const bool pmom = LT == 0 && ti.zflat;
+
+    const int n0 = fl.n0[B.grid];
+    const int f_lo = T.chunk * B.chunk;
+    const int f_hi = min(G.nface, f_lo + B.chunk);
+    const float4 *__restrict__ rec0 = fl.rec0 + G.face_off;
+    const double2 *__restrict__ eclr = fl.ecl + G.face_off;
+
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0.;
+    double mom[NMW];
+#pragma unroll
+    for (int t = 0; t < NMW; t++) mom[t] = 0.;
+    unsigned long long evc = 0;
+    int qhead = 0, qn = 0;
+
+    auto moment_pass = [&](int nq) {  // nq <= QB queued faces from qhead
+        if constexpr (LT == 1) {
+            const int fi = lane & 31;
+            if (fi < nq)
+                tay_add_half(s_q[w][(qhead + fi) & (WQ - 1)], lane >= 32, ti.cx, ti.cy, ti.cz, c1, c2, c3, c4, base,
+                             mom);
+        } else {
+            if (lane < nq) poly_add(s_q[w][(qhead + lane) & (WQ - 1)], ti.cx, ti.cy, ti.cz, qc, mom);
+        }
+    };
+
+    int f = f_lo + lane;
+    const int flast = f_hi - 1;
+    float4 pa0 = rec0[min(f, flast)];
+    double2 pio = f >= n0 ? eclr[min(f, flast)] : make_double2(0., 0.);
+    for (int fb = f_lo; fb < f_hi; fb += 64) {
+        const float4 a0 = pa0;
+        const double2 io = pio;
+        const bool in = f < f_hi;
+        const bool ecl = f >= n0;
+        const int fn = f + 64;
+        if (fb + 64 < f_hi) {  // prefetch the next step
+            pa0 = rec0[min(fn, flast)];
+            pio = fn >= n0 ? eclr[min(fn, flast)] : make_double2(0., 0.);
+        }
+        const FaceDecision fd = decide_face(a0, io, in, ecl, ecx, ecy, ecz, cull, full_cut, ti, true);
+        const bool tomom =
+            fd.full && (LT == 1 ? ti.zflat && ti.rad * ti.rad * fmaf(a0.y, a0.y, a0.x * a0.x) <=
+                                                  (TAY_RHO_W * TAY_RHO_W) * (fd.dc * fd.dc)
+                                : pmom);
+        const bool d0 = fd.act && fd.cls == 0 && !tomom, d1 = fd.act && fd.cls == 1;
+        const unsigned long long b0 = __ballot(d0), b1 = __ballot(d1), bm = __ballot(tomom);
+        wave_sync();  // the previous step's list reads are done
+        if (d0) s_d0[w][lanes_below(b0)] = a0;
+        if (d1) {
+            const int k = lanes_below(b1);
+            s_d1[w][k] = a0;
+            s_io[w][k] = io;
+        }
+        if (tomom) s_q[w][(qhead + qn + lanes_below(bm)) & (WQ - 1)] = a0;
+        wave_sync();
+        const int n_d0 = __popcll(b0), n_d1 = __popcll(b1);
+        qn += __popcll(bm);
+        evc += (unsigned long long)(n_d0 + n_d1) * (unsigned long long)live;
+        sweep<LT, false, false, MUC, false, R, SWU>(s_d0[w], s_d0[w], s_d0[w], s_io[w], n_d0, ex, ey, ez, ph, K, acc);
+        if (n_d1)
+            sweep<LT, false, false, MUC, true, R, SWU>(s_d1[w], s_d1[w], s_d1[w], s_io[w], n_d1, ex, ey, ez, ph, K, acc);
+        while (qn >= QB) {
+            moment_pass(QB);
+            qhead = (qhead + QB) & (WQ - 1);
+            qn -= QB;
+        }
+        f = fn;
+    }
+    if (qn > 0) moment_pass(qn);
+
+    // moments summed over the lanes (Claret: each half of the wave), times
+    // the binomials, into this wave's coefficient table; then every
+    // sub-phase evaluates the polynomial in (dX, dY)
+    if constexpr (LT == 1) {
+#pragma unroll
+        for (int t = 0; t < NTA; t++) {
+            double v = mom[t];
+            for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            const int k = TA_K[t], j = TA_J[t];
+            const double c = v * c_tay_binom[k * (k + 1) / 2 + j];
+            if (lane == 0) s_cf[w][k * (k + 1) / 2 + j] = c;
+            if (lane == 32 && 2 * j != k) s_cf[w][k * (k + 1) / 2 + (k - j)] = c;
+        }
+    } else {
+        if (pmom) {
+#pragma unroll
+            for (int t = 0; t < NPM; t++) {
+                double v = mom[t];
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                if (lane == 0) s_cf[w][t] = v * c_tay_binom[t];
+            }
+        }
+    }
+    wave_sync();
+    if (LT == 1 || pmom) {
+        constexpr int KMAX = LT == 1 ? WK : 5;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            // the FP64 direction is loaded again (registers)
+            const int el = sub0 + r * 64 + lane;
+            const int j = ok[r] ? (B.list_off < 0 ? el : lists[B.list_off + el]) : 0;
+            const size_t idx = (size_t)B.inst * S + j;
+            const double X = ok[r] ? pa.ex[idx] - ti.cx : 0., Y = ok[r] ? pa.ey[idx] - ti.cy : 0.;
+            // sum_k sum_j cf[k, j] X^(k-j) Y^j = sum_a X^a Q_a(Y),
+            // Q_a(Y) = sum_b cf[a + b, b] Y^b: Horner in Y inside Horner in X
+            double e = 0.;
+#pragma unroll
+            for (int a = KMAX; a >= 0; a--) {
+                double qa = 0.;
+#pragma unroll
+                for (int b = KMAX - a; b >= 0; b--) qa = fma(qa, Y, s_cf[w][(a + b) * (a + b + 1) / 2 + b]);
+                e = fma(e, X, qa);
+            }
+            acc[r] += e;
+        }
+    }
+    double *out = partial + B.part_off + (size_t)T.chunk * (size_t)(B.ntile * TILE);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int el = sub0 + r * 64 + lane;
+        if (ok[r]) out[el] = acc[r];
+    }
+    if (lane == 0 && evc) atomicAdd(fl.evald + (LT | (MUC ? 8 : 0)) * 64 + (blockIdx.x & 63), evc);
+}
+
 // FP64 element in the reference's operand order (comp_light_flat.cpp:81-168);
 // zero when the reference's masks make it zero.
 template <int LT, bool BEAM, bool GLENS>
@@ -898,9 +1255,28 @@ __global__ void k_flux_reduce(const BatchDev *__restrict__ batches, int S, int t
 }
 
 template <int R>
-void launch_variant(hipStream_t s, int variant, int fp64, const TaskDev *tasks, int ntask, const BatchDev *batches,
-                    const GridDev *grids, FlatArrays fl, PhaseArrays pa, int S, const int *lists, const int *counts,
-                    double *partial) {
+void launch_variant(hipStream_t s, int variant, int fp64, int wave, const TaskDev *tasks, int ntask,
+                    const BatchDev *batches, const GridDev *grids, FlatArrays fl, PhaseArrays pa, int S,
+                    const int *lists, const int *counts, double *partial) {
+    if (!fp64 && wave && (variant & 6) == 0) {  // no beaming, no lensing: per-wave tiles
+        const bool mc = (variant & 8) != 0;
+        if (variant & 1) {
+            if (mc)
+                hipLaunchKernelGGL((k_flux_wave<1, true, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches, grids, fl,
+                                   pa, S, lists, counts, partial);
+            else
+                hipLaunchKernelGGL((k_flux_wave<1, false, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches, grids, fl,
+                                   pa, S, lists, counts, partial);
+        } else {
+            if (mc)
+                hipLaunchKernelGGL((k_flux_wave<0, true, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches, grids, fl,
+                                   pa, S, lists, counts, partial);
+            else
+                hipLaunchKernelGGL((k_flux_wave<0, false, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches, grids, fl,
+                                   pa, S, lists, counts, partial);
+        }
+        return;
+    }
 #define LC_FLUX_CASE(V, LT, BM, GL)                                                                                 \
     case V:                                                                                                         \
         if (fp64)                                                                                                   \
@@ -931,16 +1307,16 @@ void launch_variant(hipStream_t s, int variant, int fp64, const TaskDev *tasks, 
 }  // namespace
 
 // variant = ltype | beam << 1 | glens << 2 | (mucrit > 0) << 3; rphase in {1, 2, 4}
-void launch_flux(hipStream_t s, int variant, int fp64, int rphase, const TaskDev *tasks, int ntask,
+void launch_flux(hipStream_t s, int variant, int fp64, int rphase, int wave, const TaskDev *tasks, int ntask,
                  const BatchDev *batches, const GridDev *grids, FlatArrays fl, PhaseArrays pa, int S,
                  const int *lists, const int *counts, double *partial) {
     if (ntask <= 0) return;
     if (rphase == 1)
-        launch_variant<1>(s, variant, fp64, tasks, ntask, batches, grids, fl, pa, S, lists, counts, partial);
+        launch_variant<1>(s, variant, fp64, wave, tasks, ntask, batches, grids, fl, pa, S, lists, counts, partial);
     else if (rphase == 4)
-        launch_variant<4>(s, variant, fp64, tasks, ntask, batches, grids, fl, pa, S, lists, counts, partial);
+        launch_variant<4>(s, variant, fp64, wave, tasks, ntask, batches, grids, fl, pa, S, lists, counts, partial);
     else
-        launch_variant<2>(s, variant, fp64, tasks, ntask, batches, grids, fl, pa, S, lists, counts, partial);
+        launch_variant<2>(s, variant, fp64, wave, tasks, ntask, batches, grids, fl, pa, S, lists, counts, partial);
 }
 
 __global__ __launch_bounds__(256) void k_make_tasks(const int *__restrict__ first, const int *__restrict__ order,

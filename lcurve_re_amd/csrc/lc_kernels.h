@@ -233,7 +233,9 @@ void launch_phase_bits(hipStream_t s, const InstDev *insts, int ninst, DataDev d
 void launch_route(hipStream_t s, const InstDev *insts, int ninst, int S, PhaseArrays pa, int *lists, int *counts);
 // variant = ltype | beaming << 1 | lensing << 2 | (mucrit > 0) << 3;
 // rphase = sub-phases per lane
-void launch_flux(hipStream_t s, int variant, int fp64, int rphase, const TaskDev *tasks, int ntask,
+// wave: mixed-precision variants without beaming or lensing run k_flux_wave
+// (per-wave phase tiles) instead of k_flux_mixed.
+void launch_flux(hipStream_t s, int variant, int fp64, int rphase, int wave, const TaskDev *tasks, int ntask,
                  const BatchDev *batches, const GridDev *grids, FlatArrays fl, PhaseArrays pa, int S,
                  const int *lists, const int *counts, double *partial);
 // Expands the task table on the device: the tasks of batch order[k] are
