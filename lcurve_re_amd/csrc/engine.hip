@@ -2103,7 +2103,7 @@ lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const l
         double best = time_of(r0, c0) / 1.02;
         int br = ctx->last_rphase, bc = ctx->last_chunk;
         for (int r : {1, 2, 4})
-            for (int c : {256, 512, 1024, 2048}) {
+            for (int c : {256, 512, 1024, 2048, 4096, 8192}) {
                 if (r == br && c == bc) continue;
                 const double ms = time_of(r, c);
                 if (ms < best) {

@@ -809,7 +809,7 @@ __device__ __forceinline__ void poly_add(const float4 &rec, double cx, double cy
 }
 
 template <int LT, bool MUC, int R>
-__global__ __launch_bounds__(NT, 4) void k_flux_wave(const TaskDev *__restrict__ tasks,
+__global__ __launch_bounds__(NT, 4) void k_flux_wave(const TaskDev *__restrict__ tasks, int ntask,
                                                   const BatchDev *__restrict__ batches,
                                                   const GridDev *__restrict__ grids, FlatArrays fl, PhaseArrays pa,
                                                   int S, const int *__restrict__ lists,
@@ -829,7 +829,15 @@ __global__ __launch_bounds__(NT, 4) void k_flux_wave(const TaskDev *__restrict__
     // wave-uniform values are kept in scalar registers (readfirstlane)
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63);
-    const TaskDev T = tasks[blockIdx.x];
+    // XCD-aware order: workgroups are dispatched to the 8 XCDs in turn
+    // (block b to XCD b % 8), so XCD x takes the x-th eighth of the task
+    // table in order; tasks are chunk-major (the tiles of one face chunk are
+    // consecutive), so one XCD runs a chunk's tiles back to back and its L2
+    // serves the chunk's records to all of them
+    const int per_xcd = (ntask + 7) >> 3;
+    const int ti_ = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (ti_ >= ntask) return;
+    const TaskDev T = tasks[ti_];
     const BatchDev B = batches[T.batch];
     const GridDev &G = grids[B.grid];
     const int count = B.count_idx < 0 ? S : counts[B.count_idx];
@@ -1260,20 +1268,21 @@ void launch_variant(hipStream_t s, int variant, int fp64, int wave, const TaskDe
                     const int *lists, const int *counts, double *partial) {
     if (!fp64 && wave && (variant & 6) == 0) {  // no beaming, no lensing: per-wave tiles
         const bool mc = (variant & 8) != 0;
+        const int grid8 = ((ntask + 7) >> 3) << 3;  // a whole number of blocks per XCD
         if (variant & 1) {
             if (mc)
-                hipLaunchKernelGGL((k_flux_wave<1, true, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches, grids, fl,
-                                   pa, S, lists, counts, partial);
+                hipLaunchKernelGGL((k_flux_wave<1, true, R>), dim3(grid8), dim3(NT), 0, s, tasks, ntask, batches, grids,
+                                   fl, pa, S, lists, counts, partial);
             else
-                hipLaunchKernelGGL((k_flux_wave<1, false, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches, grids, fl,
-                                   pa, S, lists, counts, partial);
+                hipLaunchKernelGGL((k_flux_wave<1, false, R>), dim3(grid8), dim3(NT), 0, s, tasks, ntask, batches, grids,
+                                   fl, pa, S, lists, counts, partial);
         } else {
             if (mc)
-                hipLaunchKernelGGL((k_flux_wave<0, true, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches, grids, fl,
-                                   pa, S, lists, counts, partial);
+                hipLaunchKernelGGL((k_flux_wave<0, true, R>), dim3(grid8), dim3(NT), 0, s, tasks, ntask, batches, grids,
+                                   fl, pa, S, lists, counts, partial);
             else
-                hipLaunchKernelGGL((k_flux_wave<0, false, R>), dim3(ntask), dim3(NT), 0, s, tasks, batches, grids, fl,
-                                   pa, S, lists, counts, partial);
+                hipLaunchKernelGGL((k_flux_wave<0, false, R>), dim3(grid8), dim3(NT), 0, s, tasks, ntask, batches, grids,
+                                   fl, pa, S, lists, counts, partial);
         }
         return;
     }
@@ -1332,11 +1341,11 @@ __global__ __launch_bounds__(256) void k_make_tasks(const int *__restrict__ firs
         else
             hi = mid - 1;
     }
-    const int b = order[lo], local = t - first[lo], nchunk = batches[b].nchunk;
+    const int b = order[lo], local = t - first[lo], ntile = batches[b].ntile;
     TaskDev T;
     T.batch = b;
-    T.tile = local / nchunk;
-    T.chunk = local - T.tile * nchunk;
+    T.chunk = local / ntile;  // chunk-major: a chunk's tiles are consecutive
+    T.tile = local - T.chunk * ntile;
     T.pad = 0;
     tasks[t] = T;
 }

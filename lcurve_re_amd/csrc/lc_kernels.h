@@ -239,8 +239,8 @@ void launch_flux(hipStream_t s, int variant, int fp64, int rphase, int wave, con
                  const BatchDev *batches, const GridDev *grids, FlatArrays fl, PhaseArrays pa, int S,
                  const int *lists, const int *counts, double *partial);
 // Expands the task table on the device: the tasks of batch order[k] are
-// first[k] .. first[k+1]-1 (first[nb] = ntask), numbered tile-major
-// (tile * nchunk + chunk).
+// first[k] .. first[k+1]-1 (first[nb] = ntask), numbered chunk-major
+// (chunk * ntile + tile).
 void launch_make_tasks(hipStream_t s, const int *first, const int *order, int nb, const BatchDev *batches,
                        int ntask, TaskDev *tasks);
 void launch_flux_reduce(hipStream_t s, const BatchDev *batches, int nbatch, int S, int threads_per_tile,
