@@ -617,16 +617,21 @@ __global__ __launch_bounds__(256) void k_continuum(const GridDev *__restrict__ g
     const double ro = G.r_cont_other;
     double temp, geom;
     const double g = (double)fa.grav[f];
+    // x^4 as two squarings and x^(1/4) as two square roots instead of pow
+    // (set_star_continuum.cpp:172-192, 271-290 call pow): each differs from pow by
+    // an ulp or so, and the two pow calls were half the kernel's time
+    const double tg = t * pow(g, G.gdcbol);
+    const double tg2 = tg * tg, ti2 = G.t_irr * G.t_irr;
     if (mu >= ro) {
         geom = sqr(ro / r) * mu;
-        temp = pow(pow(t * pow(g, G.gdcbol), 4.0) + G.absorb * pow(G.t_irr, 4.0) * geom, 0.25);
+        temp = sqrt(sqrt(tg2 * tg2 + G.absorb * (ti2 * ti2) * geom));
     } else if (mu > -ro) {
         double x0 = -mu / ro;
         geom = sqr(ro / r) * ro * (sqrt(1. - x0 * x0) * (2 + x0 * x0) / 3 - x0 * (PI / 2 - asin(x0))) / PI;
-        temp = pow(pow(t * pow(g, G.gdcbol), 4.0) + G.absorb * pow(G.t_irr, 4.0) * geom, 0.25);
+        temp = sqrt(sqrt(tg2 * tg2 + G.absorb * (ti2 * ti2) * geom));
     } else {
         geom = 0.;
-        temp = t * pow(g, G.gdcbol);
+        temp = tg;
     }
     const double area = (double)fa.area[f];
     float flux = (float)(area * planck(G.wave, temp));
