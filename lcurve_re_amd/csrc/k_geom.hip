@@ -701,9 +701,16 @@ __global__ __launch_bounds__(1024) void k_flat_build(const GridDev *__restrict__
             const int pos = plain ? before0 + done0 + ex : n0 + before1 + done1 + ((int)threadIdx.x - ex);
             const int f = base + i, p = base + pos;
             double dx = fa.dx[f], dy = fa.dy[f], dz = fa.dz[f];
-            double px = fa.px[f], py = fa.py[f], pz = fa.pz[f];
             float flux = fa.flux[f];
             fl.rec0[p] = make_float4((float)dx, (float)dy, (float)dz, flux);
+            // positions only where a record needs them (a third of the reads)
+            const bool need_pos = G.beam != 0.0 || G.glens || fp64;
+            double px = 0., py = 0., pz = 0.;
+            if (need_pos) {
+                px = fa.px[f];
+                py = fa.py[f];
+                pz = fa.pz[f];
+            }
             if (G.beam != 0.0) {  // read only by the beaming variants of the flux kernel
                 double vx = -G.VFAC * G.spin_beam * py;
                 double vy = G.star == PRIMARY ? G.VFAC * (G.spin_beam * px - G.XCOFM)
