@@ -684,7 +684,8 @@ __global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict_
 // width, twice as many moment faces).  There is no workgroup barrier: each
 // wave streams the chunk's records itself (64 faces per step, one per
 // lane; the four waves of a workgroup read the same lines, which the CU's
-// cache serves), compacts the faces it sweeps into its own LDS lists and
+// cache serves, and the XCD-aware block order below keeps a chunk's phase
+// tiles on one XCD's L2), compacts the faces it sweeps into its own LDS lists and
 // queues its moment faces in an LDS ring, from which it sums them a full
 // wave at a time (Claret: 32 faces, two lanes each; polynomial law: 64).
 //
@@ -692,10 +693,9 @@ __global__ __launch_bounds__(NT, 4) void k_flux_mixed(const TaskDev *__restrict_
 // d_x dX + d_y dY, here of degree 8 with |delta| <= 0.27 mu_c (the same
 // accuracy; a ninth degree would not fit the registers of four waves per
 // SIMD).  The lower half of the wave accumulates the 25 monomials (k, j)
-// with j <= k - j, the upper half
-// the same code with d_x and d_y exchanged, which gives the mirrored
-// monomials (k, k - j); the self-mirrored ones (2j = k) are taken from the
-// lower half only.  Polynomial law: mu I(mu) = Q(mu) of degree 5, so its
+// with j <= k - j, the upper half runs the same code with d_x and d_y
+// exchanged, which gives the mirrored monomials (k, k - j); the
+// self-mirrored ones (2j = k) are taken from the lower half only.  Polynomial law: mu I(mu) = Q(mu) of degree 5, so its
 // Taylor series about mu_c = d.e_c ends at degree 5 and is exact:
 //   sum_i f_i Q(d_i.e) = sum_{k<=5, j<=k} C(k, j) M_kj dX^(k-j) dY^j,
 //   M_kj = sum_i f_i Q^(k)(mu_ci)/k! d_xi^(k-j) d_yi^j,
