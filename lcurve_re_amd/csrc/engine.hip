@@ -682,6 +682,7 @@ struct lc_ctx {
     int flux_rphase = 0;
     int flux_chunk = 0;
     int last_rphase = 1, last_chunk = 1024;
+    long long last_part = 0;  // the last call's chunk partial sums (doubles)
     // mixed-precision sums without beaming or lensing: per-wave phase tiles
     // (k_flux_wave); 0: the workgroup-tile kernel k_flux_mixed
     int flux_wave = 1;
@@ -1138,7 +1139,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     int chunk = fp64 ? 1024 : ctx->flux_chunk;
     if (chunk <= 0) {  // the largest face chunk that still gives every CU several tasks
         chunk = 2048;
-        while (chunk > 256 && count_tasks(chunk) < 2048) chunk /= 2;
+        while (chunk > 64 && count_tasks(chunk) < 2048) chunk /= 2;
     }
     ctx->last_rphase = R;
     ctx->last_chunk = chunk;
@@ -1308,6 +1309,7 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
     }
     sz.add<double>(SM * 2);  // out
     sz.add<double>((size_t)std::max<long long>(part_total, 1));
+    ctx->last_part = part_total;
     sz.add<BatchDev>(std::max<size_t>(batches.size(), 1));
     sz.add<TaskDev>(std::max(ntask, 1));
     sz.add<int>(task_first.size());
@@ -2102,9 +2104,14 @@ lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const l
         // the current shape is reported as what it resolved to here
         double best = time_of(r0, c0) / 1.02;
         int br = ctx->last_rphase, bc = ctx->last_chunk;
+        // the partial sums grow as 1/chunk: chunks below 256 faces pay off
+        // only for small calls, so a candidate whose partial buffer would
+        // pass 4 GiB is not tried (the arena keeps its largest size)
+        const double part_bytes_x_chunk = 8.0 * (double)ctx->last_part * (double)bc;
         for (int r : {1, 2, 4})
-            for (int c : {256, 512, 1024, 2048, 4096, 8192}) {
+            for (int c : {64, 128, 256, 512, 1024, 2048, 4096, 8192}) {
                 if (r == br && c == bc) continue;
+                if (part_bytes_x_chunk / c > 4.0 * 1024 * 1024 * 1024) continue;
                 const double ms = time_of(r, c);
                 if (ms < best) {
                     best = ms;

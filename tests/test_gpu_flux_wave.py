@@ -4,7 +4,7 @@ engine, against the workgroup-tile kernel (k_flux_mixed, selected with
 LCURVE_AMD_FLUX_WAVE=0 when a context is created) and against the FP64
 oracle, over the tile shapes the autotuner chooses from (sub-phases per
 lane R = 1, 2, 4: wave tiles of 64, 128, 256 sub-phases; face chunks of
-256 and 4096).  Batched draws of the batched workload's prior with both
+64, 256 and 4096).  Batched draws of the batched workload's prior with both
 limb-darkening laws on both stars and a mu cut-off, and a flat sum whose
 sub-phases mix two inclinations (the moment sums need one inclination per
 wave tile; the other faces are summed directly).
@@ -79,7 +79,7 @@ def batch():
 
 
 @pytest.mark.parametrize("R", [1, 2, 4])
-@pytest.mark.parametrize("chunk", [256, 4096])
+@pytest.mark.parametrize("chunk", [64, 256, 4096])
 def test_wave_tiles_vs_fp64_path(batch, R, chunk):
     import lcurve_re_amd as L
     models, data, fp64 = batch
@@ -101,7 +101,7 @@ def test_wave_tiles_vs_workgroup_tiles_and_oracle(batch):
     models, data, fp64 = batch
     got = {}
     for wave in ("1", "0"):
-        eng = _engine({"LCURVE_AMD_FLUX_WAVE": wave})
+        eng = _engine({"LCURVE_AMD_FLUX_WAVE": wave, "LCURVE_AMD_FLUX_CHUNK": "128"})
         try:
             calc, _, st = eng.light_curve_comp_batch(models, data, scale=False, rdata=True,
                                                      precision=L.LC_PREC_MIXED)
