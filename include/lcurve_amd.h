@@ -217,13 +217,15 @@ lc_status lc_ctx_set_stage_cache(lc_ctx *ctx, size_t max_faces);
 lc_status lc_ctx_set_graphs(lc_ctx *ctx, int on);
 /* No reference counterpart.  Autotuner of the flux kernel's tile shape:
  * runs the given workload with every (sub-phases per lane R in {1, 2, 4})
- * x (face chunk in {256, 512, 1024, 2048}) and keeps for this ctx the pair with
- * the shortest flux-stage time (reported through best_rphase / best_chunk
- * when not NULL); a pair must beat the current shape by 2%.  The default
- * is chosen per call: R = 2 when a grid has beaming, else 1, and the
- * largest chunk of 2048, 1024, 512, 256 that still makes 2048 flux tasks; when
- * it wins, the shape it resolved to on this workload is kept and reported.
- * FP64 mode always uses 1024-face chunks. */
+ * x (face chunk in {64, 128, ..., 8192}; chunks whose partial-sum buffer
+ * would pass 4 GiB are skipped) and keeps for this ctx the pair with the
+ * shortest time of the flux sums and the merge of their chunk partials
+ * (reported through best_rphase / best_chunk when not NULL); a pair must
+ * beat the current shape by 2%.  The default is chosen per call: R = 2
+ * when a grid has beaming, else 1, and the largest chunk of 2048, 1024,
+ * ..., 64 that still makes 2048 flux tasks; when it wins, the shape it
+ * resolved to on this workload is kept and reported.  FP64 mode always
+ * uses 1024-face chunks. */
 lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const lc_datum *data, size_t n,
                           const lc_eval_opts *opts, int32_t *best_rphase, int32_t *best_chunk);
 /* Library version string; usable without a GPU. */

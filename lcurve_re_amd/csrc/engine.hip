@@ -692,6 +692,19 @@ struct lc_ctx {
     // longest one sets the time and stopping others early saves nothing
     long long ecl_stop_min_faces = 1LL << 17;
     long long ecl_round_min = 1LL << 21;  // faces x 2 from which early-stopping searches run in rounds
+    // lanes per search of the bisections without early stops
+    // (k_eclipse_bisect_wide; 0 or 1: one lane per search); -1: 16, 8 or 4,
+    // the most that keep faces x lanes within ecl_wide_budget, else one
+    // (measured on the small detached model at nlat 40 ... 120: the wide
+    // search pays off while it does not oversubscribe the device)
+    int ecl_wide = -1;
+    long long ecl_wide_budget = 1LL << 18;
+    int wide_lanes(long long nface) const {
+        if (ecl_wide >= 0) return ecl_wide;
+        for (int g : {16, 8, 4})
+            if (nface * g <= ecl_wide_budget) return g;
+        return 0;
+    }
     GeoCache geo;
     // kernel sequence of the last repeated launch shape, as a graph
     bool graphs = true;
@@ -1615,10 +1628,11 @@ void run_batch(lc_ctx *ctx, const EvalRequest &rq) {
         launch_faces(s, d_grids, d_rings, nring, nhalf_total, fa, d_cand, d_cand_count, d_status);
         mark(ev[1]);
         launch_eclipse(s, d_grids, ngrid, fa, d_cand, d_cand_count, nface_total, ew, d_status, ecl_rounds,
-                       any_ecl_stop);
+                       any_ecl_stop, ctx->wide_lanes(nface_total));
         if (ncomp) {  // the components' eclipses by the stars, then their ranges
             launch_comp_pretest(s, d_cgrids, 2 * ncomp, 2 * NC, cs, d_ccand, d_ccount);
-            launch_eclipse(s, d_cgrids, 2 * ncomp, cs, d_ccand, d_ccount, 2 * NC, cew, d_status, false, false);
+            launch_eclipse(s, d_cgrids, 2 * ncomp, cs, d_ccand, d_ccount, 2 * NC, cew, d_status, false, false,
+                           ctx->wide_lanes(2LL * NC));
             launch_comp_build(s, d_comps, ncomp, NC, cs, ca, d_status);
         }
         // reused grids: cache hits first (a duplicate may copy from one), then
@@ -1995,6 +2009,7 @@ lc_status lc_ctx_create(int device, lc_ctx **out) {
         if (std::getenv("LCURVE_AMD_FULL_ECLIPSE")) c->ecl_stop = false;
         if (const char *g = std::getenv("LCURVE_AMD_GRAPHS")) c->graphs = std::atoi(g) != 0;
         if (const char *r = std::getenv("LCURVE_AMD_ECL_ROUNDS_MIN")) c->ecl_round_min = std::atoll(r);
+        if (const char *w = std::getenv("LCURVE_AMD_ECL_WIDE")) c->ecl_wide = std::atoi(w);
         if (const char *r = std::getenv("LCURVE_AMD_ECL_STOP_MIN_FACES")) c->ecl_stop_min_faces = std::atoll(r);
         *out = c.release();
         return LC_OK;
@@ -2069,8 +2084,10 @@ lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const l
         EvalRequest rq{models, m, data, n, *opts, calc.data(), io.data(), status.data()};
         const lc_profile saved = ctx->prof;
         const int r0 = ctx->flux_rphase, c0 = ctx->flux_chunk;
-        // candidates are timed from the flux-stage events of eager calls
-        // (a replayed graph is timed only as a whole)
+        // candidates are timed from the stage events of eager calls (a
+        // replayed graph is timed only as a whole): the flux sums and the
+        // stage after them, whose merge of the chunk partials grows as the
+        // chunks shrink
         const bool graphs = ctx->graphs;
         ctx->graphs = false;
         struct Restore {  // the context's own settings if a timing run throws
@@ -2093,9 +2110,9 @@ lc_status lc_ctx_autotune(lc_ctx *ctx, const lc_model *models, size_t m, const l
             ctx->flux_chunk = c;
             double ms = 0.;
             for (int rep = 0; rep < 2; rep++) {
-                const double f0 = ctx->prof.flux_ms_total;
+                const double f0 = ctx->prof.flux_ms_total + ctx->prof.other_ms_total;
                 run_eval(ctx, rq);
-                ms += ctx->prof.flux_ms_total - f0;
+                ms += ctx->prof.flux_ms_total + ctx->prof.other_ms_total - f0;
             }
             return ms;
         };
@@ -2576,7 +2593,8 @@ int star_grid_stage(lc_ctx *ctx, const lc_model *m, int star, int fine, lc_star_
     launch_faces(s, d_grid, d_rings, (int)rings.size(), ho, fa, d_cand, d_count, d_status);
     HIPCHK(hipEventRecord(ctx->ev[1], s));
     // full searches gain nothing from rounds; LCURVE_AMD_ECL_ROUNDS_MIN=0 forces them (tests)
-    launch_eclipse(s, d_grid, 1, fa, d_cand, d_count, nf, ew, d_status, ctx->ecl_round_min <= 0, false);
+    launch_eclipse(s, d_grid, 1, fa, d_cand, d_count, nf, ew, d_status, ctx->ecl_round_min <= 0, false,
+                   ctx->wide_lanes(nf));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
     HIPCHK(hipGetLastError());
     int st = 0;

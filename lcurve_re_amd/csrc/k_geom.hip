@@ -504,6 +504,109 @@ __global__ __launch_bounds__(128, LC_ECL_WAVES) void k_eclipse_bisect(const Grid
     }
 }
 
+// The bisections of calls too small to fill the device (no early stops, one
+// round): there the longest chain of fblink calls sets the time.  GL lanes
+// share a search.  Each step evaluates fblink at the GL - 1 points the next
+// LV = log2(GL) bisection steps can visit, in heap order (node 1: the
+// midpoint; node j's children: 2j where the point is not hidden, so pout
+// moves, and 2j + 1 where it is, so pin moves), one point per lane; then
+// every lane of the group replays the LV steps from the ballots.  A node's
+// point is formed by the same FP64 midpoints as in the one-lane loop, and
+// only the nodes on the replayed path count (their status too), so the
+// result equals that loop's bit for bit; the others are discarded work.
+// Every wave runs whole groups and leaves its loops together (the shuffles
+// and ballots see all 64 lanes).
+template <int GL>
+__global__ __launch_bounds__(128, LC_ECL_WAVES) void k_eclipse_bisect_wide(const GridDev *__restrict__ grids,
+                                                                          FaceArrays fa, EclipseWork ew,
+                                                                          int *inst_status) {
+    static_assert(GL == 4 || GL == 8 || GL == 16, "lanes per search");
+    constexpr int LV = GL == 4 ? 2 : (GL == 8 ? 3 : 4);
+    constexpr int PER_WAVE = 64 / GL;
+    const int n = *ew.hit_count * 2;
+    const int lane = (int)(threadIdx.x & 63);
+    const int node_j = lane & (GL - 1);  // this lane's node; 0 evaluates nothing
+    const int gbase = lane & ~(GL - 1);
+    const int depth = node_j ? 31 - __clz(node_j) : 0;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
+    for (int i0 = wave * PER_WAVE; i0 < n; i0 += nwave * PER_WAVE) {
+        const int i = i0 + lane / GL;
+        const bool valid = i < n;
+        int other = PRIMARY, inst = 0;
+        double q = 0., spin = 0., cosi = 0., sini = 0., delta = 0., rref = 0., pref = 0., ACC = 0.;
+        double pin = 0., pout = 0.;
+        V3 r = v3(0., 0., 0.);
+        if (valid) {
+            const int k = i >> 1;
+            const int f = ew.face[k];
+            const GridDev &G = grids[ew.grid[k]];
+            other = G.star == PRIMARY ? SECONDARY : PRIMARY;
+            inst = G.inst;
+            q = G.q;
+            spin = G.spin_other;
+            cosi = G.cosi;
+            sini = G.sini;
+            delta = G.delta;
+            rref = G.rref_other;
+            pref = G.pref_other;
+            r = v3(fa.px[f], fa.py[f], fa.pz[f]);
+            ACC = ew.acc[k];
+            pin = ew.phi[k];
+            pout = ew.lim[i];
+        }
+        bool live = valid && fabs(pin - pout) > delta;
+        int st_fail = ST_OK;
+        while (__ballot(live)) {
+            bool h = false;
+            int st = ST_OK;
+            if (live && node_j) {
+                double a = pin, b = pout;  // walk from the root to this node
+                for (int d = depth - 1; d >= 0; d--) {
+                    const double m = (a + b) / 2.;
+                    if ((node_j >> d) & 1)
+                        a = m;
+                    else
+                        b = m;
+                }
+                st = fblink(q, other, spin, ACC, set_earth(cosi, sini, (a + b) / 2.), r, rref, pref, h);
+            }
+            const unsigned long long hm = __ballot(h), em = __ballot(st != ST_OK);
+            int node = 1;
+#pragma unroll
+            for (int d = 0; d < LV; d++) {
+                const int st_node = __shfl(st, gbase + node, 64);
+                if (live) {
+                    if (!(fabs(pin - pout) > delta)) {
+                        live = false;
+                    } else if ((em >> (gbase + node)) & 1ull) {
+                        st_fail = st_node;
+                        live = false;
+                    } else {
+                        const double pmid = (pin + pout) / 2.;
+                        const bool hh = (hm >> (gbase + node)) & 1ull;
+                        if (hh)
+                            pin = pmid;
+                        else
+                            pout = pmid;
+                        node = 2 * node + (hh ? 1 : 0);
+                    }
+                }
+            }
+            live = live && fabs(pin - pout) > delta;
+        }
+        if (valid && node_j == 0) {
+            if (st_fail) {
+                set_status(inst_status, inst, st_fail);
+                ew.res[i] = 0.;
+            } else {
+                const double x = (pin + pout) / 2.;
+                ew.res[i] = x - floor(x);
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_eclipse_final(FaceArrays fa, EclipseWork ew) {
     const int n = *ew.hit_count;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
@@ -954,7 +1057,7 @@ void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int
 
 void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int2 *cand,
                     const int *cand_count, int max_cand, EclipseWork ew, int *inst_status, bool rounds,
-                    bool stop) {
+                    bool stop, int wide) {
     if (max_cand <= 0 || ngrid <= 0) return;
     const int blocks = std::min((max_cand + 127) / 128, 16384);
     hipLaunchKernelGGL((k_eclipse_potmin<true, LC_FIRST_NT>), dim3(std::min((max_cand + LC_FIRST_NT - 1) / LC_FIRST_NT, 16384)),
@@ -965,6 +1068,17 @@ void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays f
     if (stop)
         hipLaunchKernelGGL(k_eclipse_open, dim3(std::min((2 * max_cand + OPEN_NT * OPEN_PER - 1) / (OPEN_NT * OPEN_PER), 4096)),
                            dim3(OPEN_NT), 0, s, grids, fa, ew);
+    if (!rounds && !stop && (wide == 4 || wide == 8 || wide == 16)) {
+        const int bw = (int)std::min<long long>((2LL * max_cand * wide + 127) / 128, 16384);
+        if (wide == 4)
+            hipLaunchKernelGGL(k_eclipse_bisect_wide<4>, dim3(bw), dim3(128), 0, s, grids, fa, ew, inst_status);
+        else if (wide == 8)
+            hipLaunchKernelGGL(k_eclipse_bisect_wide<8>, dim3(bw), dim3(128), 0, s, grids, fa, ew, inst_status);
+        else
+            hipLaunchKernelGGL(k_eclipse_bisect_wide<16>, dim3(bw), dim3(128), 0, s, grids, fa, ew, inst_status);
+        hipLaunchKernelGGL(k_eclipse_final, dim3(std::min((max_cand + 255) / 256, 4096)), dim3(256), 0, s, fa, ew);
+        return;
+    }
     const int nround = rounds ? ECL_ROUNDS : 1;
     for (int round = 0; round < nround; round++) {
         const int nstep = round == nround - 1 ? (1 << 30) : (round == 0 ? ECL_FIRST_STEPS : ECL_STEPS);

@@ -167,7 +167,7 @@ void launch_faces(hipStream_t s, const GridDev *grids, const RingDev *rings, int
 // round 0, which then runs over the list of the others.
 void launch_eclipse(hipStream_t s, const GridDev *grids, int ngrid, FaceArrays fa, const int2 *cand,
                     const int *cand_count, int max_cand, EclipseWork ew, int *inst_status, bool rounds,
-                    bool stop);
+                    bool stop, int wide);
 // Copies positions, normals, areas, gravities and eclipse ranges of whole
 // grids (not the fluxes, which the continuum kernel recomputes).
 void launch_copy_faces(hipStream_t s, const FaceCopy *items, int nitem, FaceArrays src, FaceArrays dst);
