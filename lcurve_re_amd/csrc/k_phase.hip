@@ -161,7 +161,7 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
 // scal[inst*8]: 0 sfac, 1 chisq, 2 wnok, 3 chisq/wnok written.
 // Scatter of the sub-phase sums onto the data points, third light,
 // optimal scale and chi**2 (light_curve_comp.cpp:476-586, re_scale.cpp:16-43),
-// in three kernels over blocks of SCAT_PTS points so that one large
+// in three kernels over blocks of SCAT_PTS or 4 SCAT_PTS points so that one large
 // instance fills the GPU:
 //   k_scatter_sum    calc before scaling, and per block the scale sums;
 //   k_scatter_scale  the scale (every block sums the instance's partials in
@@ -249,7 +249,7 @@ __device__ __forceinline__ void point_sums(const InstDev &I, const DataDev &d, c
 
 __global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__ insts, DataDev d, PhaseArrays pa,
                                                     const double *__restrict__ out, const double *__restrict__ comp,
-                                                    double *calc, double *part) {
+                                                    double *calc, double *part, int pts) {
     __shared__ double lds[256];
     const int inst = blockIdx.y, nblk = gridDim.x;
     const InstDev &I = insts[inst];
@@ -263,8 +263,8 @@ __global__ __launch_bounds__(256) void k_scatter_sum(const InstDev *__restrict__
     double g5[20];
 #pragma unroll
     for (int q = 0; q < 20; q++) g5[q] = 0.;
-    const int p1 = min(d.n, (int)(blockIdx.x + 1) * SCAT_PTS);
-    for (int np = blockIdx.x * SCAT_PTS + threadIdx.x; np < p1; np += 256) {
+    const int p1 = min(d.n, (int)(blockIdx.x + 1) * pts);
+    for (int np = blockIdx.x * pts + threadIdx.x; np < p1; np += 256) {
         if (I.iscale) {
             // rows of the weighted design matrix: weight > 0 and
             // sigma = ferr / sqrt(weight) > 0 (svdfit.cpp:34-53)
@@ -472,7 +472,7 @@ __device__ __forceinline__ void inst_scale(const InstDev &I, const double *pi, i
 
 __global__ __launch_bounds__(256) void k_scatter_scale(const InstDev *__restrict__ insts, DataDev d, PhaseArrays pa,
                                                       const double *__restrict__ out, const double *__restrict__ comp,
-                                                      double *calc, double *part) {
+                                                      double *calc, double *part, int pts) {
     __shared__ double lds[256];
     const int inst = blockIdx.y, nblk = gridDim.x;
     const InstDev &I = insts[inst];
@@ -489,8 +489,8 @@ __global__ __launch_bounds__(256) void k_scatter_scale(const InstDev *__restrict
     const double zc = I.scale ? 0. : I.zcols;
     double *c = calc + (size_t)inst * d.n;
     double chi = 0., wn2 = 0.;
-    const int p1 = min(d.n, (int)(blockIdx.x + 1) * SCAT_PTS);
-    for (int np = blockIdx.x * SCAT_PTS + threadIdx.x; np < p1; np += 256) {
+    const int p1 = min(d.n, (int)(blockIdx.x + 1) * pts);
+    for (int np = blockIdx.x * pts + threadIdx.x; np < p1; np += 256) {
         double v;
         if (I.iscale && I.has_comp) {
             // sfac[0] f0 + sfac[1] f1 + sfac[2] f2 + sfac[3] f3 (+ sfac[4] f4),
@@ -590,9 +590,12 @@ void launch_route(hipStream_t s, const InstDev *insts, int ninst, int S, PhaseAr
 void launch_scatter(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa, const double *out,
                     const double *comp, double *calc, double *scal, double *part) {
     if (ninst <= 0) return;
-    const int nblk = std::max(1, (d.n + SCAT_PTS - 1) / SCAT_PTS);
-    hipLaunchKernelGGL(k_scatter_sum, dim3(nblk, ninst), dim3(256), 0, s, insts, d, pa, out, comp, calc, part);
-    hipLaunchKernelGGL(k_scatter_scale, dim3(nblk, ninst), dim3(256), 0, s, insts, d, pa, out, comp, calc, part);
+    // four points per thread, or one where that leaves the device short of
+    // workgroups (small calls: the blocks' sequential loops set the time)
+    const int pts = (long long)((d.n + 4 * SCAT_PTS - 1) / (4 * SCAT_PTS)) * ninst >= 1024 ? 4 * SCAT_PTS : SCAT_PTS;
+    const int nblk = std::max(1, (d.n + pts - 1) / pts);
+    hipLaunchKernelGGL(k_scatter_sum, dim3(nblk, ninst), dim3(256), 0, s, insts, d, pa, out, comp, calc, part, pts);
+    hipLaunchKernelGGL(k_scatter_scale, dim3(nblk, ninst), dim3(256), 0, s, insts, d, pa, out, comp, calc, part, pts);
     hipLaunchKernelGGL(k_scatter_final, dim3(ninst), dim3(64), 0, s, insts, nblk, part, scal);
 }
 

@@ -260,10 +260,11 @@ void launch_flux_ragged(hipStream_t s, int variant, const int *order, int nb, in
 void launch_flux_ragged_tail(hipStream_t s, int variant, const GridDev *grid, const BatchDev *batch, FlatArrays fl,
                              const int *moff, const double2 *ranges, PhaseArrays pa, int S, int n1, int nface,
                              int chunk, int c_first, int nchunk, int nchunk_main, size_t stride, double *partial);
-// part: workspace of ninst * ceil(n / SCAT_PTS) * SCAT_PARTS doubles
+// part: workspace of ninst * ceil(n / SCAT_PTS) * SCAT_PARTS doubles (blocks
+// of SCAT_PTS points, one per thread, or 4 SCAT_PTS in calls that fill the device)
 // slots 0-7: scale sums, chi**2, the two-star iscale sums; 8-27: the
 // five-column iscale Gram matrix (15) and right-hand side (5)
-constexpr int SCAT_PTS = 1024, SCAT_PARTS = 28;
+constexpr int SCAT_PTS = 256, SCAT_PARTS = 28;
 // comp: the disc, edge and spot sums of instances with has_comp (else unused)
 void launch_scatter(hipStream_t s, const InstDev *insts, int ninst, DataDev d, PhaseArrays pa, const double *out,
                     const double *comp, double *calc, double *scal, double *part);
